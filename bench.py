@@ -1,0 +1,231 @@
+"""rcontrib Mrays/s on one MI355X: shoebox room + 200 tessellated boxes
+(25,200 quads = 50,400 triangles), 2,500 sensors, -I+ -ab 4 -ad 4096
+-lw 1e-4 -aa 0 -c 1, Reinhart MF:1 sky (145 bins) + ground (1 bin).
+
+A step is one full rcontrib pass over the 2,500 sensors with the scene and
+its BVH resident on the GPU.  Rays are ray segments (closest-hit queries),
+counted on the device.  Also reported: the DC x sky product
+(2500x146 by 146x8760, RGB) on MFMA, the dominant kernel's roofline figures,
+and the reference rcontrib timed on the host cores over a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from pyradiance_amd import _lib, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
+
+# Algorithmic bytes per ray segment: ray record in + hit out (48 B), one
+# 64-byte node per BVH internal node visited, one 48-byte primitive record
+# per primitive tested, one 32-byte path-state read-modify-write.
+B_RAY, B_NODE, B_PRIM, B_STATE = 48.0, 64.0, 48.0, 32.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(seed: int = 0):
+    rad, sensors = scenes.config2(seed)
+    mods = []
+    m = _lib.RgModifier()
+    m.name, m.bin_kind, m.nbins, m.mf, m.rhs = b"skyglow", 1, 145, 1.0, 1.0
+    m.nrm[:] = (0.0, 0.0, -1.0)
+    m.up[:] = (0.0, 1.0, 0.0)
+    g = _lib.RgModifier()
+    g.name, g.bin_kind, g.nbins, g.cbin = b"groundglow", 0, 1, 0
+    mods = [m, g]
+    return rad, sensors, mods
+
+
+def gpu_params():
+    p = _lib.default_params(True)      # rcontrib defaults after override_options
+    p.ambounce, p.ambdiv, p.minweight = 4, 4096, 1e-4
+    p.seed = 1
+    return p
+
+
+def cpu_baseline(rad: str, sensors: np.ndarray, nsample: int, cores: int):
+    """Reference rcontrib (oracle/_ref) on `nsample` sensors of the same workload."""
+    ref = ROOT / "oracle" / "_ref"
+    if not (ref / "rcontrib").exists() or not (ref / "oconv").exists():
+        return None
+    cal = ROOT / "oracle" / "reinhart_sky.cal"
+    with tempfile.TemporaryDirectory() as td:
+        radf = Path(td) / "room.rad"
+        octf = Path(td) / "room.oct"
+        radf.write_text(rad)
+        with open(octf, "wb") as f:
+            subprocess.run([str(ref / "oconv"), "-f", str(radf)], stdout=f, check=True)
+        sample = sensors[:nsample]
+        inp = "\n".join(" ".join(f"{v:.17g}" for v in r) for r in sample).encode()
+        args = ["-h", "-I+", "-ab", "4", "-ad", "4096", "-lw", "1e-4", "-aa", "0", "-c", "1",
+                "-f", str(cal), "-p", "MF=1", "-bn", "Nskybins", "-b", "skybin", "-m", "skyglow",
+                "-b", "0", "-bn", "1", "-m", "groundglow"]
+        env = dict(os.environ, RAYPATH=f".:{ref / 'lib'}:{ROOT / 'oracle'}")
+        # ray count of this sample (single process prints the counter)
+        nr_sample = max(4, nsample // max(cores, 1))
+        inp_part = "\n".join(" ".join(f"{v:.17g}" for v in rr) for rr in sample[:nr_sample]).encode()
+        t0 = time.time()
+        r = subprocess.run([str(ref / "rcontrib_count"), "-n", "1", *args, str(octf)], input=inp_part,
+                           capture_output=True, env=env)
+        t1 = time.time() - t0
+        nrays_part = None
+        for line in r.stderr.decode().splitlines():
+            if line.startswith("NRAYS"):
+                nrays_part = int(line.split()[1])
+        # all cores over the whole sample
+        t0 = time.time()
+        r2 = subprocess.run([str(ref / "rcontrib"), "-n", str(cores), *args, str(octf)], input=inp,
+                            capture_output=True, env=env)
+        wall = time.time() - t0
+        if r2.returncode != 0 or nrays_part is None:
+            return None
+        # octree load: a one-sensor run
+        t0 = time.time()
+        subprocess.run([str(ref / "rcontrib"), "-n", "1", *args, str(octf)], input=inp.split(b"\n")[0] + b"\n",
+                       capture_output=True, env=env)
+        load = time.time() - t0
+        rays_total = nrays_part * (nsample / nr_sample)
+        return {
+            "value": rays_total / max(wall - load, 1e-9) / 1e6,
+            "unit": "Mrays/s",
+            "cores": cores,
+            "kind": "reference",
+            "sample": f"{nsample} of the 2500 sensors, rcontrib -n {cores} wall {wall:.2f}s minus "
+                      f"octree load {load:.2f}s (one-sensor run); rays from a -n 1 counting run on "
+                      f"{nr_sample} sensors ({nrays_part} rays, {t1:.2f}s) scaled by {nsample / nr_sample:.1f}",
+        }
+
+
+def dcsky_bench(dc: np.ndarray, seed: int = 0):
+    nb = dc.shape[1] // 3
+    A = dc.reshape(dc.shape[0], nb, 3).astype(np.float32)
+    B = scenes.sky_matrix(seed, nb)
+    _lib.dcsky(A, B, 1)     # warm-up
+    t0 = time.time()
+    reps = 5
+    for _ in range(reps):
+        Cm = _lib.dcsky(A, B, 1)
+    wall = (time.time() - t0) / reps
+    ref = np.einsum("mkc,knc->mnc", A.astype(np.float64), B.astype(np.float64), optimize=True)
+    denom = np.maximum(np.abs(ref), 1e-30)
+    nz = np.abs(ref) > 1e-6 * np.abs(ref).max()
+    rel = np.abs(Cm - ref)[nz] / denom[nz]
+    M, K, N = A.shape[0], nb, B.shape[1]
+    return {"M": M, "K": K, "N": N, "ms_host_to_host": wall * 1e3,
+            "max_rel_err_vs_fp64": float(rel.max()), "mean_rel_err_vs_fp64": float(rel.mean()),
+            "flops": 2.0 * M * K * N * 3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=480)
+    ap.add_argument("--sensors", type=int, default=2500)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the forward pass shards "
+              "by sensor rows with no communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    rad, sensors, mods = workload(0)
+    sensors = sensors[: args.sensors]
+    ctx = _lib.Context(0)
+    t0 = time.time()
+    ctx.load_text(rad).commit()
+    t_commit = time.time() - t0
+    info = ctx.info()
+    log(f"scene: {info}, commit+BVH {t_commit * 1e3:.1f} ms")
+    ctx.set_params(gpu_params())
+    flags = _lib.RT_IMM_IRRAD
+
+    for i in range(args.warmup):
+        ctx.contrib(sensors, mods, 1, flags)
+        log(f"warmup {i} done: {ctx.last_nrays()} rays")
+    times, rays = [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        out = ctx.contrib(sensors, mods, 1, flags)
+        times.append(time.perf_counter() - t0)
+        rays.append(ctx.last_nrays())
+        log(f"step {i}: {times[-1] * 1e3:.1f} ms, {rays[-1]} rays")
+    ms = 1e3 * float(np.mean(times))
+    value = float(np.sum(rays)) / float(np.sum(times)) / 1e6
+
+    # roofline of the generation kernel: event timing run, then a counting run
+    ctx.set_profiling(1)
+    ctx.contrib(sensors, mods, 1, flags)
+    st_t = ctx.last_stats()
+    ctx.set_profiling(2)
+    ctx.contrib(sensors, mods, 1, flags)
+    st_c = ctx.last_stats()
+    ctx.set_profiling(0)
+    nr = st_c["rays"]
+    per_ray = B_RAY + B_NODE * st_c["nodes"] / nr + B_PRIM * st_c["prims"] / nr + B_STATE
+    avg_launch_ms = st_t["bounce_ms"] / max(st_t["launches"], 1)
+    bytes_per_launch = per_ray * st_t["rays"] / max(st_t["launches"], 1)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    log(f"k_bounce: {st_t['launches']:.0f} launches, {st_t['bounce_ms']:.1f} ms total; "
+        f"{st_c['nodes'] / nr:.1f} nodes, {st_c['prims'] / nr:.1f} prims per ray; {per_ray:.0f} B/ray")
+
+    dcs = dcsky_bench(out)
+    log(f"dcsky: {dcs}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cores = min(os.cpu_count() or 1, 16)
+        try:
+            cpu = cpu_baseline(rad, sensors, args.cpu_sample, cores)
+        except Exception as e:  # the GPU figure stands on its own
+            log(f"cpu baseline failed: {e}")
+
+    line = {
+        "metric": "rcontrib Mrays/s on 1 MI355X (config 2: -ab 4 -ad 4096, 2500 sensors, 50k triangles)",
+        "value": value,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 geometry / f32 BVH+colour",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "rcontrib -I+ -ab 4 -ad 4096 -lw 1e-4 -aa 0 -c 1, Reinhart MF:1 + ground",
+                   "sensors": int(len(sensors)), "primitives": int(info["primitives"]),
+                   "rays_per_step": int(np.mean(rays)), "scene_commit_ms": t_commit * 1e3},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_bounce", "avg_launch_ms": avg_launch_ms,
+                     "bytes_per_ray": per_ray, "nodes_per_ray": st_c["nodes"] / nr,
+                     "prims_per_ray": st_c["prims"] / nr},
+        "dcsky": dcs,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

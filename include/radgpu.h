@@ -1,0 +1,114 @@
+/*
+ * radgpu.h - C ABI of the MI355X rtrace / rcontrib / dctimestep hot path.
+ *
+ * Plain pointers and sizes only.  Every call returns 0 on success and a
+ * negative status on failure; rg_last_error() then returns the message.
+ * Nothing in the library calls exit(): the reference's error() -> quit()
+ * chain (common/rterror.h:30-35) becomes a status code here.
+ *
+ * A context owns one scene, its device tables and BVH, and one HIP stream.
+ * Contexts are independent (the reference allows one simulation manager per
+ * process, rt/RtraceSimulManager.cpp:237); calls on one context must not
+ * overlap.
+ */
+#ifndef RADGPU_H
+#define RADGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rg_ctx rg_ctx;
+
+/* Ray-tracing options: the subset of RAYPARAMS (rt/ray.h:157-190) the GPU
+ * path honours.  Field meaning and units follow the reference options. */
+typedef struct rg_params {
+  int ambounce;        /* -ab */
+  int ambdiv;          /* -ad */
+  int maxdepth;        /* -lr */
+  int backvis;         /* -bv */
+  double minweight;    /* -lw */
+  double dstrsrc;      /* -dj */
+  double srcsizerat;   /* -ds (local sources are sampled unpartitioned) */
+  double specthresh;   /* -st */
+  double specjitter;   /* -ss */
+  double ambval[3];    /* -av */
+  int directvis;       /* -dv */
+  int contrib;         /* rcontrib -V */
+  unsigned long long seed;
+} rg_params;
+
+/* rcontrib modifier with its bin function (rt/rcontrib.c:92 addmodifier).
+ * bin_kind: 0 constant bin `cbin`, 1 Reinhart rbin (util/reinhartb.cal),
+ * 2/3/4 Klems full/half/quarter kbin (util/klems_*.cal). */
+typedef struct rg_modifier {
+  const char *name;
+  int bin_kind;
+  int nbins;
+  int cbin;
+  double mf;           /* Reinhart MF */
+  double rhs;          /* RHS: +1 right-handed */
+  double nrm[3];       /* rNx,rNy,rNz (Reinhart) / Nx,Ny,Nz (Klems) */
+  double up[3];        /* Ux,Uy,Uz */
+} rg_modifier;
+
+/* Replaces process start-up + ray_init() (rt/raycalls.c:172-200). */
+int rg_ctx_create(rg_ctx **out, int device);
+void rg_ctx_destroy(rg_ctx *ctx);
+/* Message of the last failed call on ctx (or of rg_ctx_create if ctx==NULL). */
+const char *rg_last_error(const rg_ctx *ctx);
+
+/* Scene input, replaces readoct() (common/readoct.c:36) and readobj()
+ * (common/readobj.c:28): `path` is an octree or a scene file.  Inline
+ * "!command" lines run only when allow_exec is non-zero. */
+int rg_scene_load(rg_ctx *ctx, const char *path, int allow_exec);
+/* Appends scene text, as readobj() on a stream. */
+int rg_scene_load_text(rg_ctx *ctx, const char *text, int allow_exec);
+/* Compiles the objects, uploads them and builds the BVH on the GPU
+ * (replaces marksources(), rt/source.c:47, and the octree). */
+int rg_scene_commit(rg_ctx *ctx);
+/* info[0..5] = objects, primitives, materials, sources, BVH nodes, warnings */
+int rg_scene_info(rg_ctx *ctx, long long *info, int n);
+
+/* Replaces getrenderopt() option state (rt/renderopts.c:124). */
+int rg_set_params(rg_ctx *ctx, const rg_params *p);
+/* Library defaults of rt/raycalls.c:118-153 (what rtrace uses). */
+void rg_default_params(rg_params *p);
+/* rcontrib defaults after override_options() (rt/rcontrib.c:24-58, rt/rcmain.c:164-171). */
+void rg_default_params_rcontrib(rg_params *p);
+
+/* rtrace: one value per ray (rt/rtrace.c:436 rtcompute).
+ * rays: n x 6 doubles (origin, direction).  flags: 8 = -I (RTimmIrrad of
+ * rt/RtraceSimulManager.h:86), 16 = -i.  out: n x 3 doubles (RGB value). */
+int rg_trace(rg_ctx *ctx, const double *rays, long long n, int flags, double *out);
+
+/* rcontrib: contributions binned per modifier (rt/rcontrib.c:272-338,
+ * rt/rc2.c:294).  rays: nrec*accum x 6 doubles; each record averages
+ * `accum` consecutive rays.  out: nrec x (3 * sum of nbins) doubles in
+ * modifier order.  flags as rg_trace. */
+int rg_contrib(rg_ctx *ctx, const double *rays, long long nrec, int accum, int flags,
+               const rg_modifier *mods, int nmods, double *out);
+
+/* Ray segments traced by the last rg_trace / rg_contrib (closest-hit
+ * queries, the reference's nrays of rt/raytrace.c:606). */
+int rg_last_nrays(rg_ctx *ctx, unsigned long long *nrays);
+
+/* Measurement support (no reference counterpart).  level 0: off, 1: time
+ * every generation kernel with HIP events on the context stream, 2: also
+ * count BVH nodes visited and primitives tested per ray. */
+int rg_set_profiling(rg_ctx *ctx, int level);
+/* out[0..6] = rays, generation-kernel launches, threads launched, summed
+ * generation-kernel milliseconds, nodes visited, primitives tested, batches */
+int rg_last_stats(rg_ctx *ctx, double *out, int n);
+
+/* DC x sky product, replaces cm_multiply() (util/cmatrix.c:421-474):
+ * C[M][N][3] = A[M][K][3] . B[K][N][3], each channel independently.
+ * Host pointers.  mode 0 = FP32 accumulate, 1 = BF16 hi/lo split MFMA. */
+int rg_dcsky(const float *A, const float *B, float *C, int M, int K, int N, int mode);
+/* Same with device pointers on the current HIP device. */
+int rg_dcsky_device(const float *A, const float *B, float *C, int M, int K, int N, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,192 @@
+"""ctypes binding of libradgpu.so (declarations in include/radgpu.h).
+
+The library is the product path: there is no CPU fallback.  If the shared
+library is missing or no HIP device is present, calls raise RuntimeError.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIBPATH = Path(os.environ.get("RADGPU_LIB", Path(__file__).resolve().parent / "libradgpu.so"))
+
+# symbols declared in include/radgpu.h
+EXPORTS = (
+    "rg_ctx_create", "rg_ctx_destroy", "rg_last_error", "rg_scene_load", "rg_scene_load_text",
+    "rg_scene_commit", "rg_scene_info", "rg_set_params", "rg_default_params",
+    "rg_default_params_rcontrib", "rg_trace", "rg_contrib", "rg_last_nrays", "rg_set_profiling",
+    "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
+)
+
+RT_IMM_IRRAD = 8      # -I  (RTimmIrrad, rt/RtraceSimulManager.h:86)
+RT_LAMB_IRRAD = 16    # -i
+
+
+class RgParams(C.Structure):
+    _fields_ = [
+        ("ambounce", C.c_int), ("ambdiv", C.c_int), ("maxdepth", C.c_int), ("backvis", C.c_int),
+        ("minweight", C.c_double), ("dstrsrc", C.c_double), ("srcsizerat", C.c_double),
+        ("specthresh", C.c_double), ("specjitter", C.c_double), ("ambval", C.c_double * 3),
+        ("directvis", C.c_int), ("contrib", C.c_int), ("seed", C.c_ulonglong),
+    ]
+
+
+class RgModifier(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p), ("bin_kind", C.c_int), ("nbins", C.c_int), ("cbin", C.c_int),
+        ("mf", C.c_double), ("rhs", C.c_double), ("nrm", C.c_double * 3), ("up", C.c_double * 3),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libradgpu.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBPATH.exists():
+        raise RuntimeError(f"{LIBPATH} is not built (run `python -m pyradiance_amd.build`)")
+    L = C.CDLL(str(LIBPATH))
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
+    L.rg_ctx_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.rg_ctx_destroy.argtypes = [vp]
+    L.rg_ctx_destroy.restype = None
+    L.rg_last_error.argtypes = [vp]
+    L.rg_last_error.restype = C.c_char_p
+    L.rg_scene_load.argtypes = [vp, C.c_char_p, C.c_int]
+    L.rg_scene_load_text.argtypes = [vp, C.c_char_p, C.c_int]
+    L.rg_scene_commit.argtypes = [vp]
+    L.rg_scene_info.argtypes = [vp, C.POINTER(C.c_longlong), C.c_int]
+    L.rg_set_params.argtypes = [vp, C.POINTER(RgParams)]
+    L.rg_default_params.argtypes = [C.POINTER(RgParams)]
+    L.rg_default_params.restype = None
+    L.rg_default_params_rcontrib.argtypes = [C.POINTER(RgParams)]
+    L.rg_default_params_rcontrib.restype = None
+    L.rg_trace.argtypes = [vp, dp, C.c_longlong, C.c_int, dp]
+    L.rg_contrib.argtypes = [vp, dp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, dp]
+    L.rg_last_nrays.argtypes = [vp, C.POINTER(C.c_ulonglong)]
+    L.rg_set_profiling.argtypes = [vp, C.c_int]
+    L.rg_last_stats.argtypes = [vp, dp, C.c_int]
+    L.rg_dcsky.argtypes = [fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.rg_dcsky_device.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    _lib = L
+    return L
+
+
+def default_params(rcontrib: bool = False) -> RgParams:
+    p = RgParams()
+    if rcontrib:
+        lib().rg_default_params_rcontrib(C.byref(p))
+    else:
+        lib().rg_default_params(C.byref(p))
+    return p
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One scene on one GPU (an rg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        if self._L.rg_ctx_create(C.byref(h), device) != 0:
+            raise RuntimeError(self._L.rg_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise RuntimeError(self._L.rg_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def load(self, path: str | os.PathLike, allow_exec: bool = False) -> "Context":
+        self._check(self._L.rg_scene_load(self._h, str(path).encode(), int(allow_exec)))
+        return self
+
+    def load_text(self, text: str, allow_exec: bool = False) -> "Context":
+        self._check(self._L.rg_scene_load_text(self._h, text.encode(), int(allow_exec)))
+        return self
+
+    def commit(self) -> "Context":
+        self._check(self._L.rg_scene_commit(self._h))
+        return self
+
+    def info(self) -> dict:
+        v = (C.c_longlong * 6)()
+        self._check(self._L.rg_scene_info(self._h, v, 6))
+        keys = ("objects", "primitives", "materials", "sources", "bvh_nodes", "warnings")
+        return dict(zip(keys, list(v)))
+
+    def set_params(self, p: RgParams):
+        self._check(self._L.rg_set_params(self._h, C.byref(p)))
+
+    def trace(self, rays: np.ndarray, flags: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
+        out = np.zeros((rays.shape[0], 3))
+        self._check(self._L.rg_trace(self._h, _dptr(rays), rays.shape[0], flags, _dptr(out)))
+        return out
+
+    def contrib(self, rays: np.ndarray, mods: list, accum: int = 1, flags: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
+        nrec = rays.shape[0] // accum
+        arr = (RgModifier * len(mods))(*mods)
+        ncol = 3 * sum(m.nbins for m in mods)
+        out = np.zeros((nrec, ncol))
+        self._check(self._L.rg_contrib(self._h, _dptr(rays), nrec, accum, flags, arr, len(mods), _dptr(out)))
+        return out
+
+    def set_profiling(self, level: int):
+        self._check(self._L.rg_set_profiling(self._h, level))
+
+    def last_stats(self) -> dict:
+        v = (C.c_double * 7)()
+        self._check(self._L.rg_last_stats(self._h, v, 7))
+        keys = ("rays", "launches", "threads", "bounce_ms", "nodes", "prims", "batches")
+        return dict(zip(keys, list(v)))
+
+    def last_nrays(self) -> int:
+        n = C.c_ulonglong()
+        self._check(self._L.rg_last_nrays(self._h, C.byref(n)))
+        return int(n.value)
+
+
+def dcsky(A: np.ndarray, B: np.ndarray, mode: int = 1) -> np.ndarray:
+    """C[m,n,:] = sum_k A[m,k,:] * B[k,n,:] per channel on the GPU."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    M, K, _ = A.shape
+    K2, N, _ = B.shape
+    if K2 != K or A.shape[2] != 3 or B.shape[2] != 3:
+        raise ValueError("matrix dimensions do not match")
+    Cm = np.empty((M, N, 3), dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    L = lib()
+    if L.rg_dcsky(A.ctypes.data_as(fp), B.ctypes.data_as(fp), Cm.ctypes.data_as(fp), M, K, N, mode) != 0:
+        raise RuntimeError(L.rg_last_error(None).decode())
+    return Cm

@@ -1,0 +1,261 @@
+// Ray-primitive tests and the closest-hit BVH walk.
+//
+// The primitive tests restate the reference's FP64 intersection code:
+//   polygon  rt/o_face.c:16-62 + common/face.c:121-162 (crossing-number test)
+//   sphere   rt/sphere.c:16-83, quadratic() common/zeroes.c:19-55
+//   cone/cup/cylinder/tube/ring  rt/o_cone.c:17-146, getcone() common/cone.c
+// and the tie rule for coincident hits of rt/raytrace.c:536-575 (rayreject).
+// BVH boxes are FP32, widened at build time, and only prune work; every
+// accepted hit is decided by the FP64 test.
+#pragma once
+#include "rg_math.hpp"
+
+namespace rg {
+
+// Two child boxes per internal node: one 64-byte fetch tests both children.
+struct alignas(16) BVHNode {
+  float lmin[3]; int32_t left;    // child ref: >=0 internal node, <0 leaf ~pos
+  float lmax[3]; int32_t right;
+  float rmin[3]; int32_t pad0;
+  float rmax[3]; int32_t pad1;
+};
+
+struct DevScene {
+  const Prim* prims;
+  const double* geom;
+  const Material* mats;
+  const Source* srcs;
+  const BVHNode* nodes;
+  const int32_t* leaf_prim;     // sorted leaf position -> primitive index
+  int32_t nprims;
+  int32_t nmats;
+  int32_t nsrcs;
+  int32_t root;                 // root node (<0: single leaf)
+  int32_t lamb;                 // index of the built-in Lambertian material
+  int32_t pad;
+  double cusize;                // scene cube size (ambient distance clamp)
+};
+
+struct HitRec {
+  double t;                     // rot
+  double rod;
+  int32_t prim;                 // -1: none
+};
+
+RG_HD bool ray_reject(const DevScene& S, int prim, double t, double rod, const HitRec& h) {
+  if ((t <= FTINY) | (t > h.t + FTINY)) return true;
+  if (t < h.t - FTINY) return false;
+  if (h.prim == prim) return true;
+  if (h.prim < 0) return false;
+  int mn = S.prims[prim].mat, mr = S.prims[h.prim].mat;
+  if (mn < 0) {
+    if (mr >= 0) return true;
+  } else if (mr < 0) {
+    return false;
+  } else if (S.mats[mn].transp) {
+    if (!S.mats[mr].transp) return true;
+  } else if (S.mats[mr].transp) {
+    return false;
+  }
+  if (rod <= 0) {
+    if (h.rod > 0) return true;
+  } else if (h.rod <= 0) {
+    return false;
+  }
+  return S.prims[h.prim].omod >= S.prims[prim].omod;
+}
+
+// polygon geometry block: n[3], offset, ax, then nv vertices
+RG_HD bool in_face(V3 p, const double* g, int nv) {
+  int ax = (int)g[4];
+  int xi = ax + 1; if (xi >= 3) xi -= 3;
+  int yi = xi + 1; if (yi >= 3) yi -= 3;
+  double x = comp(p, xi), y = comp(p, yi);
+  const double* va = g + 5;
+  const double* p0 = va + 3 * (nv - 1);
+  const double* p1 = va;
+  int ncross = 0;
+  for (int n = nv; n--;) {
+    if (fabs(p0[yi] - y) <= FTINY && fabs(p1[yi] - y) <= FTINY && ((p0[xi] > x) ^ (p1[xi] > x)))
+      return true;
+    if ((p0[yi] > y) ^ (p1[yi] > y)) {
+      int tst = (p0[xi] > x) + (p1[xi] > x);
+      if (tst == 2) {
+        ncross++;
+      } else if (tst) {
+        double prodA = (p0[yi] - y) * (p1[xi] - x);
+        double prodB = (p0[xi] - x) * (p1[yi] - y);
+        if (fabs(prodA - prodB) <= FTINY) return true;
+        ncross += (p1[yi] > p0[yi]) ^ (prodA > prodB);
+      } else if (fabs(p0[xi] - x) <= FTINY && fabs(p1[xi] - x) <= FTINY) {
+        return true;
+      }
+    }
+    p0 = p1;
+    p1 += 3;
+  }
+  return ncross & 1;
+}
+
+RG_HD int quadratic(double r[2], double a, double b, double c) {
+  int first;
+  if (a < -FTINY) first = 1;
+  else if (a > FTINY) first = 0;
+  else if (fabs(b) > FTINY) { r[0] = -c / b; return 1; }
+  else return 0;
+  b *= 0.5;
+  double disc = b * b - a * c;
+  if (disc < -FTINY * FTINY) return 0;
+  if (disc <= FTINY * FTINY) { r[0] = -b / a; return 1; }
+  disc = sqrt(disc);
+  r[first] = (-b - disc) / a;
+  r[1 - first] = (-b + disc) / a;
+  return 2;
+}
+
+// Test one primitive; updates h when it wins.
+RG_HD void hit_prim(const DevScene& S, int pi, V3 o, V3 d, HitRec& h) {
+  const Prim& P = S.prims[pi];
+  const double* g = S.geom + P.goff;
+  switch (P.kind) {
+    case PK_POLYGON: {
+      V3 n = load3(g);
+      double rdot = -dot(d, n);
+      if ((rdot <= FTINY) & (rdot >= -FTINY)) return;
+      double t = (dot(o, n) - g[3]) / rdot;
+      if (ray_reject(S, pi, t, rdot, h)) return;
+      V3 p = o + d * t;
+      if (!in_face(p, g, P.nv)) return;
+      h.t = t; h.rod = rdot; h.prim = pi;
+      return;
+    }
+    case PK_SPHERE:
+    case PK_BUBBLE: {
+      double a = dot(d, d);
+      V3 oc = o - load3(g);
+      double b = 2.0 * dot(d, oc);
+      double c = dot(oc, oc) - g[3] * g[3];
+      double root[2];
+      int nr = quadratic(root, a, b, c);
+      int i;
+      double t = 0;
+      for (i = 0; i < nr; i++)
+        if ((t = root[i]) > FTINY) break;
+      if (i >= nr) return;
+      double rod = 1 - 2 * ((i > 0) ^ (P.kind == PK_BUBBLE));
+      if (ray_reject(S, pi, t, rod, h)) return;
+      // the reference's rod for the hit is recomputed from the normal
+      double rad = g[3] * (1 - 2 * (P.kind == PK_BUBBLE));
+      V3 p = o + d * t;
+      V3 nrm = (p - load3(g)) * (1.0 / rad);
+      h.t = t; h.rod = -dot(d, nrm); h.prim = pi;
+      return;
+    }
+    case PK_RING: {
+      // g: centre[3], axis[3] (unit), r0, r1
+      V3 ad = load3(g + 3);
+      double dz = dot(d, ad);
+      if ((dz <= FTINY) & (dz >= -FTINY)) return;
+      double t = -dot(o - load3(g), ad) / dz;
+      if (ray_reject(S, pi, t, -dz, h)) return;
+      V3 p = o + d * t - load3(g);
+      double a = dot(p, p);   // in-plane radius^2 (p lies in the ring plane)
+      if (a > g[7] * g[7] || a < g[6] * g[6]) return;
+      h.t = t; h.rod = -dz; h.prim = pi;
+      return;
+    }
+    default: {
+      // cone, cup, cylinder, tube. g: p0[3], ad[3] (unit axis), al, r0, r1, sl
+      V3 p0 = load3(g), ad = load3(g + 3);
+      double al = g[6], r0 = g[7], r1 = g[8];
+      bool cyl = (P.kind == PK_CYLINDER) | (P.kind == PK_TUBE);
+      V3 w = o - p0;
+      double dd = dot(d, ad), wd = dot(w, ad);
+      V3 dp = d - ad * dd, wp = w - ad * wd;        // components normal to the axis
+      double a, b, c;
+      if (cyl) {
+        a = dot(dp, dp);
+        b = 2.0 * dot(dp, wp);
+        c = dot(wp, wp) - r0 * r0;
+      } else {
+        // radius along the axis: rr(h) = r0 + k*h, k = (r1-r0)/al
+        double k = (r1 - r0) / al;
+        double rw = r0 + k * wd, rd = k * dd;
+        a = dot(dp, dp) - rd * rd;
+        b = 2.0 * (dot(dp, wp) - rw * rd);
+        c = dot(wp, wp) - rw * rw;
+      }
+      double root[2];
+      int nr = quadratic(root, a, b, c);
+      bool inward = (P.kind == PK_CUP) | (P.kind == PK_TUBE);
+      for (int rn = 0; rn < nr; rn++) {
+        double t = root[rn];
+        if (t <= FTINY) continue;
+        if (t > h.t + FTINY) break;
+        V3 p = o + d * t;
+        double hb = dot(p - p0, ad);
+        if (hb < 0.0 || hb > al) continue;
+        if (!cyl) {      // reject the mirror nappe of the double cone
+          double rr = r0 + (r1 - r0) * hb / al;
+          if (rr < 0.0) continue;
+        }
+        double rod_sign = 1 - 2 * ((rn > 0) ^ inward);
+        if (ray_reject(S, pi, t, rod_sign, h)) break;
+        V3 rdx = p - p0;
+        V3 nrm;
+        if (cyl) {
+          double aa = (P.kind == PK_CYLINDER) ? r0 : -r0;
+          nrm = (rdx - ad * hb) * (1.0 / aa);
+        } else {
+          double cc = r1 - r0;
+          double aa = r0 + hb * cc / al;
+          if (P.kind == PK_CUP) { cc = -cc; aa = -aa; }
+          nrm = (rdx - ad * hb) * (1.0 / aa);
+          nrm = (nrm * al - ad * cc) * (1.0 / g[9]);
+        }
+        double l2 = dot(nrm, nrm);
+        if ((l2 > 1. + FTINY) | (l2 < 1. - FTINY)) nrm = nrm * (1. / (.5 + .5 * l2));
+        h.t = t; h.rod = -dot(d, nrm); h.prim = pi;
+        return;
+      }
+      return;
+    }
+  }
+}
+
+// Surface normal at a hit (the reference's ron), FP64.
+RG_HD V3 prim_normal(const DevScene& S, int pi, V3 p) {
+  const Prim& P = S.prims[pi];
+  const double* g = S.geom + P.goff;
+  switch (P.kind) {
+    case PK_POLYGON: return load3(g);
+    case PK_SPHERE:
+    case PK_BUBBLE: {
+      double rad = g[3] * (1 - 2 * (P.kind == PK_BUBBLE));
+      return (p - load3(g)) * (1.0 / rad);
+    }
+    case PK_RING: return load3(g + 3);
+    default: {
+      V3 p0 = load3(g), ad = load3(g + 3);
+      double al = g[6], r0 = g[7], r1 = g[8];
+      V3 rdx = p - p0;
+      double hb = dot(rdx, ad);
+      V3 nrm;
+      if ((P.kind == PK_CYLINDER) | (P.kind == PK_TUBE)) {
+        double aa = (P.kind == PK_CYLINDER) ? r0 : -r0;
+        nrm = (rdx - ad * hb) * (1.0 / aa);
+      } else {
+        double cc = r1 - r0;
+        double aa = r0 + hb * cc / al;
+        if (P.kind == PK_CUP) { cc = -cc; aa = -aa; }
+        nrm = (rdx - ad * hb) * (1.0 / aa);
+        nrm = (nrm * al - ad * cc) * (1.0 / g[9]);
+      }
+      double l2 = dot(nrm, nrm);
+      if ((l2 > 1. + FTINY) | (l2 < 1. - FTINY)) nrm = nrm * (1. / (.5 + .5 * l2));
+      return nrm;
+    }
+  }
+}
+
+}  // namespace rg

@@ -1,0 +1,986 @@
+// Wavefront renderer for the rtrace / rcontrib ray tree on CDNA4 (gfx950).
+//
+// One fused kernel per generation of the tree: each thread takes one child of
+// a spawn record (binary search over the exclusive scan of child counts),
+// applies the spawn rules of rayorigin() (rt/raytrace.c:40-135), walks the BVH
+// for the closest hit, shades the hit and appends the spawn records of the
+// next generation.  Spawn records are allocated with one atomic per wavefront
+// (ballot + mbcnt), emission is added into the sensor's output with FP64
+// atomics.  The BVH stack lives in LDS, column-major per lane.
+#include <hip/hip_runtime.h>
+#include "rg_shade.hpp"
+#include "rg_render.hpp"
+
+namespace rg {
+
+constexpr int BLOCK = 128;
+constexpr int STACK = 64;
+
+// ---- wave-level helpers -----------------------------------------------------
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+// Allocate one slot per active lane with a single atomic per wavefront.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr) {
+  unsigned long long m = __ballot(1);
+  uint32_t leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (__lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  return base + lane_rank(m);
+}
+__device__ __forceinline__ void wave_count(unsigned long long* ctr) {
+  unsigned long long m = __ballot(1);
+  uint32_t leader = __ffsll((long long)m) - 1;
+  if (__lane_id() == leader) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+
+struct Sink {
+  Spawn* buf;
+  uint32_t* count;
+  uint32_t cap;
+  uint32_t* err;
+};
+
+__device__ __forceinline__ void emit(const Sink& sk, const Spawn& s) {
+  uint32_t i = wave_alloc(sk.count);
+  if (i < sk.cap) sk.buf[i] = s;
+  else atomicOr(sk.err, ERR_CAPACITY);
+}
+
+__device__ __forceinline__ void deposit(const RunCfg& c, uint32_t sensor, int col, C3 v) {
+  double* o = c.out + (size_t)sensor * c.ncols + col;
+  unsafeAtomicAdd(o + 0, (double)v.r);
+  unsafeAtomicAdd(o + 1, (double)v.g);
+  unsafeAtomicAdd(o + 2, (double)v.b);
+}
+
+// ---- BVH closest hit (FP32 boxes, FP64 primitive tests) --------------------
+__device__ __forceinline__ bool slab(const float* lo, const float* hi, float ox, float oy, float oz,
+                                     float ix, float iy, float iz, float tmax, float& tn) {
+  float t0x = (lo[0] - ox) * ix, t1x = (hi[0] - ox) * ix;
+  float t0y = (lo[1] - oy) * iy, t1y = (hi[1] - oy) * iy;
+  float t0z = (lo[2] - oz) * iz, t1z = (hi[2] - oz) * iz;
+  float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+  float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+  tn = tnear;
+  return (tnear <= tfar) & (tfar >= 0.0f) & (tnear <= tmax);
+}
+
+__device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* stk, uint32_t* err,
+                            unsigned long long* visits) {
+  uint32_t nvis = 0, nprim = 0;
+  h.t = FHUGE; h.rod = 1.0; h.prim = -1;
+  if (S.nprims == 0) return;
+  if (S.root < 0) { hit_prim(S, S.leaf_prim[~S.root], o, d, h); return; }
+  auto tested = [&]() { ++nprim; };
+  const float ox = (float)o.x, oy = (float)o.y, oz = (float)o.z;
+  auto inv = [](double v) -> float {
+    float f = (float)v;
+    if (fabsf(f) < 1e-20f) f = f < 0 ? -1e-20f : 1e-20f;
+    return 1.0f / f;
+  };
+  const float ix = inv(d.x), iy = inv(d.y), iz = inv(d.z);
+  const int lane = threadIdx.x;
+  int sp = 0;
+  int node = S.root;
+  // every internal node is entered at most once per ray
+  for (int iter = 0;; ++iter) {
+    if (iter > S.nprims + 8) { atomicOr(err, ERR_STACK); break; }
+    const BVHNode N = S.nodes[node];
+    ++nvis;
+    float tmax = (float)(h.t + 2 * FTINY) * 1.000001f;
+    float tl, tr;
+    bool hl = slab(N.lmin, N.lmax, ox, oy, oz, ix, iy, iz, tmax, tl);
+    bool hr = slab(N.rmin, N.rmax, ox, oy, oz, ix, iy, iz, tmax, tr);
+    if (hl & (N.left < 0)) { hit_prim(S, S.leaf_prim[~N.left], o, d, h); hl = false; tested(); }
+    if (hr & (N.right < 0)) { hit_prim(S, S.leaf_prim[~N.right], o, d, h); hr = false; tested(); }
+    if (hl & hr) {
+      int nearc = tl <= tr ? N.left : N.right;
+      int farc = tl <= tr ? N.right : N.left;
+      if (sp < STACK) {
+        stk[sp * BLOCK + lane] = farc;
+        ++sp;
+      } else {
+        atomicOr(err, ERR_STACK);
+      }
+      node = nearc;
+    } else if (hl) {
+      node = N.left;
+    } else if (hr) {
+      node = N.right;
+    } else {
+      if (sp == 0) break;
+      --sp;
+      node = stk[sp * BLOCK + lane];
+    }
+  }
+  if (visits) {
+    atomicAdd(visits, (unsigned long long)nvis);
+    atomicAdd(visits + 1, (unsigned long long)nprim);
+  }
+}
+
+// ---- spawn helpers ----------------------------------------------------------
+// Child ray of type rt with own coefficient rc (nullptr: rayorigin(...,NULL)).
+__device__ Spawn make_child(const RayS& r, V3 org, uint16_t rt, const C3* rc, uint64_t slot, uint8_t flags) {
+  Spawn s;
+  s.org[0] = org.x; s.org[1] = org.y; s.org[2] = org.z;
+  s.d0[0] = s.d0[1] = s.d0[2] = 0.f;
+  s.d1[0] = s.d1[1] = s.d1[2] = 0.f;
+  double rw = 1.0;
+  C3 own = c3(1, 1, 1);
+  if (rc) {
+    own = *rc;
+    rw = cmax(own);
+    if (rw > 1.0) rw = 1.0;
+    flags |= SPF_RR;
+  }
+  C3 cum = r.coef * own;
+  s.coef[0] = cum.r; s.coef[1] = cum.g; s.coef[2] = cum.b;
+  s.cw = (float)(r.w * rw);
+  s.crc = (float)cmax(own);
+  s.sensor = r.sensor;
+  s.count = 1;
+  s.ns = 0;
+  s.rtype = rt;
+  s.crtype = r.crtype | rt;
+  if (rt & RAYREFL) { s.rlvl = (uint8_t)(r.rlvl + 1); s.rsrc = -1; }
+  else { s.rlvl = (uint8_t)r.rlvl; s.rsrc = (int16_t)r.rsrc; }
+  s.adepth = (uint8_t)r.adepth;
+  s.flags = flags;
+  s.pad = 0;
+  s.key = child_key(r.key, slot);
+  return s;
+}
+
+__device__ __forceinline__ void set_dir(Spawn& s, V3 d) {
+  s.d0[0] = (float)d.x; s.d0[1] = (float)d.y; s.d0[2] = (float)d.z;
+}
+
+// rayorigin() admission for a child built from a spawn record.
+// Returns false when the child is not traced.
+__device__ bool admit(const RunCfg& c, RayS& r, uint8_t flags) {
+  if (!(r.w > 0.0f)) return (flags & SPF_FORCE) != 0;
+  if (r.crtype & SHADOW) return true;
+  const Params& p = c.p;
+  if ((p.maxdepth <= 0) & ((flags & SPF_RR) != 0)) {
+    if ((p.maxdepth < 0) & (r.rlvl > -p.maxdepth)) return false;
+    if (r.w >= p.minweight) return true;
+    if (rnd(r.key, RN_RR) > r.w / p.minweight) return false;
+    double sc = p.minweight / r.w;
+    r.coef = r.coef * sc;
+    r.crc = (float)(r.crc * sc);
+    r.w = (float)p.minweight;
+    return true;
+  }
+  if (flags & SPF_FORCE) return true;
+  int md = p.maxdepth < 0 ? -p.maxdepth : p.maxdepth;
+  return (r.w >= p.minweight) & (r.rlvl <= md);
+}
+
+// ---- light sources -----------------------------------------------------------
+// Object identity of what a ray hit: prim index, or -(2+source) for distant.
+__device__ __forceinline__ int src_obj(const Source& s, int si) { return s.prim >= 0 ? s.prim : -(2 + si); }
+
+// m_light() rules (rt/source.c:683-793).  Returns emission in *em, or sets
+// *pass for a transparent pass-through; false when the ray is discarded.
+__device__ bool light_rules(const DevScene& S, const RunCfg& c, const RayS& r, const Material& m, int hitobj,
+                            double rot, double rod, C3* em, bool* pass) {
+  *pass = false;
+  auto distglow = [&](double dist) {
+    return m.kind == MK_GLOW && m.f[3] >= -FTINY && dist > m.f[3];
+  };
+  // badcomponent
+  if ((r.crtype & (AMBIENT | SPECULAR)) && !((r.crtype & SHADOW) || rod < 0.0 || distglow(rot))) return false;
+  // wrongsource
+  if (r.rsrc >= 0) {
+    const Source& sr = S.srcs[r.rsrc];
+    if (src_obj(sr, r.rsrc) != hitobj) {
+      bool illumblock = !(sr.flags & SVIRTUAL) && rod > 0.0 &&
+                        (S.mats[sr.mat].kind == MK_ILLUM || S.mats[sr.mat].kind == MK_GLOW);
+      if (m.kind != MK_ILLUM || illumblock) return false;
+    }
+  }
+  // passillum
+  if (m.kind == MK_ILLUM && (r.rsrc < 0 || src_obj(S.srcs[r.rsrc], r.rsrc) != hitobj)) {
+    *pass = true;
+    return true;
+  }
+  if (rod < 0.0) {           // behind the emitter: black, or pass through with -bv-
+    *em = c3(0, 0, 0);
+    if (!c.p.backvis) *pass = true;
+    return true;
+  }
+  *em = c3((float)m.f[0], (float)m.f[1], (float)m.f[2]);
+  return true;
+}
+
+// sourcehit() of rt/source.c:317-377 for a ray that left the scene.
+__device__ int source_hit(const DevScene& S, const RayS& r) {
+  int glowsrc = -1, transrc = -1;
+  int first = 0, last = S.nsrcs - 1;
+  if (r.rsrc >= 0) first = last = r.rsrc;
+  for (int i = first; i <= last; ++i) {
+    const Source& s = S.srcs[i];
+    if ((s.flags & (SDISTANT | SVIRTUAL)) != SDISTANT) continue;
+    if (2. * PI_D * (1. - dot(load3(s.sloc), r.d)) > s.ss2) continue;
+    if (i == r.rsrc) return i;
+    if (s.flags & SSKIP) { if (glowsrc < 0) glowsrc = i; continue; }
+    const Material& m = S.mats[s.mat];
+    if (m.kind == MK_ILLUM && m.illum_alt < 0) { if (transrc < 0) transrc = i; continue; }
+    return i;
+  }
+  if (transrc >= 0 && (r.crtype & (AMBIENT | SPECULAR))) return -1;
+  return glowsrc;
+}
+
+// rcontrib's trace callback (rt/rcontrib.c:272-316) in coefficient mode.
+__device__ void trace_contrib(const DevScene& S, const RunCfg& c, const RayS& r, int modslot, int hitobj,
+                              bool zeroed, C3 rcol) {
+  if (c.mode != 1 || modslot < 0 || zeroed) return;
+  if (r.rsrc >= 0 && src_obj(S.srcs[r.rsrc], r.rsrc) != hitobj) return;
+  C3 contr = r.coef;
+  if (c.p.contrib) {
+    if (!((r.crc * rcol.r > FTINY) || (r.crc * rcol.g > FTINY) || (r.crc * rcol.b > FTINY))) return;
+    contr = contr * rcol;
+  } else if (r.crc <= FTINY) {
+    return;
+  }
+  const BinSpec& b = c.bins[modslot];
+  int bn = eval_bin(r.d, b);
+  if (bn < 0) return;
+  deposit(c, r.sensor, 3 * (b.col0 + bn), contr);
+}
+
+// ---- materials ------------------------------------------------------------
+struct HitS {
+  V3 rop, ron;     // hit point, normal (flipped by the material when needed)
+  double rod;      // -DOT(rdir, ron)
+  double rot;
+  int prim;        // -1 for the irradiance pseudo-hit
+  int flat;
+};
+
+#define FRESNE(ci) (exp(-5.85 * (ci)) - 0.00202943064)
+constexpr double FRESTHRESH = 0.017999;
+enum { SP_REFL = 01, SP_TRAN = 02, SP_PURE = 04, SP_FLAT = 010, SP_RBLT = 020, SP_TBLT = 040 };
+
+struct NormDat {
+  int specfl;
+  C3 mcolor, scolor;
+  V3 prdir, pnorm;
+  double alpha2, rdiff, rspec, trans, tdiff, tspec, pdot;
+};
+
+// dirnorm() of rt/normal.c:72-173
+__device__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdir, V3 ldir, double omega) {
+  C3 scval = c3(0, 0, 0);
+  double ldot = dot(np.pnorm, ldir);
+  if (ldot < 0.0 ? np.trans <= FTINY : np.trans >= 1.0 - FTINY) return scval;
+  double lrdiff = np.rdiff, ltdiff = np.tdiff;
+  if ((np.specfl & SP_PURE) && np.rspec >= FRESTHRESH && ((lrdiff > FTINY) | (ltdiff > FTINY))) {
+    double dtmp = 1. - FRESNE(fabs(ldot));
+    lrdiff *= dtmp;
+    ltdiff *= dtmp;
+  }
+  if ((ldot > FTINY) & (lrdiff > FTINY)) scval = scval + np.mcolor * (ldot * omega * lrdiff * (1.0 / PI_D));
+  if ((ldot < -FTINY) & (ltdiff > FTINY)) scval = scval + np.mcolor * (-ldot * omega * ltdiff * (1.0 / PI_D));
+  if ((ldot > FTINY) & ((np.specfl & (SP_REFL | SP_PURE)) == SP_REFL)) {
+    double dtmp = np.alpha2;
+    if (np.specfl & SP_FLAT) dtmp += (1. - c.p.dstrsrc) * omega * (0.25 / PI_D);
+    V3 vtmp = ldir - rdir;
+    double d2 = dot(vtmp, np.pnorm);
+    d2 *= d2;
+    double d3 = dot(vtmp, vtmp);
+    double d4 = (d3 - d2) / d2;
+    dtmp = exp(-d4 / dtmp) * d3 / (PI_D * d2 * d2 * dtmp);
+    if (dtmp > FTINY) scval = scval + np.scolor * (dtmp * ldot * omega);
+  }
+  if ((ldot < -FTINY) & ((np.specfl & (SP_TRAN | SP_PURE)) == SP_TRAN)) {
+    double dtmp = np.alpha2 + omega * (1.0 / PI_D);
+    dtmp = exp((2. * dot(np.prdir, ldir) - 2.) / dtmp) / (PI_D * dtmp);
+    if (dtmp > FTINY) scval = scval + np.mcolor * (dtmp * np.tspec * omega * sqrt(-ldot / np.pdot));
+  }
+  return scval;
+}
+
+// direct() of rt/source.c:398-557 with every source tested (no shadow
+// threshold / untested-source statistics): one shadow spawn per source with
+// non-zero potential.
+__device__ void direct_light(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, const NormDat& nd,
+                             const Sink& sk) {
+  for (int sn = 0; sn < S.nsrcs; ++sn) {
+    const Source& src = S.srcs[sn];
+    if (src.flags & SSKIP) continue;
+    if ((src.flags & (SPROX | SDISTANT)) == SPROX) {
+      V3 dv = hs.rop - load3(src.sloc);
+      double pr = src.prox + src.srad;
+      if (dot(dv, dv) > pr * pr) continue;
+    }
+    uint64_t kshadow = child_key(r.key, SLOT_SHADOW + sn);
+    // nextssamp() of rt/srcsamp.c:36-144 with a single partition
+    double vpos[3] = {0, 0, 0};
+    if (c.p.dstrsrc > FTINY) {
+      vpos[0] = rnd(kshadow, RN_SRC0);
+      vpos[1] = rnd(kshadow, RN_SRC1);
+      vpos[2] = (src.flags & SFLAT) ? 0.5 : rnd(kshadow, RN_SRC2);
+      for (int i = 0; i < 3; ++i) vpos[i] = c.p.dstrsrc * (1. - 2. * vpos[i]);
+    }
+    if ((src.flags & SCIR) && (c.p.dstrsrc > 0.7)) {
+      double trim[3];
+      if (src.flags & (SFLAT | SDISTANT)) {
+        double d = 1.12837917;
+        trim[0] = d * sqrt(1.0 - 0.5 * vpos[1] * vpos[1]);
+        trim[1] = d * sqrt(1.0 - 0.5 * vpos[0] * vpos[0]);
+        trim[2] = 0.0;
+      } else {
+        trim[2] = trim[0] = vpos[0] * vpos[0];
+        double d = vpos[1] * vpos[1];
+        if (d > trim[2]) trim[2] = d;
+        trim[0] += d;
+        d = vpos[2] * vpos[2];
+        if (d > trim[2]) trim[2] = d;
+        trim[0] += d;
+        if (trim[0] > FTINY * FTINY) {
+          d = 1.0 / 0.7236;
+          trim[2] = trim[1] = trim[0] = d * sqrt(trim[2] / trim[0]);
+        } else {
+          trim[2] = trim[1] = trim[0] = 0.0;
+        }
+      }
+      for (int i = 0; i < 3; ++i) vpos[i] *= trim[i];
+    }
+    V3 dir = load3(src.sloc) + load3(src.ss[0]) * vpos[0] + load3(src.ss[1]) * vpos[1] + load3(src.ss[2]) * vpos[2];
+    if (!(src.flags & SDISTANT)) dir = dir - hs.rop;
+    double dist = normalize(dir);
+    if (dist == 0.0) continue;
+    double dom;
+    if (src.flags & SFLAT) dom = -dot(load3(src.snorm), dir);
+    else if (src.flags & SCYL) {
+      V3 dv = load3(src.ss[0]);
+      double dd = dot(dir, dv);
+      dd *= dd / dot(dv, dv);
+      dom = sqrt(1. - dd);
+    } else dom = 1.0;
+    if (src.flags & SDISTANT) {
+      dom *= src.ss2;
+    } else {
+      if (dom <= 1e-4) continue;
+      dom *= src.ss2 / (dist * dist);
+      if ((src.flags & SPROX) && dist > src.prox) continue;
+    }
+    C3 coef = dirnorm(nd, c, r.d, dir, dom);
+    if (!(cmax(coef) > 0.0)) continue;
+    // potential: the ray must reach the source (srcvalue, rt/source.c:261-299)
+    RayS sr = r;
+    sr.d = dir;
+    sr.rsrc = sn;
+    sr.crtype = r.crtype | RSHADOW;
+    const Material& sm = S.mats[src.mat];
+    C3 em = c3(0, 0, 0);
+    bool pass;
+    if (src.flags & SDISTANT) {
+      if (2. * PI_D * (1. - dot(load3(src.sloc), dir)) > src.ss2) continue;
+      if (!light_rules(S, c, sr, sm, src_obj(src, sn), FHUGE, 1.0, &em, &pass) || pass) continue;
+    } else {
+      HitRec h1; h1.t = FHUGE; h1.rod = 1.0; h1.prim = -1;
+      hit_prim(S, src.prim, hs.rop, dir, h1);
+      if (h1.prim < 0) continue;
+      if (!light_rules(S, c, sr, sm, src.prim, h1.t, h1.rod, &em, &pass) || pass) continue;
+    }
+    if (!(cbright(coef * em) > 0.0)) continue;
+    bool thru = (hs.rod > 0) ^ (dot(hs.ron, dir) > 0);
+    Spawn s = make_child(r, hs.rop, thru ? TSHADOW : RSHADOW, nullptr, SLOT_SHADOW + sn, 0);
+    // own rcoef of the shadow ray is the source coefficient (rt/source.c:511-513)
+    C3 cum = r.coef * coef;
+    s.coef[0] = cum.r; s.coef[1] = cum.g; s.coef[2] = cum.b;
+    s.crc = (float)cmax(coef);
+    s.rsrc = (int16_t)sn;
+    set_dir(s, dir);
+    emit(sk, s);
+  }
+}
+
+// multambient() + samp_hemi() set-up (rt/ambient.c:229-330, rt/ambcomp.c:351-424)
+// for -aa 0: one hemisphere spawn of n*n stratified children.
+__device__ void ambient(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, C3 aval, V3 nrm,
+                        const Sink& sk) {
+  const Params& p = c.p;
+  bool dumb = (p.ambdiv <= 0) || (r.adepth >= p.ambounce);
+  double rdot = dot(nrm, hs.ron);
+  int sgn = 1 - 2 * (rdot < 0);
+  if (!dumb) {
+    double d = cmax(aval);
+    if (d <= FTINY) dumb = true;
+    else {
+      double wt = r.w;
+      d *= 0.8 * r.w / (p.ambdiv * p.minweight + 1e-20);
+      if (wt > d) wt = d;
+      int n = (int)(sqrt(p.ambdiv * wt) + 0.5);
+      if (n < 1) n = 1;
+      if (n > 65535) n = 65535;
+      bool back = sgn < 0;
+      V3 onrm = back ? -hs.ron : hs.ron;
+      C3 acoef = aval * (1.0 / ((double)n * n));
+      Spawn s = make_child(r, hs.rop, back ? TAMBIENT : RAMBIENT, &acoef, 0, back ? SPF_BACK : 0);
+      s.key = r.key;   // children derive their keys from the parent key
+      V3 ux, uy;
+      tangent_frame(onrm, rnd(r.key, back ? RN_FRAME_BACK : RN_FRAME_FRONT), ux, uy);
+      s.d0[0] = (float)onrm.x; s.d0[1] = (float)onrm.y; s.d0[2] = (float)onrm.z;
+      s.d1[0] = (float)ux.x; s.d1[1] = (float)ux.y; s.d1[2] = (float)ux.z;
+      s.ns = (uint16_t)n;
+      s.count = (uint32_t)n * (uint32_t)n;
+      s.adepth = (uint8_t)(r.adepth + 1);
+      emit(sk, s);
+      return;
+    }
+  }
+  // dumbamb: constant ambient value -av
+  if (c.mode == 0) {
+    C3 av = aval * c3((float)p.ambval[0], (float)p.ambval[1], (float)p.ambval[2]);
+    if (cmax(av) > 0) deposit(c, r.sensor, 0, r.coef * av);
+  }
+}
+
+// gaussamp() of rt/normal.c:364-495 with one sample per component (-ss <= 1.5)
+__device__ void gauss_sample(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, const NormDat& np,
+                             const Sink& sk) {
+  if ((np.specfl & (SP_REFL | SP_RBLT)) != SP_REFL && (np.specfl & (SP_TRAN | SP_TBLT)) != SP_TRAN) return;
+  V3 u, v;
+  tangent_frame(np.pnorm, rnd(r.key, RN_GFRAME), u, v);
+  if ((np.specfl & (SP_REFL | SP_RBLT)) == SP_REFL) {
+    C3 sc = np.scolor;
+    Spawn s = make_child(r, hs.rop, RSPECULAR, &sc, SLOT_RSPEC, 0);
+    bool ok = false;
+    V3 dir;
+    for (int ntrials = 0; ntrials < 10; ++ntrials) {
+      double rv0 = rnd(s.key, RN_GAUSS + 2 * ntrials), rv1 = rnd(s.key, RN_GAUSS + 2 * ntrials + 1);
+      double d = 2.0 * PI_D * rv0;
+      double cosp = cos(d), sinp = sin(d);
+      if ((0. <= c.p.specjitter) & (c.p.specjitter < 1.)) rv1 = 1.0 - c.p.specjitter * rv1;
+      d = (rv1 <= FTINY) ? 1.0 : sqrt(np.alpha2 * -log(rv1));
+      V3 h = np.pnorm + (u * cosp + v * sinp) * d;
+      d = -2.0 * dot(h, r.d) / (1.0 + d * d);
+      dir = r.d + h * d;
+      if (dot(dir, hs.ron) <= FTINY) continue;
+      normalize(dir);
+      ok = true;
+      break;
+    }
+    if (ok) { set_dir(s, dir); emit(sk, s); }
+  }
+  if ((np.specfl & (SP_TRAN | SP_TBLT)) == SP_TRAN) {
+    C3 tc = np.mcolor * np.tspec;
+    Spawn s = make_child(r, hs.rop, TSPECULAR, &tc, SLOT_TSPEC, 0);
+    bool ok = false;
+    V3 dir;
+    for (int ntrials = 0; ntrials < 10; ++ntrials) {
+      double rv0 = rnd(s.key, RN_GAUSS + 2 * ntrials), rv1 = rnd(s.key, RN_GAUSS + 2 * ntrials + 1);
+      double d = 2.0 * PI_D * rv0;
+      double cosp = cos(d), sinp = sin(d);
+      if ((0. <= c.p.specjitter) & (c.p.specjitter < 1.)) rv1 = 1.0 - c.p.specjitter * rv1;
+      d = (rv1 <= FTINY) ? 1.0 : sqrt(np.alpha2 * -log(rv1));
+      dir = np.prdir + (u * cosp + v * sinp) * d;
+      if (dot(dir, hs.ron) >= -FTINY) continue;
+      normalize(dir);
+      ok = true;
+      break;
+    }
+    if (ok) { set_dir(s, dir); emit(sk, s); }
+  }
+}
+
+// m_normal() of rt/normal.c:177-360 (plastic, metal, trans)
+__device__ void shade_normal(const DevScene& S, const RunCfg& c, const RayS& r, HitS hs, const Material& m,
+                             const Sink& sk) {
+  if ((r.crtype & SHADOW) && m.kind != MK_TRANS) return;
+  if (hs.rod < 0.0) {
+    if (!c.p.backvis) {
+      Spawn s = make_child(r, hs.rop, TRANS, nullptr, SLOT_PASS, SPF_FORCE);
+      set_dir(s, r.d);
+      emit(sk, s);
+      return;
+    }
+    hs.rod = -hs.rod;
+    hs.ron = -hs.ron;
+  }
+  NormDat nd;
+  nd.mcolor = c3((float)m.f[0], (float)m.f[1], (float)m.f[2]);
+  nd.specfl = 0;
+  nd.alpha2 = m.f[4] * m.f[4];
+  if (nd.alpha2 <= FTINY) nd.specfl |= SP_PURE;
+  nd.pnorm = hs.ron;
+  nd.pdot = hs.rod;
+  if (hs.prim >= 0 && hs.flat) nd.specfl |= SP_FLAT;
+  if (nd.pdot < .001) nd.pdot = .001;
+  nd.rspec = m.f[3];
+  double fest = 0.;
+  if ((nd.specfl & SP_PURE) && nd.rspec >= FRESTHRESH) {
+    fest = FRESNE(nd.pdot);
+    nd.rspec += fest * (1. - nd.rspec);
+  }
+  nd.prdir = r.d;
+  if (m.kind == MK_TRANS) {
+    nd.trans = m.f[5] * (1.0 - nd.rspec);
+    nd.tspec = nd.trans * m.f[6];
+    nd.tdiff = nd.trans - nd.tspec;
+    if (nd.tspec > FTINY) {
+      nd.specfl |= SP_TRAN;
+      if (!(nd.specfl & SP_PURE) && c.p.specthresh >= nd.tspec - FTINY) nd.specfl |= SP_TBLT;
+    }
+  } else {
+    nd.tdiff = nd.tspec = nd.trans = 0.0;
+  }
+  nd.rdiff = 1.0 - nd.trans - nd.rspec;
+  if ((nd.specfl & (SP_TRAN | SP_PURE | SP_TBLT)) == (SP_TRAN | SP_PURE)) {
+    C3 tc = nd.mcolor * nd.tspec;
+    Spawn s = make_child(r, hs.rop, TRANS, &tc, SLOT_TRANS, 0);
+    set_dir(s, nd.prdir);
+    emit(sk, s);
+  }
+  if (r.crtype & SHADOW) return;
+  nd.scolor = c3(0, 0, 0);
+  if (nd.rspec > FTINY) {
+    nd.specfl |= SP_REFL;
+    if (m.kind != MK_METAL) {
+      nd.scolor = c3((float)nd.rspec, (float)nd.rspec, (float)nd.rspec);
+    } else if (fest > FTINY) {
+      double d = m.f[3] * (1. - fest);
+      nd.scolor = c3((float)(fest + nd.mcolor.r * d), (float)(fest + nd.mcolor.g * d), (float)(fest + nd.mcolor.b * d));
+    } else {
+      nd.scolor = nd.mcolor * nd.rspec;
+    }
+    if (!(nd.specfl & SP_PURE) && c.p.specthresh >= nd.rspec - FTINY) nd.specfl |= SP_RBLT;
+  }
+  if ((nd.specfl & (SP_REFL | SP_PURE | SP_RBLT)) == (SP_REFL | SP_PURE)) {
+    C3 sc = nd.scolor;
+    Spawn s = make_child(r, hs.rop, REFLECTED, &sc, SLOT_REFL, 0);
+    V3 dir = r.d + nd.pnorm * (2. * nd.pdot);
+    normalize(dir);
+    set_dir(s, dir);
+    emit(sk, s);
+  }
+  if ((nd.specfl & SP_PURE) && nd.rdiff <= FTINY && nd.tdiff <= FTINY) return;
+  if (!(nd.specfl & SP_PURE)) gauss_sample(S, c, r, hs, nd, sk);
+  if (nd.rdiff > FTINY) {
+    C3 a = nd.mcolor * nd.rdiff;
+    if (nd.specfl & SP_RBLT) a = a + nd.scolor;
+    ambient(S, c, r, hs, a, nd.pnorm, sk);
+  }
+  if (nd.tdiff > FTINY) {
+    C3 a = nd.mcolor * ((nd.specfl & SP_TBLT) ? nd.trans : nd.tdiff);
+    ambient(S, c, r, hs, a, -nd.pnorm, sk);
+  }
+  direct_light(S, c, r, hs, nd, sk);
+}
+
+// m_glass() of rt/glass.c:46-165
+__device__ void shade_glass(const RunCfg& c, const RayS& r, HitS hs, const Material& m, const Sink& sk) {
+  double rindex = m.nfargs == 4 ? m.f[3] : 1.52;
+  if (!c.p.backvis && hs.rod <= 0.0) {
+    Spawn s = make_child(r, hs.rop, TRANS, nullptr, SLOT_PASS, SPF_FORCE);
+    set_dir(s, r.d);
+    emit(sk, s);
+    return;
+  }
+  double mc[3] = {m.f[0], m.f[1], m.f[2]};
+  bool hastrans = fmax(fmax(mc[0], mc[1]), mc[2]) > 1e-15;
+  if (hastrans) {
+    for (int i = 0; i < 3; ++i) if (mc[i] < 1e-15) mc[i] = 1e-15;
+  } else if (r.crtype & SHADOW) {
+    return;
+  }
+  if (hs.rod < 0.0) { hs.rod = -hs.rod; hs.ron = -hs.ron; }
+  V3 pnorm = hs.ron;
+  double pdot = hs.rod;
+  double cos2 = sqrt((1.0 - 1.0 / (rindex * rindex)) + pdot * pdot / (rindex * rindex));
+  if (hastrans) for (int i = 0; i < 3; ++i) mc[i] = pow(mc[i], 1.0 / cos2);
+  double r1e = (pdot - rindex * cos2) / (pdot + rindex * cos2);
+  r1e *= r1e;
+  double r1m = (1.0 / pdot - rindex / cos2) / (1.0 / pdot + rindex / cos2);
+  r1m *= r1m;
+  if (hastrans) {
+    double ct[3];
+    for (int i = 0; i < 3; ++i) {
+      double d = mc[i];
+      ct[i] = .5 * (1.0 - r1e) * (1.0 - r1e) * d / (1.0 - r1e * r1e * d * d) +
+              .5 * (1.0 - r1m) * (1.0 - r1m) * d / (1.0 - r1m * r1m * d * d);
+    }
+    C3 tc = c3((float)ct[0], (float)ct[1], (float)ct[2]);
+    Spawn s = make_child(r, hs.rop, TRANS, &tc, SLOT_TRANS, 0);
+    set_dir(s, r.d);
+    emit(sk, s);
+  }
+  if (r.crtype & SHADOW) return;
+  double cr[3];
+  for (int i = 0; i < 3; ++i) {
+    double d = mc[i];
+    d *= d;
+    cr[i] = .5 * r1e * (1.0 + (1.0 - 2.0 * r1e) * d) / (1.0 - r1e * r1e * d) +
+            .5 * r1m * (1.0 + (1.0 - 2.0 * r1m) * d) / (1.0 - r1m * r1m * d);
+  }
+  C3 rc = c3((float)cr[0], (float)cr[1], (float)cr[2]);
+  Spawn s = make_child(r, hs.rop, REFLECTED, &rc, SLOT_REFL, 0);
+  V3 dir = r.d + pnorm * (2. * pdot);
+  normalize(dir);
+  set_dir(s, dir);
+  emit(sk, s);
+}
+
+// rayshade() for a surface hit, followed by rcontrib's trace callback.
+__device__ void shade_hit(const DevScene& S, const RunCfg& c, const RayS& r, const HitRec& h, const Sink& sk) {
+  const Prim& P = S.prims[h.prim];
+  HitS hs;
+  hs.rot = h.t;
+  hs.rop = r.o + r.d * h.t;
+  hs.ron = prim_normal(S, h.prim, hs.rop);
+  hs.rod = -dot(r.d, hs.ron);
+  hs.prim = h.prim;
+  hs.flat = P.flat;
+  if (P.mat < 0) { atomicOr(c.err, ERR_NOMAT); return; }
+  const Material& m = S.mats[P.mat];
+  bool zeroed = false;
+  C3 rcol = c3(0, 0, 0);
+  // -i: Lambertian replacement at the first qualifying surface (raytirrad)
+  bool tst_irrad = c.irrad && !(r.crtype & ~(PRIMARY | TRANS));
+  if (tst_irrad && m.kind != MK_LIGHT && m.kind != MK_GLOW && m.kind != MK_ILLUM) {
+    if (m.transp) {
+      Spawn s = make_child(r, hs.rop, TRANS, nullptr, SLOT_PASS, SPF_FORCE);
+      set_dir(s, r.d);
+      emit(sk, s);
+    } else {
+      shade_normal(S, c, r, hs, S.mats[S.lamb], sk);
+    }
+  } else {
+    switch (m.kind) {
+      case MK_PLASTIC:
+      case MK_METAL:
+      case MK_TRANS:
+        shade_normal(S, c, r, hs, m, sk);
+        break;
+      case MK_GLASS:
+        shade_glass(c, r, hs, m, sk);
+        break;
+      default: {   // light, glow, illum
+        C3 em = c3(0, 0, 0);
+        bool pass;
+        if (!light_rules(S, c, r, m, h.prim, hs.rot, hs.rod, &em, &pass)) {
+          zeroed = true;
+        } else if (pass) {
+          if (m.kind == MK_ILLUM && m.illum_alt >= 0) {
+            const Material& alt = S.mats[m.illum_alt];
+            if (alt.kind == MK_GLASS) shade_glass(c, r, hs, alt, sk);
+            else if (alt.kind <= MK_TRANS) shade_normal(S, c, r, hs, alt, sk);
+          } else {
+            Spawn s = make_child(r, hs.rop, TRANS, nullptr, SLOT_PASS, SPF_FORCE);
+            set_dir(s, r.d);
+            emit(sk, s);
+          }
+        } else {
+          rcol = em;
+          if (c.mode == 0) deposit(c, r.sensor, 0, r.coef * em);
+        }
+      }
+    }
+  }
+  trace_contrib(S, c, r, P.modslot, h.prim, zeroed, rcol);
+}
+
+// A ray that hit nothing: distant sources (sourcehit + m_light).
+__device__ void shade_miss(const DevScene& S, const RunCfg& c, const RayS& r) {
+  int si = source_hit(S, r);
+  if (si < 0) return;
+  const Source& src = S.srcs[si];
+  const Material& m = S.mats[src.mat];
+  C3 em = c3(0, 0, 0);
+  bool pass;
+  bool ok = light_rules(S, c, r, m, -(2 + si), FHUGE, 1.0, &em, &pass);
+  if (!ok || pass) {
+    trace_contrib(S, c, r, src.modslot, -(2 + si), true, em);
+    return;
+  }
+  if (c.mode == 0) deposit(c, r.sensor, 0, r.coef * em);
+  trace_contrib(S, c, r, src.modslot, -(2 + si), false, em);
+}
+
+// ---- kernels -----------------------------------------------------------------
+// Generation 0 for -I: the pseudo-hit on a Lambertian surface at each sensor
+// (rt/rtrace.c:416-431, rt/rcontrib.c:321-338).
+__global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene S, RunCfg c, const double* rays, uint32_t nsens,
+                                                      uint32_t s0, Sink sk) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nsens) return;
+  const double* ry = rays + 6 * (size_t)i;
+  V3 org = load3(ry), dir = load3(ry + 3);
+  normalize(dir);
+  RayS r;
+  r.o = org + dir * 1.1e-4;
+  r.d = -dir;
+  r.coef = c3(1, 1, 1);
+  r.w = 1.0f;
+  r.crc = 1.0f;
+  r.sensor = (s0 + i) / (uint32_t)c.accum;
+  r.crtype = r.rtype = PRIMARY;
+  r.rsrc = -1;
+  r.rlvl = 0;
+  r.adepth = 0;
+  r.key = sensor_key(c.p.seed, s0 + i);
+  HitS hs;
+  hs.rot = 1e-5;
+  hs.rop = org + dir * 1e-4;
+  hs.ron = dir;
+  hs.rod = 1.0;
+  hs.prim = -1;
+  hs.flat = 0;
+  shade_normal(S, c, r, hs, S.mats[S.lamb], sk);
+}
+
+// Generation 0 for radiance rays: primary rays as explicit spawns.
+__global__ void __launch_bounds__(BLOCK) k_primary_seed(RunCfg c, const double* rays, uint32_t nsens, uint32_t s0,
+                                                        Spawn* out) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nsens) return;
+  const double* ry = rays + 6 * (size_t)i;
+  V3 dir = load3(ry + 3);
+  normalize(dir);
+  Spawn s;
+  s.org[0] = ry[0]; s.org[1] = ry[1]; s.org[2] = ry[2];
+  set_dir(s, dir);
+  s.d1[0] = s.d1[1] = s.d1[2] = 0.f;
+  s.coef[0] = s.coef[1] = s.coef[2] = 1.f;
+  s.cw = 1.f;
+  s.crc = 1.f;
+  s.sensor = (s0 + i) / (uint32_t)c.accum;
+  s.count = 1;
+  s.ns = 0;
+  s.crtype = s.rtype = PRIMARY;
+  s.rsrc = -1;
+  s.rlvl = 0;
+  s.adepth = 0;
+  s.flags = SPF_FORCE;
+  s.pad = 0;
+  s.key = sensor_key(c.p.seed, s0 + i);
+  out[i] = s;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_counts(const Spawn* sp, uint32_t n, uint32_t* counts) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) counts[i] = sp[i].count;
+}
+
+// One generation: children [c0, c0+nthreads) of the spawn records `in`.
+__global__ void __launch_bounds__(BLOCK) k_bounce(DevScene S, RunCfg c, const Spawn* in, const uint32_t* offs,
+                                                  uint32_t nspawn, uint64_t c0, uint32_t nthreads, Sink sk) {
+  __shared__ int32_t stk[STACK * BLOCK];
+  uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= nthreads) return;
+  uint64_t gid = c0 + t;
+  // spawn record owning child gid: last s with offs[s] <= gid
+  uint32_t lo = 0, hi = nspawn;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)offs[mid] <= gid) lo = mid; else hi = mid;
+  }
+  const Spawn& sp = in[lo];
+  uint32_t ci = (uint32_t)(gid - offs[lo]);
+  RayS r;
+  r.o = v3(sp.org[0], sp.org[1], sp.org[2]);
+  r.coef = c3(sp.coef[0], sp.coef[1], sp.coef[2]);
+  r.w = sp.cw;
+  r.crc = sp.crc;
+  r.sensor = sp.sensor;
+  r.crtype = sp.crtype;
+  r.rtype = sp.rtype;
+  r.rsrc = sp.rsrc;
+  r.rlvl = sp.rlvl;
+  r.adepth = sp.adepth;
+  if (sp.ns) {
+    // stratified cosine-weighted hemisphere sample (rt/ambcomp.c:177-216)
+    uint32_t ns = sp.ns;
+    uint32_t i = ci / ns, j = ci % ns;
+    r.key = child_key(sp.key, ((sp.flags & SPF_BACK) ? SLOT_HEMI_BACK : SLOT_HEMI_FRONT) + ci);
+    double ss0 = rnd(r.key, RN_AMB0), ss1 = rnd(r.key, RN_AMB1);
+    double spt[2];
+    square2disk(spt, (j + ss1) / ns, (i + ss0) / ns);
+    double zd = sqrt(fmax(0.0, 1. - spt[0] * spt[0] - spt[1] * spt[1]));
+    V3 onrm = v3(sp.d0[0], sp.d0[1], sp.d0[2]);
+    V3 ux = v3(sp.d1[0], sp.d1[1], sp.d1[2]);
+    V3 uy = cross(onrm, ux);
+    r.d = ux * spt[0] + uy * spt[1] + onrm * zd;
+  } else {
+    r.key = sp.key;
+    r.d = v3(sp.d0[0], sp.d0[1], sp.d0[2]);
+  }
+  normalize(r.d);
+  if (!admit(c, r, sp.flags)) return;
+  wave_count(c.nrays);
+  HitRec h;
+  closest_hit(S, r.o, r.d, h, stk, c.err, c.visits);
+  if (h.prim >= 0) shade_hit(S, c, r, h, sk);
+  else shade_miss(S, c, r);
+}
+
+// ---- exclusive scan of uint32 counts (three-phase, LDS per block) ----------
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+    uint32_t a = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  total = sh[SCAN_BLOCK - 1];
+  uint32_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                           uint32_t* tile_sums) {
+  __shared__ uint32_t sh[SCAN_BLOCK];
+  size_t base = (size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS], s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = base + k < n ? in[base + k] : 0; s += v[k]; }
+  uint32_t total;
+  uint32_t ex = block_excl_scan(s, sh, total);
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(uint32_t* out, uint32_t n, const uint32_t* tile_offs) {
+  size_t base = (size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint32_t a = tile_offs[blockIdx.x];
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    if (base + k < n) out[base + k] += a;
+}
+
+}  // namespace rg
+
+// ---- host side -----------------------------------------------------------------
+namespace rg {
+
+static void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// out[i] = sum(in[0..i)), out[n] = total.  `out` holds n+1 entries.
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level) {
+  if (n == 0) {
+    check(hipMemsetAsync(out, 0, sizeof(uint32_t), st), "scan memset");
+    return;
+  }
+  uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  uint32_t* sums = (uint32_t*)arena.get("scan_sums" + std::to_string(level), sizeof(uint32_t) * (ntiles + 1), st, true);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_BLOCK), 0, st, in, out, n, sums);
+  if (ntiles > 1) {
+    uint32_t* soffs = (uint32_t*)arena.get("scan_offs" + std::to_string(level), sizeof(uint32_t) * (ntiles + 1), st, true);
+    exclusive_scan(sums, soffs, ntiles, arena, st, level + 1);
+    hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(SCAN_BLOCK), 0, st, out, n, soffs);
+    check(hipMemcpyAsync(out + n, soffs + ntiles, sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "scan total");
+  } else {
+    check(hipMemcpyAsync(out + n, sums, sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "scan total");
+  }
+}
+
+void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad,
+            DevArena& arena, hipStream_t st, RenderStats* stats) {
+  // Work proceeds in waves of at most `window` children; each child appends
+  // at most `per_child` spawn records, so a wave never overflows its buffer.
+  const uint32_t per_child = 6u + (uint32_t)S.nsrcs;
+  const uint32_t window = 1u << 20;
+  const long max_batches = 1L << 22;
+  uint32_t* d_cnt = (uint32_t*)arena.get("ctr", 64, st, true);
+  uint32_t* h_cnt = arena.host_u32(8);
+
+  struct Batch { Spawn* buf; uint32_t n; int slot; };
+  std::vector<Batch> stack;
+  int nslots = 0;
+  std::vector<int> free_slots;
+  auto new_slot = [&]() -> int {
+    if (!free_slots.empty()) { int s = free_slots.back(); free_slots.pop_back(); return s; }
+    return nslots++;
+  };
+  auto slot_buf = [&](int s, size_t cap) -> Spawn* {
+    return (Spawn*)arena.get("spawn" + std::to_string(s), sizeof(Spawn) * cap, st, false);
+  };
+  size_t batch_cap = std::max<size_t>((size_t)window * per_child, (size_t)nsens);
+
+  // generation 0
+  {
+    int s = new_slot();
+    Spawn* b = slot_buf(s, batch_cap);
+    check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
+    uint32_t grid = (nsens + BLOCK - 1) / BLOCK;
+    if (imm_irrad) {
+      Sink sk{b, d_cnt, (uint32_t)batch_cap, cfg.err};
+      hipLaunchKernelGGL(k_irrad_seed, dim3(grid), dim3(BLOCK), 0, st, S, cfg, d_rays, nsens, 0u, sk);
+      check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
+      check(hipStreamSynchronize(st), "seed");
+      stack.push_back({b, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s});
+    } else {
+      hipLaunchKernelGGL(k_primary_seed, dim3(grid), dim3(BLOCK), 0, st, cfg, d_rays, nsens, 0u, b);
+      stack.push_back({b, nsens, s});
+    }
+  }
+  long gens = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (stats && stats->timing) {
+    check(hipEventCreate(&ev0), "event");
+    check(hipEventCreate(&ev1), "event");
+  }
+  while (!stack.empty()) {
+    Batch bt = stack.back();
+    stack.pop_back();
+    if (bt.n == 0) { free_slots.push_back(bt.slot); continue; }
+    if (++gens > max_batches) throw std::runtime_error("ray tree did not terminate");
+    uint32_t* counts = (uint32_t*)arena.get("counts" + std::to_string(bt.slot), sizeof(uint32_t) * (bt.n + 1), st, false);
+    uint32_t* offs = (uint32_t*)arena.get("offs" + std::to_string(bt.slot), sizeof(uint32_t) * (bt.n + 1), st, false);
+    hipLaunchKernelGGL(k_counts, dim3((bt.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, bt.buf, bt.n, counts);
+    exclusive_scan(counts, offs, bt.n, arena, st, 0);
+    check(hipMemcpyAsync(h_cnt, offs + bt.n, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "total");
+    check(hipStreamSynchronize(st), "scan");
+    uint64_t total = h_cnt[0];
+    std::vector<Batch> produced;
+    for (uint64_t c0 = 0; c0 < total; c0 += window) {
+      uint32_t nth = (uint32_t)std::min<uint64_t>(window, total - c0);
+      int s = new_slot();
+      Spawn* ob = slot_buf(s, batch_cap);
+      check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
+      Sink sk{ob, d_cnt, (uint32_t)batch_cap, cfg.err};
+      if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
+      hipLaunchKernelGGL(k_bounce, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf, offs, bt.n,
+                         c0, nth, sk);
+      check(hipGetLastError(), "k_bounce launch");
+      if (stats && stats->timing) check(hipEventRecord(ev1, st), "event");
+      check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
+      check(hipStreamSynchronize(st), "bounce");
+      if (stats && stats->timing) {
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, ev0, ev1), "event time");
+        stats->bounce_ms += ms;
+      }
+      if (stats) { stats->launches++; stats->threads += nth; }
+      uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap);
+      if (nout) produced.push_back({ob, nout, s});
+      else free_slots.push_back(s);
+    }
+    free_slots.push_back(bt.slot);
+    // depth first: children of the last window are processed next
+    for (auto& p : produced) stack.push_back(p);
+  }
+  if (stats) stats->generations = gens;
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+}
+
+}  // namespace rg

@@ -1,0 +1,89 @@
+// Host-side declarations for the renderer, the BVH builder and the device
+// memory arena that keeps per-context buffers alive between calls.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "rg_shade.hpp"
+
+namespace rg {
+
+// Named device buffers owned by a context; grown on demand, freed on destroy.
+class DevArena {
+ public:
+  ~DevArena() { release(); }
+  void* get(const std::string& name, size_t bytes, hipStream_t st, bool zero) {
+    auto it = bufs_.find(name);
+    if (it == bufs_.end() || it->second.bytes < bytes) {
+      if (it != bufs_.end()) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(it->second.p);
+        bufs_.erase(it);
+      }
+      void* p = nullptr;
+      size_t want = bytes ? bytes : 16;
+      hipError_t e = hipMalloc(&p, want);
+      if (e != hipSuccess)
+        throw std::runtime_error("device allocation of " + std::to_string(want) + " bytes failed: " +
+                                 hipGetErrorString(e));
+      bufs_[name] = {p, want};
+      it = bufs_.find(name);
+    }
+    if (zero) (void)hipMemsetAsync(it->second.p, 0, bytes ? bytes : 16, st);
+    return it->second.p;
+  }
+  uint32_t* host_u32(size_t n) {
+    if (host_n_ < n) {
+      if (host_) (void)hipHostFree(host_);
+      host_ = nullptr;
+      if (hipHostMalloc((void**)&host_, n * sizeof(uint32_t), 0) != hipSuccess)
+        throw std::runtime_error("pinned host allocation failed");
+      host_n_ = n;
+    }
+    return host_;
+  }
+  void release() {
+    for (auto& kv : bufs_) (void)hipFree(kv.second.p);
+    bufs_.clear();
+    if (host_) (void)hipHostFree(host_);
+    host_ = nullptr;
+    host_n_ = 0;
+  }
+  size_t bytes() const {
+    size_t t = 0;
+    for (auto& kv : bufs_) t += kv.second.bytes;
+    return t;
+  }
+
+ private:
+  struct Buf { void* p; size_t bytes; };
+  std::map<std::string, Buf> bufs_;
+  uint32_t* host_ = nullptr;
+  size_t host_n_ = 0;
+};
+
+struct RenderStats {
+  long generations = 0;
+  long launches = 0;
+  unsigned long long threads = 0;
+  bool timing = false;       // time every k_bounce launch with HIP events
+  double bounce_ms = 0;      // summed k_bounce durations
+};
+
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level);
+
+void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad,
+            DevArena& arena, hipStream_t st, RenderStats* stats);
+
+// BVH over the primitives' FP32 bounds (lo/hi: 3*n floats, already widened).
+// Fills nodes (n-1 entries) and leaf_prim (n entries); returns the root ref.
+int build_lbvh(const float* d_lo, const float* d_hi, uint32_t n, const float scene_lo[3], const float scene_hi[3],
+               BVHNode* d_nodes, int32_t* d_leaf_prim, DevArena& arena, hipStream_t st);
+
+// DC x sky product C[M,N,3] = A[M,K,3] x B[K,N,3], per channel.
+// mode 0: FP32 FMA, mode 1: BF16 hi/lo split on MFMA with FP32 accumulation.
+void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mode, hipStream_t st);
+
+}  // namespace rg

@@ -1,0 +1,433 @@
+// Scene compilation: the host half of subsystem (a).
+//
+//   findmaterial()          rt/initotypes.c:113-145
+//   getface()               common/face.c:35-117
+//   getcone() radii rules   common/cone.c:20-120
+//   o_sphere() radius rule  rt/sphere.c:31-39
+//   marksources()           rt/source.c:47-142
+//   ssetsrc/fsetsrc/sphsetsrc/rsetsrc + setflatss  rt/srcsupp.c:60-260
+#include "rg_scene.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <unordered_map>
+
+namespace rg {
+
+namespace {
+
+struct D3 {
+  double v[3];
+};
+inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline void cross3(double* r, const double* a, const double* b) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+inline double norm3(double* v) {
+  double l2 = dot3(v, v);
+  if (l2 <= 0) return 0;
+  double l = sqrt(l2), inv = 1.0 / l;
+  v[0] *= inv; v[1] *= inv; v[2] *= inv;
+  return l;
+}
+// getperpendicular(randomize=0), common/fvect.c:159-192
+bool perpendicular3(double* vp, const double* v) {
+  int i;
+  for (i = 3; i--;)
+    if ((-0.6 < v[i]) & (v[i] < 0.6)) break;
+  if (i < 0) return false;
+  double v1[3] = {0, 0, 0};
+  v1[i] = 1.0;
+  cross3(vp, v1, v);
+  return norm3(vp) > 0.0;
+}
+
+[[noreturn]] void objerr(const RObj& o, const std::string& m) {
+  throw RadError(o.type + " \"" + o.name + "\": " + m);
+}
+
+int material_kind(const std::string& t) {
+  if (t == "plastic") return MK_PLASTIC;
+  if (t == "metal") return MK_METAL;
+  if (t == "trans") return MK_TRANS;
+  if (t == "glass") return MK_GLASS;
+  if (t == "light") return MK_LIGHT;
+  if (t == "glow") return MK_GLOW;
+  if (t == "illum") return MK_ILLUM;
+  return -1;
+}
+
+}  // namespace
+
+CompiledScene compile_scene(const SceneDB& db) {
+  CompiledScene cs;
+  const auto& objs = db.objs;
+  const int nobj = (int)objs.size();
+  std::unordered_map<int, int> matidx;   // material object -> table index
+
+  // findmaterial(): follow the modifier chain to the first material
+  auto findmaterial = [&](int oi) -> int {
+    int guard = 0;
+    int m = objs[oi].omod;
+    while (m >= 0) {
+      if (++guard > 10000) objerr(objs[oi], "modifier loop");
+      const RObj& mo = objs[m];
+      if (SceneDB::is_material(mo.type)) return m;
+      if (mo.type == "alias") {
+        if (!mo.sargs.empty()) {
+          int r = db.modifier(mo.sargs[0]);
+          if (r < 0) objerr(mo, "bad reference");
+          m = r;
+          continue;
+        }
+        m = mo.omod;
+        continue;
+      }
+      // patterns, textures and mixtures are not part of the GPU material set
+      objerr(mo, "modifier type \"" + mo.type + "\" is not supported on the GPU path");
+    }
+    return -1;
+  };
+
+  std::function<int(int)> material_index = [&](int mo) -> int {
+    if (mo < 0) return -1;
+    auto it = matidx.find(mo);
+    if (it != matidx.end()) return it->second;
+    const RObj& m = objs[mo];
+    int kind = material_kind(m.type);
+    if (kind < 0) objerr(m, "material type \"" + m.type + "\" is not supported on the GPU path");
+    Material M;
+    memset(&M, 0, sizeof M);
+    M.kind = kind;
+    M.nfargs = (int)m.fargs.size();
+    M.illum_alt = -1;
+    int need = kind == MK_TRANS ? 7 : (kind == MK_PLASTIC || kind == MK_METAL) ? 5
+             : kind == MK_GLOW ? 4 : kind == MK_GLASS ? -1 : 3;
+    if (need > 0 && M.nfargs != need) objerr(m, "bad number of arguments");
+    if (kind == MK_GLASS && M.nfargs != 3 && M.nfargs != 4) objerr(m, "bad arguments");
+    for (int i = 0; i < M.nfargs && i < 8; ++i) M.f[i] = m.fargs[i];
+    M.transp = (kind == MK_TRANS || kind == MK_GLASS) ? 1 : 0;
+    int idx = (int)cs.mats.size();
+    cs.mats.push_back(M);
+    matidx[mo] = idx;
+    if (kind == MK_ILLUM && !m.sargs.empty() && m.sargs[0] != "void") {
+      int alt = db.modifier(m.sargs[0]);
+      if (alt < 0) objerr(m, "unknown alternate material");
+      int ai = material_index(alt);
+      cs.mats[idx].illum_alt = ai;
+    }
+    return idx;
+  };
+
+  for (int i = 0; i < 3; ++i) { cs.bmin[i] = 1e300; cs.bmax[i] = -1e300; }
+  auto grow = [&](const double* lo, const double* hi) {
+    for (int i = 0; i < 3; ++i) {
+      cs.bmin[i] = std::min(cs.bmin[i], lo[i]);
+      cs.bmax[i] = std::max(cs.bmax[i], hi[i]);
+    }
+  };
+
+  // pass 1: primitives
+  std::vector<int> obj_prim(nobj, -1);
+  std::vector<std::array<double, 6>> dbounds;
+  for (int oi = 0; oi < nobj; ++oi) {
+    const RObj& o = objs[oi];
+    if (!SceneDB::is_surface(o.type)) {
+      if (o.type == "instance" || o.type == "mesh")
+        objerr(o, "instances and meshes are not supported on the GPU path");
+      continue;
+    }
+    if (o.type == "source") continue;
+    if (o.omod < 0) continue;   // no modifier: never shaded
+    int mo = findmaterial(oi);
+    Prim P;
+    memset(&P, 0, sizeof P);
+    P.mat = material_index(mo);
+    P.omod = o.omod;
+    P.modslot = -1;
+    P.srcidx = -1;
+    P.goff = (int)cs.geom.size();
+    P.nv = 0;
+    P.flat = 0;
+    double lo[3], hi[3];
+    const auto& f = o.fargs;
+    if (o.type == "polygon") {
+      if (f.size() < 9 || f.size() % 3) objerr(o, "bad # arguments");
+      int nv = (int)f.size() / 3;
+      auto V = [&](int k) { return &f[3 * k]; };
+      if (nv > 3) {
+        double d2 = 0;
+        for (int a = 0; a < 3; ++a) d2 += (V(0)[a] - V(nv - 1)[a]) * (V(0)[a] - V(nv - 1)[a]);
+        if (d2 <= FTINY * FTINY) nv--;
+      }
+      double n[3] = {0, 0, 0}, v1[3], v2[3], v3[3];
+      for (int a = 0; a < 3; ++a) v1[a] = V(1)[a] - V(0)[a];
+      for (int k = 2; k < nv; ++k) {
+        for (int a = 0; a < 3; ++a) v2[a] = V(k)[a] - V(0)[a];
+        cross3(v3, v1, v2);
+        for (int a = 0; a < 3; ++a) n[a] += v3[a];
+        memcpy(v1, v2, sizeof v1);
+      }
+      double area = norm3(n);
+      if (area == 0.0) {
+        cs.warnings.push_back("polygon \"" + o.name + "\": zero area");
+        continue;
+      }
+      double offset = dot3(n, V(0));
+      for (int k = 1; k < nv; ++k) offset += dot3(n, V(k));
+      offset /= (double)nv;
+      int ax = fabs(n[1]) > fabs(n[0]);
+      if (fabs(n[2]) > fabs(n[ax])) ax = 2;
+      P.kind = PK_POLYGON;
+      P.nv = nv;
+      P.flat = 1;
+      cs.geom.insert(cs.geom.end(), {n[0], n[1], n[2], offset, (double)ax});
+      for (int a = 0; a < 3; ++a) { lo[a] = 1e300; hi[a] = -1e300; }
+      for (int k = 0; k < nv; ++k)
+        for (int a = 0; a < 3; ++a) {
+          cs.geom.push_back(V(k)[a]);
+          lo[a] = std::min(lo[a], V(k)[a]);
+          hi[a] = std::max(hi[a], V(k)[a]);
+        }
+    } else if (o.type == "sphere" || o.type == "bubble") {
+      if (f.size() != 4) objerr(o, "bad # arguments");
+      bool bubble = o.type == "bubble";
+      double r = f[3];
+      if (r < -FTINY) { bubble = !bubble; r = -r; cs.warnings.push_back(o.name + ": negative radius"); }
+      else if (r <= FTINY) objerr(o, "zero radius");
+      P.kind = bubble ? PK_BUBBLE : PK_SPHERE;
+      cs.geom.insert(cs.geom.end(), {f[0], f[1], f[2], r});
+      for (int a = 0; a < 3; ++a) { lo[a] = f[a] - r; hi[a] = f[a] + r; }
+    } else {
+      // cone family (getcone)
+      bool ring = o.type == "ring";
+      bool cylfam = o.type == "cylinder" || o.type == "tube";
+      std::string t = o.type;
+      double P0[3], P1[3], R0, R1, ad[3];
+      if (cylfam) {
+        if (f.size() != 7) objerr(o, "bad # arguments");
+        double r = f[6];
+        if (r < -FTINY) { t = (t == "cylinder") ? "tube" : "cylinder"; r = -r; }
+        else if (r <= FTINY) { cs.warnings.push_back(o.name + ": illegal radii"); continue; }
+        for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
+        R0 = R1 = r;
+      } else {
+        if (f.size() != 8) objerr(o, "bad # arguments");
+        double c6 = f[6], c7 = f[7];
+        int s0 = c6 < -FTINY ? -1 : (c6 > FTINY ? 1 : 0);
+        int s1 = c7 < -FTINY ? -1 : (c7 > FTINY ? 1 : 0);
+        if (s0 + s1 == 0) { cs.warnings.push_back(o.name + ": illegal radii"); continue; }
+        if ((s0 < 0) | (s1 < 0)) {
+          if (ring) { cs.warnings.push_back(o.name + ": illegal radii"); continue; }
+          t = (t == "cone") ? "cup" : "cone";
+        }
+        c6 *= s0;
+        c7 *= s1;
+        if (c7 - c6 > FTINY) {
+          for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
+          R0 = c6; R1 = c7;
+        } else if (c6 - c7 > FTINY) {
+          for (int a = 0; a < 3; ++a) { P0[a] = ring ? f[a] : f[3 + a]; P1[a] = ring ? f[3 + a] : f[a]; }
+          R0 = c7; R1 = c6;
+        } else {
+          if (ring) { cs.warnings.push_back(o.name + ": illegal radii"); continue; }
+          t = (t == "cone") ? "cylinder" : "tube";
+          for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
+          R0 = R1 = c6;
+          cylfam = true;
+        }
+      }
+      double al;
+      if (ring) {
+        for (int a = 0; a < 3; ++a) ad[a] = f[3 + a];
+        al = norm3(ad);
+      } else {
+        for (int a = 0; a < 3; ++a) ad[a] = P1[a] - P0[a];
+        al = norm3(ad);
+      }
+      if (al == 0.0) { cs.warnings.push_back(o.name + ": unknown orientation"); continue; }
+      if (ring) {
+        P.kind = PK_RING;
+        P.flat = 1;
+        cs.geom.insert(cs.geom.end(), {f[0], f[1], f[2], ad[0], ad[1], ad[2], R0, R1});
+        for (int a = 0; a < 3; ++a) { lo[a] = f[a] - R1; hi[a] = f[a] + R1; }
+      } else {
+        P.kind = t == "cone" ? PK_CONE : t == "cup" ? PK_CUP : t == "cylinder" ? PK_CYLINDER : PK_TUBE;
+        double sl = cylfam ? al : sqrt((R1 - R0) * (R1 - R0) + al * al);
+        cs.geom.insert(cs.geom.end(), {P0[0], P0[1], P0[2], ad[0], ad[1], ad[2], al, R0, R1, sl});
+        double rr = std::max(R0, R1);
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(P0[a], P1[a]) - rr;
+          hi[a] = std::max(P0[a], P1[a]) + rr;
+        }
+      }
+    }
+    obj_prim[oi] = (int)cs.prims.size();
+    cs.prims.push_back(P);
+    cs.prim_modname.push_back(objs[o.omod].name);
+    dbounds.push_back({lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
+    grow(lo, hi);
+  }
+  if (cs.prims.empty()) {
+    for (int i = 0; i < 3; ++i) { cs.bmin[i] = 0; cs.bmax[i] = 1; }
+  }
+  double ext = std::max({cs.bmax[0] - cs.bmin[0], cs.bmax[1] - cs.bmin[1], cs.bmax[2] - cs.bmin[2], 1e-3});
+  cs.cusize = ext;
+  // FP32 bounds, widened so that the FP32 slab test stays conservative
+  double eps = 1e-5 + 4e-6 * ext;
+  cs.lo.resize(3 * cs.prims.size());
+  cs.hi.resize(3 * cs.prims.size());
+  for (size_t k = 0; k < cs.prims.size(); ++k)
+    for (int a = 0; a < 3; ++a) {
+      float l = (float)(dbounds[k][a] - eps), h = (float)(dbounds[k][3 + a] + eps);
+      cs.lo[3 * k + a] = nextafterf(l, -INFINITY);
+      cs.hi[3 * k + a] = nextafterf(h, INFINITY);
+    }
+
+  // pass 2: light sources (marksources)
+  for (int oi = 0; oi < nobj; ++oi) {
+    const RObj& o = objs[oi];
+    if (!SceneDB::is_surface(o.type) || o.omod < 0) continue;
+    int mo = findmaterial(oi);
+    if (mo < 0) continue;
+    const RObj& m = objs[mo];
+    if (m.type != "light" && m.type != "glow" && m.type != "illum" && m.type != "spotlight") continue;
+    if (m.type == "spotlight") objerr(m, "spotlight sources are not supported on the GPU path");
+    size_t need = m.type == "glow" ? 4 : 3;
+    if (m.fargs.size() != need) objerr(m, "bad # arguments");
+    if (m.fargs[0] <= FTINY && (m.fargs[1] <= FTINY) & (m.fargs[2] <= FTINY)) continue;
+    if (m.type == "glow" && o.type != "source" && m.fargs[3] <= FTINY) continue;
+    Source S;
+    memset(&S, 0, sizeof S);
+    S.prim = -1;
+    S.mat = material_index(mo);
+    S.objmod = o.omod;
+    S.modslot = -1;
+    const auto& f = o.fargs;
+    auto setflatss = [&]() {
+      double u[3];
+      if (!perpendicular3(u, S.snorm)) objerr(o, "bad source orientation");
+      double mult = .5 * sqrt(S.ss2);
+      for (int a = 0; a < 3; ++a) S.ss[0][a] = u[a] * mult;
+      cross3(S.ss[1], S.snorm, S.ss[0]);
+    };
+    if (o.type == "source") {
+      if (f.size() != 4) objerr(o, "bad arguments");
+      S.flags |= SDISTANT | SCIR;
+      for (int a = 0; a < 3; ++a) S.sloc[a] = f[a];
+      if (norm3(S.sloc) == 0.0) objerr(o, "zero direction");
+      double theta = PI_D / 180.0 / 2.0 * f[3];
+      if (theta <= FTINY) objerr(o, "zero size");
+      S.ss2 = 2.0 * PI_D * (1.0 - cos(theta));
+      S.srad = sqrt(S.ss2 / PI_D);
+      memcpy(S.snorm, S.sloc, sizeof S.snorm);
+      setflatss();
+      S.ss[2][0] = S.ss[2][1] = S.ss[2][2] = 0.0;
+    } else {
+      int pi = obj_prim[oi];
+      if (pi < 0) continue;
+      const Prim& P = cs.prims[pi];
+      const double* g = cs.geom.data() + P.goff;
+      S.prim = pi;
+      if (P.kind == PK_POLYGON) {
+        int nv = P.nv;
+        const double* va = g + 5;
+        for (int a = 0; a < 3; ++a) {
+          S.sloc[a] = 0;
+          for (int k = 0; k < nv; ++k) S.sloc[a] += va[3 * k + a];
+          S.sloc[a] /= (double)nv;
+        }
+        S.flags |= SFLAT;
+        for (int a = 0; a < 3; ++a) S.snorm[a] = g[a];
+        // area = half the fan cross-product length
+        double n[3] = {0, 0, 0}, v1[3], v2[3], v3[3];
+        for (int a = 0; a < 3; ++a) v1[a] = va[3 + a] - va[a];
+        for (int k = 2; k < nv; ++k) {
+          for (int a = 0; a < 3; ++a) v2[a] = va[3 * k + a] - va[a];
+          cross3(v3, v1, v2);
+          for (int a = 0; a < 3; ++a) n[a] += v3[a];
+          memcpy(v1, v2, sizeof v1);
+        }
+        S.ss2 = 0.5 * sqrt(dot3(n, n));
+        S.srad = 0;
+        for (int k = 0; k < nv; ++k) {
+          double d2 = 0;
+          for (int a = 0; a < 3; ++a) d2 += (va[3 * k + a] - S.sloc[a]) * (va[3 * k + a] - S.sloc[a]);
+          S.srad = std::max(S.srad, d2);
+        }
+        S.srad = sqrt(S.srad);
+        if (nv == 4) {
+          for (int a = 0; a < 3; ++a) {
+            S.ss[0][a] = .5 * (va[3 + a] - va[a]);
+            S.ss[1][a] = .5 * (va[9 + a] - va[a]);
+          }
+        } else if (nv == 3) {
+          auto dist2line = [&](const double* p, const double* p1, const double* p2) {
+            double d[3], c[3], e[3];
+            for (int a = 0; a < 3; ++a) { d[a] = p2[a] - p1[a]; e[a] = p[a] - p1[a]; }
+            cross3(c, d, e);
+            return dot3(c, c) / dot3(d, d);
+          };
+          int near0 = 2;
+          double dmin = dist2line(S.sloc, va + 6, va);
+          for (int k = 0; k < 2; ++k) {
+            double d2 = dist2line(S.sloc, va + 3 * k, va + 3 * (k + 1));
+            if (d2 >= dmin) continue;
+            near0 = k;
+            dmin = d2;
+          }
+          int k1 = (near0 + 1) % 3;
+          for (int a = 0; a < 3; ++a) S.ss[0][a] = va[3 * k1 + a] - va[3 * near0 + a];
+          norm3(S.ss[0]);
+          dmin = sqrt(dmin);
+          for (int a = 0; a < 3; ++a) S.ss[0][a] *= dmin;
+          cross3(S.ss[1], S.snorm, S.ss[0]);
+        } else {
+          setflatss();
+        }
+      } else if (P.kind == PK_SPHERE || P.kind == PK_BUBBLE) {
+        S.flags |= SCIR;
+        for (int a = 0; a < 3; ++a) S.sloc[a] = g[a];
+        S.srad = g[3];
+        S.ss2 = PI_D * S.srad * S.srad;
+        for (int a = 0; a < 3; ++a) S.ss[a][a] = 0.7236 * g[3];
+      } else if (P.kind == PK_RING) {
+        if (g[6] > 0.0) objerr(o, "cannot hit source center");
+        for (int a = 0; a < 3; ++a) { S.sloc[a] = g[a]; S.snorm[a] = g[3 + a]; }
+        S.flags |= SFLAT | SCIR;
+        S.srad = g[7];
+        S.ss2 = PI_D * S.srad * S.srad;
+        setflatss();
+      } else {
+        objerr(o, "illegal material (this surface type cannot be a light source on the GPU path)");
+      }
+      cs.prims[pi].srcidx = (int)cs.srcs.size();
+    }
+    if (m.type == "glow") {
+      S.flags |= SPROX;
+      S.prox = m.fargs[3];
+      if (S.flags & SDISTANT) S.flags |= SSKIP;
+    }
+    cs.srcs.push_back(S);
+    cs.src_modname.push_back(objs[o.omod].name);
+  }
+  if (cs.srcs.size() > 32000) throw RadError("too many light sources for the GPU path");
+
+  // built-in Lambertian used by -I / -i (ρ = π, rt/raytrace.c:24-28)
+  Material L;
+  memset(&L, 0, sizeof L);
+  L.kind = MK_PLASTIC;
+  L.nfargs = 5;
+  L.f[0] = L.f[1] = L.f[2] = PI_D;
+  L.illum_alt = -1;
+  cs.lamb = (int)cs.mats.size();
+  cs.mats.push_back(L);
+  return cs;
+}
+
+}  // namespace rg

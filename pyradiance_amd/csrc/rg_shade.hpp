@@ -1,0 +1,156 @@
+// Shading, source sampling and bin functions for the wavefront renderer.
+//
+// The renderer runs Radiance's ray tree breadth-first.  Every ray carries its
+// cumulative coefficient (the product of rcoef up to the primary ray that
+// raycontrib() of rt/raytrace.c:408-442 would form), so emission found at a
+// leaf is added straight into the sensor's output: rtrace's returned value
+// and rcontrib's bins are both sums over leaves of coefficient x emission.
+#pragma once
+#include "rg_geom.hpp"
+
+namespace rg {
+
+// A spawn record: one parent event that produces `count` child rays.
+struct alignas(16) Spawn {
+  double org[3];     // child origin (parent hit point)
+  float d0[3];       // explicit direction, or hemisphere normal
+  float d1[3];       // hemisphere tangent ux
+  float coef[3];     // cumulative coefficient of each child (before roulette)
+  float cw;          // child weight before roulette
+  float crc;         // largest channel of the child's own rcoef
+  uint32_t sensor;
+  uint32_t count;    // number of children
+  uint16_t ns;       // hemisphere divisions per axis (0: explicit ray)
+  uint16_t crtype;   // child's cumulative ray type
+  uint16_t rtype;    // child's ray type
+  int16_t rsrc;      // child's aimed source
+  uint8_t rlvl;      // child's reflection level
+  uint8_t adepth;    // child's ambient recursion depth
+  uint8_t flags;     // SPF_*
+  uint8_t pad;
+  uint64_t key;      // explicit: child key; hemisphere: parent key
+};
+static_assert(sizeof(Spawn) == 96, "Spawn layout");
+
+enum : uint8_t { SPF_RR = 1, SPF_FORCE = 2, SPF_BACK = 4 };
+
+// State of one traced ray, reconstructed from its spawn record.
+struct RayS {
+  V3 o, d;
+  C3 coef;         // cumulative coefficient
+  float w;         // rweight
+  float crc;       // sintens of own rcoef
+  uint32_t sensor;
+  uint16_t crtype, rtype;
+  int rsrc;
+  int rlvl;
+  int adepth;
+  uint64_t key;
+};
+
+struct RunCfg {
+  Params p;
+  int32_t mode;        // 0 rtrace value, 1 rcontrib bins
+  int32_t irrad;       // -i (Lambertian replacement at first hit)
+  int32_t ncols;       // output doubles per sensor
+  int32_t nbinspec;
+  int32_t accum;       // rays per output record (rcontrib -c)
+  int32_t pad;
+  const BinSpec* bins; // per tracked-modifier slot
+  double* out;         // [nsensor][ncols]
+  unsigned long long* nrays;
+  unsigned long long* visits;  // optional: [0] BVH nodes visited, [1] primitives tested
+  uint32_t* err;       // error flags
+};
+
+enum : uint32_t { ERR_STACK = 1, ERR_CAPACITY = 2, ERR_NOMAT = 4, ERR_GEN = 8 };
+
+// ---- bins (util/reinhartb.cal:19-47, util/klems_full.cal:10-43) ------------
+RG_HD double cal_asin_deg(double x) {   // Asin() of reinhartb.cal
+  const double DEGREE = PI_D / 180;
+  double v = (x - 1 > 0) ? PI_D / 2 : ((-1 - x > 0) ? -PI_D / 2 : asin(x));
+  return v / DEGREE;
+}
+RG_HD double cal_acos_deg(double x) {   // Acos() of klems_full.cal
+  const double DEGREE = PI_D / 180;
+  double v = (x - 1 > 0) ? 0 : ((-1 - x > 0) ? PI_D : acos(x));
+  return v / DEGREE;
+}
+RG_HD double cal_atan2_deg(double y, double x) {   // Atan2() / posangle()
+  const double DEGREE = PI_D / 180;
+  double a = atan2(y, x);
+  return ((-a > 0) ? a + 2 * PI_D : a) / DEGREE;
+}
+
+RG_HD int reinhart_rnaz(int r, double mf) {
+  const int tnaz[7] = {30, 30, 24, 24, 18, 12, 6};
+  if (r - (7 * mf - .5) > 0) return 1;
+  int k = (int)floor((r + .5) / mf);   // select() is 1-based: tnaz(k+1) = tnaz[k]
+  if (k < 0 || k > 6) return 0;
+  return (int)(mf * tnaz[k]);
+}
+
+RG_HD double reinhart_bin(V3 D, const BinSpec& b) {
+  double rNx = b.nrm[0], rNy = b.nrm[1], rNz = b.nrm[2];
+  double Ux = b.up[0], Uy = b.up[1], Uz = b.up[2];
+  double inc_dz = -D.x * rNx - D.y * rNy - D.z * rNz;
+  double inc_rx = -b.rhs * (D.x * (Uy * rNz - Uz * rNy) + D.y * (Uz * rNx - Ux * rNz) + D.z * (Ux * rNy - Uy * rNx));
+  double inc_ry = D.x * Ux + D.y * Uy + D.z * Uz + inc_dz * (rNx * Ux + rNy * Uy + rNz * Uz);
+  double r_alt = cal_asin_deg(inc_dz);
+  double r_azi = cal_atan2_deg(inc_rx, inc_ry);
+  if (!(r_alt > 0)) return -1;
+  double alpha = 90 / (b.mf * 7 + .5);
+  int r_row = (int)floor(r_alt / alpha);
+  int naz = reinhart_rnaz(r_row, b.mf);
+  if (naz <= 0) return -1;
+  double r_inc = 360.0 / naz;
+  double r_azn = (359.9999 - .5 * r_inc - r_azi > 0) ? floor((r_azi + .5 * r_inc) / r_inc) : 0;
+  double acc = 0;
+  for (int r = 0; r < r_row; ++r) acc += reinhart_rnaz(r, b.mf);
+  return acc + r_azn;
+}
+
+RG_HD double klems_bin(V3 D, const BinSpec& b, int which) {
+  // kpola/knaz of klems_full.cal, khpola/khnaz of klems_half.cal and
+  // kqpola/kqnaz of klems_quarter.cal; element 0 is the row count (select(0,..))
+  const double kpola_f[10] = {9, 5, 15, 25, 35, 45, 55, 65, 75, 90};
+  const double knaz_f[10] = {9, 1, 8, 16, 20, 24, 24, 24, 16, 12};
+  const double kpola_h[10] = {7, 6.5, 19.5, 32.5, 45.5, 58.5, 71.5, 90, 0, 0};
+  const double knaz_h[10] = {7, 1, 8, 12, 16, 20, 12, 8, 0, 0};
+  const double kpola_q[10] = {5, 9, 27, 45, 63, 90, 0, 0, 0, 0};
+  const double knaz_q[10] = {5, 1, 8, 12, 12, 8, 0, 0, 0, 0};
+  const double* kpola = which == 0 ? kpola_f : (which == 1 ? kpola_h : kpola_q);
+  const double* knaz = which == 0 ? knaz_f : (which == 1 ? knaz_h : knaz_q);
+  double Nx = b.nrm[0], Ny = b.nrm[1], Nz = b.nrm[2];
+  double Ux = b.up[0], Uy = b.up[1], Uz = b.up[2];
+  double pol = cal_acos_deg(-D.x * Nx - D.y * Ny - D.z * Nz);
+  double azi = cal_atan2_deg(-D.x * Ux - D.y * Uy - D.z * Uz + (Nx * D.x + Ny * D.y + Nz * D.z) * (Nx * Ux + Ny * Uy + Nz * Uz),
+                             -b.rhs * (D.x * (Uy * Nz - Uz * Ny) + D.y * (Uz * Nx - Ux * Nz) + D.z * (Ux * Ny - Uy * Nx)));
+  if (pol - 90 > 0) return -1;
+  int nrows = (int)kpola[0];
+  int r = 1;   // kfindrow(1, pol)
+  while (!(r - kpola[0] + .5 > 0) && (pol - kpola[r] > 0)) r++;
+  if (r > nrows) r = nrows;
+  double acc = 0;
+  for (int k = 1; k < r; ++k) acc += knaz[k];
+  double inc = 360 / knaz[r];
+  double azn = ((360 - .5 * inc) - azi > 0) ? floor((azi + .5 * inc) / inc) : 0;
+  return acc + azn;
+}
+
+RG_HD int eval_bin(V3 D, const BinSpec& b) {
+  double bval;
+  switch (b.kind) {
+    case BK_CONST: bval = b.cbin; break;
+    case BK_REINHART: bval = reinhart_bin(D, b); break;
+    case BK_KLEMS_FULL: bval = klems_bin(D, b, 0); break;
+    case BK_KLEMS_HALF: bval = klems_bin(D, b, 1); break;
+    default: bval = klems_bin(D, b, 2); break;
+  }
+  if (bval <= -.5) return -1;
+  int bn = (int)(bval + .5);
+  if (bn >= b.nbins) return -1;
+  return bn;
+}
+
+}  // namespace rg

@@ -1,0 +1,118 @@
+// Plain data shared by the host scene compiler and the HIP kernels.
+//
+// Everything here is POD so the same bytes are uploaded to HBM unchanged.
+// Geometry is kept in FP64 (the reference's RREAL, common/fvect.h:16) for the
+// primitive tests; the BVH bounds are FP32 and widened outward so the FP32 box
+// test never rejects a primitive the FP64 test would accept.
+#pragma once
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define RG_HD __host__ __device__ __forceinline__
+#else
+#define RG_HD inline
+#endif
+
+namespace rg {
+
+constexpr double FTINY = 1e-6;   // common/fvect.h:17
+constexpr double FHUGE = 1e10;   // common/fvect.h:20
+constexpr double PI_D = 3.14159265358979323846;
+
+// ray type flags (rt/ray.h:27-44)
+enum : uint16_t {
+  PRIMARY = 01, RSHADOW = 02, REFLECTED = 04, REFRACTED = 010, TRANS = 020,
+  RAMBIENT = 040, RSPECULAR = 0100, TSHADOW = 0200, TAMBIENT = 0400,
+  TSPECULAR = 01000,
+  SHADOW = RSHADOW | TSHADOW,
+  AMBIENT = RAMBIENT | TAMBIENT,
+  SPECULAR = RSPECULAR | TSPECULAR,
+  RAYREFL = RSHADOW | REFLECTED | RAMBIENT | RSPECULAR,
+};
+
+// primitive kinds
+enum PrimKind : int32_t {
+  PK_POLYGON = 0, PK_SPHERE = 1, PK_BUBBLE = 2, PK_RING = 3,
+  PK_CONE = 4, PK_CUP = 5, PK_CYLINDER = 6, PK_TUBE = 7,
+};
+
+// material kinds (only the set the GPU path shades)
+enum MatKind : int32_t {
+  MK_PLASTIC = 0, MK_METAL = 1, MK_TRANS = 2, MK_GLASS = 3,
+  MK_LIGHT = 4, MK_GLOW = 5, MK_ILLUM = 6, MK_NONE = 7,
+};
+
+struct Material {
+  int32_t kind;      // MatKind
+  int32_t nfargs;
+  double f[8];       // real arguments (plastic 5, trans 7, glass 3/4, light 3, glow 4)
+  int32_t illum_alt; // illum: alternate material index (-1 = void / transparent)
+  int32_t transp;    // istransp() of rt/otypes.h (trans/glass/illum-void)
+};
+
+// Source flags (rt/source.h)
+enum : int32_t {
+  SDISTANT = 01, SSKIP = 02, SPROX = 04, SSPOT = 010, SVIRTUAL = 020,
+  SFLAT = 040, SCYL = 0100, SFOLLOW = 0200, SCIR = 01000,
+};
+
+struct Source {
+  double sloc[3];    // direction (distant) or centre
+  double snorm[3];
+  double ss[3][3];   // sampling vectors SU, SV, SW
+  double ss2;        // solid angle (distant) or area
+  double srad;
+  double prox;       // glow proximity radius
+  int32_t flags;
+  int32_t prim;      // primitive index for local sources, -1 for distant
+  int32_t mat;       // material index of the emitter
+  int32_t modslot;   // tracked-modifier slot of the source object (-1 none)
+  int32_t objmod;    // modifier object number (tie rule / identity)
+  int32_t pad;
+};
+
+// Per-primitive record (all kinds); geometry lives in a shared FP64 pool.
+struct Prim {
+  int32_t kind;      // PrimKind
+  int32_t mat;       // resolved material index (-1: none)
+  int32_t omod;      // modifier object number, the "later modifier wins" key
+  int32_t modslot;   // tracked-modifier slot (rcontrib), -1 if untracked
+  int32_t goff;      // offset into geometry pool (doubles)
+  int32_t nv;        // polygon vertex count
+  int32_t flat;      // isflat(): polygon / ring
+  int32_t srcidx;    // source index if this primitive is a local source, else -1
+};
+
+// Bin functions evaluated per binned ray (util/reinhartb.cal, util/klems_*.cal)
+enum BinKind : int32_t { BK_CONST = 0, BK_REINHART = 1, BK_KLEMS_FULL = 2,
+                         BK_KLEMS_HALF = 3, BK_KLEMS_QUARTER = 4 };
+
+struct BinSpec {
+  int32_t kind;      // BinKind
+  int32_t nbins;
+  int32_t col0;      // first output column (in units of bins)
+  int32_t cbin;      // constant bin for BK_CONST
+  double mf;         // Reinhart subdivision
+  double rhs;        // +1 right-handed, -1 left-handed
+  double nrm[3];     // rNx,rNy,rNz (Reinhart) or Nx,Ny,Nz (Klems)
+  double up[3];      // Ux,Uy,Uz
+};
+
+// Ray-tracing parameters (subset of RAYPARAMS, rt/ray.h:157-190)
+struct Params {
+  int32_t ambounce;     // -ab
+  int32_t ambdiv;       // -ad
+  int32_t maxdepth;     // -lr
+  int32_t backvis;      // -bv
+  double minweight;     // -lw
+  double dstrsrc;       // -dj
+  double srcsizerat;    // -ds
+  double specthresh;    // -st
+  double specjitter;    // -ss
+  double ambval[3];     // -av
+  int32_t directvis;    // -dv
+  int32_t contrib;      // rcontrib -V
+  uint64_t seed;
+};
+
+}  // namespace rg

@@ -1,0 +1,332 @@
+"""Drop-in rtrace / Rcontrib with the signatures of pyradiance (src/pyradiance/rt.py).
+
+The argument lists and byte formats follow the reference wrappers
+(rt.py:165-235 for Rcontrib, rt.py:278-362 for rtrace); the work runs on the
+GPU through libradgpu.so instead of a subprocess.  Errors surface as
+RuntimeError("An error occurred with exit code N: <message>"), the shape the
+reference's handle_called_process_error produces (anci.py:13-30).
+"""
+
+from __future__ import annotations
+
+import re
+import time
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from . import params as _params
+from .scenes import reinhart_nbins
+
+VERSION = "RADIANCE 6.0a pyradiance_amd (MI355X)"
+
+_contexts: dict = {}
+
+
+def _rterr(msg: str, code: int = 1) -> RuntimeError:
+    return RuntimeError(f"An error occurred with exit code {code}: {msg}")
+
+
+def _scene(octree: Path | str) -> "_lib.Context":
+    """One committed context per scene file, reused while the file is unchanged."""
+    p = Path(octree)
+    try:
+        st = p.stat()
+    except OSError:
+        raise _rterr(f"cannot open octree file \"{octree}\"")
+    key = (str(p.resolve()), st.st_mtime_ns, st.st_size)
+    ctx = _contexts.get(key[0])
+    if ctx is not None and ctx[0] == key:
+        return ctx[1]
+    c = _lib.Context(0)
+    try:
+        c.load(str(p)).commit()
+    except RuntimeError as e:
+        raise _rterr(str(e))
+    _contexts[key[0]] = (key, c)
+    return c
+
+
+def _read_rays(rays: bytes, inform: str) -> np.ndarray:
+    if inform == "a":
+        vals = np.array(rays.split(), dtype=np.float64)
+    elif inform == "f":
+        vals = np.frombuffer(rays, dtype=np.float32).astype(np.float64)
+    elif inform == "d":
+        vals = np.frombuffer(rays, dtype=np.float64)
+    else:
+        raise _rterr(f"unsupported input format -f{inform}")
+    if vals.size % 6:
+        raise _rterr("unexpected EOF on input")
+    return vals.reshape(-1, 6)
+
+
+def _octree_header(octree: Path | str) -> list[str]:
+    """Header lines of the scene file (checkheader() copy, common/header.c)."""
+    try:
+        with open(octree, "rb") as f:
+            head = f.read(65536)
+    except OSError:
+        return []
+    if not head.startswith(b"#?RADIANCE"):
+        return []
+    end = head.find(b"\n\n")
+    lines = head[:end].decode(errors="replace").split("\n")[1:] if end > 0 else []
+    return [ln for ln in lines if ln and not ln.startswith("FORMAT=")]
+
+
+def _header(prog: str, argv: list[str], octree, outform: str, extra: str = "") -> bytes:
+    fmt = {"a": "ascii", "f": "float", "d": "double", "c": "32-bit_rle_rgbe"}[outform]
+    lines = ["#?RADIANCE", *_octree_header(octree), " ".join([prog, *argv, str(octree)]),
+             f"SOFTWARE= {VERSION}", time.strftime("CAPDATE= %Y:%m:%d %H:%M:%S"), "NCOMP=3"]
+    if extra:
+        lines.append(extra.rstrip("\n"))
+    if outform in "fd":
+        lines.append(f"BigEndian={int(np.little_endian is False)}")
+    lines.append(f"FORMAT={fmt}")
+    return ("\n".join(lines) + "\n\n").encode()
+
+
+def _format(values: np.ndarray, outform: str, fmt: str, per_line: int) -> bytes:
+    """Records of `per_line` values: ASCII with `fmt` + TAB, or binary."""
+    if outform == "a":
+        if values.size == 0:
+            return b""
+        lines = values.reshape(-1, per_line)
+        row = (fmt + "\t") * per_line + "\n"
+        return "".join(row % tuple(r) for r in lines).encode()
+    if outform == "f":
+        return values.astype("<f4").tobytes()
+    if outform == "d":
+        return values.astype("<f8").tobytes()
+    raise _rterr(f"unsupported output format -f?{outform}")
+
+
+def _seed(uncorrelated: bool) -> int:
+    return int(time.time_ns() & 0xFFFFFFFF) if uncorrelated else 0
+
+
+def rtrace(
+    rays: bytes,
+    octree: Path | str,
+    header: bool = True,
+    inform: str = "a",
+    outform: str = "a",
+    irradiance: bool = False,
+    irradiance_lambertian: bool = False,
+    outspec: None | str = None,
+    trace_exclude: str = "",
+    trace_include: str = "",
+    trace_exclude_file: None | str | Path = None,
+    trace_include_file: None | str | Path = None,
+    uncorrelated: bool = False,
+    xres: None | int = None,
+    yres: None | int = None,
+    nproc: None | int = None,
+    params: None | Sequence[str] = None,
+    report: bool = False,
+    version: bool = False,
+) -> bytes:
+    """rtrace on the GPU (reference: src/pyradiance/rt.py:278-362).
+
+    The result bytes are laid out as bin/rtrace writes them: a header, then
+    one "%e\\t%e\\t%e\\t\\n" line (or 3 float/double values) per input ray.
+    Only the value output (-ov) is produced on the GPU path.
+    """
+    if version:
+        return (VERSION + "\n").encode()
+    if not isinstance(rays, bytes):
+        raise TypeError("Rays must be bytes")
+    argv: list[str] = []
+    if not header:
+        argv.append("-h")
+    if irradiance:
+        argv.append("-I")
+    elif irradiance_lambertian:
+        argv.append("-i")
+    argv.append(f"-f{inform}{outform}")
+    if outspec:
+        argv.append(f"-o{outspec}")
+    if trace_exclude or trace_include or trace_exclude_file or trace_include_file:
+        raise _rterr("trace include/exclude lists are not supported on the GPU path")
+    if uncorrelated:
+        argv.append("-u+")
+    if params is not None:
+        argv.extend(params)
+    try:
+        opt = _params.parse(argv, rcontrib=False)
+    except ValueError as e:
+        raise _rterr(str(e))
+    if opt.outspec != "v":
+        raise _rterr(f"output specification -o{opt.outspec} is not supported on the GPU path (only -ov)")
+    ctx = _scene(octree)
+    opt.params.seed = _seed(opt.uncorrelated)
+    vals = _read_rays(rays, opt.inform)
+    flags = (_lib.RT_IMM_IRRAD if opt.imm_irrad else 0) | (_lib.RT_LAMB_IRRAD if opt.lamb_irrad else 0)
+    try:
+        ctx.set_params(opt.params)
+        dirs = np.linalg.norm(vals[:, 3:], axis=1) if len(vals) else np.zeros(0)
+        good = dirs > 0
+        out = np.zeros((len(vals), 3))
+        if good.any():
+            out[good] = ctx.trace(vals[good], flags)
+    except RuntimeError as e:
+        raise _rterr(str(e))
+    body = _format(out, opt.outform, "%e", 3)
+    if not opt.header:
+        return body
+    return _header("rtrace", argv, octree, opt.outform) + body
+
+
+def _parse_assign(s: str | None) -> dict:
+    out = {}
+    if not s:
+        return out
+    for part in re.split(r"[,;]", s):
+        part = part.strip()
+        if not part:
+            continue
+        k, _, v = part.partition("=")
+        out[k.strip()] = float(v)
+    return out
+
+
+_KLEMS = {"klems_full.cal": (2, "kbin", 145, "Nkbins"), "klems_half.cal": (3, "khbin", 77, "Nkhbins"),
+          "klems_quarter.cal": (4, "kqbin", 41, "Nkqbins")}
+_KDIR = {"N": ((0, -1, 0), (0, 0, 1)), "E": ((-1, 0, 0), (0, 0, 1)), "S": ((0, 1, 0), (0, 0, 1)),
+         "W": ((1, 0, 0), (0, 0, 1)), "D": ((0, 0, -1), (0, 1, 0))}
+
+
+def _modifier_spec(name: str, calfile, expression, nbins, binv, param) -> "_lib.RgModifier":
+    """Map a modifier's -f/-e/-p/-bn/-b arguments onto a GPU bin function."""
+    m = _lib.RgModifier()
+    m.name = name.encode()
+    m.rhs = 1.0
+    vars_ = _parse_assign(param)
+    vars_.update(_parse_assign(expression))
+    cal = Path(str(calfile)).name if calfile else None
+    if "RHS" in vars_:
+        m.rhs = vars_["RHS"]
+    if cal == "reinhartb.cal":
+        mf = vars_.get("MF", 1.0)
+        m.bin_kind, m.mf = 1, mf
+        m.nrm[:] = (vars_.get("rNx", 0.0), vars_.get("rNy", 0.0), vars_.get("rNz", -1.0))
+        m.up[:] = (vars_.get("Ux", 0.0), vars_.get("Uy", 1.0), vars_.get("Uz", 0.0))
+        if binv not in (None, "rbin"):
+            raise ValueError(f"bin expression {binv!r} is not supported with reinhartb.cal")
+        n = reinhart_nbins(int(mf))
+        m.nbins = n if nbins in (None, "Nrbins") else int(float(nbins))
+        return m
+    if cal in _KLEMS:
+        kind, fname, n, nname = _KLEMS[cal]
+        m.bin_kind = kind
+        b = (binv or "").replace(" ", "")
+        mm = re.fullmatch(fname + r"([NESWD])", b)
+        if mm:
+            m.nrm[:], m.up[:] = _KDIR[mm.group(1)]
+        else:
+            mm = re.fullmatch(fname + r"\(([^)]*)\)", b)
+            if not mm:
+                raise ValueError(f"bin expression {binv!r} is not supported with {cal}")
+            v = [float(x) for x in mm.group(1).split(",")]
+            if len(v) != 6:
+                raise ValueError(f"bad {fname} arguments")
+            m.nrm[:], m.up[:] = v[:3], v[3:]
+        m.nbins = n if nbins in (None, nname) else int(float(nbins))
+        return m
+    if cal is not None:
+        raise ValueError(f"bin file {cal} is not supported on the GPU path")
+    m.bin_kind = 0
+    m.cbin = int(float(binv)) if binv is not None else 0
+    m.nbins = int(float(nbins)) if nbins is not None else 1
+    return m
+
+
+class Rcontrib:
+    """rcontrib on the GPU (reference: src/pyradiance/rt.py:165-235)."""
+
+    def __init__(
+        self,
+        inp: bytes,
+        octree: Path | str,
+        nproc: int = 1,
+        yres: None | int = None,
+        inform: None | str = None,
+        outform: None | str = None,
+        report: int = 0,
+        params: None | Sequence[str] = None,
+    ):
+        self.octree = octree
+        self.inp = inp
+        self.argv: list[str] = ["-n", str(nproc)]
+        if params is not None:
+            self.argv.extend(params)
+        if None not in (inform, outform):
+            self.argv.append(f"-f{inform}{outform}")
+        if yres is not None:
+            self.argv.extend(["-y", str(yres)])
+        if report:
+            self.argv.extend(["-t", str(report)])
+        self.modifiers: list = []
+
+    def add_modifier(
+        self,
+        modifier: None | str = None,
+        modifier_path: None | str = None,
+        calfile: None | str = None,
+        expression: None | str = None,
+        nbins: None | str = None,
+        binv: None | str = None,
+        param: None | str = None,
+        xres: None | int = None,
+        yres: None | int = None,
+        output: None | str = None,
+    ):
+        if modifier is not None:
+            names = [modifier]
+        elif modifier_path is not None:
+            names = Path(modifier_path).read_text().split()
+        else:
+            raise ValueError("Modifier or modifier path must be provided.")
+        if output is not None:
+            raise _rterr("per-modifier output files (-o) are not supported on the GPU path")
+        for nm in names:
+            self.modifiers.append((nm, calfile, expression, nbins, binv, param))
+        return self
+
+    def matrix(self) -> np.ndarray:
+        """Contribution matrix (records x 3*sum(nbins)) as float64, no formatting."""
+        return self._run()[0]
+
+    def _run(self):
+        try:
+            opt = _params.parse(self.argv, rcontrib=True)
+            mods = [_modifier_spec(*m) for m in self.modifiers]
+        except ValueError as e:
+            raise _rterr(str(e))
+        if not mods:
+            raise _rterr("missing required modifier argument")
+        ctx = _scene(self.octree)
+        opt.params.seed = _seed(opt.uncorrelated)
+        inp = self.inp if isinstance(self.inp, bytes) else bytes(self.inp)
+        vals = _read_rays(inp, opt.inform)
+        acc = max(opt.accumulate, 1)
+        nrec = len(vals) // acc
+        if nrec * acc != len(vals):
+            vals = vals[:nrec * acc]
+        flags = (_lib.RT_IMM_IRRAD if opt.imm_irrad else 0) | (_lib.RT_LAMB_IRRAD if opt.lamb_irrad else 0)
+        try:
+            ctx.set_params(opt.params)
+            out = ctx.contrib(vals, mods, acc, flags)
+        except RuntimeError as e:
+            raise _rterr(str(e))
+        return out, opt
+
+    def __call__(self) -> bytes:
+        out, opt = self._run()
+        body = _format(out.ravel(), opt.outform, "%.6e", out.shape[1] if out.size else 3)
+        if not opt.header:
+            return body
+        return _header("rcontrib", self.argv, self.octree, opt.outform) + body

@@ -1,0 +1,237 @@
+"""Seeded synthetic workloads: shoebox rooms, furniture, sensor grids, skies.
+
+Every generator is deterministic for a given seed (numpy PCG64) and writes
+plain Radiance scene text (``.rad``), which both the reference ``oconv`` and
+this package's own loader read.  Geometry conventions follow Radiance:
+polygon normals come from the right-hand rule over the vertex order
+(common/face.c:17-24), +Y is north, +Z is up (util/reinhartb.cal:24-25).
+
+The south wall carries its window as one polygon with a doubled seam edge,
+which the crossing-number test of common/face.c:121-162 treats as a hole.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+__all__ = [
+    "Room",
+    "room_rad",
+    "sun_rad",
+    "sky_rad",
+    "sun_direction",
+    "sensor_grid",
+    "box_quads_rad",
+    "config1",
+    "config2",
+    "furniture_rad",
+    "sky_matrix",
+    "reinhart_nbins",
+]
+
+
+@dataclass
+class Room:
+    width: float = 6.0      # x extent (m)
+    depth: float = 4.0      # y extent (m)
+    height: float = 3.0     # z extent (m)
+    rho_floor: float = 0.2
+    rho_wall: float = 0.5
+    rho_ceiling: float = 0.7
+    window: bool = True
+    win_w: float = 2.0
+    win_h: float = 1.2
+    win_sill: float = 0.9
+    glass_tau: float = 0.6
+    extra: list = field(default_factory=list)
+
+
+def _poly(mod: str, name: str, verts) -> str:
+    flat = " ".join(f"{c:.9g}" for v in verts for c in v)
+    return f"{mod} polygon {name}\n0\n0\n{3 * len(verts)} {flat}\n"
+
+
+def room_rad(room: Room | None = None) -> str:
+    """Six inward-facing plastic walls, optional glass window in the south wall."""
+    r = room or Room()
+    W, D, H = r.width, r.depth, r.height
+    out = [
+        f"void plastic floor_mat\n0\n0\n5 {r.rho_floor} {r.rho_floor} {r.rho_floor} 0 0\n",
+        f"void plastic wall_mat\n0\n0\n5 {r.rho_wall} {r.rho_wall} {r.rho_wall} 0 0\n",
+        f"void plastic ceiling_mat\n0\n0\n5 {r.rho_ceiling} {r.rho_ceiling} {r.rho_ceiling} 0 0\n",
+        f"void glass window_glass\n0\n0\n3 {r.glass_tau} {r.glass_tau} {r.glass_tau}\n",
+        _poly("floor_mat", "floor", [(0, 0, 0), (W, 0, 0), (W, D, 0), (0, D, 0)]),
+        _poly("ceiling_mat", "ceiling", [(0, 0, H), (0, D, H), (W, D, H), (W, 0, H)]),
+        _poly("wall_mat", "north_wall", [(0, D, 0), (W, D, 0), (W, D, H), (0, D, H)]),
+        _poly("wall_mat", "west_wall", [(0, 0, 0), (0, D, 0), (0, D, H), (0, 0, H)]),
+        _poly("wall_mat", "east_wall", [(W, 0, 0), (W, 0, H), (W, D, H), (W, D, 0)]),
+    ]
+    if r.window:
+        x0 = 0.5 * (W - r.win_w)
+        x1 = x0 + r.win_w
+        z0 = r.win_sill
+        z1 = z0 + r.win_h
+        south = [(0, 0, 0), (0, 0, H), (W, 0, H), (W, 0, 0), (0, 0, 0),
+                 (x0, 0, z0), (x1, 0, z0), (x1, 0, z1), (x0, 0, z1), (x0, 0, z0)]
+        out.append(_poly("wall_mat", "south_wall", south))
+        out.append(_poly("window_glass", "window",
+                         [(x0, 0, z0), (x0, 0, z1), (x1, 0, z1), (x1, 0, z0)]))
+    else:
+        out.append(_poly("wall_mat", "south_wall",
+                         [(0, 0, 0), (0, 0, H), (W, 0, H), (W, 0, 0)]))
+    out.extend(r.extra)
+    return "".join(out)
+
+
+def sun_direction(alt_deg: float, azi_deg: float) -> np.ndarray:
+    """Unit vector toward the sun; azimuth clockwise from north (+Y) toward east (+X)."""
+    a = math.radians(alt_deg)
+    z = math.radians(azi_deg)
+    return np.array([math.sin(z) * math.cos(a), math.cos(z) * math.cos(a), math.sin(a)])
+
+
+def sun_rad(direction, radiance=(1e6, 1e6, 1e6), angle=0.533) -> str:
+    d = direction
+    return (f"void light solar\n0\n0\n3 {radiance[0]:.9g} {radiance[1]:.9g} {radiance[2]:.9g}\n"
+            f"solar source sun\n0\n0\n4 {d[0]:.12g} {d[1]:.12g} {d[2]:.12g} {angle}\n")
+
+
+def sky_rad(sky=(50.0, 50.0, 50.0), ground=None) -> str:
+    out = (f"void glow skyglow\n0\n0\n4 {sky[0]:.9g} {sky[1]:.9g} {sky[2]:.9g} 0\n"
+           "skyglow source sky\n0\n0\n4 0 0 1 180\n")
+    if ground is not None:
+        out += (f"void glow groundglow\n0\n0\n4 {ground[0]:.9g} {ground[1]:.9g} {ground[2]:.9g} 0\n"
+                "groundglow source ground\n0\n0\n4 0 0 -1 180\n")
+    return out
+
+
+def sensor_grid(nx: int, ny: int, x0: float, y0: float, pitch: float, z: float,
+                normal=(0.0, 0.0, 1.0)) -> np.ndarray:
+    """(nx*ny, 6) float64 array of origin+direction records, row-major in y then x."""
+    xs = x0 + pitch * np.arange(nx)
+    ys = y0 + pitch * np.arange(ny)
+    gx, gy = np.meshgrid(xs, ys)
+    n = gx.size
+    rays = np.empty((n, 6))
+    rays[:, 0] = gx.ravel()
+    rays[:, 1] = gy.ravel()
+    rays[:, 2] = z
+    rays[:, 3:] = normal
+    return rays
+
+
+def box_quads_rad(mod: str, name: str, lo, hi, nu: int = 3, nv: int = 7) -> str:
+    """Axis-aligned box with outward normals, each face split into nu*nv quads."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    out = []
+    # (fixed axis, value, u axis, v axis, outward sign)
+    faces = [(0, lo[0], 1, 2, -1), (0, hi[0], 1, 2, +1),
+             (1, lo[1], 0, 2, -1), (1, hi[1], 0, 2, +1),
+             (2, lo[2], 0, 1, -1), (2, hi[2], 0, 1, +1)]
+    k = 0
+    for ax, val, ua, va, sgn in faces:
+        us = np.linspace(lo[ua], hi[ua], nu + 1)
+        vs = np.linspace(lo[va], hi[va], nv + 1)
+        for i in range(nu):
+            for j in range(nv):
+                quad = []
+                for (a, b) in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                    p = [0.0, 0.0, 0.0]
+                    p[ax] = val
+                    p[ua] = us[i + a]
+                    p[va] = vs[j + b]
+                    quad.append(tuple(p))
+                # right-hand rule over (ua, va) gives +normal along the third axis
+                # when (ua, va, ax) is a cyclic order; flip to keep normals outward.
+                cyc = (ua, va, ax) in ((0, 1, 2), (1, 2, 0), (2, 0, 1))
+                if (sgn > 0) != cyc:
+                    quad = quad[::-1]
+                out.append(_poly(mod, f"{name}.{k}", quad))
+                k += 1
+    return "".join(out)
+
+
+def furniture_rad(rng: np.random.Generator, nboxes: int, room: Room,
+                  nu: int = 3, nv: int = 7, prefix: str = "box") -> str:
+    """Seeded axis-aligned boxes, sides U(0.3,1.5) m, clipped inside the room.
+
+    Each box has its own gray plastic with rho ~ U(0.1, 0.8).  Box bottoms sit
+    a few mm above the floor so no face is coplanar with a room surface.
+    """
+    out = []
+    for b in range(nboxes):
+        size = rng.uniform(0.3, 1.5, 3)
+        size = np.minimum(size, [room.width - 0.02, room.depth - 0.02, room.height - 0.05])
+        x = rng.uniform(0.01, room.width - size[0] - 0.01)
+        y = rng.uniform(0.01, room.depth - size[1] - 0.01)
+        z = rng.uniform(0.002, 0.01)
+        rho = rng.uniform(0.1, 0.8)
+        out.append(f"void plastic {prefix}_mat{b}\n0\n0\n5 {rho:.6g} {rho:.6g} {rho:.6g} 0 0\n")
+        out.append(box_quads_rad(f"{prefix}_mat{b}", f"{prefix}{b}", (x, y, z),
+                                 (x + size[0], y + size[1], z + size[2]), nu, nv))
+    return "".join(out)
+
+
+def config1(seed: int = 0):
+    """rtrace -I -ab 0 -dj 0 -ds 0 workload: room + window + sun + glow sky, 11x7 sensors.
+
+    Returns (rad_text, sensors (77,6), sun_dir).
+    """
+    rng = np.random.default_rng(seed)
+    alt = rng.uniform(20.0, 60.0)
+    azi = rng.uniform(120.0, 240.0)
+    d = sun_direction(alt, azi)
+    rad = room_rad() + sun_rad(d) + sky_rad()
+    sensors = sensor_grid(11, 7, 0.5, 0.5, 0.5, 0.8)
+    return rad, sensors, d
+
+
+def config2(seed: int = 0, nboxes: int = 200, nsensors: int = 2500, nu: int = 3, nv: int = 7):
+    """rcontrib -ab 4 -ad 4096 workload: room + window + tessellated boxes + binned sky.
+
+    Sensors: the 60x40 interior lattice at 0.1 m pitch (x=0.05..5.95,
+    y=0.05..3.95), then points of the half-offset lattice (x=0.1.., y=0.1..)
+    until ``nsensors`` are reached, all at z=0.8 facing +Z.
+    """
+    rng = np.random.default_rng(seed)
+    room = Room()
+    rad = room_rad(room) + furniture_rad(rng, nboxes, room, nu, nv) + sky_rad((1, 1, 1), ground=(1, 1, 1))
+    main = sensor_grid(60, 40, 0.05, 0.05, 0.1, 0.8)
+    extra = sensor_grid(59, 39, 0.1, 0.1, 0.1, 0.8)
+    sensors = np.concatenate([main, extra])[:nsensors]
+    return rad, sensors
+
+
+def reinhart_nbins(mf: int) -> int:
+    """Number of sky bins of Reinhart MF:n (util/reinhartb.cal:37-47), ground excluded."""
+    tnaz = [30, 30, 24, 24, 18, 12, 6]
+    total = 0
+    for r in range(7 * mf + 1):
+        if r - (7 * mf - 0.5) > 0:
+            total += 1
+        else:
+            total += mf * tnaz[int(math.floor((r + 0.5) / mf))]
+    return total
+
+
+def sky_matrix(seed: int, nbins: int, nhours: int = 8760, latitude: float = 37.8) -> np.ndarray:
+    """Synthetic (nbins, nhours, 3) float32 sky matrix.
+
+    Lognormal(0,1) patch radiances; every hour whose sun altitude at the given
+    latitude is <= 0 is zeroed (simple solar-position model, local solar time).
+    """
+    rng = np.random.default_rng(seed)
+    sky = rng.lognormal(0.0, 1.0, (nbins, nhours, 3)).astype(np.float32)
+    h = np.arange(nhours)
+    day = h // 24
+    hour = (h % 24) + 0.5
+    decl = np.radians(23.45) * np.sin(2 * np.pi * (284 + day + 1) / 365.0)
+    lat = np.radians(latitude)
+    ha = np.radians(15.0 * (hour - 12.0))
+    sin_alt = np.sin(lat) * np.sin(decl) + np.cos(lat) * np.cos(decl) * np.cos(ha)
+    sky[:, sin_alt <= 0.0, :] = 0.0
+    return sky

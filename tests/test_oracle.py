@@ -1,0 +1,166 @@
+"""The oracle (oracle/radoracle.c) pinned against the reference's own outputs,
+textbook closed forms and exact special cases.  CPU only."""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from pyradiance_amd import scenes
+
+
+def glass_transmission(cos_i: float, tau: float, n: float = 1.52) -> float:
+    """Thin-slab transmittance from the Fresnel equations (textbook form):
+    incoherent sum over internal reflections, averaged over s and p."""
+    sin_t = math.sqrt(max(0.0, 1.0 - cos_i * cos_i)) / n
+    cos_t = math.sqrt(1.0 - sin_t * sin_t)
+    rs = ((cos_i - n * cos_t) / (cos_i + n * cos_t)) ** 2
+    rp = ((n * cos_i - cos_t) / (n * cos_i + cos_t)) ** 2
+    a = tau ** (1.0 / cos_t)            # internal transmissivity along the refracted path
+    return sum(0.5 * (1 - r) ** 2 * a / (1 - (r * a) ** 2) for r in (rs, rp))
+
+
+def config1_closed_form(sensors, sun, room=None):
+    """E = L * Omega * cos(theta_z) * T(cos theta_y) * V for the room of scenes.config1."""
+    r = room or scenes.Room()
+    L = 1e6
+    omega = 2 * math.pi * (1 - math.cos(math.radians(0.533) / 2))
+    x0 = 0.5 * (r.width - r.win_w)
+    x1, z0 = x0 + r.win_w, r.win_sill
+    z1 = z0 + r.win_h
+    out = np.zeros(len(sensors))
+    for i, s in enumerate(sensors):
+        p = s[:3] + 1e-4 * s[3:]
+        if sun[1] >= 0:
+            continue
+        t = -p[1] / sun[1]
+        x, z = p[0] + t * sun[0], p[2] + t * sun[2]
+        if x0 < x < x1 and z0 < z < z1:
+            out[i] = L * omega * sun[2] * glass_transmission(abs(sun[1]), r.glass_tau)
+    return out
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_config1_reference_fixture_matches_closed_form(seed):
+    g = np.load(GOLDEN / f"c1_seed{seed}.npz")
+    cf = config1_closed_form(g["sensors"], g["sun"])
+    ref = g["values"][:, 0]
+    assert (ref > 0).sum() == (cf > 0).sum()
+    assert np.allclose(ref, cf, rtol=2e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_config1_oracle_matches_reference(oracle, seed):
+    g = np.load(GOLDEN / f"c1_seed{seed}.npz")
+    rad, sens, sun = scenes.config1(seed)
+    assert np.allclose(sens, g["sensors"]) and np.allclose(sun, g["sun"])
+    out, nrays = oracle.Scene(rad).trace(sens, oracle.params(ambounce=0, dstrsrc=0.0, srcsizerat=0.0), 8)
+    ref = g["values"]
+    assert np.abs(out - ref).max() <= 1e-6 * ref.max() + 1e-9
+    cf = config1_closed_form(sens, sun)
+    assert np.allclose(out[:, 0], cf, rtol=1e-9, atol=1e-12)
+    # one shadow ray per sensor, plus one continuation through the glass when lit
+    assert nrays == len(sens) + int((cf > 0).sum())
+
+
+def test_config1_planted_mistake_is_caught(oracle):
+    """A wrong glass index (a plausible slip) must break the closed-form check."""
+    rad, sens, sun = scenes.config1(0)
+    bad = rad.replace("3 0.6 0.6 0.6\n", "4 0.6 0.6 0.6 1.4\n")
+    assert bad != rad
+    out, _ = oracle.Scene(bad).trace(sens, oracle.params(ambounce=0, dstrsrc=0.0, srcsizerat=0.0), 8)
+    cf = config1_closed_form(sens, sun)
+    lit = cf > 0
+    assert np.abs(out[lit, 0] / cf[lit] - 1).max() > 1e-3
+
+
+@pytest.mark.parametrize("mf", [1, 2, 4])
+def test_reinhart_bins_match_reference(oracle, mf):
+    g = np.load(GOLDEN / "bins_reinhart.npz")
+    m = oracle.mod("skyglow", 1, scenes.reinhart_nbins(mf), mf=mf)
+    assert np.array_equal(oracle.bins(m, g["dirs"]), g[f"mf{mf}"])
+
+
+@pytest.mark.parametrize("name,nrm,up", [("kbinS", (0, 1, 0), (0, 0, 1)), ("kbinD", (0, 0, -1), (0, 1, 0)),
+                                         ("kbinE", (-1, 0, 0), (0, 0, 1))])
+def test_klems_bins_match_reference(oracle, name, nrm, up):
+    g = np.load(GOLDEN / "bins_klems.npz")
+    m = oracle.mod("skyglow", 2, 145, nrm=nrm, up=up)
+    assert np.array_equal(oracle.bins(m, g["dirs"]), g[name])
+
+
+def test_reinhart_bin_counts():
+    # Nrbins of util/reinhartb.cal; MF:4 = 2305 (docs/howtos/guide2.md:54)
+    assert [scenes.reinhart_nbins(m) for m in (1, 2, 4)] == [145, 577, 2305]
+
+
+def test_uniform_sky_irradiance_is_pi(oracle):
+    """Unoccluded upward sensor under a unit-radiance glow sky: E = pi exactly
+    (stratified cosine sampling of a constant integrand has no variance)."""
+    rad = scenes.sky_rad((1, 1, 1), ground=(1, 1, 1))
+    s = oracle.Scene(rad)
+    mods = [oracle.mod("skyglow", 1, 145), oracle.mod("groundglow", 0, 1)]
+    rays = np.array([[0, 0, 1, 0, 0, 1], [0, 0, 0, 0.6, 0, 0.8]], float)
+    p = oracle.params(ambounce=1, ambdiv=1024, minweight=1e-4, dstrsrc=0.9, specthresh=0.02)
+    out, _ = s.contrib(rays, p, mods, 1, 8)
+    tot = out[:, 0::3].sum(1)
+    assert np.allclose(tot, math.pi, rtol=1e-12)
+    # tilted sensor sees part of the ground: the split is a solid-angle integral
+    ground = out[1, -3]
+    assert 0.0 < ground < math.pi
+
+
+def test_oracle_statistics_match_reference(oracle):
+    """Per-(sensor,bin) means of 48 independent oracle estimates against 48
+    reference rcontrib estimates (stat_room.npz): z-scores behave like N(0,1)."""
+    from numpy.random import default_rng
+    g = np.load(GOLDEN / "stat_room.npz")
+    rng = default_rng(1)
+    room = scenes.Room()
+    rad = scenes.room_rad(room) + scenes.furniture_rad(rng, 5, room, 1, 1) + scenes.sky_rad((1, 1, 1), (1, 1, 1))
+    R = int(g["repeats"])
+    rays = np.repeat(g["points"], R, axis=0)
+    mods = [oracle.mod("skyglow", 1, 145), oracle.mod("groundglow", 0, 1)]
+    p = oracle.params(ambounce=2, ambdiv=256, minweight=1e-3, dstrsrc=0.9, specthresh=0.02, seed=11)
+    out, _ = oracle.Scene(rad).contrib(rays, p, mods, 1, 8)
+    out = out.reshape(len(g["points"]), R, -1)
+    ref = g["values"]
+    se = np.sqrt(ref.var(1, ddof=1) / R + out.var(1, ddof=1) / R)
+    ok = se > 0
+    z = np.abs(ref.mean(1) - out.mean(1))[ok] / se[ok]
+    assert (z > 3).mean() < 0.02
+    assert 0.6 < (z ** 2).mean() < 1.6
+    tr, to = ref.sum(2), out.sum(2)
+    zt = np.abs(tr.mean(1) - to.mean(1)) / np.sqrt(tr.var(1, ddof=1) / R + to.var(1, ddof=1) / R)
+    assert (zt < 3.5).all()
+
+
+def test_closest_hit_analytic(oracle):
+    rad = ("void plastic m 0 0 5 .5 .5 .5 0 0\n"
+           "m sphere ball 0 0 4 0 0 0 1\n"
+           "m polygon floor 0 0 12 -5 -5 -2 5 -5 -2 5 5 -2 -5 5 -2\n"
+           "m ring disk 0 0 8 0 0 3 0 0 1 0.5 2\n")
+    s = oracle.Scene(rad)
+    rays = np.array([[0, 0, 5, 0, 0, -1],      # ring hole, then sphere top at t=4
+                     [1, 0, 5, 0, 0, -1],      # ring (r=1 in [0.5, 2]) at t=2
+                     [3, 0, 0, 0, 0, -1],      # floor at t=2
+                     [0, 0, 0, 1, 0, 0],       # inside sphere: t=1
+                     [9, 9, 9, 0, 0, 1]], float)
+    prim, t = s.closest(rays)
+    assert list(prim) == [0, 2, 1, 0, -1]
+    assert np.allclose(t[:4], [4, 2, 2, 1], rtol=1e-12)
+
+
+def test_rng_known_answers(oracle):
+    """The path-keyed generator the GPU path also uses (fixed definition)."""
+    L = oracle.lib()
+    k = L.ro_sensor_key(1, 0)
+    assert k == oracle.lib().ro_sensor_key(1, 0)
+    c = L.ro_child_key(k, 5)
+    vals = [L.ro_rnd(c, i) for i in range(4)]
+    assert all(0.0 <= v < 1.0 for v in vals)
+    assert len(set(vals)) == 4
+    # uniformity over many keys
+    u = np.array([L.ro_rnd(L.ro_child_key(k, j), 0) for j in range(20000)])
+    assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.005
