@@ -183,10 +183,10 @@ def main():
     ctx.set_profiling(0)
     nr = st_c["rays"]
     per_ray = B_RAY + B_NODE * st_c["nodes"] / nr + B_PRIM * st_c["prims"] / nr + B_STATE
-    avg_launch_ms = st_t["bounce_ms"] / max(st_t["launches"], 1)
+    avg_launch_ms = st_t["trace_ms"] / max(st_t["launches"], 1)
     bytes_per_launch = per_ray * st_t["rays"] / max(st_t["launches"], 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
-    log(f"k_bounce: {st_t['launches']:.0f} launches, {st_t['bounce_ms']:.1f} ms total; "
+    log(f"k_trace: {st_t['launches']:.0f} launches, {st_t['trace_ms']:.1f} ms total; k_shade {st_t['shade_ms']:.1f} ms; "
         f"{st_c['nodes'] / nr:.1f} nodes, {st_c['prims'] / nr:.1f} prims per ray; {per_ray:.0f} B/ray")
 
     dcs = dcsky_bench(out)
@@ -218,7 +218,8 @@ def main():
                    "rays_per_step": int(np.mean(rays)), "scene_commit_ms": t_commit * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_bounce", "avg_launch_ms": avg_launch_ms,
+                     "kernel": "k_trace", "avg_launch_ms": avg_launch_ms,
+                     "shade_kernel_ms_per_step": st_t["shade_ms"], "trace_kernel_ms_per_step": st_t["trace_ms"],
                      "bytes_per_ray": per_ray, "nodes_per_ray": st_c["nodes"] / nr,
                      "prims_per_ray": st_c["prims"] / nr},
         "dcsky": dcs,

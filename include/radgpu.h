@@ -97,8 +97,9 @@ int rg_last_nrays(rg_ctx *ctx, unsigned long long *nrays);
  * every generation kernel with HIP events on the context stream, 2: also
  * count BVH nodes visited and primitives tested per ray. */
 int rg_set_profiling(rg_ctx *ctx, int level);
-/* out[0..6] = rays, generation-kernel launches, threads launched, summed
- * generation-kernel milliseconds, nodes visited, primitives tested, batches */
+/* out[0..7] = rays, generation launches, threads launched, summed traversal-
+ * kernel milliseconds, nodes visited, primitives tested, batches, summed
+ * shading-kernel milliseconds */
 int rg_last_stats(rg_ctx *ctx, double *out, int n);
 
 /* DC x sky product, replaces cm_multiply() (util/cmatrix.c:421-474):

@@ -165,9 +165,9 @@ class Context:
         self._check(self._L.rg_set_profiling(self._h, level))
 
     def last_stats(self) -> dict:
-        v = (C.c_double * 7)()
-        self._check(self._L.rg_last_stats(self._h, v, 7))
-        keys = ("rays", "launches", "threads", "bounce_ms", "nodes", "prims", "batches")
+        v = (C.c_double * 8)()
+        self._check(self._L.rg_last_stats(self._h, v, 8))
+        keys = ("rays", "launches", "threads", "trace_ms", "nodes", "prims", "batches", "shade_ms")
         return dict(zip(keys, list(v)))
 
     def last_nrays(self) -> int:

@@ -326,10 +326,10 @@ int rg_set_profiling(rg_ctx* c, int level) {
 
 int rg_last_stats(rg_ctx* c, double* out, int n) {
   RG_TRY(c, {
-    double v[7] = {(double)c->last_nrays, (double)c->last_stats.launches, (double)c->last_stats.threads,
+    double v[8] = {(double)c->last_nrays, (double)c->last_stats.launches, (double)c->last_stats.threads,
                    c->last_stats.bounce_ms, (double)c->last_visits[0], (double)c->last_visits[1],
-                   (double)c->last_stats.generations};
-    for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
+                   (double)c->last_stats.generations, c->last_stats.shade_ms};
+    for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
   });
 }
 

@@ -13,8 +13,10 @@
 
 namespace rg {
 
-constexpr int BLOCK = 128;
-constexpr int STACK = 64;
+constexpr int BLOCK = 128;       // shading and seed kernels
+constexpr int TBLOCK = 256;      // traversal kernel
+constexpr int LSTACK = 16;       // per-lane traversal stack entries held in LDS
+constexpr int PSTACK = 48;       // further entries in private (scratch) memory
 
 // ---- wave-level helpers -----------------------------------------------------
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
@@ -67,13 +69,16 @@ __device__ __forceinline__ bool slab(const float* lo, const float* hi, float ox,
   return (tnear <= tfar) & (tfar >= 0.0f) & (tnear <= tmax);
 }
 
+// Stack: the top LSTACK entries of each lane live in LDS (column-major, one
+// bank per lane); deeper entries spill to a private array that only deep
+// subtrees touch.
+template <int BS>
 __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* stk, uint32_t* err,
                             unsigned long long* visits) {
   uint32_t nvis = 0, nprim = 0;
   h.t = FHUGE; h.rod = 1.0; h.prim = -1;
   if (S.nprims == 0) return;
   if (S.root < 0) { hit_prim(S, S.leaf_prim[~S.root], o, d, h); return; }
-  auto tested = [&]() { ++nprim; };
   const float ox = (float)o.x, oy = (float)o.y, oz = (float)o.z;
   auto inv = [](double v) -> float {
     float f = (float)v;
@@ -82,6 +87,7 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
   };
   const float ix = inv(d.x), iy = inv(d.y), iz = inv(d.z);
   const int lane = threadIdx.x;
+  int32_t deep[PSTACK];
   int sp = 0;
   int node = S.root;
   // every internal node is entered at most once per ray
@@ -93,17 +99,15 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
     float tl, tr;
     bool hl = slab(N.lmin, N.lmax, ox, oy, oz, ix, iy, iz, tmax, tl);
     bool hr = slab(N.rmin, N.rmax, ox, oy, oz, ix, iy, iz, tmax, tr);
-    if (hl & (N.left < 0)) { hit_prim(S, S.leaf_prim[~N.left], o, d, h); hl = false; tested(); }
-    if (hr & (N.right < 0)) { hit_prim(S, S.leaf_prim[~N.right], o, d, h); hr = false; tested(); }
+    if (hl & (N.left < 0)) { hit_prim(S, S.leaf_prim[~N.left], o, d, h); hl = false; ++nprim; }
+    if (hr & (N.right < 0)) { hit_prim(S, S.leaf_prim[~N.right], o, d, h); hr = false; ++nprim; }
     if (hl & hr) {
       int nearc = tl <= tr ? N.left : N.right;
       int farc = tl <= tr ? N.right : N.left;
-      if (sp < STACK) {
-        stk[sp * BLOCK + lane] = farc;
-        ++sp;
-      } else {
-        atomicOr(err, ERR_STACK);
-      }
+      if (sp < LSTACK) stk[sp * BS + lane] = farc;
+      else if (sp < LSTACK + PSTACK) deep[sp - LSTACK] = farc;
+      else { atomicOr(err, ERR_STACK); break; }
+      ++sp;
       node = nearc;
     } else if (hl) {
       node = N.left;
@@ -112,7 +116,7 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
     } else {
       if (sp == 0) break;
       --sp;
-      node = stk[sp * BLOCK + lane];
+      node = sp < LSTACK ? stk[sp * BS + lane] : deep[sp - LSTACK];
     }
   }
   if (visits) {
@@ -769,22 +773,8 @@ __global__ void __launch_bounds__(BLOCK) k_counts(const Spawn* sp, uint32_t n, u
   if (i < n) counts[i] = sp[i].count;
 }
 
-// One generation: children [c0, c0+nthreads) of the spawn records `in`.
-__global__ void __launch_bounds__(BLOCK) k_bounce(DevScene S, RunCfg c, const Spawn* in, const uint32_t* offs,
-                                                  uint32_t nspawn, uint64_t c0, uint32_t nthreads, Sink sk) {
-  __shared__ int32_t stk[STACK * BLOCK];
-  uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= nthreads) return;
-  uint64_t gid = c0 + t;
-  // spawn record owning child gid: last s with offs[s] <= gid
-  uint32_t lo = 0, hi = nspawn;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if ((uint64_t)offs[mid] <= gid) lo = mid; else hi = mid;
-  }
-  const Spawn& sp = in[lo];
-  uint32_t ci = (uint32_t)(gid - offs[lo]);
-  RayS r;
+// Child ci of spawn record sp: origin, direction and rayorigin() admission.
+__device__ __forceinline__ bool build_ray(const RunCfg& c, const Spawn& sp, uint32_t ci, RayS& r) {
   r.o = v3(sp.org[0], sp.org[1], sp.org[2]);
   r.coef = c3(sp.coef[0], sp.coef[1], sp.coef[2]);
   r.w = sp.cw;
@@ -813,10 +803,69 @@ __global__ void __launch_bounds__(BLOCK) k_bounce(DevScene S, RunCfg c, const Sp
     r.d = v3(sp.d0[0], sp.d0[1], sp.d0[2]);
   }
   normalize(r.d);
-  if (!admit(c, r, sp.flags)) return;
+  return admit(c, r, sp.flags);
+}
+
+__device__ __forceinline__ uint32_t find_spawn(const uint32_t* offs, uint32_t nspawn, uint64_t gid) {
+  uint32_t lo = 0, hi = nspawn;      // last s with offs[s] <= gid
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)offs[mid] <= gid) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Traversal result of one child ray (prim -2: not traced).
+struct alignas(16) TraceOut {
+  double t;
+  int32_t prim;
+  float dx, dy, dz;
+  uint32_t spawn;
+  uint32_t pad;
+};
+
+// One generation, part 1: children [c0, c0+nthreads) of `in` -> closest hits.
+__global__ void __launch_bounds__(TBLOCK) k_trace(DevScene S, RunCfg c, const Spawn* in, const uint32_t* offs,
+                                                  uint32_t nspawn, uint64_t c0, uint32_t nthreads, TraceOut* outp) {
+  __shared__ int32_t stk[LSTACK * TBLOCK];
+  uint32_t t = blockIdx.x * TBLOCK + threadIdx.x;
+  if (t >= nthreads) return;
+  uint64_t gid = c0 + t;
+  uint32_t si = find_spawn(offs, nspawn, gid);
+  RayS r;
+  TraceOut o;
+  o.spawn = si;
+  o.pad = 0;
+  if (!build_ray(c, in[si], (uint32_t)(gid - offs[si]), r)) {
+    o.prim = -2;
+    o.t = 0;
+    o.dx = o.dy = o.dz = 0.f;
+    outp[t] = o;
+    return;
+  }
   wave_count(c.nrays);
   HitRec h;
-  closest_hit(S, r.o, r.d, h, stk, c.err, c.visits);
+  closest_hit<TBLOCK>(S, r.o, r.d, h, stk, c.err, c.visits);
+  o.t = h.t;
+  o.prim = h.prim;
+  o.dx = (float)r.d.x; o.dy = (float)r.d.y; o.dz = (float)r.d.z;
+  outp[t] = o;
+}
+
+// One generation, part 2: shade the hits and append the next spawn records.
+__global__ void __launch_bounds__(BLOCK) k_shade(DevScene S, RunCfg c, const Spawn* in, const uint32_t* offs,
+                                                 uint64_t c0, uint32_t nthreads, const TraceOut* tr, Sink sk) {
+  uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= nthreads) return;
+  const TraceOut o = tr[t];
+  if (o.prim == -2) return;
+  uint32_t si = o.spawn;
+  RayS r;
+  build_ray(c, in[si], (uint32_t)(c0 + t - offs[si]), r);
+  HitRec h;
+  h.t = o.t;
+  h.prim = o.prim;
+  h.rod = 1.0;
   if (h.prim >= 0) shade_hit(S, c, r, h, sk);
   else shade_miss(S, c, r);
 }
@@ -933,11 +982,13 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
     }
   }
   long gens = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   if (stats && stats->timing) {
     check(hipEventCreate(&ev0), "event");
     check(hipEventCreate(&ev1), "event");
+    check(hipEventCreate(&ev2), "event");
   }
+  TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * window, st, false);
   while (!stack.empty()) {
     Batch bt = stack.back();
     stack.pop_back();
@@ -958,16 +1009,22 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
       Sink sk{ob, d_cnt, (uint32_t)batch_cap, cfg.err};
       if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
-      hipLaunchKernelGGL(k_bounce, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf, offs, bt.n,
-                         c0, nth, sk);
-      check(hipGetLastError(), "k_bounce launch");
+      hipLaunchKernelGGL(k_trace, dim3((nth + TBLOCK - 1) / TBLOCK), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n,
+                         c0, nth, tout);
+      check(hipGetLastError(), "k_trace launch");
       if (stats && stats->timing) check(hipEventRecord(ev1, st), "event");
+      hipLaunchKernelGGL(k_shade, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf, offs, c0, nth,
+                         tout, sk);
+      check(hipGetLastError(), "k_shade launch");
+      if (stats && stats->timing) check(hipEventRecord(ev2, st), "event");
       check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
       check(hipStreamSynchronize(st), "bounce");
       if (stats && stats->timing) {
-        float ms = 0;
+        float ms = 0, ms2 = 0;
         check(hipEventElapsedTime(&ms, ev0, ev1), "event time");
+        check(hipEventElapsedTime(&ms2, ev1, ev2), "event time");
         stats->bounce_ms += ms;
+        stats->shade_ms += ms2;
       }
       if (stats) { stats->launches++; stats->threads += nth; }
       uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap);
@@ -981,6 +1038,7 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   if (stats) stats->generations = gens;
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
+  if (ev2) (void)hipEventDestroy(ev2);
 }
 
 }  // namespace rg

@@ -68,8 +68,9 @@ struct RenderStats {
   long generations = 0;
   long launches = 0;
   unsigned long long threads = 0;
-  bool timing = false;       // time every k_bounce launch with HIP events
-  double bounce_ms = 0;      // summed k_bounce durations
+  bool timing = false;       // time every generation's kernels with HIP events
+  double bounce_ms = 0;      // summed traversal-kernel (k_trace) durations
+  double shade_ms = 0;       // summed shading-kernel (k_shade) durations
 };
 
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level);

@@ -82,15 +82,24 @@ RG_HD double cal_atan2_deg(double y, double x) {   // Atan2() / posangle()
   return ((-a > 0) ? a + 2 * PI_D : a) / DEGREE;
 }
 
-RG_HD int reinhart_rnaz(int r, double mf) {
-  const int tnaz[7] = {30, 30, 24, 24, 18, 12, 6};
+// tnaz() of reinhartb.cal:35 and the Klems row tables (element 0 = row count)
+__device__ __constant__ static const int kTnaz[7] = {30, 30, 24, 24, 18, 12, 6};
+__device__ __constant__ static const double kKlems[6][10] = {
+    {9, 5, 15, 25, 35, 45, 55, 65, 75, 90},         // kpola, klems_full.cal:13
+    {9, 1, 8, 16, 20, 24, 24, 24, 16, 12},          // knaz
+    {7, 6.5, 19.5, 32.5, 45.5, 58.5, 71.5, 90, 0, 0},   // khpola, klems_half.cal:15
+    {7, 1, 8, 12, 16, 20, 12, 8, 0, 0},             // khnaz
+    {5, 9, 27, 45, 63, 90, 0, 0, 0, 0},             // kqpola, klems_quarter.cal:15
+    {5, 1, 8, 12, 12, 8, 0, 0, 0, 0}};              // kqnaz
+
+__device__ inline int reinhart_rnaz(int r, double mf) {
   if (r - (7 * mf - .5) > 0) return 1;
   int k = (int)floor((r + .5) / mf);   // select() is 1-based: tnaz(k+1) = tnaz[k]
   if (k < 0 || k > 6) return 0;
-  return (int)(mf * tnaz[k]);
+  return (int)(mf * kTnaz[k]);
 }
 
-RG_HD double reinhart_bin(V3 D, const BinSpec& b) {
+__device__ inline double reinhart_bin(V3 D, const BinSpec& b) {
   double rNx = b.nrm[0], rNy = b.nrm[1], rNz = b.nrm[2];
   double Ux = b.up[0], Uy = b.up[1], Uz = b.up[2];
   double inc_dz = -D.x * rNx - D.y * rNy - D.z * rNz;
@@ -110,17 +119,9 @@ RG_HD double reinhart_bin(V3 D, const BinSpec& b) {
   return acc + r_azn;
 }
 
-RG_HD double klems_bin(V3 D, const BinSpec& b, int which) {
-  // kpola/knaz of klems_full.cal, khpola/khnaz of klems_half.cal and
-  // kqpola/kqnaz of klems_quarter.cal; element 0 is the row count (select(0,..))
-  const double kpola_f[10] = {9, 5, 15, 25, 35, 45, 55, 65, 75, 90};
-  const double knaz_f[10] = {9, 1, 8, 16, 20, 24, 24, 24, 16, 12};
-  const double kpola_h[10] = {7, 6.5, 19.5, 32.5, 45.5, 58.5, 71.5, 90, 0, 0};
-  const double knaz_h[10] = {7, 1, 8, 12, 16, 20, 12, 8, 0, 0};
-  const double kpola_q[10] = {5, 9, 27, 45, 63, 90, 0, 0, 0, 0};
-  const double knaz_q[10] = {5, 1, 8, 12, 12, 8, 0, 0, 0, 0};
-  const double* kpola = which == 0 ? kpola_f : (which == 1 ? kpola_h : kpola_q);
-  const double* knaz = which == 0 ? knaz_f : (which == 1 ? knaz_h : knaz_q);
+__device__ inline double klems_bin(V3 D, const BinSpec& b, int which) {
+  const double* kpola = kKlems[2 * which];
+  const double* knaz = kKlems[2 * which + 1];
   double Nx = b.nrm[0], Ny = b.nrm[1], Nz = b.nrm[2];
   double Ux = b.up[0], Uy = b.up[1], Uz = b.up[2];
   double pol = cal_acos_deg(-D.x * Nx - D.y * Ny - D.z * Nz);
@@ -138,7 +139,7 @@ RG_HD double klems_bin(V3 D, const BinSpec& b, int which) {
   return acc + azn;
 }
 
-RG_HD int eval_bin(V3 D, const BinSpec& b) {
+__device__ inline int eval_bin(V3 D, const BinSpec& b) {
   double bval;
   switch (b.kind) {
     case BK_CONST: bval = b.cbin; break;

@@ -1,0 +1,250 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle, the
+reference's golden vectors and size-independent properties."""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from pyradiance_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def L():
+    from pyradiance_amd import _lib
+    _lib.lib()
+    return _lib
+
+
+def _ctx(L, rad):
+    c = L.Context(0)
+    c.load_text(rad).commit()
+    return c
+
+
+def _rc_params(L, **kw):
+    p = L.default_params(True)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _sky_mods(L, mf=1):
+    m = L.RgModifier()
+    m.name, m.bin_kind, m.nbins, m.mf, m.rhs = b"skyglow", 1, scenes.reinhart_nbins(mf), float(mf), 1.0
+    m.nrm[:] = (0, 0, -1)
+    m.up[:] = (0, 1, 0)
+    g = L.RgModifier()
+    g.name, g.bin_kind, g.nbins, g.cbin = b"groundglow", 0, 1, 0
+    return [m, g]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_config1_irradiance_matches_reference(L, oracle, seed):
+    _say(f"config1 seed {seed}")
+    g = np.load(GOLDEN / f"c1_seed{seed}.npz")
+    rad, sens, _ = scenes.config1(seed)
+    c = _ctx(L, rad)
+    p = L.default_params(False)
+    p.ambounce, p.dstrsrc, p.srcsizerat = 0, 0.0, 0.0
+    c.set_params(p)
+    out = c.trace(sens, L.RT_IMM_IRRAD)
+    ref = g["values"]
+    assert np.abs(out - ref).max() <= 1e-4 * ref.max()
+    orc, nr = oracle.Scene(rad).trace(sens, oracle.params(ambounce=0, dstrsrc=0.0, srcsizerat=0.0), 8)
+    assert np.abs(out - orc).max() <= 1e-6 * orc.max()
+    assert c.last_nrays() == nr          # ray counts: bit-exact
+
+
+def _mixed_scene():
+    return ("void plastic m 0 0 5 .5 .5 .5 0 0\n"
+            "void glass g 0 0 3 .8 .8 .8\n"
+            "m sphere s1 0 0 4 0 0 0 1\n"
+            "m bubble s2 0 0 4 3 0 0 0.7\n"
+            "m polygon floor 0 0 12 -6 -6 -2 6 -6 -2 6 6 -2 -6 6 -2\n"
+            "m polygon tri 0 0 9 -3 -3 0 -1 -3 0 -2 -1 1\n"
+            "g polygon pane 0 0 12 -4 2 -1 -2 2 -1 -2 2 1 -4 2 1\n"
+            "m ring r1 0 0 8 0 3 0 0 1 0 0.2 1.2\n"
+            "m cylinder c1 0 0 7 -3 0 -2 -3 0 2 0.5\n"
+            "m tube t1 0 0 7 3 3 -2 3 3 2 0.6\n"
+            "m cone k1 0 0 8 0 -3 -2 0 -3 1 1.0 0.2\n"
+            "m cup k2 0 0 8 2 -3 -2 2 -3 0 0.1 0.8\n")
+
+
+def test_closest_hit_matches_brute_force(L, oracle):
+    _say("closest hit, mixed primitives")
+    rad = _mixed_scene()
+    rng = np.random.default_rng(7)
+    n = 4000
+    org = rng.uniform(-5, 5, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = np.hstack([org, d])
+    prim, t = oracle.Scene(rad).closest(rays)
+    # GPU: primary rays into an all-black scene carrying one tracked modifier
+    # per primitive would need bins; instead compare through rtrace -i, whose
+    # value is non-zero exactly when a primitive is hit under a bright sky.
+    sky = "void glow skyglow 0 0 4 1 1 1 0\nskyglow source sky 0 0 4 0 0 1 360\n"
+    c = _ctx(L, rad + sky)
+    p = L.default_params(False)
+    c.set_params(p)
+    out = c.trace(rays, 0)
+    o2, _ = oracle.Scene(rad + sky).trace(rays, oracle.params(), 0)
+    same = np.isclose(out, o2, rtol=1e-6, atol=1e-9).all(1)
+    assert same.mean() >= 0.999
+    # rays that escape see the sky (value 1), hits on plastic with -ab 0 are black
+    escaped = prim < 0
+    assert np.allclose(out[escaped], 1.0)
+
+
+def test_rcontrib_matches_oracle_same_samples(L, oracle):
+    _say("rcontrib -ab 2 vs oracle")
+    rng = np.random.default_rng(1)
+    room = scenes.Room()
+    rad = scenes.room_rad(room) + scenes.furniture_rad(rng, 5, room, 1, 1) + scenes.sky_rad((1, 1, 1), (1, 1, 1))
+    pts = np.array([[1.0, 1.0, 0.8, 0, 0, 1], [3.0, 2.0, 0.8, 0, 0, 1], [5.0, 3.0, 1.5, 0, 0, 1],
+                    [3.0, 0.5, 1.5, 0, -1, 0]])
+    rays = np.repeat(pts, 8, axis=0)
+    mods = _sky_mods(L)
+    c = _ctx(L, rad)
+    c.set_params(_rc_params(L, ambounce=2, ambdiv=256, minweight=1e-3, seed=3))
+    out = c.contrib(rays, mods, 1, L.RT_IMM_IRRAD)
+    omods = [oracle.mod("skyglow", 1, 145), oracle.mod("groundglow", 0, 1)]
+    orc, nr = oracle.Scene(rad).contrib(rays, oracle.params(ambounce=2, ambdiv=256, minweight=1e-3, dstrsrc=0.9,
+                                                            specthresh=0.02, seed=3), omods, 1, 8)
+    # identical samples: totals agree tightly, entries agree except where a
+    # sample's path diverged on an FP32-vs-FP64 edge decision
+    assert np.allclose(out.sum(1), orc.sum(1), rtol=2e-3)
+    close = np.isclose(out, orc, rtol=1e-4, atol=1e-7)
+    assert close.mean() > 0.99
+    assert abs(c.last_nrays() - nr) <= 0.002 * nr
+
+
+def test_rtrace_ambient_matches_oracle(L, oracle):
+    _say("rtrace -I -ab 1 with sun and sky vs oracle")
+    rad, sens, _ = scenes.config1(0)
+    c = _ctx(L, rad)
+    p = L.default_params(False)
+    p.ambounce, p.ambdiv, p.dstrsrc = 1, 128, 0.5
+    p.seed = 9
+    c.set_params(p)
+    out = c.trace(sens, L.RT_IMM_IRRAD)
+    orc, nr = oracle.Scene(rad).trace(sens, oracle.params(ambounce=1, ambdiv=128, dstrsrc=0.5, seed=9), 8)
+    assert np.allclose(out.sum(1), orc.sum(1), rtol=1e-3, atol=1e-6)
+
+
+def test_uniform_sky_pi(L):
+    _say("uniform sky = pi")
+    c = _ctx(L, scenes.sky_rad((1, 1, 1), ground=(1, 1, 1)))
+    c.set_params(_rc_params(L, ambounce=1, ambdiv=4096, minweight=1e-4))
+    rays = np.array([[0, 0, 1.0, 0, 0, 1], [5, 5, 0, 0, 0, 1], [0, 0, 0, 1, 0, 0]], float)
+    out = c.contrib(rays, _sky_mods(L), 1, L.RT_IMM_IRRAD)
+    assert np.allclose(out[:, 0::3].sum(1), math.pi, rtol=1e-6)
+    assert np.allclose(out[2, -3], math.pi / 2, rtol=0.02)    # half the hemisphere is ground
+
+
+@pytest.mark.parametrize("mf", [1, 2, 4])
+def test_bins_match_reference(L, mf):
+    _say(f"Reinhart MF {mf} bins")
+    g = np.load(GOLDEN / "bins_reinhart.npz")
+    d = g["dirs"]
+    c = _ctx(L, "void glow skyglow 0 0 4 1 1 1 0\nskyglow source sky 0 0 4 0 0 1 180\n")
+    c.set_params(_rc_params(L, ambounce=0))
+    out = c.contrib(np.hstack([np.zeros_like(d), d]), _sky_mods(L, mf)[:1], 1, 0)
+    v = out[:, 0::3]
+    got = np.where(v.max(1) > 0.5, v.argmax(1), -1)
+    assert np.array_equal(got, g[f"mf{mf}"])
+
+
+def test_klems_bins_match_reference(L):
+    _say("Klems bins")
+    g = np.load(GOLDEN / "bins_klems.npz")
+    d = g["dirs"]
+    c = _ctx(L, "void glow skyglow 0 0 4 1 1 1 0\nskyglow source sky 0 0 4 0 0 1 360\n")
+    c.set_params(_rc_params(L, ambounce=0))
+    for name, nrm, up in (("kbinS", (0, 1, 0), (0, 0, 1)), ("kbinD", (0, 0, -1), (0, 1, 0))):
+        m = L.RgModifier()
+        m.name, m.bin_kind, m.nbins, m.rhs = b"skyglow", 2, 145, 1.0
+        m.nrm[:], m.up[:] = nrm, up
+        out = c.contrib(np.hstack([np.zeros_like(d), d]), [m], 1, 0)[:, 0::3]
+        got = np.where(out.max(1) > 0.5, out.argmax(1), -1)
+        assert np.array_equal(got, g[name]), name
+
+
+def test_dcsky_against_fp64(L):
+    _say("dcsky")
+    rng = np.random.default_rng(0)
+    for (M, K, N) in [(37, 19, 51), (250, 146, 877), (64, 2305, 96)]:
+        A = rng.random((M, K, 3), dtype=np.float32)
+        B = rng.lognormal(0, 1, (K, N, 3)).astype(np.float32)
+        ref = np.einsum("mkc,knc->mnc", A.astype(np.float64), B.astype(np.float64), optimize=True)
+        c32 = L.dcsky(A, B, 0)
+        cbf = L.dcsky(A, B, 1)
+        e32 = np.abs(c32 - ref).max() / np.abs(ref).max()
+        ebf = (np.abs(cbf - ref) / np.abs(ref)).max()
+        assert e32 < 1e-5, (M, K, N, e32)
+        assert ebf < 1e-4, (M, K, N, ebf)
+
+
+def test_config2_full_size_properties(L):
+    """Full config-2 workload: every record is a valid irradiance split
+    (non-negative, total <= pi per unit sky), repeatable for a fixed seed."""
+    _say("config2 full size")
+    rad, sens = scenes.config2(0)
+    c = _ctx(L, rad)
+    assert c.info()["primitives"] == 8 + 200 * 6 * 21 - 1
+    c.set_params(_rc_params(L, ambounce=4, ambdiv=4096, minweight=1e-4, seed=1))
+    out = c.contrib(sens, _sky_mods(L), 1, L.RT_IMM_IRRAD)
+    n1 = c.last_nrays()
+    assert out.shape == (2500, 438)
+    assert (out >= 0).all()
+    tot = out.sum(1) / 3
+    assert (tot <= math.pi * 1.02).all() and tot.max() > 0.01
+    out2 = c.contrib(sens, _sky_mods(L), 1, L.RT_IMM_IRRAD)
+    assert c.last_nrays() == n1
+    assert np.allclose(out, out2, rtol=1e-9, atol=1e-15)
+    assert n1 > 2500 * 4096
+
+
+def test_python_api_bytes(L, tmp_path):
+    _say("python API")
+    from pyradiance_amd import Rcontrib, rtrace
+    rad, sens, _ = scenes.config1(0)
+    f = tmp_path / "room.rad"
+    f.write_text(rad)
+    inp = "\n".join(" ".join(f"{v:.10g}" for v in r) for r in sens[:5]).encode()
+    out = rtrace(inp, f, irradiance=True, params=["-ab", "0", "-dj", "0", "-ds", "0"])
+    head, _, body = out.partition(b"\n\n")
+    assert head.startswith(b"#?RADIANCE") and b"FORMAT=ascii" in head
+    lines = body.decode().strip("\n").split("\n")
+    assert len(lines) == 5 and all(len(ln.split("\t")) == 4 for ln in lines)
+    g = np.load(GOLDEN / "c1_seed0.npz")
+    vals = np.array(body.split(), float).reshape(-1, 3)
+    assert np.allclose(vals, g["values"][:5], rtol=1e-5, atol=1e-9)
+    raw = rtrace(inp, f, header=False, irradiance=True, outform="f", params=["-ab", "0", "-dj", "0", "-ds", "0"])
+    assert len(raw) == 5 * 3 * 4
+    sky = tmp_path / "sky.rad"
+    sky.write_text(scenes.sky_rad((1, 1, 1), (1, 1, 1)))
+    rc = Rcontrib(b"0 0 0 0 0 1\n", sky, params=["-I+", "-ab", "1", "-ad", "1024", "-lw", "1e-4", "-h"])
+    rc.add_modifier(modifier="skyglow", calfile="reinhartb.cal", param="MF=1", nbins="Nrbins", binv="rbin")
+    rc.add_modifier(modifier="groundglow", nbins="1", binv="0")
+    res = rc()
+    v = np.array(res.split(), float)
+    assert v.size == 146 * 3 and abs(v[0::3].sum() - math.pi) < 1e-4
+
+
+def test_errors_are_status_codes(L):
+    c = L.Context(0)
+    with pytest.raises(RuntimeError, match="undefined modifier"):
+        c.load_text("nosuch polygon p 0 0 9 0 0 0 1 0 0 0 1 0\n")
+    c2 = L.Context(0)
+    c2.load_text("void mirror mm 0 0 3 1 1 1\nmm polygon p 0 0 9 0 0 0 1 0 0 0 1 0\n")
+    with pytest.raises(RuntimeError, match="not supported"):
+        c2.commit()
