@@ -135,14 +135,93 @@ def dcsky_bench(dc: np.ndarray, seed: int = 0):
             "flops": 2.0 * M * K * N * 3}
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One device-to-device copy of `nbytes` (the runtime's plain copy kernel),
+    used to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes} bytes device to device")
+
+
+def batched_bench(nsun: int = 255, nscene2: int = 4):
+    """Plain (one scene per call) vs batched (one wavefront over many scenes)."""
+    res = {}
+    # config 1 over many sun positions
+    ctxs, rays = [], []
+    for seed in range(nsun):
+        rad, sens, _ = scenes.config1(seed)
+        c = _lib.Context(0)
+        c.load_text(rad).commit()
+        ctxs.append(c)
+        rays.append(sens)
+    p = _lib.default_params(False)
+    p.ambounce, p.dstrsrc, p.srcsizerat = 0, 0.0, 0.0
+    for c in ctxs:
+        c.set_params(p)
+    _lib.trace_batch(ctxs, rays, _lib.RT_IMM_IRRAD)     # warm-up
+    t0 = time.perf_counter()
+    nr = 0
+    for c, r in zip(ctxs, rays):
+        c.trace(r, _lib.RT_IMM_IRRAD)
+        nr += c.last_nrays()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _lib.trace_batch(ctxs, rays, _lib.RT_IMM_IRRAD)
+    t_batch = time.perf_counter() - t0
+    nb = ctxs[0].last_nrays()
+    res["config1_suns"] = {"scenes": nsun, "sensors_per_scene": 77, "rays": nb,
+                           "plain_ms": t_plain * 1e3, "batched_ms": t_batch * 1e3,
+                           "plain_Mrays_s": nr / t_plain / 1e6, "batched_Mrays_s": nb / t_batch / 1e6,
+                           "speedup": t_plain / t_batch}
+    for c in ctxs:
+        c.close()
+    # config 2 with several furniture seeds
+    ctxs, rays = [], []
+    for seed in range(nscene2):
+        rad, sens, mods = workload(seed)
+        c = _lib.Context(0)
+        c.load_text(rad).commit()
+        c.set_params(gpu_params())
+        ctxs.append(c)
+        rays.append(sens)
+    t0 = time.perf_counter()
+    nr = 0
+    for c, r in zip(ctxs, rays):
+        c.contrib(r, mods, 1, _lib.RT_IMM_IRRAD)
+        nr += c.last_nrays()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _lib.contrib_batch(ctxs, rays, mods, 1, _lib.RT_IMM_IRRAD)
+    t_batch = time.perf_counter() - t0
+    nb = ctxs[0].last_nrays()
+    res["config2_scenes"] = {"scenes": nscene2, "rays": nb, "plain_ms": t_plain * 1e3, "batched_ms": t_batch * 1e3,
+                             "plain_Mrays_s": nr / t_plain / 1e6, "batched_Mrays_s": nb / t_batch / 1e6,
+                             "speedup": t_plain / t_batch}
+    for c in ctxs:
+        c.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=480)
     ap.add_argument("--sensors", type=int, default=2500)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the forward pass shards "
@@ -189,8 +268,15 @@ def main():
     log(f"k_trace: {st_t['launches']:.0f} launches, {st_t['trace_ms']:.1f} ms total; k_shade {st_t['shade_ms']:.1f} ms; "
         f"{st_c['nodes'] / nr:.1f} nodes, {st_c['prims'] / nr:.1f} prims per ray; {per_ray:.0f} B/ray")
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     dcs = dcsky_bench(out)
     log(f"dcsky: {dcs}")
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench()
+        log(f"batched: {batched}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -223,6 +309,7 @@ def main():
                      "bytes_per_ray": per_ray, "nodes_per_ray": st_c["nodes"] / nr,
                      "prims_per_ray": st_c["prims"] / nr},
         "dcsky": dcs,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -89,6 +89,16 @@ int rg_trace(rg_ctx *ctx, const double *rays, long long n, int flags, double *ou
 int rg_contrib(rg_ctx *ctx, const double *rays, long long nrec, int accum, int flags,
                const rg_modifier *mods, int nmods, double *out);
 
+/* Batched rcontrib: the records of nctx scenes (one committed context each,
+ * same device) traced in one wavefront.  rays holds scene 0's nrec[0]*accum
+ * rays, then scene 1's, ...; out holds the records in the same order.
+ * Parameters and the ray counter are those of ctxs[0]. */
+int rg_contrib_batch(rg_ctx *const *ctxs, int nctx, const double *rays, const long long *nrec, int accum,
+                     int flags, const rg_modifier *mods, int nmods, double *out);
+
+/* Batched rtrace: n[i] rays of scene i, concatenated; out in the same order. */
+int rg_trace_batch(rg_ctx *const *ctxs, int nctx, const double *rays, const long long *n, int flags, double *out);
+
 /* Ray segments traced by the last rg_trace / rg_contrib (closest-hit
  * queries, the reference's nrays of rt/raytrace.c:606). */
 int rg_last_nrays(rg_ctx *ctx, unsigned long long *nrays);
@@ -97,6 +107,16 @@ int rg_last_nrays(rg_ctx *ctx, unsigned long long *nrays);
  * every generation kernel with HIP events on the context stream, 2: also
  * count BVH nodes visited and primitives tested per ray. */
 int rg_set_profiling(rg_ctx *ctx, int level);
+
+/* Spatial index builder used by the next rg_scene_commit (no reference
+ * counterpart: the reference indexes scenes with an octree,
+ * rt/raytrace.c:595-760).  RG_BVH_AUTO: host binned-SAH tree up to 4M
+ * primitives, device LBVH above; RG_BVH_SAH / RG_BVH_LBVH force one.  Both
+ * give the same closest hits; only the traversal cost differs. */
+#define RG_BVH_AUTO 0
+#define RG_BVH_SAH 1
+#define RG_BVH_LBVH 2
+int rg_set_bvh_builder(rg_ctx *ctx, int kind);
 /* out[0..7] = rays, generation launches, threads launched, summed traversal-
  * kernel milliseconds, nodes visited, primitives tested, batches, summed
  * shading-kernel milliseconds */

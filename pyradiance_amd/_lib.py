@@ -18,9 +18,11 @@ LIBPATH = Path(os.environ.get("RADGPU_LIB", Path(__file__).resolve().parent / "l
 EXPORTS = (
     "rg_ctx_create", "rg_ctx_destroy", "rg_last_error", "rg_scene_load", "rg_scene_load_text",
     "rg_scene_commit", "rg_scene_info", "rg_set_params", "rg_default_params",
-    "rg_default_params_rcontrib", "rg_trace", "rg_contrib", "rg_last_nrays", "rg_set_profiling",
-    "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
+    "rg_default_params_rcontrib", "rg_trace", "rg_contrib", "rg_contrib_batch", "rg_trace_batch", "rg_last_nrays",
+    "rg_set_profiling", "rg_set_bvh_builder", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
 )
+
+BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
 
 RT_IMM_IRRAD = 8      # -I  (RTimmIrrad, rt/RtraceSimulManager.h:86)
 RT_LAMB_IRRAD = 16    # -i
@@ -72,8 +74,12 @@ def lib() -> C.CDLL:
     L.rg_default_params_rcontrib.restype = None
     L.rg_trace.argtypes = [vp, dp, C.c_longlong, C.c_int, dp]
     L.rg_contrib.argtypes = [vp, dp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, dp]
+    L.rg_contrib_batch.argtypes = [C.POINTER(vp), C.c_int, dp, C.POINTER(C.c_longlong), C.c_int, C.c_int,
+                                   C.POINTER(RgModifier), C.c_int, dp]
+    L.rg_trace_batch.argtypes = [C.POINTER(vp), C.c_int, dp, C.POINTER(C.c_longlong), C.c_int, dp]
     L.rg_last_nrays.argtypes = [vp, C.POINTER(C.c_ulonglong)]
     L.rg_set_profiling.argtypes = [vp, C.c_int]
+    L.rg_set_bvh_builder.argtypes = [vp, C.c_int]
     L.rg_last_stats.argtypes = [vp, dp, C.c_int]
     L.rg_dcsky.argtypes = [fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.rg_dcsky_device.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -164,6 +170,11 @@ class Context:
     def set_profiling(self, level: int):
         self._check(self._L.rg_set_profiling(self._h, level))
 
+    def set_bvh_builder(self, kind: int):
+        """BVH_AUTO, BVH_SAH or BVH_LBVH for the next commit()."""
+        self._check(self._L.rg_set_bvh_builder(self._h, kind))
+        return self
+
     def last_stats(self) -> dict:
         v = (C.c_double * 8)()
         self._check(self._L.rg_last_stats(self._h, v, 8))
@@ -174,6 +185,49 @@ class Context:
         n = C.c_ulonglong()
         self._check(self._L.rg_last_nrays(self._h, C.byref(n)))
         return int(n.value)
+
+
+def contrib_batch(ctxs: list, rays: list, mods: list, accum: int = 1, flags: int = 0) -> list:
+    """rcontrib over several scenes in one wavefront; parameters from ctxs[0].
+
+    rays[i] are scene i's (nrec_i*accum, 6) rays; returns one array per scene.
+    """
+    if len(ctxs) != len(rays) or not ctxs:
+        raise ValueError("one ray array per context")
+    arrs = [np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 6) for r in rays]
+    nrec = (C.c_longlong * len(arrs))(*[a.shape[0] // accum for a in arrs])
+    allr = np.ascontiguousarray(np.concatenate([a[: (a.shape[0] // accum) * accum] for a in arrs]))
+    ncol = 3 * sum(m.nbins for m in mods)
+    out = np.zeros((sum(nrec), ncol))
+    hs = (C.c_void_p * len(ctxs))(*[c._h for c in ctxs])
+    arr = (RgModifier * len(mods))(*mods)
+    L = lib()
+    if L.rg_contrib_batch(hs, len(ctxs), _dptr(allr), nrec, accum, flags, arr, len(mods), _dptr(out)) != 0:
+        raise RuntimeError(L.rg_last_error(ctxs[0]._h).decode())
+    res, o = [], 0
+    for n in nrec:
+        res.append(out[o:o + n])
+        o += n
+    return res
+
+
+def trace_batch(ctxs: list, rays: list, flags: int = 0) -> list:
+    """rtrace over several scenes in one wavefront; parameters from ctxs[0]."""
+    if len(ctxs) != len(rays) or not ctxs:
+        raise ValueError("one ray array per context")
+    arrs = [np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 6) for r in rays]
+    n = (C.c_longlong * len(arrs))(*[a.shape[0] for a in arrs])
+    allr = np.ascontiguousarray(np.concatenate(arrs))
+    out = np.zeros((allr.shape[0], 3))
+    hs = (C.c_void_p * len(ctxs))(*[c._h for c in ctxs])
+    L = lib()
+    if L.rg_trace_batch(hs, len(ctxs), _dptr(allr), n, flags, _dptr(out)) != 0:
+        raise RuntimeError(L.rg_last_error(ctxs[0]._h).decode())
+    res, o = [], 0
+    for k in n:
+        res.append(out[o:o + k])
+        o += k
+    return res
 
 
 def dcsky(A: np.ndarray, B: np.ndarray, mode: int = 1) -> np.ndarray:

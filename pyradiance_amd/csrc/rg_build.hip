@@ -98,7 +98,7 @@ __device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j) 
 }
 
 __global__ void __launch_bounds__(BB) k_karras(const uint32_t* keys, int n, BVHNode* nodes, int32_t* parent_int,
-                                               int32_t* parent_leaf) {
+                                               int32_t* parent_leaf, int32_t* chl, int32_t* chr) {
   int i = blockIdx.x * BB + threadIdx.x;
   if (i >= n - 1) return;
   int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -124,24 +124,30 @@ __global__ void __launch_bounds__(BB) k_karras(const uint32_t* keys, int n, BVHN
   else { right = gamma + 1; parent_int[gamma + 1] = i; }
   nodes[i].left = left;
   nodes[i].right = right;
+  chl[i] = left;
+  chr[i] = right;
   nodes[i].pad0 = 0;
   nodes[i].pad1 = 0;
 }
 
 __global__ void __launch_bounds__(BB) k_refit(const float* lo, const float* hi, const uint32_t* vals, int n,
                                               BVHNode* nodes, const int32_t* parent_int, const int32_t* parent_leaf,
-                                              float* nbox, uint32_t* flags, int32_t* leaf_prim) {
+                                              const int32_t* chl, const int32_t* chr, float* nbox, uint32_t* flags,
+                                              const int32_t* ids, int32_t* leaf_prim) {
   int k = blockIdx.x * BB + threadIdx.x;
   if (k >= n) return;
-  leaf_prim[k] = (int32_t)vals[k];
+  leaf_prim[k] = ids[vals[k]];
   int p = parent_leaf[k];
+  // the parent refers to this leaf by primitive index from now on
+  if (chl[p] == ~k) nodes[p].left = ~ids[vals[k]];
+  else nodes[p].right = ~ids[vals[k]];
   while (p >= 0) {
     __threadfence();
     if (atomicAdd(&flags[p], 1u) == 0) return;   // first arrival: the sibling finishes
     __threadfence();
     BVHNode& N = nodes[p];
     float b[2][6];
-    int ch[2] = {N.left, N.right};
+    int ch[2] = {chl[p], chr[p]};
     for (int c = 0; c < 2; ++c) {
       int r = ch[c];
       if (r < 0) {
@@ -166,8 +172,10 @@ static void chk(hipError_t e, const char* w) {
   if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
 }
 
-int build_lbvh(const float* d_lo, const float* d_hi, uint32_t n, const float scene_lo[3], const float scene_hi[3],
-               BVHNode* d_nodes, int32_t* d_leaf_prim, DevArena& arena, hipStream_t st) {
+__global__ void k_leaf_one(const int32_t* ids, int32_t* leaf_prim) { leaf_prim[0] = ids[0]; }
+
+int build_lbvh(const float* d_lo, const float* d_hi, const int32_t* d_ids, uint32_t n, const float scene_lo[3],
+               const float scene_hi[3], BVHNode* d_nodes, int32_t* d_leaf_prim, DevArena& arena, hipStream_t st) {
   if (n == 0) return -1;
   uint32_t* ka = (uint32_t*)arena.get("bvh_ka", 4 * n, st, false);
   uint32_t* va = (uint32_t*)arena.get("bvh_va", 4 * n, st, false);
@@ -192,7 +200,7 @@ int build_lbvh(const float* d_lo, const float* d_hi, uint32_t n, const float sce
   }
   chk(hipGetLastError(), "lbvh sort");
   if (n == 1) {
-    chk(hipMemsetAsync(d_leaf_prim, 0, sizeof(int32_t), st), "memset");
+    hipLaunchKernelGGL(k_leaf_one, dim3(1), dim3(1), 0, st, d_ids, d_leaf_prim);
     return ~0;
   }
   int32_t* pint = (int32_t*)arena.get("bvh_pint", 4 * (size_t)n, st, false);
@@ -200,9 +208,12 @@ int build_lbvh(const float* d_lo, const float* d_hi, uint32_t n, const float sce
   float* nbox = (float*)arena.get("bvh_nbox", 24 * (size_t)n, st, false);
   uint32_t* flags = (uint32_t*)arena.get("bvh_flags", 4 * (size_t)n, st, true);
   chk(hipMemsetAsync(pint, 0xff, 4 * (size_t)n, st), "memset");
-  hipLaunchKernelGGL(k_karras, dim3((n - 1 + BB - 1) / BB), dim3(BB), 0, st, ka, (int)n, d_nodes, pint, pleaf);
-  hipLaunchKernelGGL(k_refit, dim3(nblocks), dim3(BB), 0, st, d_lo, d_hi, va, (int)n, d_nodes, pint, pleaf, nbox, flags,
-                     d_leaf_prim);
+  int32_t* chl = (int32_t*)arena.get("bvh_chl", 4 * (size_t)n, st, false);
+  int32_t* chr = (int32_t*)arena.get("bvh_chr", 4 * (size_t)n, st, false);
+  hipLaunchKernelGGL(k_karras, dim3((n - 1 + BB - 1) / BB), dim3(BB), 0, st, ka, (int)n, d_nodes, pint, pleaf, chl,
+                     chr);
+  hipLaunchKernelGGL(k_refit, dim3(nblocks), dim3(BB), 0, st, d_lo, d_hi, va, (int)n, d_nodes, pint, pleaf, chl, chr,
+                     nbox, flags, d_ids, d_leaf_prim);
   chk(hipGetLastError(), "lbvh build");
   return 0;
 }

@@ -1,0 +1,171 @@
+// Binned SAH builder (host).  Each split chooses, over three axes and 32
+// centroid bins, the partition minimising area(L)*n(L) + area(R)*n(R); ranges
+// whose centroids coincide, and ranges below the median-split depth, are split
+// at the object median of the longest centroid axis.  Nodes are written in
+// depth-first order (left subtree right after its parent) so a walk down the
+// near side stays within a few cache lines.
+#include "rg_bvh.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+namespace rg {
+
+namespace {
+
+struct Box {
+  float lo[3], hi[3];
+  void reset() {
+    for (int a = 0; a < 3; ++a) { lo[a] = 3.0e38f; hi[a] = -3.0e38f; }
+  }
+  void grow(const float* l, const float* h) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], l[a]);
+      hi[a] = std::max(hi[a], h[a]);
+    }
+  }
+  void grow(const Box& b) { grow(b.lo, b.hi); }
+  double area() const {
+    double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+    if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+    return dx * dy + dy * dz + dz * dx;
+  }
+};
+
+struct Ref {
+  float lo[3], hi[3];
+  float c[3];
+  int32_t id;
+};
+
+struct Task {
+  uint32_t b, e;       // ref range
+  int32_t parent;      // parent node (-1: root)
+  int side;            // 0: left slot, 1: right slot
+  int depth;
+};
+
+constexpr int NBINS = 32;
+
+// bin of a centroid coordinate, clamped in float before the conversion
+inline int bin_of(float c, float lo, float scale) {
+  float k = (c - lo) * scale;
+  k = std::min(std::max(k, 0.0f), (float)(NBINS - 1));
+  return (int)k;
+}
+
+void set_child(BVHNode& N, int side, const Box& bx, int32_t ref) {
+  float* mn = side ? N.rmin : N.lmin;
+  float* mx = side ? N.rmax : N.lmax;
+  for (int a = 0; a < 3; ++a) { mn[a] = bx.lo[a]; mx[a] = bx.hi[a]; }
+  if (side) N.right = ref; else N.left = ref;
+}
+
+}  // namespace
+
+int32_t build_sah_bvh(const float* lo, const float* hi, const int32_t* ids, uint32_t n, std::vector<BVHNode>& nodes,
+                      int max_depth) {
+  nodes.clear();
+  if (n == 0) return -1;
+  if (n == 1) return ~0;
+  std::vector<Ref> refs(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    Ref& r = refs[i];
+    for (int a = 0; a < 3; ++a) {
+      r.lo[a] = lo[3 * i + a];
+      r.hi[a] = hi[3 * i + a];
+      r.c[a] = 0.5f * (r.lo[a] + r.hi[a]);
+    }
+    r.id = ids[i];
+  }
+  nodes.reserve(n - 1);
+  std::vector<Task> stack;
+  stack.push_back({0, n, -1, 0, 0});
+  const int median_depth = std::max(1, max_depth - 26);   // leaves room for log2(n) median levels
+  while (!stack.empty()) {
+    Task t = stack.back();
+    stack.pop_back();
+    Box bx, cb;
+    bx.reset();
+    cb.reset();
+    for (uint32_t i = t.b; i < t.e; ++i) {
+      bx.grow(refs[i].lo, refs[i].hi);
+      cb.grow(refs[i].c, refs[i].c);
+    }
+    if (t.e - t.b == 1) {
+      set_child(nodes[t.parent], t.side, bx, ~refs[t.b].id);
+      continue;
+    }
+    int32_t me = (int32_t)nodes.size();
+    if (t.depth > max_depth) throw std::runtime_error("BVH deeper than the traversal stack");
+    nodes.emplace_back();
+    std::memset(&nodes.back(), 0, sizeof(BVHNode));
+    if (t.parent >= 0) set_child(nodes[t.parent], t.side, bx, me);
+
+    int axis = 0;
+    float ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = cb.hi[a] - cb.lo[a];
+    if (ext[1] > ext[axis]) axis = 1;
+    if (ext[2] > ext[axis]) axis = 2;
+    uint32_t mid = t.b + (t.e - t.b) / 2;
+    bool done = false;
+    if (ext[axis] > 0.0f && t.depth < median_depth) {
+      // binned SAH over all three axes
+      double best = 1e300;
+      int best_axis = -1, best_split = 0;
+      for (int a = 0; a < 3; ++a) {
+        if (!(ext[a] > 0.0f)) continue;
+        Box bins[NBINS];
+        uint32_t cnt[NBINS] = {};
+        for (int k = 0; k < NBINS; ++k) bins[k].reset();
+        const float scale = NBINS / ext[a];
+        for (uint32_t i = t.b; i < t.e; ++i) {
+          int k = bin_of(refs[i].c[a], cb.lo[a], scale);
+          cnt[k]++;
+          bins[k].grow(refs[i].lo, refs[i].hi);
+        }
+        double rarea[NBINS];
+        uint32_t rcnt[NBINS];
+        Box acc;
+        acc.reset();
+        uint32_t c = 0;
+        for (int k = NBINS - 1; k > 0; --k) {
+          acc.grow(bins[k]);
+          c += cnt[k];
+          rarea[k] = acc.area();
+          rcnt[k] = c;
+        }
+        acc.reset();
+        c = 0;
+        for (int k = 0; k < NBINS - 1; ++k) {
+          acc.grow(bins[k]);
+          c += cnt[k];
+          if (c == 0 || rcnt[k + 1] == 0) continue;
+          double cost = acc.area() * c + rarea[k + 1] * rcnt[k + 1];
+          if (cost < best) { best = cost; best_axis = a; best_split = k + 1; }
+        }
+      }
+      if (best_axis >= 0) {
+        const int a = best_axis;
+        const float scale = NBINS / ext[a];
+        const float l0 = cb.lo[a];
+        auto it = std::partition(refs.begin() + t.b, refs.begin() + t.e,
+                                 [&](const Ref& r) { return bin_of(r.c[a], l0, scale) < best_split; });
+        mid = (uint32_t)(it - refs.begin());
+        done = mid > t.b && mid < t.e;
+      }
+    }
+    if (!done) {
+      mid = t.b + (t.e - t.b) / 2;
+      std::nth_element(refs.begin() + t.b, refs.begin() + mid, refs.begin() + t.e,
+                       [axis](const Ref& x, const Ref& y) { return x.c[axis] < y.c[axis]; });
+    }
+    // right first so the left subtree follows its parent in memory
+    stack.push_back({mid, t.e, me, 1, t.depth + 1});
+    stack.push_back({t.b, mid, me, 0, t.depth + 1});
+  }
+  return 0;
+}
+
+}  // namespace rg

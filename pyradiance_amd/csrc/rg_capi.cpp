@@ -1,15 +1,23 @@
 // C ABI: contexts, scene upload, rtrace / rcontrib / dctimestep entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <unordered_map>
 
 #include "../../include/radgpu.h"
+#include "rg_bvh.hpp"
 #include "rg_render.hpp"
 #include "rg_scene.hpp"
 
 using namespace rg;
+
+// Scenes up to this many BVH primitives get the host SAH tree by default;
+// larger ones the device LBVH (build time grows as n log n on one core).
+constexpr uint32_t kSahMaxPrims = 4u << 20;
+// Deepest tree the traversal stack (16 LDS + 48 private entries) can walk.
+constexpr int kMaxTreeDepth = 60;
 
 struct rg_ctx {
   int device = 0;
@@ -23,6 +31,7 @@ struct rg_ctx {
   Params params{};
   std::string err;
   unsigned long long last_nrays = 0;
+  int bvh_builder = RG_BVH_AUTO;
   int profile = 0;              // 0 off, 1 time k_bounce launches, 2 also count visits
   RenderStats last_stats;
   unsigned long long last_visits[2] = {0, 0};
@@ -141,14 +150,62 @@ int rg_scene_commit(rg_ctx* c) {
     S.nsrcs = (int)cs.srcs.size();
     S.lamb = cs.lamb;
     S.cusize = cs.cusize;
-    uint32_t n = (uint32_t)cs.prims.size();
-    float* dlo = upload(c->scene_mem, "blo", cs.lo, c->st);
-    float* dhi = upload(c->scene_mem, "bhi", cs.hi, c->st);
+    // Room-sized primitives (box diagonal > 1/8 of the scene's, at most 32,
+    // largest first) stay out of the BVH: inside the tree their boxes would
+    // inflate every ancestor; each ray tests them first instead.
+    std::vector<int32_t> big, small_ids;
+    {
+      double sd2 = 0;
+      for (int a = 0; a < 3; ++a) sd2 += (cs.bmax[a] - cs.bmin[a]) * (cs.bmax[a] - cs.bmin[a]);
+      std::vector<std::pair<double, int32_t>> cand;
+      for (size_t k = 0; k < cs.prims.size(); ++k) {
+        double d2 = 0;
+        for (int a = 0; a < 3; ++a) d2 += (double)(cs.hi[3 * k + a] - cs.lo[3 * k + a]) * (cs.hi[3 * k + a] - cs.lo[3 * k + a]);
+        if (d2 * 64.0 > sd2) cand.push_back({d2, (int32_t)k});
+      }
+      std::sort(cand.begin(), cand.end(), [](auto& a, auto& b) { return a.first > b.first; });
+      if (cand.size() > 32) cand.resize(32);
+      if (cand.size() == cs.prims.size() && !cand.empty()) cand.pop_back();   // keep the tree non-empty
+      std::vector<char> isbig(cs.prims.size(), 0);
+      for (auto& pr : cand) { isbig[pr.second] = 1; big.push_back(pr.second); }
+      std::sort(big.begin(), big.end());
+      for (size_t k = 0; k < cs.prims.size(); ++k)
+        if (!isbig[k]) small_ids.push_back((int32_t)k);
+    }
+    std::vector<float> slo_v, shi_v;
+    slo_v.reserve(3 * small_ids.size());
+    shi_v.reserve(3 * small_ids.size());
+    float slo[3] = {1e30f, 1e30f, 1e30f}, shi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int32_t k : small_ids)
+      for (int a = 0; a < 3; ++a) {
+        slo_v.push_back(cs.lo[3 * k + a]);
+        shi_v.push_back(cs.hi[3 * k + a]);
+        slo[a] = std::min(slo[a], cs.lo[3 * k + a]);
+        shi[a] = std::max(shi[a], cs.hi[3 * k + a]);
+      }
+    uint32_t n = (uint32_t)small_ids.size();
+    float* dlo = upload(c->scene_mem, "blo", slo_v, c->st);
+    float* dhi = upload(c->scene_mem, "bhi", shi_v, c->st);
+    int32_t* dids = upload(c->scene_mem, "bids", small_ids, c->st);
+    S.big = upload(c->scene_mem, "big", big, c->st);
+    S.nbig = (int)big.size();
+    S.nbvh = (int)n;
     BVHNode* nodes = (BVHNode*)c->scene_mem.get("nodes", sizeof(BVHNode) * std::max<uint32_t>(n, 1), c->st, true);
     int32_t* leaf = (int32_t*)c->scene_mem.get("leaf", sizeof(int32_t) * std::max<uint32_t>(n, 1), c->st, false);
-    float slo[3], shi[3];
-    for (int a = 0; a < 3; ++a) { slo[a] = (float)cs.bmin[a]; shi[a] = (float)cs.bmax[a]; }
-    S.root = n ? build_lbvh(dlo, dhi, n, slo, shi, nodes, leaf, c->arena, c->st) : -1;
+    bool sah = c->bvh_builder == RG_BVH_SAH || (c->bvh_builder == RG_BVH_AUTO && n <= kSahMaxPrims);
+    if (n == 0) {
+      S.root = -1;
+    } else if (sah) {
+      // host SAH tree: leaf refs already hold primitive indices
+      std::vector<BVHNode> hn;
+      S.root = build_sah_bvh(slo_v.data(), shi_v.data(), small_ids.data(), n, hn, kMaxTreeDepth);
+      if (!hn.empty())
+        hipck(hipMemcpyAsync(nodes, hn.data(), sizeof(BVHNode) * hn.size(), hipMemcpyHostToDevice, c->st), "nodes");
+      hipck(hipMemcpyAsync(leaf, small_ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st), "leaves");
+      hipck(hipStreamSynchronize(c->st), "nodes");
+    } else {
+      S.root = build_lbvh(dlo, dhi, dids, n, slo, shi, nodes, leaf, c->arena, c->st);
+    }
     S.nodes = nodes;
     S.leaf_prim = leaf;
     hipck(hipStreamSynchronize(c->st), "scene commit");
@@ -187,7 +244,8 @@ int rg_set_params(rg_ctx* c, const rg_params* p) {
 }
 
 static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int flags, int mode,
-                const std::vector<BinSpec>& bins, int ncols, double* out) {
+                const std::vector<BinSpec>& bins, int ncols, double* out,
+                const std::vector<DevScene>* batch = nullptr, const std::vector<uint32_t>* batch_start = nullptr) {
   if (!c->committed) throw std::runtime_error("scene not committed");
   if (nrays < 0 || nrays > 0x7fffffffLL) throw std::runtime_error("bad ray count");
   if (accum < 1) throw std::runtime_error("bad accumulation count");
@@ -217,6 +275,19 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   cfg.nrays = d_nr;
   cfg.err = d_err;
   cfg.accum = accum;
+  cfg.nscenes = 0;
+  cfg.scenes = nullptr;
+  cfg.scene_start = nullptr;
+  if (batch) {
+    DevScene* d_sc = (DevScene*)c->arena.get("batch_scenes", sizeof(DevScene) * batch->size(), c->st, false);
+    uint32_t* d_ss = (uint32_t*)c->arena.get("batch_start", sizeof(uint32_t) * batch_start->size(), c->st, false);
+    hipck(hipMemcpyAsync(d_sc, batch->data(), sizeof(DevScene) * batch->size(), hipMemcpyHostToDevice, c->st), "scenes");
+    hipck(hipMemcpyAsync(d_ss, batch_start->data(), sizeof(uint32_t) * batch_start->size(), hipMemcpyHostToDevice,
+                         c->st), "scene starts");
+    cfg.nscenes = (int)batch->size();
+    cfg.scenes = d_sc;
+    cfg.scene_start = d_ss;
+  }
   cfg.visits = nullptr;
   unsigned long long* d_vis = nullptr;
   if (c->profile >= 2) {
@@ -225,7 +296,11 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   }
   c->last_stats = RenderStats();
   c->last_stats.timing = c->profile >= 1;
-  if (nrays) render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, c->arena, c->st, &c->last_stats);
+  int max_srcs = c->dscene.nsrcs;
+  if (batch)
+    for (const DevScene& b : *batch) max_srcs = std::max(max_srcs, b.nsrcs);
+  if (nrays)
+    render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, c->arena, c->st, &c->last_stats);
   if (d_vis) hipck(hipMemcpyAsync(c->last_visits, d_vis, 16, hipMemcpyDeviceToHost, c->st), "visits");
   else c->last_visits[0] = c->last_visits[1] = 0;
   uint32_t herr = 0;
@@ -249,62 +324,74 @@ int rg_trace(rg_ctx* c, const double* rays, long long n, int flags, double* out)
   });
 }
 
+// Bin specs for the modifier list; fills the name -> slot map.
+static std::vector<BinSpec> bin_specs(const rg_modifier* mods, int nmods, std::unordered_map<std::string, int>& slot,
+                                      int& ncols) {
+  std::vector<BinSpec> bins(nmods);
+  int col = 0;
+  for (int i = 0; i < nmods; ++i) {
+    const rg_modifier& m = mods[i];
+    if (!m.name || !m.name[0]) throw std::runtime_error("modifier without a name");
+    if (!strcmp(m.name, "void")) throw std::runtime_error("cannot track 'void' modifier");
+    if (slot.count(m.name)) throw std::runtime_error(std::string("duplicate modifier '") + m.name + "'");
+    if (m.nbins < 1) throw std::runtime_error("bad bin count");
+    if (m.bin_kind < 0 || m.bin_kind > 4) throw std::runtime_error("bad bin kind");
+    slot[m.name] = i;
+    BinSpec& b = bins[i];
+    memset(&b, 0, sizeof b);
+    b.kind = m.bin_kind;
+    b.nbins = m.nbins;
+    b.col0 = col;
+    b.cbin = m.cbin;
+    b.mf = m.mf;
+    b.rhs = m.rhs;
+    for (int a = 0; a < 3; ++a) { b.nrm[a] = m.nrm[a]; b.up[a] = m.up[a]; }
+    col += m.nbins;
+  }
+  ncols = 3 * col;
+  return bins;
+}
+
+// The context's scene with tracked-modifier slots assigned by modifier name
+// (rt/rcontrib.c:289) and every source followed (rt/rcontrib.c:235-237).
+static DevScene contrib_scene(rg_ctx* c, const std::unordered_map<std::string, int>& slot, hipStream_t st) {
+  if (!c->committed) throw std::runtime_error("scene not committed");
+  CompiledScene& cs = c->cs;
+  std::vector<Prim> prims = cs.prims;
+  for (size_t k = 0; k < prims.size(); ++k) {
+    auto it = slot.find(cs.prim_modname[k]);
+    prims[k].modslot = it == slot.end() ? -1 : it->second;
+  }
+  std::vector<Source> srcs = cs.srcs;
+  for (size_t k = 0; k < srcs.size(); ++k) {
+    auto it = slot.find(cs.src_modname[k]);
+    srcs[k].modslot = it == slot.end() ? -1 : it->second;
+    srcs[k].flags |= SFOLLOW;
+  }
+  if (c->params.contrib) {
+    for (size_t k = 0; k < prims.size(); ++k) {
+      int mk = prims[k].mat >= 0 ? cs.mats[prims[k].mat].kind : -1;
+      if (prims[k].modslot >= 0 && mk != MK_LIGHT && mk != MK_GLOW && mk != MK_ILLUM)
+        throw std::runtime_error("-V+ is supported for light-source modifiers only on the GPU path");
+    }
+  }
+  DevScene S = c->dscene;
+  S.prims = upload(c->arena, "prims_c", prims, st);
+  S.srcs = upload(c->arena, "srcs_c", srcs, st);
+  return S;
+}
+
 int rg_contrib(rg_ctx* c, const double* rays, long long nrec, int accum, int flags, const rg_modifier* mods,
                int nmods, double* out) {
   RG_TRY(c, {
-    if (!c->committed) throw std::runtime_error("scene not committed");
-    std::vector<BinSpec> bins(nmods);
     std::unordered_map<std::string, int> slot;
-    int col = 0;
-    for (int i = 0; i < nmods; ++i) {
-      const rg_modifier& m = mods[i];
-      if (!m.name || !m.name[0]) throw std::runtime_error("modifier without a name");
-      if (!strcmp(m.name, "void")) throw std::runtime_error("cannot track 'void' modifier");
-      if (slot.count(m.name)) throw std::runtime_error(std::string("duplicate modifier '") + m.name + "'");
-      if (m.nbins < 1) throw std::runtime_error("bad bin count");
-      if (m.bin_kind < 0 || m.bin_kind > 4) throw std::runtime_error("bad bin kind");
-      slot[m.name] = i;
-      BinSpec& b = bins[i];
-      memset(&b, 0, sizeof b);
-      b.kind = m.bin_kind;
-      b.nbins = m.nbins;
-      b.col0 = col;
-      b.cbin = m.cbin;
-      b.mf = m.mf;
-      b.rhs = m.rhs;
-      for (int a = 0; a < 3; ++a) { b.nrm[a] = m.nrm[a]; b.up[a] = m.up[a]; }
-      col += m.nbins;
-    }
-    // assign tracked-modifier slots by modifier name (rt/rcontrib.c:289)
-    CompiledScene& cs = c->cs;
-    std::vector<Prim> prims = cs.prims;
-    bool any = false;
-    for (size_t k = 0; k < prims.size(); ++k) {
-      auto it = slot.find(cs.prim_modname[k]);
-      prims[k].modslot = it == slot.end() ? -1 : it->second;
-      any |= prims[k].modslot >= 0;
-    }
-    std::vector<Source> srcs = cs.srcs;
-    for (size_t k = 0; k < srcs.size(); ++k) {
-      auto it = slot.find(cs.src_modname[k]);
-      srcs[k].modslot = it == slot.end() ? -1 : it->second;
-      srcs[k].flags |= SFOLLOW;      // rt/rcontrib.c:235-237
-    }
-    if (c->params.contrib) {
-      for (size_t k = 0; k < prims.size(); ++k) {
-        int mk = prims[k].mat >= 0 ? cs.mats[prims[k].mat].kind : -1;
-        if (prims[k].modslot >= 0 && mk != MK_LIGHT && mk != MK_GLOW && mk != MK_ILLUM)
-          throw std::runtime_error("-V+ is supported for light-source modifiers only on the GPU path");
-      }
-    }
-    (void)any;
-    DevScene S = c->dscene;
-    S.prims = upload(c->arena, "prims_c", prims, c->st);
-    S.srcs = upload(c->arena, "srcs_c", srcs, c->st);
+    int ncols = 0;
+    std::vector<BinSpec> bins = bin_specs(mods, nmods, slot, ncols);
+    DevScene S = contrib_scene(c, slot, c->st);
     DevScene saved = c->dscene;
     c->dscene = S;
     try {
-      run(c, rays, nrec * (long long)accum, accum, flags, 1, bins, 3 * col, out);
+      run(c, rays, nrec * (long long)accum, accum, flags, 1, bins, ncols, out);
     } catch (...) {
       c->dscene = saved;
       throw;
@@ -313,8 +400,69 @@ int rg_contrib(rg_ctx* c, const double* rays, long long nrec, int accum, int fla
   });
 }
 
+int rg_contrib_batch(rg_ctx* const* ctxs, int nctx, const double* rays, const long long* nrec, int accum, int flags,
+                     const rg_modifier* mods, int nmods, double* out) {
+  rg_ctx* c0 = nctx > 0 ? ctxs[0] : nullptr;
+  if (!c0) {
+    g_create_err = "empty batch";
+    return -1;
+  }
+  RG_TRY(c0, {
+    if (nctx > 255) throw std::runtime_error("at most 255 scenes per batch");
+    std::unordered_map<std::string, int> slot;
+    int ncols = 0;
+    std::vector<BinSpec> bins = bin_specs(mods, nmods, slot, ncols);
+    std::vector<DevScene> scenes;
+    std::vector<uint32_t> start(1, 0);
+    long long total = 0;
+    for (int i = 0; i < nctx; ++i) {
+      if (!ctxs[i] || ctxs[i]->device != c0->device) throw std::runtime_error("batch contexts must share a device");
+      if (nrec[i] < 0) throw std::runtime_error("bad record count");
+      scenes.push_back(contrib_scene(ctxs[i], slot, ctxs[i]->st));
+      hipck(hipStreamSynchronize(ctxs[i]->st), "batch upload");
+      total += nrec[i] * (long long)accum;
+      if (total > 0x7fffffffLL) throw std::runtime_error("bad ray count");
+      start.push_back((uint32_t)total);
+    }
+    run(c0, rays, total, accum, flags, 1, bins, ncols, out, &scenes, &start);
+  });
+}
+
+int rg_trace_batch(rg_ctx* const* ctxs, int nctx, const double* rays, const long long* n, int flags, double* out) {
+  rg_ctx* c0 = nctx > 0 ? ctxs[0] : nullptr;
+  if (!c0) {
+    g_create_err = "empty batch";
+    return -1;
+  }
+  RG_TRY(c0, {
+    if (nctx > 255) throw std::runtime_error("at most 255 scenes per batch");
+    std::vector<DevScene> scenes;
+    std::vector<uint32_t> start(1, 0);
+    long long total = 0;
+    for (int i = 0; i < nctx; ++i) {
+      if (!ctxs[i] || ctxs[i]->device != c0->device) throw std::runtime_error("batch contexts must share a device");
+      if (!ctxs[i]->committed) throw std::runtime_error("scene not committed");
+      if (n[i] < 0) throw std::runtime_error("bad ray count");
+      scenes.push_back(ctxs[i]->dscene);
+      total += n[i];
+      if (total > 0x7fffffffLL) throw std::runtime_error("bad ray count");
+      start.push_back((uint32_t)total);
+    }
+    std::vector<BinSpec> none;
+    run(c0, rays, total, 1, flags, 0, none, 3, out, &scenes, &start);
+  });
+}
+
 int rg_last_nrays(rg_ctx* c, unsigned long long* n) {
   RG_TRY(c, { *n = c->last_nrays; });
+}
+
+int rg_set_bvh_builder(rg_ctx* c, int kind) {
+  RG_TRY(c, {
+    if (kind != RG_BVH_AUTO && kind != RG_BVH_SAH && kind != RG_BVH_LBVH)
+      throw std::runtime_error("unknown BVH builder");
+    c->bvh_builder = kind;
+  });
 }
 
 int rg_set_profiling(rg_ctx* c, int level) {

@@ -109,20 +109,37 @@ __global__ void __launch_bounds__(DB) k_dcsky_mfma(const unsigned short* Ah, con
         }
     }
   }
-  // D layout: lane holds column (lane & 15), rows 4*(lane>>4) + i
+  // D layout: lane holds column (lane & 15), rows 4*(lane>>4) + i.  The
+  // block's 64 x 64 x 3 tile is staged in LDS in C's [m][n][c] order, then
+  // written row by row with coalesced stores.
+  __shared__ float ct[BM][BN * 3 + 4];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) {
-      int n = n0 + 16 * b + r16;
-      if (n >= N) continue;
+      int nl = wn * 32 + 16 * b + r16;
       for (int i = 0; i < 4; ++i) {
-        int m = m0 + 16 * a + (lane >> 4) * 4 + i;
-        if (m >= M) continue;
-        float* o = C + ((size_t)m * N + n) * 3;
-        o[0] = acc[0][a][b][i];
-        o[1] = acc[1][a][b][i];
-        o[2] = acc[2][a][b][i];
+        int ml = wm * 32 + 16 * a + (lane >> 4) * 4 + i;
+        ct[ml][3 * nl + 0] = acc[0][a][b][i];
+        ct[ml][3 * nl + 1] = acc[1][a][b][i];
+        ct[ml][3 * nl + 2] = acc[2][a][b][i];
       }
     }
+  __syncthreads();
+  const int mb = blockIdx.y * BM, nb = blockIdx.x * BN;
+  const int ncols = min(BN, N - nb) * 3;     // valid floats per tile row
+  const bool vec = (ncols == BN * 3) && (((size_t)N * 3) % 4 == 0);
+  for (int ml = wave; ml < BM; ml += DB / 64) {
+    int m = mb + ml;
+    if (m >= M) break;
+    float* row = C + ((size_t)m * N + nb) * 3;
+    if (vec) {
+      for (int e = lane; e < BN * 3 / 4; e += 64) {
+        float4 v = make_float4(ct[ml][4 * e], ct[ml][4 * e + 1], ct[ml][4 * e + 2], ct[ml][4 * e + 3]);
+        *(float4*)(row + 4 * e) = v;
+      }
+    } else {
+      for (int e = lane; e < ncols; e += 64) row[e] = ct[ml][e];
+    }
+  }
 }
 
 // FP32 reference path: 16x16 threads, 64x64 tile, LDS staged.

@@ -27,6 +27,9 @@ struct DevScene {
   const Source* srcs;
   const BVHNode* nodes;
   const int32_t* leaf_prim;     // sorted leaf position -> primitive index
+  const int32_t* big;           // primitives kept out of the BVH, tested first
+  int32_t nbig;
+  int32_t nbvh;                 // primitives inside the BVH
   int32_t nprims;
   int32_t nmats;
   int32_t nsrcs;

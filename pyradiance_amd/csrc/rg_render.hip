@@ -73,12 +73,15 @@ __device__ __forceinline__ bool slab(const float* lo, const float* hi, float ox,
 // bank per lane); deeper entries spill to a private array that only deep
 // subtrees touch.
 template <int BS>
-__device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* stk, uint32_t* err,
+__device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* stk, float* stn, uint32_t* err,
                             unsigned long long* visits) {
   uint32_t nvis = 0, nprim = 0;
   h.t = FHUGE; h.rod = 1.0; h.prim = -1;
-  if (S.nprims == 0) return;
-  if (S.root < 0) { hit_prim(S, S.leaf_prim[~S.root], o, d, h); return; }
+  // room-sized primitives first: their hit bounds the walk below
+  for (int k = 0; k < S.nbig; ++k) { hit_prim(S, S.big[k], o, d, h); ++nprim; }
+  if (S.nbvh == 0) goto done;
+  if (S.root < 0) { hit_prim(S, S.leaf_prim[~S.root], o, d, h); ++nprim; goto done; }
+  {
   const float ox = (float)o.x, oy = (float)o.y, oz = (float)o.z;
   auto inv = [](double v) -> float {
     float f = (float)v;
@@ -88,37 +91,73 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
   const float ix = inv(d.x), iy = inv(d.y), iz = inv(d.z);
   const int lane = threadIdx.x;
   int32_t deep[PSTACK];
+  float deept[PSTACK];
   int sp = 0;
   int node = S.root;
-  // every internal node is entered at most once per ray
-  for (int iter = 0;; ++iter) {
-    if (iter > S.nprims + 8) { atomicOr(err, ERR_STACK); break; }
-    const BVHNode N = S.nodes[node];
-    ++nvis;
-    float tmax = (float)(h.t + 2 * FTINY) * 1.000001f;
-    float tl, tr;
-    bool hl = slab(N.lmin, N.lmax, ox, oy, oz, ix, iy, iz, tmax, tl);
-    bool hr = slab(N.rmin, N.rmax, ox, oy, oz, ix, iy, iz, tmax, tr);
-    if (hl & (N.left < 0)) { hit_prim(S, S.leaf_prim[~N.left], o, d, h); hl = false; ++nprim; }
-    if (hr & (N.right < 0)) { hit_prim(S, S.leaf_prim[~N.right], o, d, h); hr = false; ++nprim; }
-    if (hl & hr) {
-      int nearc = tl <= tr ? N.left : N.right;
-      int farc = tl <= tr ? N.right : N.left;
-      if (sp < LSTACK) stk[sp * BS + lane] = farc;
-      else if (sp < LSTACK + PSTACK) deep[sp - LSTACK] = farc;
-      else { atomicOr(err, ERR_STACK); break; }
-      ++sp;
-      node = nearc;
-    } else if (hl) {
-      node = N.left;
-    } else if (hr) {
-      node = N.right;
-    } else {
-      if (sp == 0) break;
+  const BVHNode* __restrict__ nodes = S.nodes;
+  const int nbvh = S.nbvh;
+  // Walk internal nodes until every lane of the wavefront holds a postponed
+  // leaf (or is done), then run the FP64 primitive tests together: testing a
+  // leaf as soon as one lane meets it would stall the other 63 lanes on it.
+  // Child refs: >= 0 internal node, < 0 primitive ~ref, DONE: stack empty.
+  constexpr int32_t DONE = INT32_MIN;
+  auto pop = [&](float lim) -> int32_t {
+    // skip subtrees whose box starts beyond the current closest hit
+    while (sp > 0) {
       --sp;
-      node = sp < LSTACK ? stk[sp * BS + lane] : deep[sp - LSTACK];
+      int32_t cand;
+      float ct;
+      if (sp < LSTACK) { cand = stk[sp * BS + lane]; ct = stn[sp * BS + lane]; }
+      else { cand = deep[sp - LSTACK]; ct = deept[sp - LSTACK]; }
+      if (ct <= lim) return cand;
+    }
+    return DONE;
+  };
+  int32_t leaf = -1;             // postponed primitive, -1: none
+  int iter = 0;                  // every internal node is entered at most once
+  while (node != DONE) {
+    while (node >= 0) {
+      if (++iter > nbvh + 8) { atomicOr(err, ERR_STACK); node = DONE; break; }
+      const BVHNode N = nodes[node];
+      ++nvis;
+      float tmax = (float)(h.t + 2 * FTINY) * 1.000001f;
+      float tl, tr;
+      bool hl = slab(N.lmin, N.lmax, ox, oy, oz, ix, iy, iz, tmax, tl);
+      bool hr = slab(N.rmin, N.rmax, ox, oy, oz, ix, iy, iz, tmax, tr);
+      if (hl & hr) {
+        bool lfirst = tl <= tr;
+        int32_t farc = lfirst ? N.right : N.left;
+        float fart = lfirst ? tr : tl;
+        if (sp < LSTACK) { stk[sp * BS + lane] = farc; stn[sp * BS + lane] = fart; }
+        else if (sp < LSTACK + PSTACK) { deep[sp - LSTACK] = farc; deept[sp - LSTACK] = fart; }
+        else { atomicOr(err, ERR_STACK); node = DONE; break; }
+        ++sp;
+        node = lfirst ? N.left : N.right;
+      } else if (hl) {
+        node = N.left;
+      } else if (hr) {
+        node = N.right;
+      } else {
+        node = pop(tmax);
+      }
+      if ((node < 0) & (node != DONE) & (leaf < 0)) {
+        leaf = ~node;
+        node = pop((float)(h.t + 2 * FTINY) * 1.000001f);
+      }
+      if (__ballot(leaf < 0) == 0ull) break;
+    }
+    while (leaf >= 0) {
+      hit_prim(S, leaf, o, d, h);
+      ++nprim;
+      leaf = -1;
+      if ((node < 0) & (node != DONE)) {
+        leaf = ~node;
+        node = pop((float)(h.t + 2 * FTINY) * 1.000001f);
+      }
     }
   }
+  }
+done:
   if (visits) {
     atomicAdd(visits, (unsigned long long)nvis);
     atomicAdd(visits + 1, (unsigned long long)nprim);
@@ -153,7 +192,7 @@ __device__ Spawn make_child(const RayS& r, V3 org, uint16_t rt, const C3* rc, ui
   else { s.rlvl = (uint8_t)r.rlvl; s.rsrc = (int16_t)r.rsrc; }
   s.adepth = (uint8_t)r.adepth;
   s.flags = flags;
-  s.pad = 0;
+  s.scene = (uint8_t)r.scene;
   s.key = child_key(r.key, slot);
   return s;
 }
@@ -709,12 +748,28 @@ __device__ void shade_miss(const DevScene& S, const RunCfg& c, const RayS& r) {
 }
 
 // ---- kernels -----------------------------------------------------------------
+// Scene tables of a ray: the kernel argument, or its entry of a batched launch.
+__device__ __forceinline__ const DevScene& scene_of(const DevScene& S, const RunCfg& c, int scene) {
+  return c.nscenes ? c.scenes[scene] : S;
+}
+__device__ __forceinline__ int scene_of_ray(const RunCfg& c, uint32_t ray) {
+  if (!c.nscenes) return 0;
+  int lo = 0, hi = c.nscenes;       // last k with scene_start[k] <= ray
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (c.scene_start[mid] <= ray) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // Generation 0 for -I: the pseudo-hit on a Lambertian surface at each sensor
 // (rt/rtrace.c:416-431, rt/rcontrib.c:321-338).
-__global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene S, RunCfg c, const double* rays, uint32_t nsens,
+__global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene Sarg, RunCfg c, const double* rays, uint32_t nsens,
                                                       uint32_t s0, Sink sk) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nsens) return;
+  const int scene = scene_of_ray(c, s0 + i);
+  const DevScene& S = scene_of(Sarg, c, scene);
   const double* ry = rays + 6 * (size_t)i;
   V3 org = load3(ry), dir = load3(ry + 3);
   normalize(dir);
@@ -729,7 +784,8 @@ __global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene S, RunCfg c, cons
   r.rsrc = -1;
   r.rlvl = 0;
   r.adepth = 0;
-  r.key = sensor_key(c.p.seed, s0 + i);
+  r.scene = scene;
+  r.key = sensor_key(c.p.seed + scene, s0 + i - (c.nscenes ? c.scene_start[scene] : 0));
   HitS hs;
   hs.rot = 1e-5;
   hs.rop = org + dir * 1e-4;
@@ -763,8 +819,9 @@ __global__ void __launch_bounds__(BLOCK) k_primary_seed(RunCfg c, const double* 
   s.rlvl = 0;
   s.adepth = 0;
   s.flags = SPF_FORCE;
-  s.pad = 0;
-  s.key = sensor_key(c.p.seed, s0 + i);
+  const int scene = scene_of_ray(c, s0 + i);
+  s.scene = (uint8_t)scene;
+  s.key = sensor_key(c.p.seed + scene, s0 + i - (c.nscenes ? c.scene_start[scene] : 0));
   out[i] = s;
 }
 
@@ -785,6 +842,7 @@ __device__ __forceinline__ bool build_ray(const RunCfg& c, const Spawn& sp, uint
   r.rsrc = sp.rsrc;
   r.rlvl = sp.rlvl;
   r.adepth = sp.adepth;
+  r.scene = sp.scene;
   if (sp.ns) {
     // stratified cosine-weighted hemisphere sample (rt/ambcomp.c:177-216)
     uint32_t ns = sp.ns;
@@ -825,13 +883,28 @@ struct alignas(16) TraceOut {
 };
 
 // One generation, part 1: children [c0, c0+nthreads) of `in` -> closest hits.
-__global__ void __launch_bounds__(TBLOCK) k_trace(DevScene S, RunCfg c, const Spawn* in, const uint32_t* offs,
+__global__ void __launch_bounds__(TBLOCK) k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs,
                                                   uint32_t nspawn, uint64_t c0, uint32_t nthreads, TraceOut* outp) {
   __shared__ int32_t stk[LSTACK * TBLOCK];
+  __shared__ float stn[LSTACK * TBLOCK];
   uint32_t t = blockIdx.x * TBLOCK + threadIdx.x;
   if (t >= nthreads) return;
   uint64_t gid = c0 + t;
-  uint32_t si = find_spawn(offs, nspawn, gid);
+  // The wavefront's children are consecutive: one lane searches the whole
+  // table for the first child, the others only the few records after it
+  // (every record has at least one child).
+  unsigned long long act = __ballot(1);
+  uint32_t leader = __ffsll((long long)act) - 1;
+  uint64_t gid0 = __shfl(gid, leader);
+  uint32_t base = 0;
+  if (__lane_id() == leader) base = find_spawn(offs, nspawn, gid0);
+  base = __shfl(base, leader);
+  uint32_t lo = base, hi = (uint32_t)min((uint64_t)nspawn, (uint64_t)base + (gid - gid0) + 1);
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)offs[mid] <= gid) lo = mid; else hi = mid;
+  }
+  uint32_t si = lo;
   RayS r;
   TraceOut o;
   o.spawn = si;
@@ -845,7 +918,7 @@ __global__ void __launch_bounds__(TBLOCK) k_trace(DevScene S, RunCfg c, const Sp
   }
   wave_count(c.nrays);
   HitRec h;
-  closest_hit<TBLOCK>(S, r.o, r.d, h, stk, c.err, c.visits);
+  closest_hit<TBLOCK>(scene_of(Sarg, c, r.scene), r.o, r.d, h, stk, stn, c.err, c.visits);
   o.t = h.t;
   o.prim = h.prim;
   o.dx = (float)r.d.x; o.dy = (float)r.d.y; o.dz = (float)r.d.z;
@@ -853,7 +926,7 @@ __global__ void __launch_bounds__(TBLOCK) k_trace(DevScene S, RunCfg c, const Sp
 }
 
 // One generation, part 2: shade the hits and append the next spawn records.
-__global__ void __launch_bounds__(BLOCK) k_shade(DevScene S, RunCfg c, const Spawn* in, const uint32_t* offs,
+__global__ void __launch_bounds__(BLOCK) k_shade(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs,
                                                  uint64_t c0, uint32_t nthreads, const TraceOut* tr, Sink sk) {
   uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
   if (t >= nthreads) return;
@@ -866,6 +939,7 @@ __global__ void __launch_bounds__(BLOCK) k_shade(DevScene S, RunCfg c, const Spa
   h.t = o.t;
   h.prim = o.prim;
   h.rod = 1.0;
+  const DevScene& S = scene_of(Sarg, c, r.scene);
   if (h.prim >= 0) shade_hit(S, c, r, h, sk);
   else shade_miss(S, c, r);
 }
@@ -941,12 +1015,12 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& are
   }
 }
 
-void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad,
+void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad, int max_srcs,
             DevArena& arena, hipStream_t st, RenderStats* stats) {
   // Work proceeds in waves of at most `window` children; each child appends
   // at most `per_child` spawn records, so a wave never overflows its buffer.
-  const uint32_t per_child = 6u + (uint32_t)S.nsrcs;
-  const uint32_t window = 1u << 20;
+  const uint32_t per_child = 6u + (uint32_t)std::max(max_srcs, S.nsrcs);
+  const uint32_t window = 1u << 21;
   const long max_batches = 1L << 22;
   uint32_t* d_cnt = (uint32_t*)arena.get("ctr", 64, st, true);
   uint32_t* h_cnt = arena.host_u32(8);
@@ -962,7 +1036,9 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   auto slot_buf = [&](int s, size_t cap) -> Spawn* {
     return (Spawn*)arena.get("spawn" + std::to_string(s), sizeof(Spawn) * cap, st, false);
   };
-  size_t batch_cap = std::max<size_t>((size_t)window * per_child, (size_t)nsens);
+  // Buffers are sized to the work at hand (and kept by the arena for later
+  // calls), so a context that only ever traces a few sensors stays small.
+  size_t batch_cap = (size_t)std::max<uint32_t>(nsens, 1u) * per_child;
 
   // generation 0
   {
@@ -988,7 +1064,6 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
     check(hipEventCreate(&ev1), "event");
     check(hipEventCreate(&ev2), "event");
   }
-  TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * window, st, false);
   while (!stack.empty()) {
     Batch bt = stack.back();
     stack.pop_back();
@@ -1004,10 +1079,12 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
     std::vector<Batch> produced;
     for (uint64_t c0 = 0; c0 < total; c0 += window) {
       uint32_t nth = (uint32_t)std::min<uint64_t>(window, total - c0);
+      size_t cap = (size_t)nth * per_child;
       int s = new_slot();
-      Spawn* ob = slot_buf(s, batch_cap);
+      Spawn* ob = slot_buf(s, cap);
+      TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * nth, st, false);
       check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
-      Sink sk{ob, d_cnt, (uint32_t)batch_cap, cfg.err};
+      Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err};
       if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
       hipLaunchKernelGGL(k_trace, dim3((nth + TBLOCK - 1) / TBLOCK), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n,
                          c0, nth, tout);
@@ -1027,7 +1104,7 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
         stats->shade_ms += ms2;
       }
       if (stats) { stats->launches++; stats->threads += nth; }
-      uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap);
+      uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)cap);
       if (nout) produced.push_back({ob, nout, s});
       else free_slots.push_back(s);
     }

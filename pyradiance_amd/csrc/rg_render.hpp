@@ -75,13 +75,16 @@ struct RenderStats {
 
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level);
 
-void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad,
+// max_srcs: the largest source count over the scenes of a batch (sizes the
+// spawn buffers); S.nsrcs is used when that is larger.
+void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad, int max_srcs,
             DevArena& arena, hipStream_t st, RenderStats* stats);
 
 // BVH over the primitives' FP32 bounds (lo/hi: 3*n floats, already widened).
 // Fills nodes (n-1 entries) and leaf_prim (n entries); returns the root ref.
-int build_lbvh(const float* d_lo, const float* d_hi, uint32_t n, const float scene_lo[3], const float scene_hi[3],
-               BVHNode* d_nodes, int32_t* d_leaf_prim, DevArena& arena, hipStream_t st);
+// d_ids maps the n boxes to primitive indices written into leaf_prim.
+int build_lbvh(const float* d_lo, const float* d_hi, const int32_t* d_ids, uint32_t n, const float scene_lo[3],
+               const float scene_hi[3], BVHNode* d_nodes, int32_t* d_leaf_prim, DevArena& arena, hipStream_t st);
 
 // DC x sky product C[M,N,3] = A[M,K,3] x B[K,N,3], per channel.
 // mode 0: FP32 FMA, mode 1: BF16 hi/lo split on MFMA with FP32 accumulation.

@@ -27,7 +27,7 @@ struct alignas(16) Spawn {
   uint8_t rlvl;      // child's reflection level
   uint8_t adepth;    // child's ambient recursion depth
   uint8_t flags;     // SPF_*
-  uint8_t pad;
+  uint8_t scene;     // scene of a batched launch (0 otherwise)
   uint64_t key;      // explicit: child key; hemisphere: parent key
 };
 static_assert(sizeof(Spawn) == 96, "Spawn layout");
@@ -45,6 +45,7 @@ struct RayS {
   int rsrc;
   int rlvl;
   int adepth;
+  int scene;
   uint64_t key;
 };
 
@@ -55,7 +56,9 @@ struct RunCfg {
   int32_t ncols;       // output doubles per sensor
   int32_t nbinspec;
   int32_t accum;       // rays per output record (rcontrib -c)
-  int32_t pad;
+  int32_t nscenes;     // >0: batched launch over `scenes`
+  const DevScene* scenes;      // per-scene tables of a batched launch
+  const uint32_t* scene_start; // first input ray of each scene (nscenes+1)
   const BinSpec* bins; // per tracked-modifier slot
   double* out;         // [nsensor][ncols]
   unsigned long long* nrays;

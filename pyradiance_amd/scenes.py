@@ -30,6 +30,8 @@ __all__ = [
     "furniture_rad",
     "sky_matrix",
     "reinhart_nbins",
+    "config3",
+    "boxes_rad",
 ]
 
 
@@ -235,3 +237,70 @@ def sky_matrix(seed: int, nbins: int, nhours: int = 8760, latitude: float = 37.8
     sin_alt = np.sin(lat) * np.sin(decl) + np.cos(lat) * np.cos(decl) * np.cos(ha)
     sky[:, sin_alt <= 0.0, :] = 0.0
     return sky
+
+
+def boxes_rad(rng: np.random.Generator, nboxes: int, room: Room, mod_prefix: str = "obj",
+              nmats: int = 8, alpha: float = 2.0, smin: float = 0.05, smax: float = 2.0) -> str:
+    """`nboxes` axis-aligned boxes (6 quads each) with Pareto(alpha) sizes in
+    [smin, smax] m, placed uniformly inside the room; `nmats` gray plastics.
+    Vectorised text generation for large counts."""
+    out = [f"void plastic {mod_prefix}_m{k}\n0\n0\n5 {r:.4g} {r:.4g} {r:.4g} 0 0\n"
+           for k, r in enumerate(rng.uniform(0.1, 0.8, nmats))]
+    if nboxes <= 0:
+        return "".join(out)
+    u = rng.random((nboxes, 3))
+    size = smin * (1 - u) ** (-1.0 / alpha)
+    size = np.minimum(size, [min(smax, room.width - 0.02), min(smax, room.depth - 0.02), min(smax, room.height - 0.05)])
+    lo = np.empty((nboxes, 3))
+    lo[:, 0] = rng.uniform(0.01, 1.0, nboxes) * (room.width - size[:, 0] - 0.02) + 0.01
+    lo[:, 1] = rng.uniform(0.01, 1.0, nboxes) * (room.depth - size[:, 1] - 0.02) + 0.01
+    lo[:, 2] = rng.uniform(0.0, 1.0, nboxes) * (room.height - size[:, 2] - 0.05) + 0.003
+    hi = lo + size
+    mats = rng.integers(0, nmats, nboxes)
+    # unit-cube quads with outward normals (counter-clockwise seen from outside)
+    faces = np.array([
+        [[0, 0, 0], [0, 0, 1], [0, 1, 1], [0, 1, 0]],   # -x
+        [[1, 0, 0], [1, 1, 0], [1, 1, 1], [1, 0, 1]],   # +x
+        [[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1]],   # -y
+        [[0, 1, 0], [0, 1, 1], [1, 1, 1], [1, 1, 0]],   # +y
+        [[0, 0, 0], [0, 1, 0], [1, 1, 0], [1, 0, 0]],   # -z
+        [[0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],   # +z
+    ], float)
+    verts = lo[:, None, None, :] + faces[None] * size[:, None, None, :]     # (n, 6, 4, 3)
+    coords = verts.reshape(nboxes * 6, 12)
+    lines = []
+    for i in range(nboxes * 6):
+        b = i // 6
+        lines.append(f"{mod_prefix}_m{mats[b]} polygon {mod_prefix}{i}\n0\n0\n12 ")
+    nums = np.char.mod("%.6f", coords)
+    body = [" ".join(r) for r in nums]
+    out.append("".join(h + b + "\n" for h, b in zip(lines, body)))
+    return "".join(out)
+
+
+def config3(seed: int = 0, nscenes: int = 64, prims=(1_000, 500_000), sensors=(100, 20_000)):
+    """Batched workload: `nscenes` shoebox rooms with seeded dimensions
+    U(3,20) m, log-uniform primitive counts (6-quad boxes, Pareto sizes) and
+    log-uniform sensor counts, each with its own sun.
+    Returns a list of (rad_text, sensors (n,6))."""
+    rng = np.random.default_rng(seed)
+    res = []
+    for k in range(nscenes):
+        dims = rng.uniform(3.0, 20.0, 3)
+        dims[2] = min(dims[2], 6.0)
+        room = Room(width=dims[0], depth=dims[1], height=max(dims[2], 3.0),
+                    win_w=min(2.0, dims[0] - 0.5), win_h=1.2, win_sill=0.9)
+        nprim = int(np.exp(rng.uniform(np.log(prims[0]), np.log(prims[1]))))
+        nsens = int(np.exp(rng.uniform(np.log(sensors[0]), np.log(sensors[1]))))
+        alt, azi = rng.uniform(20, 60), rng.uniform(120, 240)
+        rad = (room_rad(room) + boxes_rad(rng, max(nprim // 6, 1), room, f"s{k}o")
+               + sun_rad(sun_direction(alt, azi)) + sky_rad((1, 1, 1), ground=(1, 1, 1)))
+        # sensors: square lattice on the floor area at 0.8 m
+        area = (room.width - 0.2) * (room.depth - 0.2)
+        pitch = np.sqrt(area / nsens)
+        nx = max(int((room.width - 0.2) / pitch), 1)
+        ny = max(int(np.ceil(nsens / nx)), 1)
+        g = sensor_grid(nx, ny, 0.1 + pitch / 2, 0.1 + pitch / 2, pitch, 0.8)[:nsens]
+        g[:, 1] = np.minimum(g[:, 1], room.depth - 0.05)
+        res.append((rad, g))
+    return res

@@ -1,0 +1,44 @@
+"""Host SAH BVH builder: structure and nearest-box walks against brute force.
+
+The builder (pyradiance_amd/csrc/rg_bvh.cpp) is plain C++; it is compiled here
+with g++ together with tests/native/bvh_check.cpp, so these checks run without
+a GPU.
+"""
+
+import ctypes as C
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.fixture(scope="module")
+def bvhcheck(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    out = tmp_path_factory.mktemp("bvh") / "bvhcheck.so"
+    subprocess.run(["g++", "-O2", "-shared", "-fPIC", "-std=c++17", f"-I{ROOT / 'pyradiance_amd/csrc'}",
+                    str(ROOT / "tests/native/bvh_check.cpp"), str(ROOT / "pyradiance_amd/csrc/rg_bvh.cpp"),
+                    "-o", str(out)], check=True)
+    L = C.CDLL(str(out))
+    L.check_sah.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    return L
+
+
+@pytest.mark.parametrize("n,clustered", [(1, 0), (2, 0), (3, 0), (257, 0), (20000, 0), (20000, 1)])
+def test_sah_tree_is_complete_and_walks_like_brute_force(bvhcheck, n, clustered):
+    depth = C.c_int()
+    bad = bvhcheck.check_sah(n, 1000, 11, 60, clustered, C.byref(depth))
+    assert bad == 0, f"failed checks mask {bad:#x}"
+    assert depth.value <= 60
+
+
+def test_sah_depth_limit_forces_median_splits(bvhcheck):
+    # a tight limit still yields a complete, correct tree within it
+    depth = C.c_int()
+    bad = bvhcheck.check_sah(50000, 300, 3, 30, 0, C.byref(depth))
+    assert bad == 0, f"failed checks mask {bad:#x}"
+    assert depth.value <= 30

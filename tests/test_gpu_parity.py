@@ -23,9 +23,12 @@ def L():
     return _lib
 
 
-def _ctx(L, rad):
+def _ctx(L, rad, builder=None):
     c = L.Context(0)
-    c.load_text(rad).commit()
+    c.load_text(rad)
+    if builder is not None:
+        c.set_bvh_builder(builder)
+    c.commit()
     return c
 
 
@@ -78,8 +81,10 @@ def _mixed_scene():
             "m cup k2 0 0 8 2 -3 -2 2 -3 0 0.1 0.8\n")
 
 
-def test_closest_hit_matches_brute_force(L, oracle):
-    _say("closest hit, mixed primitives")
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
+def test_closest_hit_matches_brute_force(L, oracle, builder):
+    _say(f"closest hit, mixed primitives, {builder} tree")
+    kind = {"sah": L.BVH_SAH, "lbvh": L.BVH_LBVH}[builder]
     rad = _mixed_scene()
     rng = np.random.default_rng(7)
     n = 4000
@@ -92,7 +97,7 @@ def test_closest_hit_matches_brute_force(L, oracle):
     # per primitive would need bins; instead compare through rtrace -i, whose
     # value is non-zero exactly when a primitive is hit under a bright sky.
     sky = "void glow skyglow 0 0 4 1 1 1 0\nskyglow source sky 0 0 4 0 0 1 360\n"
-    c = _ctx(L, rad + sky)
+    c = _ctx(L, rad + sky, kind)
     p = L.default_params(False)
     c.set_params(p)
     out = c.trace(rays, 0)
@@ -213,6 +218,25 @@ def test_config2_full_size_properties(L):
     assert n1 > 2500 * 4096
 
 
+def test_sah_and_lbvh_trees_give_the_same_contributions(L):
+    """The two spatial indexes change only the traversal order: closest hits,
+    and so every sample, agree (up to coincident-surface ties)."""
+    _say("config2 subset, SAH vs LBVH")
+    rad, sens = scenes.config2(0)
+    sens = sens[::5]
+    outs, nrs = [], []
+    for kind in (L.BVH_SAH, L.BVH_LBVH):
+        c = _ctx(L, rad, kind)
+        c.set_params(_rc_params(L, ambounce=2, ambdiv=512, minweight=1e-3, seed=5))
+        outs.append(c.contrib(sens, _sky_mods(L), 1, L.RT_IMM_IRRAD))
+        nrs.append(c.last_nrays())
+        c.close()
+    assert abs(nrs[0] - nrs[1]) <= 1e-4 * nrs[0]
+    close = np.isclose(outs[0], outs[1], rtol=1e-6, atol=1e-12).all(1)
+    assert close.mean() >= 0.99
+    assert np.allclose(outs[0].sum(), outs[1].sum(), rtol=1e-3)
+
+
 def test_python_api_bytes(L, tmp_path):
     _say("python API")
     from pyradiance_amd import Rcontrib, rtrace
@@ -248,3 +272,43 @@ def test_errors_are_status_codes(L):
     c2.load_text("void mirror mm 0 0 3 1 1 1\nmm polygon p 0 0 9 0 0 0 1 0 0 0 1 0\n")
     with pytest.raises(RuntimeError, match="not supported"):
         c2.commit()
+
+
+def test_batched_scenes_match_individual_runs(L):
+    """Several scenes in one wavefront give each scene's standalone result
+    (scene k of a batch draws the samples of seed + k)."""
+    _say("batched scenes vs individual")
+    sc = scenes.config3(5, nscenes=4, prims=(600, 6000), sensors=(20, 200))
+    ctxs = [_ctx(L, rad) for rad, _ in sc]
+    p = _rc_params(L, ambounce=2, ambdiv=256, minweight=1e-3, seed=21)
+    ctxs[0].set_params(p)
+    mods = _sky_mods(L)
+    res = L.contrib_batch(ctxs, [s for _, s in sc], mods, 1, L.RT_IMM_IRRAD)
+    nb = ctxs[0].last_nrays()
+    total = 0
+    for k, (c, (_, sens)) in enumerate(zip(ctxs, sc)):
+        p.seed = 21 + k
+        c.set_params(p)
+        single = c.contrib(sens, mods, 1, L.RT_IMM_IRRAD)
+        total += c.last_nrays()
+        assert single.shape == res[k].shape
+        assert np.allclose(single, res[k], rtol=1e-9, atol=1e-13), k
+    assert nb == total
+
+
+def test_batched_rtrace_config1_sun_positions(L):
+    """Config 1 for many sun positions in one launch equals the reference
+    fixtures scene by scene (deterministic at -ab 0 -dj 0)."""
+    _say("batched rtrace, config-1 suns")
+    ctxs, rays = [], []
+    for seed in range(3):
+        rad, sens, _ = scenes.config1(seed)
+        ctxs.append(_ctx(L, rad))
+        rays.append(sens)
+    p = L.default_params(False)
+    p.ambounce, p.dstrsrc, p.srcsizerat = 0, 0.0, 0.0
+    ctxs[0].set_params(p)
+    res = L.trace_batch(ctxs, rays, L.RT_IMM_IRRAD)
+    for seed in range(3):
+        ref = np.load(GOLDEN / f"c1_seed{seed}.npz")["values"]
+        assert np.abs(res[seed] - ref).max() <= 1e-4 * ref.max()
