@@ -9,6 +9,11 @@ counted on the device.  Also reported: the DC x sky product
 and the reference rcontrib timed on the host cores over a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+                    [--no-batched] [--bvh auto|sah|lbvh]
+
+roofline.traffic (HBM bytes per k_trace launch) is read from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload,
+profiles/k_trace_config2_hbm_traffic.md, when it names the same BVH builder.
 """
 
 from __future__ import annotations
@@ -29,6 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 from pyradiance_amd import _lib, scenes  # noqa: E402
 
+METRIC = "rcontrib Mrays/s on 1 MI355X, plain vs batched, at 3\u03c3 match; s per grad step"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
 
@@ -113,6 +119,23 @@ def cpu_baseline(rad: str, sensors: np.ndarray, nsample: int, cores: int):
                       f"octree load {load:.2f}s (one-sensor run); rays from a -n 1 counting run on "
                       f"{nr_sample} sensors ({nrays_part} rays, {t1:.2f}s) scaled by {nsample / nr_sample:.1f}",
         }
+
+
+def committed_traffic(bvh: str):
+    """HBM bytes per k_trace launch from the committed PMC summary, if it was
+    measured with the same BVH builder (its machine-readable lines are
+    `bvh: <name>` and `traffic_bytes_per_launch: <bytes>`)."""
+    f = ROOT / "profiles" / "k_trace_config2_hbm_traffic.md"
+    if not f.exists():
+        return None
+    kv = {}
+    for line in f.read_text().splitlines():
+        if ":" in line and not line.startswith(("|", " ", "#")):
+            k, _, v = line.partition(":")
+            kv[k.strip()] = v.strip()
+    if kv.get("bvh") != bvh or "traffic_bytes_per_launch" not in kv:
+        return None
+    return float(kv["traffic_bytes_per_launch"])
 
 
 def dcsky_bench(dc: np.ndarray, seed: int = 0):
@@ -208,7 +231,104 @@ def batched_bench(nsun: int = 255, nscene2: int = 4):
                              "speedup": t_plain / t_batch}
     for c in ctxs:
         c.close()
+    res["config3"] = config3_bench()
     return res
+
+
+def config3_bench(prims=(1_000, 100_000)):
+    """64 uneven scenes (rooms 3-20 m, log-uniform primitive and sensor counts,
+    own sun each), rcontrib -ab 3 -ad 1024 -aa 0 with MF:1 + ground bins:
+    one call per scene vs one batched call over all scenes.  Primitive counts
+    are drawn from `prims` (the scene text for 500k-primitive rooms takes
+    longer to generate than the whole GPU run)."""
+    t0 = time.time()
+    sc = scenes.config3(0, prims=prims)
+    ctxs, rays = [], []
+    _, _, mods = workload(0)
+    p = _lib.default_params(True)
+    p.ambounce, p.ambdiv, p.seed = 3, 1024, 1
+    nprim = 0
+    for rad, sens in sc:
+        c = _lib.Context(0)
+        c.load_text(rad).commit()
+        c.set_params(p)
+        nprim += c.info()["primitives"]
+        ctxs.append(c)
+        rays.append(sens)
+    t_setup = time.time() - t0
+    for c, r in zip(ctxs[:2], rays[:2]):       # warm-up
+        c.contrib(r, mods, 1, _lib.RT_IMM_IRRAD)
+    t0 = time.perf_counter()
+    nr = 0
+    per_scene = []
+    for c, r in zip(ctxs, rays):
+        t1 = time.perf_counter()
+        c.contrib(r, mods, 1, _lib.RT_IMM_IRRAD)
+        dt = time.perf_counter() - t1
+        nr += c.last_nrays()
+        per_scene.append({"sensors": int(len(r)), "primitives": int(c.info()["primitives"]),
+                          "rays": int(c.last_nrays()), "Mrays_s": c.last_nrays() / dt / 1e6})
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _lib.contrib_batch(ctxs, rays, mods, 1, _lib.RT_IMM_IRRAD)
+    t_batch = time.perf_counter() - t0
+    nb = ctxs[0].last_nrays()
+    for c in ctxs:
+        c.close()
+    return {"scenes": len(sc), "primitives": nprim, "sensors": int(sum(len(r) for r in rays)),
+            "prims_range": list(prims), "setup_s": t_setup, "rays_plain": nr, "rays_batched": nb,
+            "plain_ms": t_plain * 1e3, "batched_ms": t_batch * 1e3,
+            "plain_Mrays_s": nr / t_plain / 1e6, "batched_Mrays_s": nb / t_batch / 1e6,
+            "speedup": t_plain / t_batch, "per_scene": per_scene}
+
+
+def grad_bench(steps: int = 3):
+    """Differentiable workload: config-1 room with 24 plastic groups, 1000
+    sensors, -ab 3 -ad 2048, binned sky; MSE against targets rendered at the
+    true arguments, from a perturbed start.  Times a forward pass and a
+    gradient step (forward + path-replay adjoint through torch.autograd)."""
+    import torch
+    from pyradiance_amd.adjoint import DiffScene, irradiance
+    theta_true = scenes.config4_theta(0)
+    rad, sens, groups = scenes.config4(theta_true)
+    _, _, mods = workload(0)
+    p = _lib.default_params(True)
+    p.ambounce, p.ambdiv, p.seed = 3, 2048, 1
+    sc = DiffScene(rad, sens, groups, mods, p)
+    rng = np.random.default_rng(4)
+    sky = torch.tensor(scenes.sky_matrix(0, 146, nhours=12)[:, 6], dtype=torch.float32, device="cuda")
+    with torch.no_grad():
+        target = irradiance(torch.tensor(theta_true, device="cuda"), sky, sc)
+    theta0 = np.clip(theta_true + rng.normal(0, 0.05, theta_true.shape), 0.0, None)
+    th = torch.tensor(theta0, device="cuda", requires_grad=True)
+
+    def step():
+        e = irradiance(th, sky, sc)
+        loss = ((e - target) ** 2).mean()
+        loss.backward()
+        return float(loss)
+
+    step()
+    th.grad = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        with torch.no_grad():
+            irradiance(th, sky, sc)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / steps
+    nrays = sc.ctx.last_nrays()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        th.grad = None
+        loss = step()
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / steps
+    gnorm = float(th.grad.norm())
+    sc.close()
+    return {"sensors": int(len(sens)), "groups": len(groups), "params": 24 * 5 + 146 * 3, "rays_forward": nrays,
+            "forward_s": t_fwd, "grad_step_s": t_grad, "grad_over_forward": t_grad / t_fwd, "loss": loss,
+            "grad_norm": gnorm}
 
 
 def main():
@@ -218,8 +338,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=480)
     ap.add_argument("--sensors", type=int, default=2500)
+    ap.add_argument("--bvh", choices=["auto", "sah", "lbvh"], default="auto")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -231,6 +353,7 @@ def main():
     rad, sensors, mods = workload(0)
     sensors = sensors[: args.sensors]
     ctx = _lib.Context(0)
+    ctx.set_bvh_builder({"auto": _lib.BVH_AUTO, "sah": _lib.BVH_SAH, "lbvh": _lib.BVH_LBVH}[args.bvh])
     t0 = time.time()
     ctx.load_text(rad).commit()
     t_commit = time.time() - t0
@@ -276,7 +399,14 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_bench()
-        log(f"batched: {batched}")
+        log(f"batched: { {k: v for k, v in batched.items() if k != 'config3'} }")
+        c3 = {k: v for k, v in batched["config3"].items() if k != "per_scene"}
+        log(f"config3: {c3}")
+
+    grad = None
+    if not args.no_grad:
+        grad = grad_bench()
+        log(f"grad step: {grad}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -286,8 +416,10 @@ def main():
         except Exception as e:  # the GPU figure stands on its own
             log(f"cpu baseline failed: {e}")
 
+    bvh_name = args.bvh if args.bvh != "auto" else ("sah" if info["primitives"] <= (4 << 20) else "lbvh")
+    traffic = committed_traffic(bvh_name)
     line = {
-        "metric": "rcontrib Mrays/s on 1 MI355X (config 2: -ab 4 -ad 4096, 2500 sensors, 50k triangles)",
+        "metric": METRIC,
         "value": value,
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -301,15 +433,17 @@ def main():
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": "rcontrib -I+ -ab 4 -ad 4096 -lw 1e-4 -aa 0 -c 1, Reinhart MF:1 + ground",
                    "sensors": int(len(sensors)), "primitives": int(info["primitives"]),
-                   "rays_per_step": int(np.mean(rays)), "scene_commit_ms": t_commit * 1e3},
+                   "rays_per_step": int(np.mean(rays)), "scene_commit_ms": t_commit * 1e3,
+                   "bvh": bvh_name},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_trace", "avg_launch_ms": avg_launch_ms,
                      "shade_kernel_ms_per_step": st_t["shade_ms"], "trace_kernel_ms_per_step": st_t["trace_ms"],
                      "bytes_per_ray": per_ray, "nodes_per_ray": st_c["nodes"] / nr,
                      "prims_per_ray": st_c["prims"] / nr},
         "dcsky": dcs,
         "batched": batched,
+        "grad_step": grad,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -99,6 +99,29 @@ int rg_contrib_batch(rg_ctx *const *ctxs, int nctx, const double *rays, const lo
 /* Batched rtrace: n[i] rays of scene i, concatenated; out in the same order. */
 int rg_trace_batch(rg_ctx *const *ctxs, int nctx, const double *rays, const long long *n, int flags, double *out);
 
+/* Materials of the committed scene (no reference counterpart: the reference
+ * changes a material by editing the scene file and rebuilding the octree).
+ * rg_material_index: index of the material object `name`.  rg_material_get:
+ * its type (0 plastic, 1 metal, 2 trans, ...) and real arguments.
+ * rg_material_set: replace the real arguments (same count) on the device. */
+int rg_material_index(rg_ctx *ctx, const char *name, int *index);
+int rg_material_get(rg_ctx *ctx, int index, int *kind, double *f, int n);
+int rg_material_set(rg_ctx *ctx, int index, const double *f, int n);
+
+/* Path-replay adjoint of rg_contrib (no reference counterpart).  For the
+ * loss whose derivative with respect to each record's binned irradiance
+ * E[rec][c] = sum_col out[rec][col][c] * lcol[col][c] is dlde[rec][c]
+ * (nrec x 3), adds into grad[i][0..4] (nmats x 5; zeroed here) the
+ * derivative with respect to plastic material mats[i]'s real arguments
+ * (red, green, blue reflectance, specularity, roughness), tracing the same
+ * rays with the same random numbers as rg_contrib.  lcol: 3 * sum of nbins
+ * doubles (radiance of every output column).  Sampling decisions are held
+ * fixed, so the result is the exact derivative of the estimator for these
+ * random numbers; roughness only moves sample directions and has none. */
+int rg_contrib_adjoint(rg_ctx *ctx, const double *rays, long long nrec, int accum, int flags,
+                       const rg_modifier *mods, int nmods, const double *dlde, const double *lcol,
+                       const int *mats, int nmats, double *grad);
+
 /* Ray segments traced by the last rg_trace / rg_contrib (closest-hit
  * queries, the reference's nrays of rt/raytrace.c:606). */
 int rg_last_nrays(rg_ctx *ctx, unsigned long long *nrays);

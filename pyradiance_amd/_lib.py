@@ -20,6 +20,7 @@ EXPORTS = (
     "rg_scene_commit", "rg_scene_info", "rg_set_params", "rg_default_params",
     "rg_default_params_rcontrib", "rg_trace", "rg_contrib", "rg_contrib_batch", "rg_trace_batch", "rg_last_nrays",
     "rg_set_profiling", "rg_set_bvh_builder", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
+    "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint",
 )
 
 BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
@@ -83,6 +84,11 @@ def lib() -> C.CDLL:
     L.rg_last_stats.argtypes = [vp, dp, C.c_int]
     L.rg_dcsky.argtypes = [fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.rg_dcsky_device.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.rg_material_index.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
+    L.rg_material_get.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.c_int]
+    L.rg_material_set.argtypes = [vp, C.c_int, dp, C.c_int]
+    L.rg_contrib_adjoint.argtypes = [vp, dp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, dp, dp,
+                                     C.POINTER(C.c_int), C.c_int, dp]
     _lib = L
     return L
 
@@ -166,6 +172,42 @@ class Context:
         out = np.zeros((nrec, ncol))
         self._check(self._L.rg_contrib(self._h, _dptr(rays), nrec, accum, flags, arr, len(mods), _dptr(out)))
         return out
+
+    def contrib_adjoint(self, rays: np.ndarray, mods: list, dlde: np.ndarray, lcol: np.ndarray, mats: list,
+                        accum: int = 1, flags: int = 0) -> np.ndarray:
+        """d(loss)/d(plastic arguments) of materials `mats` (indices), (len(mats), 5).
+
+        dlde: (nrec, 3) derivative of the loss with respect to each record's
+        E = sum over columns of contribution x lcol; lcol: (3 * sum nbins,)
+        radiance of every output column."""
+        rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
+        nrec = rays.shape[0] // accum
+        arr = (RgModifier * len(mods))(*mods)
+        ncol = 3 * sum(m.nbins for m in mods)
+        dlde = np.ascontiguousarray(dlde, dtype=np.float64).reshape(nrec, 3)
+        lcol = np.ascontiguousarray(lcol, dtype=np.float64).reshape(ncol)
+        mi = (C.c_int * max(len(mats), 1))(*[int(m) for m in mats])
+        grad = np.zeros((len(mats), 5))
+        self._check(self._L.rg_contrib_adjoint(self._h, _dptr(rays), nrec, accum, flags, arr, len(mods), _dptr(dlde),
+                                               _dptr(lcol), mi, len(mats), _dptr(grad)))
+        return grad
+
+    def material_index(self, name: str) -> int:
+        i = C.c_int()
+        self._check(self._L.rg_material_index(self._h, name.encode(), C.byref(i)))
+        return int(i.value)
+
+    def material_get(self, index: int) -> tuple:
+        """(kind, real arguments) of material `index`."""
+        k = C.c_int()
+        f = np.zeros(8)
+        self._check(self._L.rg_material_get(self._h, index, C.byref(k), _dptr(f), 8))
+        n = {0: 5, 1: 5, 2: 7}.get(int(k.value), 8)
+        return int(k.value), f[:n].copy()
+
+    def material_set(self, index: int, f) -> None:
+        f = np.ascontiguousarray(f, dtype=np.float64).ravel()
+        self._check(self._L.rg_material_set(self._h, index, _dptr(f), f.size))
 
     def set_profiling(self, level: int):
         self._check(self._L.rg_set_profiling(self._h, level))

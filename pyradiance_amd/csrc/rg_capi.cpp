@@ -141,10 +141,10 @@ int rg_scene_commit(rg_ctx* c) {
     c->cs = compile_scene(c->db);
     CompiledScene& cs = c->cs;
     DevScene& S = c->dscene;
-    S.prims = upload(c->scene_mem, "prims", cs.prims, c->st);
-    S.geom = upload(c->scene_mem, "geom", cs.geom, c->st);
-    S.mats = upload(c->scene_mem, "mats", cs.mats, c->st);
-    S.srcs = upload(c->scene_mem, "srcs", cs.srcs, c->st);
+    S.prims = to_global(upload(c->scene_mem, "prims", cs.prims, c->st));
+    S.geom = to_global(upload(c->scene_mem, "geom", cs.geom, c->st));
+    S.mats = to_global(upload(c->scene_mem, "mats", cs.mats, c->st));
+    S.srcs = to_global(upload(c->scene_mem, "srcs", cs.srcs, c->st));
     S.nprims = (int)cs.prims.size();
     S.nmats = (int)cs.mats.size();
     S.nsrcs = (int)cs.srcs.size();
@@ -187,7 +187,7 @@ int rg_scene_commit(rg_ctx* c) {
     float* dlo = upload(c->scene_mem, "blo", slo_v, c->st);
     float* dhi = upload(c->scene_mem, "bhi", shi_v, c->st);
     int32_t* dids = upload(c->scene_mem, "bids", small_ids, c->st);
-    S.big = upload(c->scene_mem, "big", big, c->st);
+    S.big = to_global(upload(c->scene_mem, "big", big, c->st));
     S.nbig = (int)big.size();
     S.nbvh = (int)n;
     BVHNode* nodes = (BVHNode*)c->scene_mem.get("nodes", sizeof(BVHNode) * std::max<uint32_t>(n, 1), c->st, true);
@@ -206,8 +206,8 @@ int rg_scene_commit(rg_ctx* c) {
     } else {
       S.root = build_lbvh(dlo, dhi, dids, n, slo, shi, nodes, leaf, c->arena, c->st);
     }
-    S.nodes = nodes;
-    S.leaf_prim = leaf;
+    S.nodes = to_global(nodes);
+    S.leaf_prim = to_global(leaf);
     hipck(hipStreamSynchronize(c->st), "scene commit");
     c->committed = true;
   });
@@ -243,9 +243,20 @@ int rg_set_params(rg_ctx* c, const rg_params* p) {
   });
 }
 
+// Adjoint replay request: loss weights per record, radiance per output
+// column, material -> gradient row map, host gradient [rows][5].
+struct AdjSpec {
+  const double* dlde;
+  const double* lcol;
+  std::vector<int32_t> row;
+  int nrows;
+  double* grad;
+};
+
 static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int flags, int mode,
                 const std::vector<BinSpec>& bins, int ncols, double* out,
-                const std::vector<DevScene>* batch = nullptr, const std::vector<uint32_t>* batch_start = nullptr) {
+                const std::vector<DevScene>* batch = nullptr, const std::vector<uint32_t>* batch_start = nullptr,
+                const AdjSpec* adj = nullptr) {
   if (!c->committed) throw std::runtime_error("scene not committed");
   if (nrays < 0 || nrays > 0x7fffffffLL) throw std::runtime_error("bad ray count");
   if (accum < 1) throw std::runtime_error("bad accumulation count");
@@ -288,6 +299,30 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
     cfg.scenes = d_sc;
     cfg.scene_start = d_ss;
   }
+  cfg.adj = 0;
+  cfg.adj_w = nullptr;
+  cfg.adj_L = nullptr;
+  cfg.grad_row = nullptr;
+  cfg.grad = nullptr;
+  double* d_grad = nullptr;
+  if (adj) {
+    std::vector<double> w(adj->dlde, adj->dlde + 3 * nrec);
+    for (double& v : w) v /= accum;        // the record is the mean of its rays
+    double* d_w = (double*)c->arena.get("adj_w", sizeof(double) * std::max<size_t>(w.size(), 1), c->st, false);
+    double* d_L = (double*)c->arena.get("adj_L", sizeof(double) * std::max(ncols, 1), c->st, false);
+    int32_t* d_row = (int32_t*)c->arena.get("adj_row", sizeof(int32_t) * adj->row.size(), c->st, false);
+    d_grad = (double*)c->arena.get("adj_grad", sizeof(double) * 5 * std::max(adj->nrows, 1), c->st, true);
+    if (!w.empty()) hipck(hipMemcpyAsync(d_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, c->st), "adj");
+    hipck(hipMemcpyAsync(d_L, adj->lcol, sizeof(double) * ncols, hipMemcpyHostToDevice, c->st), "adj");
+    hipck(hipMemcpyAsync(d_row, adj->row.data(), sizeof(int32_t) * adj->row.size(), hipMemcpyHostToDevice, c->st),
+          "adj");
+    hipck(hipStreamSynchronize(c->st), "adjoint upload");   // `w` is freed on return
+    cfg.adj = 1;
+    cfg.adj_w = d_w;
+    cfg.adj_L = d_L;
+    cfg.grad_row = d_row;
+    cfg.grad = d_grad;
+  }
   cfg.visits = nullptr;
   unsigned long long* d_vis = nullptr;
   if (c->profile >= 2) {
@@ -306,11 +341,17 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   uint32_t herr = 0;
   hipck(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c->st), "err");
   hipck(hipMemcpyAsync(&c->last_nrays, d_nr, 8, hipMemcpyDeviceToHost, c->st), "nrays");
-  if (outn) hipck(hipMemcpyAsync(out, d_out, sizeof(double) * outn, hipMemcpyDeviceToHost, c->st), "out");
+  if (outn && out) hipck(hipMemcpyAsync(out, d_out, sizeof(double) * outn, hipMemcpyDeviceToHost, c->st), "out");
+  if (adj)
+    hipck(hipMemcpyAsync(adj->grad, d_grad, sizeof(double) * 5 * adj->nrows, hipMemcpyDeviceToHost, c->st), "grad");
   hipck(hipStreamSynchronize(c->st), "render");
   if (herr & ERR_STACK) throw std::runtime_error("BVH traversal stack overflow");
   if (herr & ERR_CAPACITY) throw std::runtime_error("spawn buffer overflow");
   if (herr & ERR_NOMAT) throw std::runtime_error("surface without material");
+  if (herr & ERR_PATH) throw std::runtime_error("adjoint: more than 7 differentiable vertices on one path");
+  if (herr & ERR_ADJ_DIRECT)
+    throw std::runtime_error("adjoint: a tracked modifier is reached by shadow rays (direct light is not differentiated)");
+  if (!out) return;
   if (accum > 1) {
     double s = 1.0 / accum;
     for (size_t i = 0; i < outn; ++i) out[i] *= s;
@@ -376,8 +417,8 @@ static DevScene contrib_scene(rg_ctx* c, const std::unordered_map<std::string, i
     }
   }
   DevScene S = c->dscene;
-  S.prims = upload(c->arena, "prims_c", prims, st);
-  S.srcs = upload(c->arena, "srcs_c", srcs, st);
+  S.prims = to_global(upload(c->arena, "prims_c", prims, st));
+  S.srcs = to_global(upload(c->arena, "srcs_c", srcs, st));
   return S;
 }
 
@@ -397,6 +438,78 @@ int rg_contrib(rg_ctx* c, const double* rays, long long nrec, int accum, int fla
       throw;
     }
     c->dscene = saved;
+  });
+}
+
+int rg_contrib_adjoint(rg_ctx* c, const double* rays, long long nrec, int accum, int flags, const rg_modifier* mods,
+                       int nmods, const double* dlde, const double* lcol, const int* mats, int nmats, double* grad) {
+  RG_TRY(c, {
+    if (!c->committed) throw std::runtime_error("scene not committed");
+    if (nmats < 0 || (nmats && (!mats || !grad))) throw std::runtime_error("bad material list");
+    if (!dlde || !lcol) throw std::runtime_error("missing loss weights or column radiances");
+    std::unordered_map<std::string, int> slot;
+    int ncols = 0;
+    std::vector<BinSpec> bins = bin_specs(mods, nmods, slot, ncols);
+    AdjSpec adj;
+    adj.dlde = dlde;
+    adj.lcol = lcol;
+    adj.row.assign(c->cs.mats.size(), -1);
+    adj.nrows = nmats;
+    adj.grad = grad;
+    for (int i = 0; i < nmats; ++i) {
+      int mi = mats[i];
+      if (mi < 0 || mi >= (int)c->cs.mats.size() || mi == c->cs.lamb) throw std::runtime_error("bad material index");
+      if (c->cs.mats[mi].kind != MK_PLASTIC) throw std::runtime_error("adjoint: only plastic materials are differentiated");
+      if (mi > 0xfff) throw std::runtime_error("adjoint: material index beyond 4095");
+      if (adj.row[mi] >= 0) throw std::runtime_error("duplicate material in the gradient list");
+      adj.row[mi] = i;
+    }
+    DevScene S = contrib_scene(c, slot, c->st);
+    DevScene saved = c->dscene;
+    c->dscene = S;
+    try {
+      run(c, rays, nrec * (long long)accum, accum, flags, 1, bins, ncols, nullptr, nullptr, nullptr, &adj);
+    } catch (...) {
+      c->dscene = saved;
+      throw;
+    }
+    c->dscene = saved;
+  });
+}
+
+int rg_material_index(rg_ctx* c, const char* name, int* index) {
+  RG_TRY(c, {
+    if (!c->committed) throw std::runtime_error("scene not committed");
+    if (!name || !index) throw std::runtime_error("null argument");
+    *index = -1;
+    for (size_t i = 0; i < c->cs.mat_names.size(); ++i)
+      if ((int)i != c->cs.lamb && c->cs.mat_names[i] == name) { *index = (int)i; break; }
+    if (*index < 0) throw std::runtime_error(std::string("no material '") + name + "'");
+  });
+}
+
+int rg_material_get(rg_ctx* c, int index, int* kind, double* f, int n) {
+  RG_TRY(c, {
+    if (!c->committed) throw std::runtime_error("scene not committed");
+    if (index < 0 || index >= (int)c->cs.mats.size()) throw std::runtime_error("bad material index");
+    const Material& m = c->cs.mats[index];
+    if (kind) *kind = m.kind;
+    for (int i = 0; i < n && i < m.nfargs; ++i) f[i] = m.f[i];
+  });
+}
+
+int rg_material_set(rg_ctx* c, int index, const double* f, int n) {
+  RG_TRY(c, {
+    if (!c->committed) throw std::runtime_error("scene not committed");
+    if (index < 0 || index >= (int)c->cs.mats.size() || index == c->cs.lamb)
+      throw std::runtime_error("bad material index");
+    Material& m = c->cs.mats[index];
+    if (n != m.nfargs) throw std::runtime_error("material argument count differs");
+    for (int i = 0; i < n; ++i) m.f[i] = f[i];
+    hipck(hipSetDevice(c->device), "hipSetDevice");
+    hipck(hipMemcpyAsync((void*)c->dscene.mats, c->cs.mats.data(), sizeof(Material) * c->cs.mats.size(),
+                         hipMemcpyHostToDevice, c->st), "material upload");
+    hipck(hipStreamSynchronize(c->st), "material upload");
   });
 }
 

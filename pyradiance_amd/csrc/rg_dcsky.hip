@@ -7,7 +7,8 @@
 // FP32 as hi*hi + hi*lo + lo*hi, which leaves a relative error near 2^-16
 // per term instead of bf16's 2^-8.  v_mfma_f32_16x16x32_bf16 (gfx950) does
 // the products; both operands are first re-laid out K-contiguous per channel
-// so every lane fetches its 8 bf16 values with one 16-byte load.
+// so every lane fetches its 8 bf16 values with one 16-byte load, and the next
+// K step is fetched while the current one is on the matrix cores.
 // mode 0: plain FP32 FMA, tiled through LDS (accuracy baseline).
 #include <hip/hip_runtime.h>
 #include "rg_render.hpp"
@@ -63,81 +64,107 @@ __global__ void k_split_b(const float* B, int K, int N, int Kp, unsigned short* 
   }
 }
 
-constexpr int DB = 256;        // 4 wavefronts, 2 x 2
-constexpr int BM = 64, BN = 64;
+constexpr int DB = 256;        // 4 wavefronts, 2 (m) x 2 (n)
+#ifndef RG_DCSKY_WB
+#define RG_DCSKY_WB 4          // 16-column tiles per wavefront along n
+#endif
+constexpr int WA = 2;          // 16-row tiles per wavefront along m
+constexpr int WB = RG_DCSKY_WB;
+constexpr int BM = 32 * WA, BN = 32 * WB;   // block tile
+constexpr int GROUP_M = 8;     // m tiles walked together (L2 reuse of B)
 
 __device__ __forceinline__ bf16x8 ld8(const unsigned short* p) {
   u16x8 v = *(const u16x8*)p;
   return __builtin_bit_cast(bf16x8, v);
 }
 
+// Block -> tile.  Workgroups are dispatched round-robin over the 8 XCDs, so
+// block b runs on XCD b % 8; each XCD is given a contiguous range of tiles,
+// walked in groups of GROUP_M row tiles, so the operand rows it re-reads stay
+// in its own L2.
+__device__ __forceinline__ bool tile_of(int Mt, int Nt, int& tm, int& tn) {
+  const int T = Mt * Nt;
+  const int per = (T + 7) / 8;
+  const int b = blockIdx.x;
+  const int slot = b >> 3;
+  const int tile = (b & 7) * per + slot;
+  if (slot >= per || tile >= T) return false;
+  const int group = tile / (GROUP_M * Nt);
+  const int first = group * GROUP_M;
+  const int gsz = min(Mt - first, GROUP_M);
+  const int in = tile - group * GROUP_M * Nt;
+  tm = first + in % gsz;
+  tn = in / gsz;
+  return true;
+}
+
+// The MFMA computes D = X * Y with X the B fragment (rows = sky columns n)
+// and Y the A fragment (columns = sensors m), so each lane ends up holding 4
+// consecutive n of one m: with the three channels that is 12 consecutive
+// floats of C, written as three 16-byte stores.
 __global__ void __launch_bounds__(DB) k_dcsky_mfma(const unsigned short* Ah, const unsigned short* Al,
                                                    const unsigned short* Bh, const unsigned short* Bl, float* C,
-                                                   int M, int N, int Kp) {
+                                                   int M, int N, int Kp, int Mt, int Nt) {
+  int tm, tn;
+  if (!tile_of(Mt, Nt, tm, tn)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * BM + wm * 32, n0 = blockIdx.x * BN + wn * 32;
+  const int m0 = tm * BM + wm * 16 * WA, n0 = tn * BN + wn * 16 * WB;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
-  f32x4 acc[3][2][2];
+  f32x4 acc[3][WA][WB];
   for (int c = 0; c < 3; ++c)
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) acc[c][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int a = 0; a < WA; ++a)
+      for (int b = 0; b < WB; ++b) acc[c][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
   // clamp rows/cols so loads stay in bounds; their results are not stored
-  int mr[2], nr[2];
-  for (int a = 0; a < 2; ++a) {
-    mr[a] = min(m0 + 16 * a + r16, M - 1);
-    nr[a] = min(n0 + 16 * a + r16, N - 1);
-  }
+  size_t ao[WA], bo[WB];
+  for (int a = 0; a < WA; ++a) ao[a] = (size_t)min(m0 + 16 * a + r16, M - 1) * Kp + kq;
+  for (int b = 0; b < WB; ++b) bo[b] = (size_t)min(n0 + 16 * b + r16, N - 1) * Kp + kq;
   for (int c = 0; c < 3; ++c) {
     const unsigned short* ah = Ah + (size_t)c * M * Kp;
     const unsigned short* al = Al + (size_t)c * M * Kp;
     const unsigned short* bh = Bh + (size_t)c * N * Kp;
     const unsigned short* bl = Bl + (size_t)c * N * Kp;
+    bf16x8 fah[WA], fal[WA], fbh[WB], fbl[WB];
+    for (int a = 0; a < WA; ++a) { fah[a] = ld8(ah + ao[a]); fal[a] = ld8(al + ao[a]); }
+    for (int b = 0; b < WB; ++b) { fbh[b] = ld8(bh + bo[b]); fbl[b] = ld8(bl + bo[b]); }
     for (int k = 0; k < Kp; k += 32) {
-      bf16x8 fah[2], fal[2], fbh[2], fbl[2];
-      for (int a = 0; a < 2; ++a) {
-        fah[a] = ld8(ah + (size_t)mr[a] * Kp + k + kq);
-        fal[a] = ld8(al + (size_t)mr[a] * Kp + k + kq);
-        fbh[a] = ld8(bh + (size_t)nr[a] * Kp + k + kq);
-        fbl[a] = ld8(bl + (size_t)nr[a] * Kp + k + kq);
-      }
-      for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) {
-          acc[c][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbh[b], acc[c][a][b], 0, 0, 0);
-          acc[c][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbl[b], acc[c][a][b], 0, 0, 0);
-          acc[c][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fal[a], fbh[b], acc[c][a][b], 0, 0, 0);
+      // prefetch the next K step while this one is on the matrix cores
+      bf16x8 nah[WA], nal[WA], nbh[WB], nbl[WB];
+      const int kn = k + 32 < Kp ? k + 32 : k;
+      for (int a = 0; a < WA; ++a) { nah[a] = ld8(ah + ao[a] + kn); nal[a] = ld8(al + ao[a] + kn); }
+      for (int b = 0; b < WB; ++b) { nbh[b] = ld8(bh + bo[b] + kn); nbl[b] = ld8(bl + bo[b] + kn); }
+      for (int a = 0; a < WA; ++a)
+        for (int b = 0; b < WB; ++b) {
+          acc[c][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbh[b], fah[a], acc[c][a][b], 0, 0, 0);
+          acc[c][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbl[b], fah[a], acc[c][a][b], 0, 0, 0);
+          acc[c][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbh[b], fal[a], acc[c][a][b], 0, 0, 0);
         }
+      for (int a = 0; a < WA; ++a) { fah[a] = nah[a]; fal[a] = nal[a]; }
+      for (int b = 0; b < WB; ++b) { fbh[b] = nbh[b]; fbl[b] = nbl[b]; }
     }
   }
-  // D layout: lane holds column (lane & 15), rows 4*(lane>>4) + i.  The
-  // block's 64 x 64 x 3 tile is staged in LDS in C's [m][n][c] order, then
-  // written row by row with coalesced stores.
-  __shared__ float ct[BM][BN * 3 + 4];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      int nl = wn * 32 + 16 * b + r16;
-      for (int i = 0; i < 4; ++i) {
-        int ml = wm * 32 + 16 * a + (lane >> 4) * 4 + i;
-        ct[ml][3 * nl + 0] = acc[0][a][b][i];
-        ct[ml][3 * nl + 1] = acc[1][a][b][i];
-        ct[ml][3 * nl + 2] = acc[2][a][b][i];
+  // D layout: lane holds D[n = 4 * (lane >> 4) + i][m = lane & 15]
+  const bool vec = (N & 3) == 0;
+  for (int a = 0; a < WA; ++a) {
+    const int m = m0 + 16 * a + r16;
+    if (m >= M) continue;
+    for (int b = 0; b < WB; ++b) {
+      const int nb = n0 + 16 * b + (lane >> 4) * 4;
+      if (nb >= N) continue;
+      float* dst = C + ((size_t)m * N + nb) * 3;
+      const f32x4 r = acc[0][a][b], g = acc[1][a][b], u = acc[2][a][b];
+      if (vec) {   // nb + 3 < N as N is a multiple of 4; 16-byte aligned
+        float4* d4 = (float4*)dst;
+        d4[0] = make_float4(r[0], g[0], u[0], r[1]);
+        d4[1] = make_float4(g[1], u[1], r[2], g[2]);
+        d4[2] = make_float4(u[2], r[3], g[3], u[3]);
+      } else {
+        for (int i = 0; i < 4 && nb + i < N; ++i) {
+          dst[3 * i + 0] = r[i];
+          dst[3 * i + 1] = g[i];
+          dst[3 * i + 2] = u[i];
+        }
       }
-    }
-  __syncthreads();
-  const int mb = blockIdx.y * BM, nb = blockIdx.x * BN;
-  const int ncols = min(BN, N - nb) * 3;     // valid floats per tile row
-  const bool vec = (ncols == BN * 3) && (((size_t)N * 3) % 4 == 0);
-  for (int ml = wave; ml < BM; ml += DB / 64) {
-    int m = mb + ml;
-    if (m >= M) break;
-    float* row = C + ((size_t)m * N + nb) * 3;
-    if (vec) {
-      for (int e = lane; e < BN * 3 / 4; e += 64) {
-        float4 v = make_float4(ct[ml][4 * e], ct[ml][4 * e + 1], ct[ml][4 * e + 2], ct[ml][4 * e + 3]);
-        *(float4*)(row + 4 * e) = v;
-      }
-    } else {
-      for (int e = lane; e < ncols; e += 64) row[e] = ct[ml][e];
     }
   }
 }
@@ -193,8 +220,11 @@ void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mo
   unsigned short *Ah = buf, *Al = buf + na, *Bh = buf + 2 * na, *Bl = buf + 2 * na + nb;
   hipLaunchKernelGGL(k_split_a, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, st, A, M, K, Kp, Ah, Al);
   hipLaunchKernelGGL(k_split_b, dim3((N + 31) / 32, Kp / 32, 3), dim3(32, 8), 0, st, B, K, N, Kp, Bh, Bl);
-  hipLaunchKernelGGL(k_dcsky_mfma, dim3((N + BN - 1) / BN, (M + BM - 1) / BM), dim3(DB), 0, st, Ah, Al, Bh, Bl, C, M,
-                     N, Kp);
+  const int Mt = (M + BM - 1) / BM, Nt = (N + BN - 1) / BN;
+  const long long tiles = (long long)Mt * Nt;
+  if (tiles > 0x7fffffffLL - 8) throw std::runtime_error("dcsky: matrix too large");
+  const unsigned nblk = (unsigned)((tiles + 7) / 8 * 8);
+  hipLaunchKernelGGL(k_dcsky_mfma, dim3(nblk), dim3(DB), 0, st, Ah, Al, Bh, Bl, C, M, N, Kp, Mt, Nt);
   dck(hipFreeAsync(buf, st), "dcsky scratch free");
 }
 

@@ -21,13 +21,13 @@ struct alignas(16) BVHNode {
 };
 
 struct DevScene {
-  const Prim* prims;
-  const double* geom;
-  const Material* mats;
-  const Source* srcs;
-  const BVHNode* nodes;
-  const int32_t* leaf_prim;     // sorted leaf position -> primitive index
-  const int32_t* big;           // primitives kept out of the BVH, tested first
+  const RG_GLOBAL Prim* prims;
+  const RG_GLOBAL double* geom;
+  const RG_GLOBAL Material* mats;
+  const RG_GLOBAL Source* srcs;
+  const RG_GLOBAL BVHNode* nodes;
+  const RG_GLOBAL int32_t* leaf_prim;   // sorted leaf position -> primitive index
+  const RG_GLOBAL int32_t* big;         // primitives kept out of the BVH, tested first
   int32_t nbig;
   int32_t nbvh;                 // primitives inside the BVH
   int32_t nprims;

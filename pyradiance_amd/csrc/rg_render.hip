@@ -42,12 +42,19 @@ struct Sink {
   uint32_t* count;
   uint32_t cap;
   uint32_t* err;
+  PathRec* pbuf;       // adjoint replay: path record of each spawn (else null)
 };
 
-__device__ __forceinline__ void emit(const Sink& sk, const Spawn& s) {
+// Append a spawn; in adjoint replay also its path: the parent's plus `entry`
+// (0: the parent's vertex is not a differentiable one).
+__device__ __forceinline__ void emit(const Sink& sk, const Spawn& s, const RayS& r, uint32_t entry = 0) {
   uint32_t i = wave_alloc(sk.count);
-  if (i < sk.cap) sk.buf[i] = s;
-  else atomicOr(sk.err, ERR_CAPACITY);
+  if (i < sk.cap) {
+    sk.buf[i] = s;
+    if (sk.pbuf) sk.pbuf[i] = path_push(r.path, entry);
+  } else {
+    atomicOr(sk.err, ERR_CAPACITY);
+  }
 }
 
 __device__ __forceinline__ void deposit(const RunCfg& c, uint32_t sensor, int col, C3 v) {
@@ -67,6 +74,10 @@ __device__ __forceinline__ bool slab(const float* lo, const float* hi, float ox,
   float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
   tn = tnear;
   return (tnear <= tfar) & (tfar >= 0.0f) & (tnear <= tmax);
+}
+
+__device__ __forceinline__ uint64_t pack_entry(int32_t ref, float t) {
+  return ((uint64_t)__float_as_uint(t) << 32) | (uint32_t)ref;
 }
 
 // Stack: the top LSTACK entries of each lane live in LDS (column-major, one
@@ -90,8 +101,10 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
   };
   const float ix = inv(d.x), iy = inv(d.y), iz = inv(d.z);
   const int lane = threadIdx.x;
-  int32_t deep[PSTACK];
-  float deept[PSTACK];
+  // Deeper entries pack (ref, tnear) in 64 bits: one private store / load
+  // each, and a different access type from the LDS pair above, so the two
+  // paths are not merged into one flat access.
+  uint64_t deep[PSTACK];
   int sp = 0;
   int node = S.root;
   const BVHNode* __restrict__ nodes = S.nodes;
@@ -107,8 +120,14 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
       --sp;
       int32_t cand;
       float ct;
-      if (sp < LSTACK) { cand = stk[sp * BS + lane]; ct = stn[sp * BS + lane]; }
-      else { cand = deep[sp - LSTACK]; ct = deept[sp - LSTACK]; }
+      if (sp < LSTACK) {
+        cand = stk[sp * BS + lane];
+        ct = stn[sp * BS + lane];
+      } else {
+        uint64_t e = deep[sp - LSTACK];
+        cand = (int32_t)(uint32_t)e;
+        ct = __uint_as_float((uint32_t)(e >> 32));
+      }
       if (ct <= lim) return cand;
     }
     return DONE;
@@ -129,7 +148,7 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
         int32_t farc = lfirst ? N.right : N.left;
         float fart = lfirst ? tr : tl;
         if (sp < LSTACK) { stk[sp * BS + lane] = farc; stn[sp * BS + lane] = fart; }
-        else if (sp < LSTACK + PSTACK) { deep[sp - LSTACK] = farc; deept[sp - LSTACK] = fart; }
+        else if (sp < LSTACK + PSTACK) deep[sp - LSTACK] = pack_entry(farc, fart);
         else { atomicOr(err, ERR_STACK); node = DONE; break; }
         ++sp;
         node = lfirst ? N.left : N.right;
@@ -166,7 +185,7 @@ done:
 
 // ---- spawn helpers ----------------------------------------------------------
 // Child ray of type rt with own coefficient rc (nullptr: rayorigin(...,NULL)).
-__device__ Spawn make_child(const RayS& r, V3 org, uint16_t rt, const C3* rc, uint64_t slot, uint8_t flags) {
+__device__ __forceinline__ Spawn make_child(const RayS& r, V3 org, uint16_t rt, const C3* rc, uint64_t slot, uint8_t flags) {
   Spawn s;
   s.org[0] = org.x; s.org[1] = org.y; s.org[2] = org.z;
   s.d0[0] = s.d0[1] = s.d0[2] = 0.f;
@@ -203,7 +222,7 @@ __device__ __forceinline__ void set_dir(Spawn& s, V3 d) {
 
 // rayorigin() admission for a child built from a spawn record.
 // Returns false when the child is not traced.
-__device__ bool admit(const RunCfg& c, RayS& r, uint8_t flags) {
+__device__ __forceinline__ bool admit(const RunCfg& c, RayS& r, uint8_t flags) {
   if (!(r.w > 0.0f)) return (flags & SPF_FORCE) != 0;
   if (r.crtype & SHADOW) return true;
   const Params& p = c.p;
@@ -215,6 +234,7 @@ __device__ bool admit(const RunCfg& c, RayS& r, uint8_t flags) {
     r.coef = r.coef * sc;
     r.crc = (float)(r.crc * sc);
     r.w = (float)p.minweight;
+    if (c.adj) path_roulette(r.path);
     return true;
   }
   if (flags & SPF_FORCE) return true;
@@ -228,7 +248,7 @@ __device__ __forceinline__ int src_obj(const Source& s, int si) { return s.prim 
 
 // m_light() rules (rt/source.c:683-793).  Returns emission in *em, or sets
 // *pass for a transparent pass-through; false when the ray is discarded.
-__device__ bool light_rules(const DevScene& S, const RunCfg& c, const RayS& r, const Material& m, int hitobj,
+__device__ __forceinline__ bool light_rules(const DevScene& S, const RunCfg& c, const RayS& r, const Material& m, int hitobj,
                             double rot, double rod, C3* em, bool* pass) {
   *pass = false;
   auto distglow = [&](double dist) {
@@ -260,7 +280,7 @@ __device__ bool light_rules(const DevScene& S, const RunCfg& c, const RayS& r, c
 }
 
 // sourcehit() of rt/source.c:317-377 for a ray that left the scene.
-__device__ int source_hit(const DevScene& S, const RayS& r) {
+__device__ __forceinline__ int source_hit(const DevScene& S, const RayS& r) {
   int glowsrc = -1, transrc = -1;
   int first = 0, last = S.nsrcs - 1;
   if (r.rsrc >= 0) first = last = r.rsrc;
@@ -278,8 +298,77 @@ __device__ int source_hit(const DevScene& S, const RayS& r) {
   return glowsrc;
 }
 
+#define FRESNE(ci) (exp(-5.85 * (ci)) - 0.00202943064)
+constexpr double FRESTHRESH = 0.017999;
+
+// Path-replay adjoint of one binned deposit.  The deposit is
+//   contr_c = L-independent path coefficient; loss weight w_c = dLoss/dE_c * L_c * contr_c,
+// and contr_c is a product of the own coefficients of the path's vertices,
+// times minweight / weight at every Russian roulette survival.  For each
+// plastic vertex the derivative of log(own_c) with respect to (rho_r, rho_g,
+// rho_b, specularity) follows from how m_normal() forms it
+// (rt/normal.c:177-360): ambient a_c = rho_c * rdiff (+ rspec when the
+// specular part is folded in below -st), specular own = rspec, with
+// rdiff = 1 - rspec and rspec = spec + fest * (1 - spec) for a pure
+// specular surface.  A roulette survival divides by the largest channel of
+// each own coefficient since the previous reset, which subtracts that
+// channel's derivative.  Sampling decisions (strata counts, roulette, lobe
+// directions) are held fixed: the result is the exact derivative of the
+// estimator for the fixed random numbers, so roughness has none here.
+__device__ __forceinline__ void adjoint_deposit(const DevScene& S, const RunCfg& c, const RayS& r, int col, C3 contr) {
+  if (r.crtype & SHADOW) { atomicOr(c.err, ERR_ADJ_DIRECT); return; }
+  if (r.path.hdr & 256u) { atomicOr(c.err, ERR_PATH); return; }
+  const double* aw = c.adj_w + 3 * (size_t)r.sensor;
+  const double* L = c.adj_L + col;
+  const double w[3] = {aw[0] * L[0] * contr.r, aw[1] * L[1] * contr.g, aw[2] * L[2] * contr.b};
+  if (w[0] == 0.0 && w[1] == 0.0 && w[2] == 0.0) return;
+  const uint32_t n = r.path.hdr & 15u;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t e = r.path.e[i];
+    const int mi = (int)(e & 0xfffu);
+    const uint32_t kind = (e >> 12) & 7u;
+    const int row = c.grad_row[mi];
+    if (row < 0) continue;
+    const Material& M = S.mats[mi];
+    const double rho[3] = {M.f[0], M.f[1], M.f[2]};
+    const double sp = M.f[3];
+    const double fest = (double)(e >> 16) * (1.0 / 65535.0);
+    const double rspec = sp + fest * (1.0 - sp);
+    const double drs = 1.0 - fest;            // d rspec / d spec
+    const double rdiff = 1.0 - rspec;
+    double D[3][4] = {};                      // d log own_c / d(rho_r, rho_g, rho_b, spec)
+    double own[3];
+    for (int ch = 0; ch < 3; ++ch) {
+      if (kind == PV_SPEC) {
+        own[ch] = rspec;
+        D[ch][3] = rspec > 0 ? drs / rspec : 0.0;
+      } else {
+        double a = rho[ch] * rdiff + (kind == PV_AMB_RBLT ? rspec : 0.0);
+        own[ch] = a;
+        if (a > 0) {
+          D[ch][ch] = rdiff / a;
+          D[ch][3] = (kind == PV_AMB_RBLT ? (1.0 - rho[ch]) : -rho[ch]) * drs / a;
+        }
+      }
+    }
+    if (e & 0x8000u) {                        // roulette divided by the largest channel
+      int cm = 0;
+      if ((float)own[1] > (float)own[cm]) cm = 1;
+      if ((float)own[2] > (float)own[cm]) cm = 2;
+      double Dm[4] = {D[cm][0], D[cm][1], D[cm][2], D[cm][3]};
+      for (int ch = 0; ch < 3; ++ch)
+        for (int q = 0; q < 4; ++q) D[ch][q] -= Dm[q];
+    }
+    double* g = c.grad + 5 * (size_t)row;
+    for (int q = 0; q < 4; ++q) {
+      double v = w[0] * D[0][q] + w[1] * D[1][q] + w[2] * D[2][q];
+      if (v != 0.0) unsafeAtomicAdd(g + q, v);
+    }
+  }
+}
+
 // rcontrib's trace callback (rt/rcontrib.c:272-316) in coefficient mode.
-__device__ void trace_contrib(const DevScene& S, const RunCfg& c, const RayS& r, int modslot, int hitobj,
+__device__ __forceinline__ void trace_contrib(const DevScene& S, const RunCfg& c, const RayS& r, int modslot, int hitobj,
                               bool zeroed, C3 rcol) {
   if (c.mode != 1 || modslot < 0 || zeroed) return;
   if (r.rsrc >= 0 && src_obj(S.srcs[r.rsrc], r.rsrc) != hitobj) return;
@@ -293,7 +382,8 @@ __device__ void trace_contrib(const DevScene& S, const RunCfg& c, const RayS& r,
   const BinSpec& b = c.bins[modslot];
   int bn = eval_bin(r.d, b);
   if (bn < 0) return;
-  deposit(c, r.sensor, 3 * (b.col0 + bn), contr);
+  if (c.adj) adjoint_deposit(S, c, r, 3 * (b.col0 + bn), contr);
+  else deposit(c, r.sensor, 3 * (b.col0 + bn), contr);
 }
 
 // ---- materials ------------------------------------------------------------
@@ -305,8 +395,6 @@ struct HitS {
   int flat;
 };
 
-#define FRESNE(ci) (exp(-5.85 * (ci)) - 0.00202943064)
-constexpr double FRESTHRESH = 0.017999;
 enum { SP_REFL = 01, SP_TRAN = 02, SP_PURE = 04, SP_FLAT = 010, SP_RBLT = 020, SP_TBLT = 040 };
 
 struct NormDat {
@@ -317,7 +405,7 @@ struct NormDat {
 };
 
 // dirnorm() of rt/normal.c:72-173
-__device__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdir, V3 ldir, double omega) {
+__device__ __forceinline__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdir, V3 ldir, double omega) {
   C3 scval = c3(0, 0, 0);
   double ldot = dot(np.pnorm, ldir);
   if (ldot < 0.0 ? np.trans <= FTINY : np.trans >= 1.0 - FTINY) return scval;
@@ -351,7 +439,7 @@ __device__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdir, V3 ldir, doub
 // direct() of rt/source.c:398-557 with every source tested (no shadow
 // threshold / untested-source statistics): one shadow spawn per source with
 // non-zero potential.
-__device__ void direct_light(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, const NormDat& nd,
+__device__ __forceinline__ void direct_light(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, const NormDat& nd,
                              const Sink& sk) {
   for (int sn = 0; sn < S.nsrcs; ++sn) {
     const Source& src = S.srcs[sn];
@@ -441,14 +529,14 @@ __device__ void direct_light(const DevScene& S, const RunCfg& c, const RayS& r, 
     s.crc = (float)cmax(coef);
     s.rsrc = (int16_t)sn;
     set_dir(s, dir);
-    emit(sk, s);
+    emit(sk, s, r);
   }
 }
 
 // multambient() + samp_hemi() set-up (rt/ambient.c:229-330, rt/ambcomp.c:351-424)
 // for -aa 0: one hemisphere spawn of n*n stratified children.
-__device__ void ambient(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, C3 aval, V3 nrm,
-                        const Sink& sk) {
+__device__ __forceinline__ void ambient(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, C3 aval, V3 nrm,
+                        const Sink& sk, uint32_t entry = 0) {
   const Params& p = c.p;
   bool dumb = (p.ambdiv <= 0) || (r.adepth >= p.ambounce);
   double rdot = dot(nrm, hs.ron);
@@ -475,7 +563,7 @@ __device__ void ambient(const DevScene& S, const RunCfg& c, const RayS& r, const
       s.ns = (uint16_t)n;
       s.count = (uint32_t)n * (uint32_t)n;
       s.adepth = (uint8_t)(r.adepth + 1);
-      emit(sk, s);
+      emit(sk, s, r, entry);
       return;
     }
   }
@@ -487,8 +575,8 @@ __device__ void ambient(const DevScene& S, const RunCfg& c, const RayS& r, const
 }
 
 // gaussamp() of rt/normal.c:364-495 with one sample per component (-ss <= 1.5)
-__device__ void gauss_sample(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, const NormDat& np,
-                             const Sink& sk) {
+__device__ __forceinline__ void gauss_sample(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, const NormDat& np,
+                             const Sink& sk, uint32_t e_spec) {
   if ((np.specfl & (SP_REFL | SP_RBLT)) != SP_REFL && (np.specfl & (SP_TRAN | SP_TBLT)) != SP_TRAN) return;
   V3 u, v;
   tangent_frame(np.pnorm, rnd(r.key, RN_GFRAME), u, v);
@@ -511,7 +599,7 @@ __device__ void gauss_sample(const DevScene& S, const RunCfg& c, const RayS& r, 
       ok = true;
       break;
     }
-    if (ok) { set_dir(s, dir); emit(sk, s); }
+    if (ok) { set_dir(s, dir); emit(sk, s, r, e_spec); }
   }
   if ((np.specfl & (SP_TRAN | SP_TBLT)) == SP_TRAN) {
     C3 tc = np.mcolor * np.tspec;
@@ -530,19 +618,20 @@ __device__ void gauss_sample(const DevScene& S, const RunCfg& c, const RayS& r, 
       ok = true;
       break;
     }
-    if (ok) { set_dir(s, dir); emit(sk, s); }
+    if (ok) { set_dir(s, dir); emit(sk, s, r); }
   }
 }
 
 // m_normal() of rt/normal.c:177-360 (plastic, metal, trans)
-__device__ void shade_normal(const DevScene& S, const RunCfg& c, const RayS& r, HitS hs, const Material& m,
+// mi: material index of m for the adjoint path (-1: not differentiated).
+__device__ __forceinline__ void shade_normal(const DevScene& S, const RunCfg& c, const RayS& r, HitS hs, const Material& m, int mi,
                              const Sink& sk) {
   if ((r.crtype & SHADOW) && m.kind != MK_TRANS) return;
   if (hs.rod < 0.0) {
     if (!c.p.backvis) {
       Spawn s = make_child(r, hs.rop, TRANS, nullptr, SLOT_PASS, SPF_FORCE);
       set_dir(s, r.d);
-      emit(sk, s);
+      emit(sk, s, r);
       return;
     }
     hs.rod = -hs.rod;
@@ -580,7 +669,7 @@ __device__ void shade_normal(const DevScene& S, const RunCfg& c, const RayS& r, 
     C3 tc = nd.mcolor * nd.tspec;
     Spawn s = make_child(r, hs.rop, TRANS, &tc, SLOT_TRANS, 0);
     set_dir(s, nd.prdir);
-    emit(sk, s);
+    emit(sk, s, r);
   }
   if (r.crtype & SHADOW) return;
   nd.scolor = c3(0, 0, 0);
@@ -596,20 +685,24 @@ __device__ void shade_normal(const DevScene& S, const RunCfg& c, const RayS& r, 
     }
     if (!(nd.specfl & SP_PURE) && c.p.specthresh >= nd.rspec - FTINY) nd.specfl |= SP_RBLT;
   }
+  // adjoint path entries of this vertex (plastic only)
+  const bool dv = c.adj && mi >= 0 && m.kind == MK_PLASTIC;
+  const uint32_t e_spec = dv ? path_entry(mi, PV_SPEC, fest) : 0u;
   if ((nd.specfl & (SP_REFL | SP_PURE | SP_RBLT)) == (SP_REFL | SP_PURE)) {
     C3 sc = nd.scolor;
     Spawn s = make_child(r, hs.rop, REFLECTED, &sc, SLOT_REFL, 0);
     V3 dir = r.d + nd.pnorm * (2. * nd.pdot);
     normalize(dir);
     set_dir(s, dir);
-    emit(sk, s);
+    emit(sk, s, r, e_spec);
   }
   if ((nd.specfl & SP_PURE) && nd.rdiff <= FTINY && nd.tdiff <= FTINY) return;
-  if (!(nd.specfl & SP_PURE)) gauss_sample(S, c, r, hs, nd, sk);
+  if (!(nd.specfl & SP_PURE)) gauss_sample(S, c, r, hs, nd, sk, e_spec);
   if (nd.rdiff > FTINY) {
     C3 a = nd.mcolor * nd.rdiff;
     if (nd.specfl & SP_RBLT) a = a + nd.scolor;
-    ambient(S, c, r, hs, a, nd.pnorm, sk);
+    const uint32_t e_amb = dv ? path_entry(mi, (nd.specfl & SP_RBLT) ? PV_AMB_RBLT : PV_AMB, fest) : 0u;
+    ambient(S, c, r, hs, a, nd.pnorm, sk, e_amb);
   }
   if (nd.tdiff > FTINY) {
     C3 a = nd.mcolor * ((nd.specfl & SP_TBLT) ? nd.trans : nd.tdiff);
@@ -619,12 +712,12 @@ __device__ void shade_normal(const DevScene& S, const RunCfg& c, const RayS& r, 
 }
 
 // m_glass() of rt/glass.c:46-165
-__device__ void shade_glass(const RunCfg& c, const RayS& r, HitS hs, const Material& m, const Sink& sk) {
+__device__ __forceinline__ void shade_glass(const RunCfg& c, const RayS& r, HitS hs, const Material& m, const Sink& sk) {
   double rindex = m.nfargs == 4 ? m.f[3] : 1.52;
   if (!c.p.backvis && hs.rod <= 0.0) {
     Spawn s = make_child(r, hs.rop, TRANS, nullptr, SLOT_PASS, SPF_FORCE);
     set_dir(s, r.d);
-    emit(sk, s);
+    emit(sk, s, r);
     return;
   }
   double mc[3] = {m.f[0], m.f[1], m.f[2]};
@@ -653,7 +746,7 @@ __device__ void shade_glass(const RunCfg& c, const RayS& r, HitS hs, const Mater
     C3 tc = c3((float)ct[0], (float)ct[1], (float)ct[2]);
     Spawn s = make_child(r, hs.rop, TRANS, &tc, SLOT_TRANS, 0);
     set_dir(s, r.d);
-    emit(sk, s);
+    emit(sk, s, r);
   }
   if (r.crtype & SHADOW) return;
   double cr[3];
@@ -668,11 +761,11 @@ __device__ void shade_glass(const RunCfg& c, const RayS& r, HitS hs, const Mater
   V3 dir = r.d + pnorm * (2. * pdot);
   normalize(dir);
   set_dir(s, dir);
-  emit(sk, s);
+  emit(sk, s, r);
 }
 
 // rayshade() for a surface hit, followed by rcontrib's trace callback.
-__device__ void shade_hit(const DevScene& S, const RunCfg& c, const RayS& r, const HitRec& h, const Sink& sk) {
+__device__ __forceinline__ void shade_hit(const DevScene& S, const RunCfg& c, const RayS& r, const HitRec& h, const Sink& sk) {
   const Prim& P = S.prims[h.prim];
   HitS hs;
   hs.rot = h.t;
@@ -691,16 +784,16 @@ __device__ void shade_hit(const DevScene& S, const RunCfg& c, const RayS& r, con
     if (m.transp) {
       Spawn s = make_child(r, hs.rop, TRANS, nullptr, SLOT_PASS, SPF_FORCE);
       set_dir(s, r.d);
-      emit(sk, s);
+      emit(sk, s, r);
     } else {
-      shade_normal(S, c, r, hs, S.mats[S.lamb], sk);
+      shade_normal(S, c, r, hs, S.mats[S.lamb], -1, sk);
     }
   } else {
     switch (m.kind) {
       case MK_PLASTIC:
       case MK_METAL:
       case MK_TRANS:
-        shade_normal(S, c, r, hs, m, sk);
+        shade_normal(S, c, r, hs, m, P.mat, sk);
         break;
       case MK_GLASS:
         shade_glass(c, r, hs, m, sk);
@@ -714,11 +807,11 @@ __device__ void shade_hit(const DevScene& S, const RunCfg& c, const RayS& r, con
           if (m.kind == MK_ILLUM && m.illum_alt >= 0) {
             const Material& alt = S.mats[m.illum_alt];
             if (alt.kind == MK_GLASS) shade_glass(c, r, hs, alt, sk);
-            else if (alt.kind <= MK_TRANS) shade_normal(S, c, r, hs, alt, sk);
+            else if (alt.kind <= MK_TRANS) shade_normal(S, c, r, hs, alt, -1, sk);
           } else {
             Spawn s = make_child(r, hs.rop, TRANS, nullptr, SLOT_PASS, SPF_FORCE);
             set_dir(s, r.d);
-            emit(sk, s);
+            emit(sk, s, r);
           }
         } else {
           rcol = em;
@@ -731,7 +824,7 @@ __device__ void shade_hit(const DevScene& S, const RunCfg& c, const RayS& r, con
 }
 
 // A ray that hit nothing: distant sources (sourcehit + m_light).
-__device__ void shade_miss(const DevScene& S, const RunCfg& c, const RayS& r) {
+__device__ __forceinline__ void shade_miss(const DevScene& S, const RunCfg& c, const RayS& r) {
   int si = source_hit(S, r);
   if (si < 0) return;
   const Source& src = S.srcs[si];
@@ -749,8 +842,8 @@ __device__ void shade_miss(const DevScene& S, const RunCfg& c, const RayS& r) {
 
 // ---- kernels -----------------------------------------------------------------
 // Scene tables of a ray: the kernel argument, or its entry of a batched launch.
-__device__ __forceinline__ const DevScene& scene_of(const DevScene& S, const RunCfg& c, int scene) {
-  return c.nscenes ? c.scenes[scene] : S;
+__device__ __forceinline__ DevScene scene_of(const DevScene& Sarg, const RunCfg& c, int scene) {
+  return c.nscenes ? c.scenes[scene] : Sarg;
 }
 __device__ __forceinline__ int scene_of_ray(const RunCfg& c, uint32_t ray) {
   if (!c.nscenes) return 0;
@@ -764,10 +857,15 @@ __device__ __forceinline__ int scene_of_ray(const RunCfg& c, uint32_t ray) {
 
 // Generation 0 for -I: the pseudo-hit on a Lambertian surface at each sensor
 // (rt/rtrace.c:416-431, rt/rcontrib.c:321-338).
+// ADJ: adjoint replay (path records on, deposits into the gradient); the
+// forward instantiation compiles all of that away.
+template <bool ADJ>
 __global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene Sarg, RunCfg c, const double* rays, uint32_t nsens,
                                                       uint32_t s0, Sink sk) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nsens) return;
+  c.adj = ADJ;
+  if (!ADJ) sk.pbuf = nullptr;
   const int scene = scene_of_ray(c, s0 + i);
   const DevScene& S = scene_of(Sarg, c, scene);
   const double* ry = rays + 6 * (size_t)i;
@@ -786,6 +884,7 @@ __global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene Sarg, RunCfg c, c
   r.adepth = 0;
   r.scene = scene;
   r.key = sensor_key(c.p.seed + scene, s0 + i - (c.nscenes ? c.scene_start[scene] : 0));
+  if (ADJ) r.path = PathRec{};
   HitS hs;
   hs.rot = 1e-5;
   hs.rop = org + dir * 1e-4;
@@ -793,7 +892,7 @@ __global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene Sarg, RunCfg c, c
   hs.rod = 1.0;
   hs.prim = -1;
   hs.flat = 0;
-  shade_normal(S, c, r, hs, S.mats[S.lamb], sk);
+  shade_normal(S, c, r, hs, S.mats[S.lamb], -1, sk);
 }
 
 // Generation 0 for radiance rays: primary rays as explicit spawns.
@@ -831,7 +930,10 @@ __global__ void __launch_bounds__(BLOCK) k_counts(const Spawn* sp, uint32_t n, u
 }
 
 // Child ci of spawn record sp: origin, direction and rayorigin() admission.
-__device__ __forceinline__ bool build_ray(const RunCfg& c, const Spawn& sp, uint32_t ci, RayS& r) {
+// pp: the spawn's path record in adjoint replay (null otherwise).
+__device__ __forceinline__ bool build_ray(const RunCfg& c, const Spawn& sp, uint32_t ci, RayS& r,
+                                          const PathRec* pp = nullptr) {
+  if (c.adj) r.path = pp ? *pp : PathRec{};
   r.o = v3(sp.org[0], sp.org[1], sp.org[2]);
   r.coef = c3(sp.coef[0], sp.coef[1], sp.coef[2]);
   r.w = sp.cw;
@@ -883,12 +985,17 @@ struct alignas(16) TraceOut {
 };
 
 // One generation, part 1: children [c0, c0+nthreads) of `in` -> closest hits.
-__global__ void __launch_bounds__(TBLOCK) k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs,
-                                                  uint32_t nspawn, uint64_t c0, uint32_t nthreads, TraceOut* outp) {
+#ifndef RG_TRACE_WAVES
+#define RG_TRACE_WAVES 4     // waves per SIMD the register budget of k_trace targets
+#endif
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(RG_TRACE_WAVES)))
+k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t nspawn, uint64_t c0,
+        uint32_t nthreads, TraceOut* outp) {
   __shared__ int32_t stk[LSTACK * TBLOCK];
   __shared__ float stn[LSTACK * TBLOCK];
   uint32_t t = blockIdx.x * TBLOCK + threadIdx.x;
   if (t >= nthreads) return;
+  c.adj = 0;      // admission does not depend on the path record
   uint64_t gid = c0 + t;
   // The wavefront's children are consecutive: one lane searches the whole
   // table for the first child, the others only the few records after it
@@ -926,15 +1033,20 @@ __global__ void __launch_bounds__(TBLOCK) k_trace(DevScene Sarg, RunCfg c, const
 }
 
 // One generation, part 2: shade the hits and append the next spawn records.
-__global__ void __launch_bounds__(BLOCK) k_shade(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs,
-                                                 uint64_t c0, uint32_t nthreads, const TraceOut* tr, Sink sk) {
+// pin: path records of `in` (adjoint replay only).
+template <bool ADJ>
+__global__ void __launch_bounds__(BLOCK) k_shade(DevScene Sarg, RunCfg c, const Spawn* in, const PathRec* pin,
+                                                 const uint32_t* offs, uint64_t c0, uint32_t nthreads,
+                                                 const TraceOut* tr, Sink sk) {
   uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
   if (t >= nthreads) return;
+  c.adj = ADJ;
+  if (!ADJ) sk.pbuf = nullptr;
   const TraceOut o = tr[t];
   if (o.prim == -2) return;
   uint32_t si = o.spawn;
   RayS r;
-  build_ray(c, in[si], (uint32_t)(c0 + t - offs[si]), r);
+  build_ray(c, in[si], (uint32_t)(c0 + t - offs[si]), r, ADJ ? pin + si : nullptr);
   HitRec h;
   h.t = o.t;
   h.prim = o.prim;
@@ -1025,7 +1137,8 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   uint32_t* d_cnt = (uint32_t*)arena.get("ctr", 64, st, true);
   uint32_t* h_cnt = arena.host_u32(8);
 
-  struct Batch { Spawn* buf; uint32_t n; int slot; };
+  const bool adj = cfg.adj != 0;
+  struct Batch { Spawn* buf; PathRec* pbuf; uint32_t n; int slot; };
   std::vector<Batch> stack;
   int nslots = 0;
   std::vector<int> free_slots;
@@ -1036,6 +1149,10 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   auto slot_buf = [&](int s, size_t cap) -> Spawn* {
     return (Spawn*)arena.get("spawn" + std::to_string(s), sizeof(Spawn) * cap, st, false);
   };
+  // adjoint replay: one path record beside every spawn record
+  auto slot_path = [&](int s, size_t cap) -> PathRec* {
+    return adj ? (PathRec*)arena.get("path" + std::to_string(s), sizeof(PathRec) * cap, st, false) : nullptr;
+  };
   // Buffers are sized to the work at hand (and kept by the arena for later
   // calls), so a context that only ever traces a few sensors stays small.
   size_t batch_cap = (size_t)std::max<uint32_t>(nsens, 1u) * per_child;
@@ -1044,17 +1161,20 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   {
     int s = new_slot();
     Spawn* b = slot_buf(s, batch_cap);
+    PathRec* pb = slot_path(s, batch_cap);
     check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
     uint32_t grid = (nsens + BLOCK - 1) / BLOCK;
     if (imm_irrad) {
-      Sink sk{b, d_cnt, (uint32_t)batch_cap, cfg.err};
-      hipLaunchKernelGGL(k_irrad_seed, dim3(grid), dim3(BLOCK), 0, st, S, cfg, d_rays, nsens, 0u, sk);
+      Sink sk{b, d_cnt, (uint32_t)batch_cap, cfg.err, pb};
+      if (adj) hipLaunchKernelGGL(k_irrad_seed<true>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, d_rays, nsens, 0u, sk);
+      else hipLaunchKernelGGL(k_irrad_seed<false>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, d_rays, nsens, 0u, sk);
       check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
       check(hipStreamSynchronize(st), "seed");
-      stack.push_back({b, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s});
+      stack.push_back({b, pb, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s});
     } else {
       hipLaunchKernelGGL(k_primary_seed, dim3(grid), dim3(BLOCK), 0, st, cfg, d_rays, nsens, 0u, b);
-      stack.push_back({b, nsens, s});
+      if (pb) check(hipMemsetAsync(pb, 0, sizeof(PathRec) * nsens, st), "memset");   // empty paths
+      stack.push_back({b, pb, nsens, s});
     }
   }
   long gens = 0;
@@ -1082,16 +1202,21 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       size_t cap = (size_t)nth * per_child;
       int s = new_slot();
       Spawn* ob = slot_buf(s, cap);
+      PathRec* opb = slot_path(s, cap);
       TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * nth, st, false);
       check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
-      Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err};
+      Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err, opb};
       if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
       hipLaunchKernelGGL(k_trace, dim3((nth + TBLOCK - 1) / TBLOCK), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n,
                          c0, nth, tout);
       check(hipGetLastError(), "k_trace launch");
       if (stats && stats->timing) check(hipEventRecord(ev1, st), "event");
-      hipLaunchKernelGGL(k_shade, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf, offs, c0, nth,
-                         tout, sk);
+      if (adj)
+        hipLaunchKernelGGL(k_shade<true>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
+                           bt.pbuf, offs, c0, nth, tout, sk);
+      else
+        hipLaunchKernelGGL(k_shade<false>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
+                           bt.pbuf, offs, c0, nth, tout, sk);
       check(hipGetLastError(), "k_shade launch");
       if (stats && stats->timing) check(hipEventRecord(ev2, st), "event");
       check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
@@ -1105,7 +1230,7 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       }
       if (stats) { stats->launches++; stats->threads += nth; }
       uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)cap);
-      if (nout) produced.push_back({ob, nout, s});
+      if (nout) produced.push_back({ob, opb, nout, s});
       else free_slots.push_back(s);
     }
     free_slots.push_back(bt.slot);

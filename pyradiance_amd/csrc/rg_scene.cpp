@@ -114,6 +114,7 @@ CompiledScene compile_scene(const SceneDB& db) {
     M.transp = (kind == MK_TRANS || kind == MK_GLASS) ? 1 : 0;
     int idx = (int)cs.mats.size();
     cs.mats.push_back(M);
+    cs.mat_names.push_back(m.name);
     matidx[mo] = idx;
     if (kind == MK_ILLUM && !m.sargs.empty() && m.sargs[0] != "void") {
       int alt = db.modifier(m.sargs[0]);
@@ -427,6 +428,7 @@ CompiledScene compile_scene(const SceneDB& db) {
   L.illum_alt = -1;
   cs.lamb = (int)cs.mats.size();
   cs.mats.push_back(L);
+  cs.mat_names.push_back("");          // not addressable by name
   return cs;
 }
 

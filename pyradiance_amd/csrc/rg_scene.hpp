@@ -11,6 +11,7 @@ struct CompiledScene {
   std::vector<Prim> prims;
   std::vector<double> geom;
   std::vector<Material> mats;
+  std::vector<std::string> mat_names;        // name of each material object
   std::vector<Source> srcs;
   std::vector<float> lo, hi;                 // widened FP32 bounds per primitive
   std::vector<std::string> prim_modname;     // name of each primitive's modifier

@@ -34,6 +34,42 @@ static_assert(sizeof(Spawn) == 96, "Spawn layout");
 
 enum : uint8_t { SPF_RR = 1, SPF_FORCE = 2, SPF_BACK = 4 };
 
+// Adjoint bookkeeping (path replay): the differentiable vertices a ray's
+// coefficient was formed at.  Each entry is one plastic vertex whose own
+// coefficient fed the ray:
+//   bits 0-11  material index          bits 12-14 kind (PV_*)
+//   bit 15     Russian roulette after this vertex rescaled by 1/weight
+//   bits 16-31 Fresnel estimate fest of a pure-specular vertex, x 65535
+// hdr: bits 0-3 entry count, 4-7 first entry since the last roulette reset,
+// bit 8 overflow (more than PATH_VERTS differentiable vertices).
+constexpr int PATH_VERTS = 7;
+enum : uint32_t { PV_AMB = 1, PV_AMB_RBLT = 2, PV_SPEC = 3 };
+struct alignas(16) PathRec {
+  uint32_t e[PATH_VERTS];
+  uint32_t hdr;
+};
+static_assert(sizeof(PathRec) == 32, "PathRec layout");
+
+RG_HD uint32_t path_entry(int mat, uint32_t kind, double fest) {
+  uint32_t q = (uint32_t)(fest <= 0 ? 0.0 : (fest >= 1 ? 65535.0 : fest * 65535.0 + 0.5));
+  return (uint32_t)(mat & 0xfff) | (kind << 12) | (q << 16);
+}
+RG_HD PathRec path_push(PathRec p, uint32_t entry) {
+  if (!entry) return p;
+  uint32_t n = p.hdr & 15u;
+  if (n >= (uint32_t)PATH_VERTS) { p.hdr |= 256u; return p; }
+  p.e[n] = entry;
+  p.hdr = (p.hdr & ~15u) | (n + 1);
+  return p;
+}
+// A roulette survivor's coefficient was divided by its weight, the product
+// of the own-coefficient maxima since the last reset: flag those vertices.
+RG_HD void path_roulette(PathRec& p) {
+  uint32_t n = p.hdr & 15u, seg = (p.hdr >> 4) & 15u;
+  for (uint32_t i = seg; i < n; ++i) p.e[i] |= 0x8000u;
+  p.hdr = (p.hdr & ~0xf0u) | (n << 4);
+}
+
 // State of one traced ray, reconstructed from its spawn record.
 struct RayS {
   V3 o, d;
@@ -47,6 +83,7 @@ struct RayS {
   int adepth;
   int scene;
   uint64_t key;
+  PathRec path;    // adjoint replay only
 };
 
 struct RunCfg {
@@ -64,9 +101,16 @@ struct RunCfg {
   unsigned long long* nrays;
   unsigned long long* visits;  // optional: [0] BVH nodes visited, [1] primitives tested
   uint32_t* err;       // error flags
+  // adjoint replay (adj != 0): instead of writing `out`, each binned
+  // deposit adds d(loss)/d(material parameter) into grad
+  int32_t adj;
+  const double* adj_w;         // [nsensor][3] d(loss)/d(E), already / accum
+  const double* adj_L;         // [ncols] radiance of each output column
+  const int32_t* grad_row;     // material index -> grad row (-1: not differentiated)
+  double* grad;                // [rows][5]: rho_r, rho_g, rho_b, specularity, roughness
 };
 
-enum : uint32_t { ERR_STACK = 1, ERR_CAPACITY = 2, ERR_NOMAT = 4, ERR_GEN = 8 };
+enum : uint32_t { ERR_STACK = 1, ERR_CAPACITY = 2, ERR_NOMAT = 4, ERR_GEN = 8, ERR_PATH = 16, ERR_ADJ_DIRECT = 32 };
 
 // ---- bins (util/reinhartb.cal:19-47, util/klems_full.cal:10-43) ------------
 RG_HD double cal_asin_deg(double x) {   // Asin() of reinhartb.cal

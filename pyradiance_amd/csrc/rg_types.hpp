@@ -15,6 +15,22 @@
 
 namespace rg {
 
+// Scene tables are reached through pointers held in structs (one DevScene per
+// scene of a batch, read from device memory).  Device code types those
+// pointers as global (address space 1) so every load through them is a
+// global load rather than a flat one (flat loads are slower and also count
+// against the LDS counter).  Host code sees plain pointers; the layout is the
+// same 64-bit address either way.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RG_GLOBAL __attribute__((address_space(1)))
+#else
+#define RG_GLOBAL
+#endif
+template <class T>
+inline const RG_GLOBAL T* to_global(const T* p) {
+  return (const RG_GLOBAL T*)p;
+}
+
 constexpr double FTINY = 1e-6;   // common/fvect.h:17
 constexpr double FHUGE = 1e10;   // common/fvect.h:20
 constexpr double PI_D = 3.14159265358979323846;

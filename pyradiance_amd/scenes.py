@@ -304,3 +304,69 @@ def config3(seed: int = 0, nscenes: int = 64, prims=(1_000, 500_000), sensors=(1
         g[:, 1] = np.minimum(g[:, 1], room.depth - 0.05)
         res.append((rad, g))
     return res
+
+
+def _quad(p, u, v):
+    p, u, v = (np.asarray(x, float) for x in (p, u, v))
+    return [tuple(p), tuple(p + u), tuple(p + u + v), tuple(p + v)]
+
+
+def config4_groups(room: Room | None = None):
+    """The config-1 room surfaces split into 24 patches (name, vertices): floor,
+    ceiling and the north, west and east walls 2 x 2 each, the south wall as
+    the four rectangles around the window.  Every quad faces into the room."""
+    r = room or Room()
+    W, D, H = r.width, r.depth, r.height
+    walls = [((0, 0, 0), (W, 0, 0), (0, D, 0)),      # floor, +z
+             ((0, 0, H), (0, D, 0), (W, 0, 0)),      # ceiling, -z
+             ((0, D, 0), (W, 0, 0), (0, 0, H)),      # north, -y
+             ((0, 0, 0), (0, D, 0), (0, 0, H)),      # west, +x
+             ((W, 0, 0), (0, 0, H), (0, D, 0))]      # east, -x
+    quads = []
+    for p, u, v in walls:
+        p, u, v = (np.asarray(x, float) for x in (p, u, v))
+        for i in range(2):
+            for j in range(2):
+                quads.append(_quad(p + u * (i / 2) + v * (j / 2), u / 2, v / 2))
+    x0 = 0.5 * (W - r.win_w)
+    x1 = x0 + r.win_w
+    z0 = r.win_sill
+    z1 = z0 + r.win_h
+    for xa, xb, za, zb in [(0, x0, 0, H), (x1, W, 0, H), (x0, x1, 0, z0), (x0, x1, z1, H)]:
+        quads.append(_quad((xa, 0, za), (0, 0, zb - za), (xb - xa, 0, 0)))
+    return [(f"g{k:02d}", q) for k, q in enumerate(quads)]
+
+
+def config4_theta(seed: int = 0, ngroups: int = 24):
+    """(ngroups, 5) plastic arguments: RGB reflectance U(0.05, 0.9),
+    specularity U(0, 0.05), roughness U(0, 0.2)."""
+    rng = np.random.default_rng(seed)
+    th = np.empty((ngroups, 5))
+    th[:, :3] = rng.uniform(0.05, 0.9, (ngroups, 3))
+    th[:, 3] = rng.uniform(0.0, 0.05, ngroups)
+    th[:, 4] = rng.uniform(0.0, 0.2, ngroups)
+    return th
+
+
+def config4(theta: np.ndarray | None = None, seed: int = 0, nsensors: int = 1000):
+    """Differentiable workload: the config-1 room with 24 plastic material
+    groups, glass window, binned glow sky and ground (no sun), 1000 sensors
+    on a 40 x 25 lattice at 0.8 m facing +Z.
+    Returns (rad_text, sensors, group_names)."""
+    room = Room()
+    if theta is None:
+        theta = config4_theta(seed)
+    groups = config4_groups(room)
+    out = []
+    for (name, q), th in zip(groups, theta):
+        out.append(f"void plastic {name}\n0\n0\n5 {th[0]:.9g} {th[1]:.9g} {th[2]:.9g} {th[3]:.9g} {th[4]:.9g}\n")
+        out.append(_poly(name, f"{name}_surf", q))
+    x0 = 0.5 * (room.width - room.win_w)
+    z0 = room.win_sill
+    out.append(f"void glass window_glass\n0\n0\n3 {room.glass_tau} {room.glass_tau} {room.glass_tau}\n")
+    out.append(_poly("window_glass", "window", [(x0, 0, z0), (x0, 0, z0 + room.win_h),
+                                                (x0 + room.win_w, 0, z0 + room.win_h), (x0 + room.win_w, 0, z0)]))
+    out.append(sky_rad((1, 1, 1), ground=(1, 1, 1)))
+    sens = sensor_grid(40, 25, 0.075, 0.08, 0.15, 0.8)
+    sens[:, 1] = 0.08 + 0.16 * np.repeat(np.arange(25), 40)
+    return "".join(out), sens[:nsensors], [g for g, _ in groups]
