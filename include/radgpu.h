@@ -122,6 +122,12 @@ int rg_contrib_adjoint(rg_ctx *ctx, const double *rays, long long nrec, int accu
                        const rg_modifier *mods, int nmods, const double *dlde, const double *lcol,
                        const int *mats, int nmats, double *grad);
 
+/* First intersection of each ray, as rtrace -oLN prints it
+ * (rt/rtrace.c:758-804): t[i] distance (1e10 on a miss), nrm[i] the
+ * surface's unflipped normal (zero on a miss), prim[i] the primitive index in
+ * scene order (-1 on a miss).  Any output pointer may be NULL. */
+int rg_first_hit(rg_ctx *ctx, const double *rays, long long n, double *t, double *nrm, int *prim);
+
 /* Ray segments traced by the last rg_trace / rg_contrib (closest-hit
  * queries, the reference's nrays of rt/raytrace.c:606). */
 int rg_last_nrays(rg_ctx *ctx, unsigned long long *nrays);

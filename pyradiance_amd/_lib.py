@@ -20,7 +20,7 @@ EXPORTS = (
     "rg_scene_commit", "rg_scene_info", "rg_set_params", "rg_default_params",
     "rg_default_params_rcontrib", "rg_trace", "rg_contrib", "rg_contrib_batch", "rg_trace_batch", "rg_last_nrays",
     "rg_set_profiling", "rg_set_bvh_builder", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
-    "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint",
+    "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint", "rg_first_hit",
 )
 
 BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
@@ -84,6 +84,7 @@ def lib() -> C.CDLL:
     L.rg_last_stats.argtypes = [vp, dp, C.c_int]
     L.rg_dcsky.argtypes = [fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.rg_dcsky_device.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.rg_first_hit.argtypes = [vp, dp, C.c_longlong, dp, dp, C.POINTER(C.c_int)]
     L.rg_material_index.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
     L.rg_material_get.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.c_int]
     L.rg_material_set.argtypes = [vp, C.c_int, dp, C.c_int]
@@ -191,6 +192,17 @@ class Context:
         self._check(self._L.rg_contrib_adjoint(self._h, _dptr(rays), nrec, accum, flags, arr, len(mods), _dptr(dlde),
                                                _dptr(lcol), mi, len(mats), _dptr(grad)))
         return grad
+
+    def first_hit(self, rays: np.ndarray):
+        """(t, normal, prim) of each ray's first intersection (rtrace -oLN)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
+        n = rays.shape[0]
+        t = np.zeros(n)
+        nrm = np.zeros((n, 3))
+        prim = np.zeros(n, dtype=np.int32)
+        self._check(self._L.rg_first_hit(self._h, _dptr(rays), n, _dptr(t), _dptr(nrm),
+                                         prim.ctypes.data_as(C.POINTER(C.c_int))))
+        return t, nrm, prim
 
     def material_index(self, name: str) -> int:
         i = C.c_int()

@@ -566,6 +566,29 @@ int rg_trace_batch(rg_ctx* const* ctxs, int nctx, const double* rays, const long
   });
 }
 
+int rg_first_hit(rg_ctx* c, const double* rays, long long n, double* t, double* nrm, int* prim) {
+  RG_TRY(c, {
+    if (!c->committed) throw std::runtime_error("scene not committed");
+    if (n < 0 || n > 0x7fffffffLL) throw std::runtime_error("bad ray count");
+    if (n == 0) return 0;
+    hipck(hipSetDevice(c->device), "hipSetDevice");
+    double* d_rays = (double*)c->arena.get("fh_rays", sizeof(double) * 6 * n, c->st, false);
+    double* d_t = (double*)c->arena.get("fh_t", sizeof(double) * n, c->st, false);
+    double* d_n = (double*)c->arena.get("fh_n", sizeof(double) * 3 * n, c->st, false);
+    int32_t* d_p = (int32_t*)c->arena.get("fh_p", sizeof(int32_t) * n, c->st, false);
+    uint32_t* d_err = (uint32_t*)c->arena.get("err", 8, c->st, true);
+    hipck(hipMemcpyAsync(d_rays, rays, sizeof(double) * 6 * n, hipMemcpyHostToDevice, c->st), "rays");
+    first_hit(c->dscene, d_rays, (uint32_t)n, d_t, d_n, d_p, d_err, c->st);
+    uint32_t herr = 0;
+    hipck(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c->st), "err");
+    if (t) hipck(hipMemcpyAsync(t, d_t, sizeof(double) * n, hipMemcpyDeviceToHost, c->st), "t");
+    if (nrm) hipck(hipMemcpyAsync(nrm, d_n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->st), "normals");
+    if (prim) hipck(hipMemcpyAsync(prim, d_p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->st), "prims");
+    hipck(hipStreamSynchronize(c->st), "first hit");
+    if (herr & ERR_STACK) throw std::runtime_error("BVH traversal stack overflow");
+  });
+}
+
 int rg_last_nrays(rg_ctx* c, unsigned long long* n) {
   RG_TRY(c, { *n = c->last_nrays; });
 }

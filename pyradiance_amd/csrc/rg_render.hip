@@ -1032,6 +1032,39 @@ k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t
   outp[t] = o;
 }
 
+static void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// First intersection only (what rtrace -oLN prints: rt/rtrace.c:758-804):
+// distance (FHUGE on a miss), the primitive's unflipped normal, its index.
+__global__ void __launch_bounds__(TBLOCK) k_first_hit(DevScene S, const double* rays, uint32_t n, double* t_out,
+                                                      double* n_out, int32_t* p_out, uint32_t* err) {
+  __shared__ int32_t stk[LSTACK * TBLOCK];
+  __shared__ float stn[LSTACK * TBLOCK];
+  uint32_t i = blockIdx.x * TBLOCK + threadIdx.x;
+  if (i >= n) return;
+  V3 o = load3(rays + 6 * (size_t)i), d = load3(rays + 6 * (size_t)i + 3);
+  normalize(d);
+  HitRec h;
+  closest_hit<TBLOCK>(S, o, d, h, stk, stn, err, nullptr);
+  V3 nrm = v3(0, 0, 0);
+  if (h.prim >= 0) nrm = prim_normal(S, h.prim, o + d * h.t);
+  t_out[i] = h.prim >= 0 ? h.t : FHUGE;
+  n_out[3 * (size_t)i + 0] = nrm.x;
+  n_out[3 * (size_t)i + 1] = nrm.y;
+  n_out[3 * (size_t)i + 2] = nrm.z;
+  p_out[i] = h.prim;
+}
+
+void first_hit(const DevScene& S, const double* d_rays, uint32_t n, double* d_t, double* d_n, int32_t* d_p,
+               uint32_t* d_err, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_first_hit, dim3((n + TBLOCK - 1) / TBLOCK), dim3(TBLOCK), 0, st, S, d_rays, n, d_t, d_n, d_p,
+                     d_err);
+  check(hipGetLastError(), "k_first_hit launch");
+}
+
 // One generation, part 2: shade the hits and append the next spawn records.
 // pin: path records of `in` (adjoint replay only).
 template <bool ADJ>
@@ -1104,9 +1137,6 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(uint32_t* out, uint32_t
 // ---- host side -----------------------------------------------------------------
 namespace rg {
 
-static void check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
-}
 
 // out[i] = sum(in[0..i)), out[n] = total.  `out` holds n+1 entries.
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level) {

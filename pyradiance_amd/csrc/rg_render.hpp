@@ -80,6 +80,11 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& are
 void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad, int max_srcs,
             DevArena& arena, hipStream_t st, RenderStats* stats);
 
+// First intersection of n rays (n x 6 doubles): distance (FHUGE on a miss),
+// unflipped normal (n x 3), primitive index (-1 on a miss).
+void first_hit(const DevScene& S, const double* d_rays, uint32_t n, double* d_t, double* d_n, int32_t* d_p,
+               uint32_t* d_err, hipStream_t st);
+
 // BVH over the primitives' FP32 bounds (lo/hi: 3*n floats, already widened).
 // Fills nodes (n-1 entries) and leaf_prim (n entries); returns the root ref.
 // d_ids maps the n boxes to primitive indices written into leaf_prim.

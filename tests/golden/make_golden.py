@@ -8,8 +8,14 @@ Fixtures (inputs and the reference's outputs only):
   bins_reinhart.npz      sky directions and the rcontrib bin (MF 1, 2, 4)
   bins_klems.npz         directions and the klems_full.cal kbinS / kbinD bins
   stat_room.npz          48 repeats of 4 sensors, rcontrib -ab 2 -ad 256 bins
-The scenes themselves come from pyradiance_amd.scenes (seeded) and are not
-stored.
+  prims.npz              mixed primitives (sphere, bubble, polygons, ring, cylinder,
+                         tube, cone, cup): rays, rtrace -oLN distance + normal,
+                         -os surface name
+  shading.npz            local light sources (polygon, sphere, ring, proximity glow),
+                         metal, trans, glass, mirror-smooth metal: rtrace radiance
+                         and -I irradiance at -ab 0 -dj 0 -ds 0 -dt 0 -st 1
+The config-1/2 scenes come from pyradiance_amd.scenes (seeded) and are not
+stored; the two scenes above are stored as text with their vectors.
 """
 
 from __future__ import annotations
@@ -95,6 +101,101 @@ def bins():
     np.savez_compressed(HERE / "bins_klems.npz", **out)
 
 
+MIXED = ("void plastic m 0 0 5 .5 .5 .5 0 0\n"
+         "void glass g 0 0 3 .8 .8 .8\n"
+         "m sphere s1 0 0 4 0 0 0 1\n"
+         "m bubble s2 0 0 4 3 0 0 0.7\n"
+         "m polygon floor 0 0 12 -6 -6 -2 6 -6 -2 6 6 -2 -6 6 -2\n"
+         "m polygon tri 0 0 9 -3 -3 0 -1 -3 0 -2 -1 1\n"
+         "g polygon pane 0 0 12 -4 2 -1 -2 2 -1 -2 2 1 -4 2 1\n"
+         "m polygon notch 0 0 18 -5 -4 -1.5 -3 -4 -1.5 -3 -4 0.5 -4 -4 -0.5 -5 -4 0.5 -5 -4 -1.5\n"
+         "m ring r1 0 0 8 0 3 0 0 1 0 0.2 1.2\n"
+         "m ring disc 0 0 8 -3 3.5 1 0.3 0.2 0.9 0 0.8\n"
+         "m cylinder c1 0 0 7 -3 0 -2 -3 0 2 0.5\n"
+         "m tube t1 0 0 7 3 3 -2 3 3 2 0.6\n"
+         "m cone k1 0 0 8 0 -3 -2 0 -3 1 1.0 0.2\n"
+         "m cup k2 0 0 8 2 -3 -2 2 -3 0 0.1 0.8\n"
+         "m cone k3 0 0 8 -1 1 3 1 2 4 0 0.7\n"
+         "m cylinder c2 0 0 7 1 -1 3 3 -2 4.5 0.4\n")
+
+
+def primitives():
+    rng = np.random.default_rng(7)
+    n = 4000
+    org = rng.uniform(-5, 5, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = np.hstack([org, d])
+    with tempfile.TemporaryDirectory() as td:
+        octf = oconv(MIXED, td)
+        r = subprocess.run([str(REF / "rtrace"), "-h", "-ab", "0", "-fad", "-oLN", octf], input=rays_text(rays),
+                           capture_output=True, env=ENV, check=True)
+        ln = np.frombuffer(r.stdout, dtype=np.float64).reshape(n, 4)
+        r = subprocess.run([str(REF / "rtrace"), "-h", "-ab", "0", "-os", octf], input=rays_text(rays),
+                           capture_output=True, env=ENV, check=True)
+        names = r.stdout.decode().split()
+    assert len(names) == n
+    np.savez_compressed(HERE / "prims.npz", scene=np.array(MIXED), rays=rays, t=ln[:, 0], normal=ln[:, 1:],
+                        name=np.array(names))
+
+
+SHADING = ("void plastic floorm 0 0 5 .4 .35 .3 .03 .08\n"
+           "void plastic plain 0 0 5 .6 .6 .6 0 0\n"
+           "void metal brushed 0 0 5 .7 .6 .5 .9 .06\n"
+           "void metal mirrorm 0 0 5 .8 .8 .8 1 0\n"
+           "void trans frosted 0 0 7 .6 .7 .8 .04 .1 .5 .3\n"
+           "void trans clear 0 0 7 .8 .8 .8 .02 0 .6 .9\n"
+           "void glass gl 0 0 3 .85 .85 .85\n"
+           "void light lamp 0 0 3 80 80 80\n"
+           "void light warm 0 0 3 60 45 30\n"
+           "void glow gp 0 0 4 4 4 4 6\n"
+           "floorm polygon floor 0 0 12 -5 -5 0 5 -5 0 5 5 0 -5 5 0\n"
+           "lamp polygon panel 0 0 12 -.5 -.5 3 -.5 .5 3 .5 .5 3 .5 -.5 3\n"
+           "warm sphere bulb 0 0 4 2 1.5 2 .15\n"
+           "warm ring halo 0 0 8 -2 2 2.5 0 0 -1 0 .25\n"
+           "gp polygon glowpanel 0 0 12 4 -1 .5 4 -1 1.5 4 1 1.5 4 1 .5\n"
+           "brushed polygon metaltop 0 0 12 .5 -2 .6 1.5 -2 .6 1.5 -1 .6 .5 -1 .6\n"
+           "mirrorm polygon mirror 0 0 12 -1 -3 .2 1 -3 .2 1 -3 1.8 -1 -3 1.8\n"
+           "frosted polygon pane1 0 0 12 -2.5 -.5 .1 -2.5 .5 .1 -2.5 .5 1.4 -2.5 -.5 1.4\n"
+           "clear polygon pane2 0 0 12 2.5 -.5 .1 2.5 -.5 1.4 2.5 .5 1.4 2.5 .5 .1\n"
+           "gl polygon window 0 0 12 -1 2.5 .1 1 2.5 .1 1 2.5 1.6 -1 2.5 1.6\n"
+           "plain cylinder post 0 0 7 -1 -1 0 -1 -1 1.5 .2\n"
+           "plain cone shade 0 0 8 1.5 1 1.2 1.5 1 1.6 .35 .15\n"
+           "plain sphere ball 0 0 4 -1.2 1.2 .4 .4\n")
+
+SHADING_ARGS = ["-ab", "0", "-dj", "0", "-ds", "0", "-dt", "0", "-dc", "1", "-dr", "0", "-st", "1", "-lr", "8",
+                "-lw", "1e-4", "-av", "0", "0", "0"]
+
+
+def shading():
+    rng = np.random.default_rng(11)
+    n = 1500
+    org = np.column_stack([rng.uniform(-3.5, 3.5, n), rng.uniform(-3.5, 3.5, n), rng.uniform(0.3, 2.6, n)])
+    d = rng.normal(size=(n, 3))
+    d[:, 2] -= 0.8                       # mostly downward: floor, tops, panes, mirror
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = np.hstack([org, d])
+    # irradiance sensors on the floor, the metal top and facing the panes
+    ns = 400
+    sens = np.column_stack([rng.uniform(-4, 4, ns), rng.uniform(-4, 4, ns), np.full(ns, 0.0)])
+    sdir = np.tile([0.0, 0.0, 1.0], (ns, 1))
+    side = rng.random(ns) < 0.25
+    sens[side, 2] = rng.uniform(0.2, 1.3, side.sum())
+    sdir[side] = rng.normal(size=(side.sum(), 3))
+    sdir /= np.linalg.norm(sdir, axis=1)[:, None]
+    sensors = np.hstack([sens, sdir])
+    with tempfile.TemporaryDirectory() as td:
+        octf = oconv(SHADING, td)
+        r = subprocess.run([str(REF / "rtrace"), "-h", "-fad", *SHADING_ARGS, octf], input=rays_text(rays),
+                           capture_output=True, env=ENV, check=True)
+        val = np.frombuffer(r.stdout, dtype=np.float64).reshape(n, 3)
+        r = subprocess.run([str(REF / "rtrace"), "-h", "-fad", "-I", *SHADING_ARGS, octf],
+                           input=rays_text(sensors), capture_output=True, env=ENV, check=True)
+        irr = np.frombuffer(r.stdout, dtype=np.float64).reshape(ns, 3)
+    np.savez_compressed(HERE / "shading.npz", scene=np.array(SHADING), args=np.array(SHADING_ARGS), rays=rays,
+                        values=val, sensors=sensors, irradiance=irr)
+
+
 def stat_room():
     rng = np.random.default_rng(1)
     room = scenes.Room()
@@ -117,4 +218,6 @@ if __name__ == "__main__":
     config1()
     bins()
     stat_room()
+    primitives()
+    shading()
     print("golden vectors written to", HERE)

@@ -109,6 +109,44 @@ def test_closest_hit_matches_brute_force(L, oracle, builder):
     assert np.allclose(out[escaped], 1.0)
 
 
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
+def test_first_hit_matches_reference(L, builder):
+    """rg_first_hit against rtrace -oLN / -os of the reference: distance,
+    unflipped normal and surface for sphere, bubble, polygons, rings,
+    cylinders, tubes, cones and cups."""
+    _say(f"first hit vs reference, {builder} tree")
+    g = np.load(GOLDEN / "prims.npz")
+    c = _ctx(L, str(g["scene"]), {"sah": L.BVH_SAH, "lbvh": L.BVH_LBVH}[builder])
+    t, nrm, prim = c.first_hit(g["rays"])
+    hit = g["t"] < 1e9
+    assert np.array_equal(prim >= 0, hit)
+    assert np.allclose(t[hit], g["t"][hit], rtol=1e-5, atol=0)
+    assert np.all(t[~hit] == 1e10)
+    assert np.allclose(nrm, g["normal"], rtol=0, atol=2e-6)
+    names = np.array([ln.split()[2] for ln in str(g["scene"]).splitlines() if ln.split()[1] not in
+                      ("plastic", "glass")])
+    assert np.array_equal(np.where(prim >= 0, names[np.maximum(prim, 0)], "*"), g["name"])
+    c.close()
+
+
+def test_local_sources_metal_trans_glass_match_reference(L):
+    """Deterministic shading against the reference rtrace: polygon, sphere
+    and ring light sources, a proximity glow, metal, trans, glass and
+    mirror-smooth metal (-ab 0 -dj 0 -ds 0 -dt 0 -st 1), radiance and -I."""
+    _say("shading vs reference")
+    g = np.load(GOLDEN / "shading.npz")
+    c = _ctx(L, str(g["scene"]))
+    p = L.default_params(False)
+    p.ambounce, p.dstrsrc, p.srcsizerat, p.specthresh, p.maxdepth, p.minweight = 0, 0.0, 0.0, 1.0, 8, 1e-4
+    p.ambval[:] = (0.0, 0.0, 0.0)
+    c.set_params(p)
+    v = c.trace(g["rays"], 0)
+    assert np.allclose(v, g["values"], rtol=1e-5, atol=1e-9)
+    e = c.trace(g["sensors"], L.RT_IMM_IRRAD)
+    assert np.allclose(e, g["irradiance"], rtol=1e-5, atol=1e-9)
+    c.close()
+
+
 def test_rcontrib_matches_oracle_same_samples(L, oracle):
     _say("rcontrib -ab 2 vs oracle")
     rng = np.random.default_rng(1)

@@ -164,3 +164,37 @@ def test_rng_known_answers(oracle):
     # uniformity over many keys
     u = np.array([L.ro_rnd(L.ro_child_key(k, j), 0) for j in range(20000)])
     assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.005
+
+
+def _shading_params(oracle):
+    # -ab 0 -dj 0 -ds 0 -dt 0 -st 1 -lr 8 -lw 1e-4 -av 0 0 0 (the fixture's arguments)
+    return oracle.params(ambounce=0, dstrsrc=0.0, srcsizerat=0.0, specthresh=1.0, maxdepth=8, minweight=1e-4)
+
+
+def test_primitive_intersections_match_reference(oracle):
+    """Sphere, bubble, polygons (convex, concave), rings, cylinders, tubes,
+    cones and cups: first-hit distances of 4000 rays against rtrace -oL."""
+    g = np.load(GOLDEN / "prims.npz")
+    prim, t = oracle.Scene(str(g["scene"])).closest(g["rays"])
+    hit = g["t"] < 1e9
+    assert np.array_equal(prim >= 0, hit)
+    # the reference intersects cones in a normalised frame: distances agree
+    # to a few 1e-6 relative at grazing incidence, to FP64 rounding elsewhere
+    assert np.allclose(t[hit], g["t"][hit], rtol=1e-5, atol=0)
+    names = np.array([ln.split()[2] for ln in str(g["scene"]).splitlines() if ln.split()[1] not in
+                      ("plastic", "glass")])
+    assert np.array_equal(np.where(prim >= 0, names[np.maximum(prim, 0)], "*"), g["name"])
+
+
+def test_local_sources_metal_trans_glass_match_reference(oracle):
+    """Polygon, sphere and ring light sources, a proximity glow, metal,
+    trans, glass and mirror-smooth metal: rtrace radiance and -I irradiance
+    at -ab 0 -dj 0 -ds 0 -dt 0 -st 1, against the reference."""
+    g = np.load(GOLDEN / "shading.npz")
+    sc = oracle.Scene(str(g["scene"]))
+    v, _ = sc.trace(g["rays"], _shading_params(oracle), 0)
+    assert np.allclose(v, g["values"], rtol=1e-5, atol=1e-9)
+    e, _ = sc.trace(g["sensors"], _shading_params(oracle), 8)
+    assert np.allclose(e, g["irradiance"], rtol=1e-5, atol=1e-9)
+    # the fixture exercises every emitter and material
+    assert (g["values"].sum(1) > 0).mean() > 0.5 and (g["irradiance"].sum(1) > 0).mean() > 0.8
