@@ -121,6 +121,42 @@ def cpu_baseline(rad: str, sensors: np.ndarray, nsample: int, cores: int):
         }
 
 
+def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, nsample: int = 20000):
+    """DC x sky product on device buffers: kernel time (HIP events on the
+    null stream, where the product runs), algorithmic and issued MFMA rates,
+    and the relative error of `nsample` random entries against an FP64 dot
+    product.  A is a synthetic coefficient matrix, B the synthetic sky."""
+    import torch
+    rng = np.random.default_rng(seed)
+    A = (rng.random((M, K, 3), dtype=np.float32) * np.float32(2e-3)).astype(np.float32)
+    B = scenes.sky_matrix(seed, K, nhours=N)
+    dA = torch.from_numpy(A).cuda()
+    dB = torch.from_numpy(B).cuda()
+    dC = torch.empty((M, N, 3), dtype=torch.float32, device="cuda")
+    _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, 1)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, 1)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    mi = rng.integers(0, M, nsample)
+    ni = rng.integers(0, N, nsample)
+    got = dC[torch.from_numpy(mi).cuda(), torch.from_numpy(ni).cuda()].cpu().numpy().astype(np.float64)
+    ref = np.einsum("skc,skc->sc", A[mi].astype(np.float64), B[:, ni].transpose(1, 0, 2).astype(np.float64))
+    nz = ref > 1e-6 * ref.max()
+    rel = np.abs(got - ref)[nz] / ref[nz]
+    flops = 2.0 * M * K * N * 3
+    byts = 12.0 * (M * K + K * N + M * N)
+    return {"M": M, "K": K, "N": N, "kernel_ms": ms, "algorithmic_TFLOPs": flops / ms / 1e9,
+            "issued_TFLOPs": 3 * flops / ms / 1e9, "mfma_util_issued": 3 * flops / ms / 1e9 / BF16_PEAK_TFLOPS,
+            "GBs": byts / ms / 1e6, "hbm_frac": byts / ms / 1e6 / HBM_PEAK_GBS,
+            "max_rel_err_vs_fp64": float(rel.max()), "mean_rel_err_vs_fp64": float(rel.mean()),
+            "sampled_entries": int(nz.sum())}
+
+
 def committed_traffic(bvh: str):
     """HBM bytes per k_trace launch from the committed PMC summary, if it was
     measured with the same BVH builder (its machine-readable lines are
@@ -296,7 +332,7 @@ def grad_bench(steps: int = 3):
     p.ambounce, p.ambdiv, p.seed = 3, 2048, 1
     sc = DiffScene(rad, sens, groups, mods, p)
     rng = np.random.default_rng(4)
-    sky = torch.tensor(scenes.sky_matrix(0, 146, nhours=12)[:, 6], dtype=torch.float32, device="cuda")
+    sky = torch.tensor(scenes.sky_matrix(0, 146, nhours=24)[:, 12], dtype=torch.float32, device="cuda")   # noon
     with torch.no_grad():
         target = irradiance(torch.tensor(theta_true, device="cuda"), sky, sc)
     theta0 = np.clip(theta_true + rng.normal(0, 0.05, theta_true.shape), 0.0, None)
@@ -306,7 +342,7 @@ def grad_bench(steps: int = 3):
         e = irradiance(th, sky, sc)
         loss = ((e - target) ** 2).mean()
         loss.backward()
-        return float(loss)
+        return float(loss.detach())
 
     step()
     th.grad = None
@@ -339,6 +375,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-config5", action="store_true", help="skip the 20000x2306x8760 DC x sky product")
     ap.add_argument("--cpu-sample", type=int, default=480)
     ap.add_argument("--sensors", type=int, default=2500)
     ap.add_argument("--bvh", choices=["auto", "sah", "lbvh"], default="auto")
@@ -394,7 +431,12 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     dcs = dcsky_bench(out)
+    dcs["device"] = dcsky_device_bench(2500, 146, 8760)
     log(f"dcsky: {dcs}")
+    dc5 = None
+    if not args.no_config5:
+        dc5 = dcsky_device_bench(20000, 2306, 8760)
+        log(f"dcsky config 5 size: {dc5}")
 
     batched = None
     if not args.no_batched:
@@ -442,6 +484,7 @@ def main():
                      "bytes_per_ray": per_ray, "nodes_per_ray": st_c["nodes"] / nr,
                      "prims_per_ray": st_c["prims"] / nr},
         "dcsky": dcs,
+        "dcsky_config5": dc5,
         "batched": batched,
         "grad_step": grad,
         "cpu_baseline": cpu,

@@ -7,6 +7,7 @@ library is missing or no HIP device is present, calls raise RuntimeError.
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
 import os
 from pathlib import Path
 
@@ -48,6 +49,26 @@ class RgModifier(C.Structure):
 _lib = None
 
 
+def _share_hip_runtime() -> None:
+    """Load the HIP runtime that PyTorch-ROCm ships, when PyTorch is
+    installed, before this library.  Both carry the soname libamdhip64.so.7,
+    but torch's libraries ask for "libamdhip64.so": loaded the other way
+    round, the process would map two runtimes and the second cannot use the
+    device ("No HIP GPUs are available").  Preloaded here, this library binds
+    to it by soname and torch's later import finds the same file.  torch
+    itself is not imported."""
+    if os.environ.get("RADGPU_HIP_RUNTIME", "") == "system":
+        return
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for d in spec.submodule_search_locations:
+        rt = Path(d) / "lib" / "libamdhip64.so"
+        if rt.exists():
+            C.CDLL(str(rt), mode=C.RTLD_GLOBAL)
+            return
+
+
 def lib() -> C.CDLL:
     """Load libradgpu.so once; raise if it is not built."""
     global _lib
@@ -55,6 +76,7 @@ def lib() -> C.CDLL:
         return _lib
     if not LIBPATH.exists():
         raise RuntimeError(f"{LIBPATH} is not built (run `python -m pyradiance_amd.build`)")
+    _share_hip_runtime()
     L = C.CDLL(str(LIBPATH))
     vp = C.c_void_p
     dp = C.POINTER(C.c_double)
@@ -298,3 +320,12 @@ def dcsky(A: np.ndarray, B: np.ndarray, mode: int = 1) -> np.ndarray:
     if L.rg_dcsky(A.ctypes.data_as(fp), B.ctypes.data_as(fp), Cm.ctypes.data_as(fp), M, K, N, mode) != 0:
         raise RuntimeError(L.rg_last_error(None).decode())
     return Cm
+
+
+def dcsky_device(A_ptr: int, B_ptr: int, C_ptr: int, M: int, K: int, N: int, mode: int = 1) -> None:
+    """The same product on device buffers (raw device addresses, e.g. a
+    torch tensor's data_ptr()): A (M,K,3), B (K,N,3), C (M,N,3) float32.
+    Runs on the null stream and returns when it has finished."""
+    L = lib()
+    if L.rg_dcsky_device(C.c_void_p(A_ptr), C.c_void_p(B_ptr), C.c_void_p(C_ptr), M, K, N, mode) != 0:
+        raise RuntimeError(L.rg_last_error(None).decode())

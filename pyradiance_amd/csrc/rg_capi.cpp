@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 
@@ -38,6 +39,20 @@ struct rg_ctx {
 };
 
 static thread_local std::string g_create_err;
+
+// Wavefront scratch (spawn windows, scans, trace records) is shared by all
+// contexts of a device: it is sized by the largest call, not multiplied by
+// the number of scenes a process keeps committed.  Calls on one device take
+// turns on it.  The arenas live until the process ends (never destroyed, so
+// no device memory is released after the runtime has shut down).
+static std::mutex g_render_mu;
+static std::unordered_map<int, DevArena*> g_render_arena;
+
+static DevArena& render_arena(int device) {
+  DevArena*& a = g_render_arena[device];
+  if (!a) a = new DevArena();
+  return *a;
+}
 
 #define RG_TRY(ctx, ...)                      \
   try {                                       \
@@ -334,8 +349,11 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   int max_srcs = c->dscene.nsrcs;
   if (batch)
     for (const DevScene& b : *batch) max_srcs = std::max(max_srcs, b.nsrcs);
-  if (nrays)
-    render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, c->arena, c->st, &c->last_stats);
+  if (nrays) {
+    std::lock_guard<std::mutex> lock(g_render_mu);
+    render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, render_arena(c->device), c->st,
+           &c->last_stats);
+  }
   if (d_vis) hipck(hipMemcpyAsync(c->last_visits, d_vis, 16, hipMemcpyDeviceToHost, c->st), "visits");
   else c->last_visits[0] = c->last_visits[1] = 0;
   uint32_t herr = 0;
