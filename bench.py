@@ -32,7 +32,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-from pyradiance_amd import _lib, scenes  # noqa: E402
+from pyradiance_amd import _lib, scenes, tune  # noqa: E402
 
 METRIC = "rcontrib Mrays/s on 1 MI355X, plain vs batched, at 3\u03c3 match; s per grad step"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
@@ -158,9 +158,11 @@ def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, nsa
 
 
 def committed_traffic(bvh: str):
-    """HBM bytes per k_trace launch from the committed PMC summary, if it was
-    measured with the same BVH builder (its machine-readable lines are
-    `bvh: <name>` and `traffic_bytes_per_launch: <bytes>`)."""
+    """HBM bytes per traced ray of the traversal kernel from the committed
+    PMC summary, if it was measured with the same BVH builder (its
+    machine-readable lines are `bvh: <name>` and
+    `traffic_bytes_per_ray: <bytes>`).  Per ray, so that it applies to any
+    generation window the launch shape uses."""
     f = ROOT / "profiles" / "k_trace_config2_hbm_traffic.md"
     if not f.exists():
         return None
@@ -169,9 +171,9 @@ def committed_traffic(bvh: str):
         if ":" in line and not line.startswith(("|", " ", "#")):
             k, _, v = line.partition(":")
             kv[k.strip()] = v.strip()
-    if kv.get("bvh") != bvh or "traffic_bytes_per_launch" not in kv:
+    if kv.get("bvh") != bvh or "traffic_bytes_per_ray" not in kv:
         return None
-    return float(kv["traffic_bytes_per_launch"])
+    return float(kv["traffic_bytes_per_ray"])
 
 
 def dcsky_bench(dc: np.ndarray, seed: int = 0):
@@ -375,6 +377,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--shape", default="", metavar="K=V,...",
+                    help="launch shape to use instead of autotuning, e.g. trace_kernel=1,refill=16")
     ap.add_argument("--no-config5", action="store_true", help="skip the 20000x2306x8760 DC x sky product")
     ap.add_argument("--cpu-sample", type=int, default=480)
     ap.add_argument("--sensors", type=int, default=2500)
@@ -398,6 +403,17 @@ def main():
     log(f"scene: {info}, commit+BVH {t_commit * 1e3:.1f} ms")
     ctx.set_params(gpu_params())
     flags = _lib.RT_IMM_IRRAD
+
+    tuned = None
+    if args.shape:
+        shape = {k: int(v) for k, v in (kv.split("=") for kv in args.shape.split(","))}
+        ctx.set_tuning(**shape)
+        tuned = {"best": dict(tune.DEFAULT, **shape), "ms": None, "tried": [(None, None)]}
+    elif not args.no_autotune:
+        tuned = tune.autotune(ctx, lambda: ctx.contrib(sensors, mods, 1, flags))
+        top = sorted(tuned["tried"], key=lambda x: x[1])[:5]
+        log(f"autotune: best {tuned['best']} {tuned['ms']:.1f} ms; per-thread kernel "
+            f"{tuned['tried'][0][1]:.1f} ms; top {[(t, round(m, 1)) for t, m in top]}")
 
     for i in range(args.warmup):
         ctx.contrib(sensors, mods, 1, flags)
@@ -459,7 +475,9 @@ def main():
             log(f"cpu baseline failed: {e}")
 
     bvh_name = args.bvh if args.bvh != "auto" else ("sah" if info["primitives"] <= (4 << 20) else "lbvh")
-    traffic = committed_traffic(bvh_name)
+    traffic_ray = committed_traffic(bvh_name)
+    # HBM bytes per launch = measured bytes per ray x rays per launch of this run
+    traffic = traffic_ray * st_t["rays"] / max(st_t["launches"], 1) if traffic_ray else None
     line = {
         "metric": METRIC,
         "value": value,
@@ -476,7 +494,9 @@ def main():
         "config": {"workload": "rcontrib -I+ -ab 4 -ad 4096 -lw 1e-4 -aa 0 -c 1, Reinhart MF:1 + ground",
                    "sensors": int(len(sensors)), "primitives": int(info["primitives"]),
                    "rays_per_step": int(np.mean(rays)), "scene_commit_ms": t_commit * 1e3,
-                   "bvh": bvh_name},
+                   "bvh": bvh_name,
+                   "launch_shape": tuned["best"] if tuned else tune.DEFAULT,
+                   "per_thread_kernel_ms": tuned["tried"][0][1] if tuned else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_trace", "avg_launch_ms": avg_launch_ms,

@@ -142,6 +142,14 @@ int rg_set_profiling(rg_ctx *ctx, int level);
  * rt/raytrace.c:595-760).  RG_BVH_AUTO: host binned-SAH tree up to 4M
  * primitives, device LBVH above; RG_BVH_SAH / RG_BVH_LBVH force one.  Both
  * give the same closest hits; only the traversal cost differs. */
+/* Launch shape of the renderer (no reference counterpart).  Keys:
+ * "trace_kernel" 0 one child ray per thread, 1 persistent wavefronts with
+ * dynamic ray fetch (default); "refill" idle lanes (1..64) that make a
+ * persistent wavefront fetch new rays (default 16); "window" children per
+ * generation launch (default 2M); "blocks_per_cu" persistent grid size.
+ * Results do not depend on these; only the time does. */
+int rg_set_tuning(rg_ctx *ctx, const char *key, long long value);
+
 #define RG_BVH_AUTO 0
 #define RG_BVH_SAH 1
 #define RG_BVH_LBVH 2

@@ -20,7 +20,7 @@ EXPORTS = (
     "rg_ctx_create", "rg_ctx_destroy", "rg_last_error", "rg_scene_load", "rg_scene_load_text",
     "rg_scene_commit", "rg_scene_info", "rg_set_params", "rg_default_params",
     "rg_default_params_rcontrib", "rg_trace", "rg_contrib", "rg_contrib_batch", "rg_trace_batch", "rg_last_nrays",
-    "rg_set_profiling", "rg_set_bvh_builder", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
+    "rg_set_profiling", "rg_set_bvh_builder", "rg_set_tuning", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
     "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint", "rg_first_hit",
 )
 
@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
     L.rg_last_nrays.argtypes = [vp, C.POINTER(C.c_ulonglong)]
     L.rg_set_profiling.argtypes = [vp, C.c_int]
     L.rg_set_bvh_builder.argtypes = [vp, C.c_int]
+    L.rg_set_tuning.argtypes = [vp, C.c_char_p, C.c_longlong]
     L.rg_last_stats.argtypes = [vp, dp, C.c_int]
     L.rg_dcsky.argtypes = [fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.rg_dcsky_device.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -245,6 +246,12 @@ class Context:
 
     def set_profiling(self, level: int):
         self._check(self._L.rg_set_profiling(self._h, level))
+
+    def set_tuning(self, **kw):
+        """Launch shape: trace_kernel (0/1), refill, window, blocks_per_cu."""
+        for k, v in kw.items():
+            self._check(self._L.rg_set_tuning(self._h, k.encode(), int(v)))
+        return self
 
     def set_bvh_builder(self, kind: int):
         """BVH_AUTO, BVH_SAH or BVH_LBVH for the next commit()."""

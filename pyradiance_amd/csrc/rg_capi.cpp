@@ -33,6 +33,7 @@ struct rg_ctx {
   std::string err;
   unsigned long long last_nrays = 0;
   int bvh_builder = RG_BVH_AUTO;
+  Tuning tuning;
   int profile = 0;              // 0 off, 1 time k_bounce launches, 2 also count visits
   RenderStats last_stats;
   unsigned long long last_visits[2] = {0, 0};
@@ -351,8 +352,8 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
     for (const DevScene& b : *batch) max_srcs = std::max(max_srcs, b.nsrcs);
   if (nrays) {
     std::lock_guard<std::mutex> lock(g_render_mu);
-    render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, render_arena(c->device), c->st,
-           &c->last_stats);
+    render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, c->tuning, render_arena(c->device),
+           c->st, &c->last_stats);
   }
   if (d_vis) hipck(hipMemcpyAsync(c->last_visits, d_vis, 16, hipMemcpyDeviceToHost, c->st), "visits");
   else c->last_visits[0] = c->last_visits[1] = 0;
@@ -609,6 +610,29 @@ int rg_first_hit(rg_ctx* c, const double* rays, long long n, double* t, double* 
 
 int rg_last_nrays(rg_ctx* c, unsigned long long* n) {
   RG_TRY(c, { *n = c->last_nrays; });
+}
+
+int rg_set_tuning(rg_ctx* c, const char* key, long long value) {
+  RG_TRY(c, {
+    if (!key) throw std::runtime_error("null tuning key");
+    std::string k = key;
+    Tuning& t = c->tuning;
+    if (k == "trace_kernel") {
+      if (value != 0 && value != 1) throw std::runtime_error("trace_kernel is 0 or 1");
+      t.trace_kernel = (int)value;
+    } else if (k == "refill") {
+      if (value < 1 || value > 64) throw std::runtime_error("refill is 1..64");
+      t.refill = (uint32_t)value;
+    } else if (k == "window") {
+      if (value < 1024 || value > (1LL << 24)) throw std::runtime_error("window is 1024..16M");
+      t.window = (uint32_t)value;
+    } else if (k == "blocks_per_cu") {
+      if (value < 1 || value > 64) throw std::runtime_error("blocks_per_cu is 1..64");
+      t.blocks_per_cu = (uint32_t)value;
+    } else {
+      throw std::runtime_error("unknown tuning key '" + k + "'");
+    }
+  });
 }
 
 int rg_set_bvh_builder(rg_ctx* c, int kind) {

@@ -1032,6 +1032,174 @@ k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t
   outp[t] = o;
 }
 
+// Persistent form of k_trace with dynamic ray fetch.  A fixed grid of
+// wavefronts loops; when at least `refill` lanes of a wavefront have finished
+// their ray (or all have), the wavefront takes the next batch of child
+// indices from `work` with one atomic and the idle lanes start new rays,
+// while the others carry on.  Lanes therefore do not idle until the longest
+// ray of their wavefront ends, which is what bounds the one-ray-per-thread
+// kernel (its VALU lane utilisation is about a third).  Same admission,
+// closest-hit walk and output record as k_trace.  BATCH: rays of several
+// scenes (each lane keeps its ray's scene tables); otherwise the kernel
+// argument's scene, held in scalar registers.
+template <int BS, bool BATCH>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(RG_TRACE_WAVES)))
+k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t nspawn, uint64_t c0,
+           uint32_t nthreads, TraceOut* outp, uint32_t* work, uint32_t refill) {
+  __shared__ int32_t stk[LSTACK * BS];
+  __shared__ float stn[LSTACK * BS];
+  constexpr int32_t DONE = INT32_MIN;
+  c.adj = 0;
+  const int lane = threadIdx.x;
+  const uint32_t wl = __lane_id();
+  uint64_t deep[PSTACK];
+  int32_t tcur = -1;              // window index of this lane's ray, -1: idle
+  bool exhausted = false;         // no child left to take (wave-uniform)
+  DevScene S = Sarg;
+  V3 o = v3(0, 0, 0), d = v3(0, 0, 1);
+  HitRec h;
+  h.t = FHUGE; h.rod = 1.0; h.prim = -1;
+  int32_t node = DONE, leaf = -1;
+  int sp = 0, iter = 0;
+  uint32_t si = 0;
+  float ox = 0, oy = 0, oz = 0, ix = 0, iy = 0, iz = 0;
+  uint32_t nvis = 0, nprim = 0;
+  for (;;) {
+    // ---- refill idle lanes ----
+    unsigned long long idle = __ballot(tcur < 0);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (!exhausted && nidle && (nidle >= refill || nidle == (uint32_t)__popcll(__ballot(1)))) {
+      const uint32_t leader = __ffsll((long long)idle) - 1;
+      uint32_t base = 0;
+      if (wl == leader) base = atomicAdd(work, nidle);
+      base = __shfl(base, leader);
+      if ((uint64_t)base + nidle >= nthreads) exhausted = true;
+      if (tcur < 0) {
+        const uint32_t my = base + lane_rank(idle);
+        const uint64_t gid0 = c0 + base;
+        uint32_t s0 = 0;
+        if (wl == leader && base < nthreads) s0 = find_spawn(offs, nspawn, gid0);
+        s0 = __shfl(s0, leader);
+        if (my < nthreads) {
+          const uint64_t gid = c0 + my;
+          uint32_t lo = s0, hi = (uint32_t)min((uint64_t)nspawn, (uint64_t)s0 + (gid - gid0) + 1);
+          while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if ((uint64_t)offs[mid] <= gid) lo = mid; else hi = mid;
+          }
+          si = lo;
+          RayS r;
+          if (!build_ray(c, in[si], (uint32_t)(gid - offs[si]), r)) {
+            TraceOut to;
+            to.spawn = si; to.pad = 0; to.prim = -2; to.t = 0;
+            to.dx = to.dy = to.dz = 0.f;
+            outp[my] = to;
+          } else {
+            wave_count(c.nrays);
+            tcur = (int32_t)my;
+            if (BATCH) S = scene_of(Sarg, c, r.scene);
+            o = r.o;
+            d = r.d;
+            h.t = FHUGE; h.rod = 1.0; h.prim = -1;
+            for (int k = 0; k < S.nbig; ++k) { hit_prim(S, S.big[k], o, d, h); ++nprim; }
+            node = DONE;
+            leaf = -1;
+            sp = 0;
+            iter = 0;
+            if (S.nbvh > 0) {
+              if (S.root < 0) { hit_prim(S, S.leaf_prim[~S.root], o, d, h); ++nprim; }
+              else node = S.root;
+            }
+            ox = (float)o.x; oy = (float)o.y; oz = (float)o.z;
+            auto inv = [](double v) -> float {
+              float f = (float)v;
+              if (fabsf(f) < 1e-20f) f = f < 0 ? -1e-20f : 1e-20f;
+              return 1.0f / f;
+            };
+            ix = inv(d.x); iy = inv(d.y); iz = inv(d.z);
+          }
+        }
+      }
+    }
+    if (__ballot(tcur >= 0) == 0ull) {
+      if (exhausted) break;
+      continue;
+    }
+    // ---- walk: internal nodes until every lane holds a postponed leaf ----
+    auto pop = [&](float lim) -> int32_t {
+      while (sp > 0) {
+        --sp;
+        int32_t cand;
+        float ct;
+        if (sp < LSTACK) {
+          cand = stk[sp * BS + lane];
+          ct = stn[sp * BS + lane];
+        } else {
+          uint64_t e = deep[sp - LSTACK];
+          cand = (int32_t)(uint32_t)e;
+          ct = __uint_as_float((uint32_t)(e >> 32));
+        }
+        if (ct <= lim) return cand;
+      }
+      return DONE;
+    };
+    const BVHNode* __restrict__ nodes = S.nodes;
+    while (node >= 0) {
+      if (++iter > S.nbvh + 8) { atomicOr(c.err, ERR_STACK); node = DONE; leaf = -1; break; }
+      const BVHNode N = nodes[node];
+      ++nvis;
+      float tmax = (float)(h.t + 2 * FTINY) * 1.000001f;
+      float tl, tr;
+      bool hl = slab(N.lmin, N.lmax, ox, oy, oz, ix, iy, iz, tmax, tl);
+      bool hr = slab(N.rmin, N.rmax, ox, oy, oz, ix, iy, iz, tmax, tr);
+      if (hl & hr) {
+        bool lfirst = tl <= tr;
+        int32_t farc = lfirst ? N.right : N.left;
+        float fart = lfirst ? tr : tl;
+        if (sp < LSTACK) { stk[sp * BS + lane] = farc; stn[sp * BS + lane] = fart; }
+        else if (sp < LSTACK + PSTACK) deep[sp - LSTACK] = pack_entry(farc, fart);
+        else { atomicOr(c.err, ERR_STACK); node = DONE; leaf = -1; break; }
+        ++sp;
+        node = lfirst ? N.left : N.right;
+      } else if (hl) {
+        node = N.left;
+      } else if (hr) {
+        node = N.right;
+      } else {
+        node = pop(tmax);
+      }
+      if ((node < 0) & (node != DONE) & (leaf < 0)) {
+        leaf = ~node;
+        node = pop((float)(h.t + 2 * FTINY) * 1.000001f);
+      }
+      if (__ballot(leaf < 0) == 0ull) break;
+    }
+    while (leaf >= 0) {
+      hit_prim(S, leaf, o, d, h);
+      ++nprim;
+      leaf = -1;
+      if ((node < 0) & (node != DONE)) {
+        leaf = ~node;
+        node = pop((float)(h.t + 2 * FTINY) * 1.000001f);
+      }
+    }
+    // ---- finished rays: write the record, free the lane ----
+    if ((tcur >= 0) & (node == DONE) & (leaf < 0)) {
+      TraceOut to;
+      to.spawn = si; to.pad = 0;
+      to.t = h.t;
+      to.prim = h.prim;
+      to.dx = (float)d.x; to.dy = (float)d.y; to.dz = (float)d.z;
+      outp[tcur] = to;
+      tcur = -1;
+    }
+  }
+  if (c.visits) {
+    atomicAdd(c.visits, (unsigned long long)nvis);
+    atomicAdd(c.visits + 1, (unsigned long long)nprim);
+  }
+}
+
 static void check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -1158,11 +1326,15 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& are
 }
 
 void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad, int max_srcs,
-            DevArena& arena, hipStream_t st, RenderStats* stats) {
+            const Tuning& tn, DevArena& arena, hipStream_t st, RenderStats* stats) {
   // Work proceeds in waves of at most `window` children; each child appends
   // at most `per_child` spawn records, so a wave never overflows its buffer.
   const uint32_t per_child = 6u + (uint32_t)std::max(max_srcs, S.nsrcs);
-  const uint32_t window = 1u << 21;
+  const uint32_t window = std::max<uint32_t>(tn.window, 1024u);
+  int dev = 0, ncu = 256;
+  check(hipGetDevice(&dev), "device");
+  check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+  const uint32_t pf_blocks = (uint32_t)std::max(1, ncu) * std::max<uint32_t>(tn.blocks_per_cu, 1u);
   const long max_batches = 1L << 22;
   uint32_t* d_cnt = (uint32_t*)arena.get("ctr", 64, st, true);
   uint32_t* h_cnt = arena.host_u32(8);
@@ -1234,11 +1406,21 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       Spawn* ob = slot_buf(s, cap);
       PathRec* opb = slot_path(s, cap);
       TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * nth, st, false);
-      check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
+      check(hipMemsetAsync(d_cnt, 0, 2 * sizeof(uint32_t), st), "memset");   // spawn count, work counter
       Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err, opb};
       if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
-      hipLaunchKernelGGL(k_trace, dim3((nth + TBLOCK - 1) / TBLOCK), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n,
-                         c0, nth, tout);
+      const uint32_t tblocks = (nth + TBLOCK - 1) / TBLOCK;
+      if (tn.trace_kernel == 1) {
+        const uint32_t nb = std::min(tblocks, pf_blocks);
+        if (cfg.nscenes)
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n,
+                             c0, nth, tout, d_cnt + 1, tn.refill);
+        else
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+      } else {
+        hipLaunchKernelGGL(k_trace, dim3(tblocks), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n, c0, nth, tout);
+      }
       check(hipGetLastError(), "k_trace launch");
       if (stats && stats->timing) check(hipEventRecord(ev1, st), "event");
       if (adj)

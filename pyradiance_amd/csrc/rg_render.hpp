@@ -75,10 +75,19 @@ struct RenderStats {
 
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level);
 
+// Launch shape of the wavefront renderer (set per context, chosen by
+// measurement: see rg_set_tuning).
+struct Tuning {
+  int trace_kernel = 1;          // 0: one child per thread, 1: persistent with dynamic ray fetch
+  uint32_t refill = 16;          // idle lanes that make a persistent wavefront fetch new rays
+  uint32_t window = 1u << 21;    // children per generation launch
+  uint32_t blocks_per_cu = 4;    // persistent grid: blocks of 256 per compute unit
+};
+
 // max_srcs: the largest source count over the scenes of a batch (sizes the
 // spawn buffers); S.nsrcs is used when that is larger.
 void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad, int max_srcs,
-            DevArena& arena, hipStream_t st, RenderStats* stats);
+            const Tuning& tn, DevArena& arena, hipStream_t st, RenderStats* stats);
 
 // First intersection of n rays (n x 6 doubles): distance (FHUGE on a miss),
 // unflipped normal (n x 3), primitive index (-1 on a miss).

@@ -275,6 +275,27 @@ def test_sah_and_lbvh_trees_give_the_same_contributions(L):
     assert np.allclose(outs[0].sum(), outs[1].sum(), rtol=1e-3)
 
 
+def test_persistent_and_per_thread_traversal_agree(L):
+    """The persistent dynamic-fetch kernel and the one-ray-per-thread kernel
+    trace the same rays: same ray count, same contributions (up to the order
+    of the FP64 additions), also with small refill thresholds and windows
+    that split the generations."""
+    _say("persistent vs per-thread traversal")
+    rad, sens = scenes.config2(1)
+    sens = sens[::5]
+    res = []
+    for kw in ({"trace_kernel": 0}, {"trace_kernel": 1}, {"trace_kernel": 1, "refill": 1, "window": 50000},
+               {"trace_kernel": 1, "refill": 64, "blocks_per_cu": 1}):
+        c = _ctx(L, rad)
+        c.set_tuning(**kw)
+        c.set_params(_rc_params(L, ambounce=2, ambdiv=512, minweight=1e-3, seed=7))
+        res.append((c.contrib(sens, _sky_mods(L), 1, L.RT_IMM_IRRAD), c.last_nrays()))
+        c.close()
+    for out, nr in res[1:]:
+        assert nr == res[0][1]
+        assert np.allclose(out, res[0][0], rtol=1e-12, atol=1e-15)
+
+
 def test_python_api_bytes(L, tmp_path):
     _say("python API")
     from pyradiance_amd import Rcontrib, rtrace
