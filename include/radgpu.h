@@ -146,7 +146,9 @@ int rg_set_profiling(rg_ctx *ctx, int level);
  * "trace_kernel" 0 one child ray per thread, 1 persistent wavefronts with
  * dynamic ray fetch (default); "refill" idle lanes (1..64) that make a
  * persistent wavefront fetch new rays (default 16); "window" children per
- * generation launch (default 2M); "blocks_per_cu" persistent grid size.
+ * generation launch (default 2M); "blocks_per_cu" persistent grid size;
+ * "max_big" (0..32, applies at the next commit) room-sized primitives kept
+ * out of the BVH and tested before the walk.
  * Results do not depend on these; only the time does. */
 int rg_set_tuning(rg_ctx *ctx, const char *key, long long value);
 

@@ -180,7 +180,7 @@ int rg_scene_commit(rg_ctx* c) {
         if (d2 * 64.0 > sd2) cand.push_back({d2, (int32_t)k});
       }
       std::sort(cand.begin(), cand.end(), [](auto& a, auto& b) { return a.first > b.first; });
-      if (cand.size() > 32) cand.resize(32);
+      if (cand.size() > c->tuning.max_big) cand.resize(c->tuning.max_big);
       if (cand.size() == cs.prims.size() && !cand.empty()) cand.pop_back();   // keep the tree non-empty
       std::vector<char> isbig(cs.prims.size(), 0);
       for (auto& pr : cand) { isbig[pr.second] = 1; big.push_back(pr.second); }
@@ -626,6 +626,9 @@ int rg_set_tuning(rg_ctx* c, const char* key, long long value) {
     } else if (k == "window") {
       if (value < 1024 || value > (1LL << 24)) throw std::runtime_error("window is 1024..16M");
       t.window = (uint32_t)value;
+    } else if (k == "max_big") {
+      if (value < 0 || value > 32) throw std::runtime_error("max_big is 0..32");
+      t.max_big = (uint32_t)value;
     } else if (k == "blocks_per_cu") {
       if (value < 1 || value > 64) throw std::runtime_error("blocks_per_cu is 1..64");
       t.blocks_per_cu = (uint32_t)value;

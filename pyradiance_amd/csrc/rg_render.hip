@@ -65,11 +65,13 @@ __device__ __forceinline__ void deposit(const RunCfg& c, uint32_t sensor, int co
 }
 
 // ---- BVH closest hit (FP32 boxes, FP64 primitive tests) --------------------
+// (lo - o) * inv as one FMA per plane: o * inv is precomputed per ray (the
+// boxes are widened by far more than the extra rounding this allows).
 __device__ __forceinline__ bool slab(const float* lo, const float* hi, float ox, float oy, float oz,
                                      float ix, float iy, float iz, float tmax, float& tn) {
-  float t0x = (lo[0] - ox) * ix, t1x = (hi[0] - ox) * ix;
-  float t0y = (lo[1] - oy) * iy, t1y = (hi[1] - oy) * iy;
-  float t0z = (lo[2] - oz) * iz, t1z = (hi[2] - oz) * iz;
+  float t0x = fmaf(lo[0], ix, -ox), t1x = fmaf(hi[0], ix, -ox);
+  float t0y = fmaf(lo[1], iy, -oy), t1y = fmaf(hi[1], iy, -oy);
+  float t0z = fmaf(lo[2], iz, -oz), t1z = fmaf(hi[2], iz, -oz);
   float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
   float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
   tn = tnear;
@@ -80,16 +82,75 @@ __device__ __forceinline__ uint64_t pack_entry(int32_t ref, float t) {
   return ((uint64_t)__float_as_uint(t) << 32) | (uint32_t)ref;
 }
 
+// Room-sized primitives, tested before the walk.  Polygons among the first
+// eight are taken nearest plane first (plane distance from FP64 dot products,
+// divided in FP32 only to order them); once the next estimate lies beyond
+// the current hit by a margin far above the estimate's error, the FP64 test
+// would reject it and every later one (t > hit + FTINY), so they are skipped.
+// Planes the ray nearly parallels, and other shapes, get the full test.
+// te: this lane's NF estimates, in LDS column-major (te[k * BS] for entry k).
+constexpr int NBIG_ORDER = 8;
+template <int BS>
+__device__ __forceinline__ void big_hits(const DevScene& S, V3 o, V3 d, HitRec& h, uint32_t& nprim, float* te) {
+  constexpr int NF = NBIG_ORDER;
+  const int nb = S.nbig;
+  uint32_t full = 0;        // big primitives that get the full test unordered
+#pragma unroll 1
+  for (int k = 0; k < NF; ++k) {
+    te[k * BS] = INFINITY;
+    if (k < nb) {
+      const Prim& P = S.prims[S.big[k]];
+      bool f = true;
+      if (P.kind == PK_POLYGON) {
+        const double* g = S.geom + P.goff;
+        double rdot = -(d.x * g[0] + d.y * g[1] + d.z * g[2]);
+        double num = o.x * g[0] + o.y * g[1] + o.z * g[2] - g[3];
+        if (fabs(rdot) > 4 * FTINY) {
+          float t = (float)num / (float)rdot;
+          if (t > -1e-3f) te[k * BS] = t;   // behind the origin: the FP64 test rejects t <= FTINY
+          f = false;
+        }
+      }
+      if (f) full |= 1u << k;
+    }
+  }
+  // one call site for the primitive test: unordered ones first, then the
+  // ordered polygons nearest first
+  int kk = 0;
+  for (;;) {
+    int pk = -1;
+    if (full) {
+      pk = __ffs(full) - 1;
+      full &= full - 1;
+    } else if (kk < nb - NF) {
+      pk = NF + kk++;
+    } else {
+      float best = INFINITY;
+      int bk = -1;
+#pragma unroll 1
+      for (int k = 0; k < NF; ++k) {
+        const float tk = te[k * BS];
+        if (tk < best) { best = tk; bk = k; }
+      }
+      if (bk < 0 || best > (float)(h.t + FTINY) * 1.0001f + 1e-4f) break;
+      te[bk * BS] = INFINITY;
+      pk = bk;
+    }
+    hit_prim(S, S.big[pk], o, d, h);
+    ++nprim;
+  }
+}
+
 // Stack: the top LSTACK entries of each lane live in LDS (column-major, one
 // bank per lane); deeper entries spill to a private array that only deep
 // subtrees touch.
 template <int BS>
-__device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* stk, float* stn, uint32_t* err,
-                            unsigned long long* visits) {
+__device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* stk, float* stn, float* tes,
+                            uint32_t* err, unsigned long long* visits) {
   uint32_t nvis = 0, nprim = 0;
   h.t = FHUGE; h.rod = 1.0; h.prim = -1;
   // room-sized primitives first: their hit bounds the walk below
-  for (int k = 0; k < S.nbig; ++k) { hit_prim(S, S.big[k], o, d, h); ++nprim; }
+  big_hits<BS>(S, o, d, h, nprim, tes + threadIdx.x);
   if (S.nbvh == 0) goto done;
   if (S.root < 0) { hit_prim(S, S.leaf_prim[~S.root], o, d, h); ++nprim; goto done; }
   {
@@ -100,6 +161,7 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
     return 1.0f / f;
   };
   const float ix = inv(d.x), iy = inv(d.y), iz = inv(d.z);
+  const float oix = ox * ix, oiy = oy * iy, oiz = oz * iz;
   const int lane = threadIdx.x;
   // Deeper entries pack (ref, tnear) in 64 bits: one private store / load
   // each, and a different access type from the LDS pair above, so the two
@@ -141,8 +203,8 @@ __device__ void closest_hit(const DevScene& S, V3 o, V3 d, HitRec& h, int32_t* s
       ++nvis;
       float tmax = (float)(h.t + 2 * FTINY) * 1.000001f;
       float tl, tr;
-      bool hl = slab(N.lmin, N.lmax, ox, oy, oz, ix, iy, iz, tmax, tl);
-      bool hr = slab(N.rmin, N.rmax, ox, oy, oz, ix, iy, iz, tmax, tr);
+      bool hl = slab(N.lmin, N.lmax, oix, oiy, oiz, ix, iy, iz, tmax, tl);
+      bool hr = slab(N.rmin, N.rmax, oix, oiy, oiz, ix, iy, iz, tmax, tr);
       if (hl & hr) {
         bool lfirst = tl <= tr;
         int32_t farc = lfirst ? N.right : N.left;
@@ -993,6 +1055,7 @@ k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t
         uint32_t nthreads, TraceOut* outp) {
   __shared__ int32_t stk[LSTACK * TBLOCK];
   __shared__ float stn[LSTACK * TBLOCK];
+  __shared__ float tes[NBIG_ORDER * TBLOCK];
   uint32_t t = blockIdx.x * TBLOCK + threadIdx.x;
   if (t >= nthreads) return;
   c.adj = 0;      // admission does not depend on the path record
@@ -1025,7 +1088,7 @@ k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t
   }
   wave_count(c.nrays);
   HitRec h;
-  closest_hit<TBLOCK>(scene_of(Sarg, c, r.scene), r.o, r.d, h, stk, stn, c.err, c.visits);
+  closest_hit<TBLOCK>(scene_of(Sarg, c, r.scene), r.o, r.d, h, stk, stn, tes, c.err, c.visits);
   o.t = h.t;
   o.prim = h.prim;
   o.dx = (float)r.d.x; o.dy = (float)r.d.y; o.dz = (float)r.d.z;
@@ -1048,6 +1111,7 @@ k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint3
            uint32_t nthreads, TraceOut* outp, uint32_t* work, uint32_t refill) {
   __shared__ int32_t stk[LSTACK * BS];
   __shared__ float stn[LSTACK * BS];
+  __shared__ float tes[NBIG_ORDER * BS];
   constexpr int32_t DONE = INT32_MIN;
   c.adj = 0;
   const int lane = threadIdx.x;
@@ -1101,7 +1165,7 @@ k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint3
             o = r.o;
             d = r.d;
             h.t = FHUGE; h.rod = 1.0; h.prim = -1;
-            for (int k = 0; k < S.nbig; ++k) { hit_prim(S, S.big[k], o, d, h); ++nprim; }
+            big_hits<BS>(S, o, d, h, nprim, tes + lane);
             node = DONE;
             leaf = -1;
             sp = 0;
@@ -1110,13 +1174,13 @@ k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint3
               if (S.root < 0) { hit_prim(S, S.leaf_prim[~S.root], o, d, h); ++nprim; }
               else node = S.root;
             }
-            ox = (float)o.x; oy = (float)o.y; oz = (float)o.z;
             auto inv = [](double v) -> float {
               float f = (float)v;
               if (fabsf(f) < 1e-20f) f = f < 0 ? -1e-20f : 1e-20f;
               return 1.0f / f;
             };
             ix = inv(d.x); iy = inv(d.y); iz = inv(d.z);
+            ox = (float)o.x * ix; oy = (float)o.y * iy; oz = (float)o.z * iz;   // origin x inverse direction
           }
         }
       }
@@ -1210,12 +1274,13 @@ __global__ void __launch_bounds__(TBLOCK) k_first_hit(DevScene S, const double* 
                                                       double* n_out, int32_t* p_out, uint32_t* err) {
   __shared__ int32_t stk[LSTACK * TBLOCK];
   __shared__ float stn[LSTACK * TBLOCK];
+  __shared__ float tes[NBIG_ORDER * TBLOCK];
   uint32_t i = blockIdx.x * TBLOCK + threadIdx.x;
   if (i >= n) return;
   V3 o = load3(rays + 6 * (size_t)i), d = load3(rays + 6 * (size_t)i + 3);
   normalize(d);
   HitRec h;
-  closest_hit<TBLOCK>(S, o, d, h, stk, stn, err, nullptr);
+  closest_hit<TBLOCK>(S, o, d, h, stk, stn, tes, err, nullptr);
   V3 nrm = v3(0, 0, 0);
   if (h.prim >= 0) nrm = prim_normal(S, h.prim, o + d * h.t);
   t_out[i] = h.prim >= 0 ? h.t : FHUGE;
@@ -1235,8 +1300,11 @@ void first_hit(const DevScene& S, const double* d_rays, uint32_t n, double* d_t,
 
 // One generation, part 2: shade the hits and append the next spawn records.
 // pin: path records of `in` (adjoint replay only).
+#ifndef RG_SHADE_WAVES
+#define RG_SHADE_WAVES 2     // waves per SIMD the register budget of k_shade targets
+#endif
 template <bool ADJ>
-__global__ void __launch_bounds__(BLOCK) k_shade(DevScene Sarg, RunCfg c, const Spawn* in, const PathRec* pin,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RG_SHADE_WAVES))) k_shade(DevScene Sarg, RunCfg c, const Spawn* in, const PathRec* pin,
                                                  const uint32_t* offs, uint64_t c0, uint32_t nthreads,
                                                  const TraceOut* tr, Sink sk) {
   uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
@@ -1255,6 +1323,71 @@ __global__ void __launch_bounds__(BLOCK) k_shade(DevScene Sarg, RunCfg c, const 
   const DevScene& S = scene_of(Sarg, c, r.scene);
   if (h.prim >= 0) shade_hit(S, c, r, h, sk);
   else shade_miss(S, c, r);
+}
+
+// ---- grouping a batched generation by scene ----------------------------------
+// Between generations of a batched launch the spawn records are regrouped by
+// scene (counting sort on Spawn.scene), so that the next windows trace one
+// scene at a time instead of interleaving every scene's tables in the caches.
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_ITEMS = 4;
+
+__global__ void __launch_bounds__(SORT_BLOCK) k_scene_hist(const Spawn* in, uint32_t n, uint32_t* hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * SORT_BLOCK * SORT_ITEMS + threadIdx.x;
+  for (int k = 0; k < SORT_ITEMS; ++k) {
+    size_t i = base + (size_t)k * SORT_BLOCK;
+    if (i < n) atomicAdd(&h[in[i].scene], 1u);
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// exclusive scan of the 256 scene counts (one block)
+__global__ void __launch_bounds__(256) k_scene_start(const uint32_t* hist, uint32_t* start) {
+  __shared__ uint32_t sh[256];
+  const int t = threadIdx.x;
+  uint32_t v = hist[t];
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    uint32_t a = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  start[t] = sh[t] - v;
+}
+
+// Each block reserves one range per scene it holds, then places its records
+// (and their adjoint path records) at LDS-ranked positions in that range.
+__global__ void __launch_bounds__(SORT_BLOCK) k_scene_scatter(const Spawn* in, const PathRec* pin, uint32_t n,
+                                                              const uint32_t* start, uint32_t* fill, Spawn* out,
+                                                              PathRec* pout) {
+  __shared__ uint32_t h[256];
+  __shared__ uint32_t b[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * SORT_BLOCK * SORT_ITEMS + threadIdx.x;
+  uint32_t sc[SORT_ITEMS], rank[SORT_ITEMS];
+  for (int k = 0; k < SORT_ITEMS; ++k) {
+    size_t i = base + (size_t)k * SORT_BLOCK;
+    sc[k] = i < n ? in[i].scene : 0u;
+    rank[k] = i < n ? atomicAdd(&h[sc[k]], 1u) : 0u;
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) b[threadIdx.x] = start[threadIdx.x] + atomicAdd(&fill[threadIdx.x], h[threadIdx.x]);
+  __syncthreads();
+  for (int k = 0; k < SORT_ITEMS; ++k) {
+    size_t i = base + (size_t)k * SORT_BLOCK;
+    if (i < n) {
+      uint32_t pos = b[sc[k]] + rank[k];
+      out[pos] = in[i];
+      if (pin) pout[pos] = pin[i];
+    }
+  }
 }
 
 // ---- exclusive scan of uint32 counts (three-phase, LDS per block) ----------
@@ -1386,21 +1519,38 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
     check(hipEventCreate(&ev1), "event");
     check(hipEventCreate(&ev2), "event");
   }
-  while (!stack.empty()) {
-    Batch bt = stack.back();
-    stack.pop_back();
-    if (bt.n == 0) { free_slots.push_back(bt.slot); continue; }
+  // Depth first, one window at a time: after each window its children are
+  // opened and walked before the next window of the same batch, so live
+  // spawn buffers are bounded by the depth of the ray tree, not by the size
+  // of a generation.
+  struct Frame { Batch bt; uint32_t* offs; uint64_t total; uint64_t c0; };
+  std::vector<Frame> frames;
+  auto open_frame = [&](const Batch& b) {
+    if (b.n == 0) { free_slots.push_back(b.slot); return; }
     if (++gens > max_batches) throw std::runtime_error("ray tree did not terminate");
-    uint32_t* counts = (uint32_t*)arena.get("counts" + std::to_string(bt.slot), sizeof(uint32_t) * (bt.n + 1), st, false);
-    uint32_t* offs = (uint32_t*)arena.get("offs" + std::to_string(bt.slot), sizeof(uint32_t) * (bt.n + 1), st, false);
-    hipLaunchKernelGGL(k_counts, dim3((bt.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, bt.buf, bt.n, counts);
-    exclusive_scan(counts, offs, bt.n, arena, st, 0);
-    check(hipMemcpyAsync(h_cnt, offs + bt.n, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "total");
+    uint32_t* counts = (uint32_t*)arena.get("counts" + std::to_string(b.slot), sizeof(uint32_t) * (b.n + 1), st, false);
+    uint32_t* offs = (uint32_t*)arena.get("offs" + std::to_string(b.slot), sizeof(uint32_t) * (b.n + 1), st, false);
+    hipLaunchKernelGGL(k_counts, dim3((b.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, b.buf, b.n, counts);
+    exclusive_scan(counts, offs, b.n, arena, st, 0);
+    check(hipMemcpyAsync(h_cnt, offs + b.n, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "total");
     check(hipStreamSynchronize(st), "scan");
-    uint64_t total = h_cnt[0];
-    std::vector<Batch> produced;
-    for (uint64_t c0 = 0; c0 < total; c0 += window) {
+    frames.push_back({b, offs, (uint64_t)h_cnt[0], 0});
+  };
+  for (const Batch& b : stack) open_frame(b);
+  stack.clear();
+  while (!frames.empty()) {
+    if (frames.back().c0 >= frames.back().total) {
+      free_slots.push_back(frames.back().bt.slot);
+      frames.pop_back();
+      continue;
+    }
+    const Batch bt = frames.back().bt;
+    uint32_t* offs = frames.back().offs;
+    const uint64_t c0 = frames.back().c0;
+    const uint64_t total = frames.back().total;
+    {
       uint32_t nth = (uint32_t)std::min<uint64_t>(window, total - c0);
+      frames.back().c0 += nth;
       size_t cap = (size_t)nth * per_child;
       int s = new_slot();
       Spawn* ob = slot_buf(s, cap);
@@ -1442,12 +1592,24 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       }
       if (stats) { stats->launches++; stats->threads += nth; }
       uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)cap);
-      if (nout) produced.push_back({ob, opb, nout, s});
+      if (nout && cfg.nscenes > 1) {
+        // regroup by scene into the slot's second buffer, which the batch then uses
+        Spawn* sb = (Spawn*)arena.get("spawn_sorted" + std::to_string(s), sizeof(Spawn) * cap, st, false);
+        PathRec* spb = adj ? (PathRec*)arena.get("path_sorted" + std::to_string(s), sizeof(PathRec) * cap, st, false)
+                           : nullptr;
+        uint32_t* hist = (uint32_t*)arena.get("scene_hist", sizeof(uint32_t) * 768, st, true);
+        const uint32_t nb = (nout + SORT_BLOCK * SORT_ITEMS - 1) / (SORT_BLOCK * SORT_ITEMS);
+        hipLaunchKernelGGL(k_scene_hist, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, nout, hist);
+        hipLaunchKernelGGL(k_scene_start, dim3(1), dim3(256), 0, st, hist, hist + 256);
+        hipLaunchKernelGGL(k_scene_scatter, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, opb, nout, hist + 256, hist + 512,
+                           sb, spb);
+        check(hipGetLastError(), "scene sort launch");
+        ob = sb;
+        opb = spb;
+      }
+      if (nout) open_frame(Batch{ob, opb, nout, s});
       else free_slots.push_back(s);
     }
-    free_slots.push_back(bt.slot);
-    // depth first: children of the last window are processed next
-    for (auto& p : produced) stack.push_back(p);
   }
   if (stats) stats->generations = gens;
   if (ev0) (void)hipEventDestroy(ev0);

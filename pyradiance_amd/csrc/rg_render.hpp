@@ -82,6 +82,7 @@ struct Tuning {
   uint32_t refill = 16;          // idle lanes that make a persistent wavefront fetch new rays
   uint32_t window = 1u << 21;    // children per generation launch
   uint32_t blocks_per_cu = 4;    // persistent grid: blocks of 256 per compute unit
+  uint32_t max_big = 32;         // room-sized primitives kept out of the BVH (applies at commit)
 };
 
 // max_srcs: the largest source count over the scenes of a batch (sizes the
