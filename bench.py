@@ -137,7 +137,7 @@ def cpu_baseline(rad: str, sensors: np.ndarray, nsample: int, cores: int):
         }
 
 
-def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, nsample: int = 20000):
+def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, nsample: int = 20000, mode: int = 1):
     """DC x sky product on device buffers: kernel time (HIP events on the
     null stream, where the product runs), algorithmic and issued MFMA rates,
     and the relative error of `nsample` random entries against an FP64 dot
@@ -149,12 +149,12 @@ def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, nsa
     dA = torch.from_numpy(A).cuda()
     dB = torch.from_numpy(B).cuda()
     dC = torch.empty((M, N, 3), dtype=torch.float32, device="cuda")
-    _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, 1)
+    _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, 1)
+        _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
@@ -166,7 +166,7 @@ def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, nsa
     rel = np.abs(got - ref)[nz] / ref[nz]
     flops = 2.0 * M * K * N * 3
     byts = 12.0 * (M * K + K * N + M * N)
-    return {"M": M, "K": K, "N": N, "kernel_ms": ms, "algorithmic_TFLOPs": flops / ms / 1e9,
+    return {"M": M, "K": K, "N": N, "mode": mode, "kernel_ms": ms, "algorithmic_TFLOPs": flops / ms / 1e9,
             "issued_TFLOPs": 3 * flops / ms / 1e9, "mfma_util_issued": 3 * flops / ms / 1e9 / BF16_PEAK_TFLOPS,
             "GBs": byts / ms / 1e6, "hbm_frac": byts / ms / 1e6 / HBM_PEAK_GBS,
             "max_rel_err_vs_fp64": float(rel.max()), "mean_rel_err_vs_fp64": float(rel.mean()),
@@ -490,10 +490,12 @@ def main():
     dcs = dcsky_bench(out)
     dcs["device"] = dcsky_device_bench(2500, 146, 8760)
     log(f"dcsky: {dcs}")
+    log(f"dcsky with LDS-staged tiles: {dcsky_device_bench(2500, 146, 8760, mode=3)}")
     dc5 = None
     if not args.no_config5:
         dc5 = dcsky_device_bench(20000, 2306, 8760)
         log(f"dcsky config 5 size: {dc5}")
+        log(f"dcsky config 5 size without LDS staging: {dcsky_device_bench(20000, 2306, 8760, mode=2)}")
 
     batched = None
     if not args.no_batched:

@@ -163,7 +163,10 @@ int rg_last_stats(rg_ctx *ctx, double *out, int n);
 
 /* DC x sky product, replaces cm_multiply() (util/cmatrix.c:421-474):
  * C[M][N][3] = A[M][K][3] . B[K][N][3], each channel independently.
- * Host pointers.  mode 0 = FP32 accumulate, 1 = BF16 hi/lo split MFMA. */
+ * Host pointers.  mode 0 = FP32 accumulate, 1 = BF16 hi/lo split MFMA
+ * (LDS-staged 128 x 128 tiles when K > 256, per-wavefront operand loads
+ * otherwise), 2 / 3 = the split product forced to per-wavefront loads /
+ * LDS-staged tiles. */
 int rg_dcsky(const float *A, const float *B, float *C, int M, int K, int N, int mode);
 /* Same with device pointers on the current HIP device. */
 int rg_dcsky_device(const float *A, const float *B, float *C, int M, int K, int N, int mode);

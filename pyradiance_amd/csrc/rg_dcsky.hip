@@ -169,6 +169,117 @@ __global__ void __launch_bounds__(DB) k_dcsky_mfma(const unsigned short* Ah, con
   }
 }
 
+// LDS-staged form.  A block of 16 wavefronts (4 x 4) computes a
+// 128 x 128 tile of all three channels; each wavefront holds a 32 x 32 tile
+// (2 x 2 MFMA tiles x 3 channels of accumulators).  For each K step of 32
+// and channel, the block's A and B hi/lo tiles (128 rows x 32 K each, 32 KB)
+// are loaded once into LDS and read by all 16 wavefronts; the next stage's
+// global loads are issued before the current stage's MFMAs, into the second
+// LDS buffer.  Rows are padded to 40 bf16 so the 16 lanes that read one
+// fragment column hit different banks.
+constexpr int LBM = 128, LBN = 128, LKS = 32;
+constexpr int LROW = 40;                      // padded row length (bf16)
+constexpr int LTILE = 128 * LROW;             // one 128 x 32 plane tile in LDS (bf16)
+
+// WM x WN wavefronts, each holding TA x TB 16 x 16 MFMA tiles of all three
+// channels (16 * TA * WM = 16 * TB * WN = 128).
+template <int WM, int WN, int TA, int TB>
+__global__ void __launch_bounds__(64 * WM * WN) k_dcsky_lds(const unsigned short* Ah, const unsigned short* Al,
+                                                            const unsigned short* Bh, const unsigned short* Bl,
+                                                            float* C, int M, int N, int Kp, int Mt, int Nt) {
+  static_assert(16 * TA * WM == LBM && 16 * TB * WN == LBN, "block tile");
+  constexpr int NT = 64 * WM * WN;
+  constexpr int NCH = 2048 / NT;              // 16-byte chunks each thread stages per stage
+  __shared__ __attribute__((aligned(16))) unsigned short lds[2][4][LTILE];   // [buffer][Ah, Al, Bh, Bl]
+  int tm, tn;
+  if (!tile_of(Mt, Nt, tm, tn)) return;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  const int mb = tm * LBM, nb = tn * LBN;
+  // staging: chunk q (16 B = 8 bf16) of the stage: plane q >> 9, row (q & 511) >> 2, part q & 3
+  int soff[NCH], splane[NCH];
+  size_t sg[NCH];
+  const size_t csz[4] = {(size_t)M * Kp, (size_t)M * Kp, (size_t)N * Kp, (size_t)N * Kp};
+  for (int j = 0; j < NCH; ++j) {
+    const int q = t + j * NT;
+    const int pl = q >> 9, row = (q & 511) >> 2, part = q & 3;
+    splane[j] = pl;
+    soff[j] = row * LROW + part * 8;
+    sg[j] = (pl < 2 ? (size_t)min(mb + row, M - 1) : (size_t)min(nb + row, N - 1)) * Kp + part * 8;
+  }
+  const unsigned short* planes[4] = {Ah, Al, Bh, Bl};
+  auto gload = [&](int stage, u16x8* reg) {
+    const int k0 = (stage / 3) * LKS, c = stage % 3;
+    for (int j = 0; j < NCH; ++j) reg[j] = *(const u16x8*)(planes[splane[j]] + c * csz[splane[j]] + sg[j] + k0);
+  };
+  auto lstore = [&](int buf, const u16x8* reg) {
+    for (int j = 0; j < NCH; ++j) *(u16x8*)&lds[buf][splane[j]][soff[j]] = reg[j];
+  };
+  f32x4 acc[3][TA][TB];
+  for (int c = 0; c < 3; ++c)
+    for (int a = 0; a < TA; ++a)
+      for (int b = 0; b < TB; ++b) acc[c][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nstage = 3 * (Kp / LKS);
+  u16x8 reg[NCH];
+  gload(0, reg);
+  lstore(0, reg);
+  __syncthreads();
+  for (int s = 0; s < nstage; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nstage) gload(s + 1, reg);
+    const int c = s % 3;
+    bf16x8 fah[TA], fal[TA], fbh[TB], fbl[TB];
+    for (int a = 0; a < TA; ++a) {
+      const int row = wm * 16 * TA + 16 * a + r16;
+      fah[a] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][0][row * LROW + kq]);
+      fal[a] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][1][row * LROW + kq]);
+    }
+    for (int b = 0; b < TB; ++b) {
+      const int row = wn * 16 * TB + 16 * b + r16;
+      fbh[b] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][2][row * LROW + kq]);
+      fbl[b] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][3][row * LROW + kq]);
+    }
+    for (int a = 0; a < TA; ++a)
+      for (int b = 0; b < TB; ++b) {
+        f32x4 v = c == 0 ? acc[0][a][b] : (c == 1 ? acc[1][a][b] : acc[2][a][b]);
+        v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbh[b], fah[a], v, 0, 0, 0);
+        v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbl[b], fah[a], v, 0, 0, 0);
+        v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbh[b], fal[a], v, 0, 0, 0);
+        if (c == 0) acc[0][a][b] = v;
+        else if (c == 1) acc[1][a][b] = v;
+        else acc[2][a][b] = v;
+      }
+    if (s + 1 < nstage) lstore(buf ^ 1, reg);
+    __syncthreads();
+  }
+  // D layout: lane holds D[n = 4 * (lane >> 4) + i][m = lane & 15]
+  const bool vec = (N & 3) == 0;
+  for (int a = 0; a < TA; ++a) {
+    const int m = mb + wm * 16 * TA + 16 * a + r16;
+    if (m >= M) continue;
+    for (int b = 0; b < TB; ++b) {
+      const int nq = nb + wn * 16 * TB + 16 * b + (lane >> 4) * 4;
+      if (nq >= N) continue;
+      float* dst = C + ((size_t)m * N + nq) * 3;
+      const f32x4 r = acc[0][a][b], g = acc[1][a][b], u = acc[2][a][b];
+      if (vec) {
+        float4* d4 = (float4*)dst;
+        d4[0] = make_float4(r[0], g[0], u[0], r[1]);
+        d4[1] = make_float4(g[1], u[1], r[2], g[2]);
+        d4[2] = make_float4(u[2], r[3], g[3], u[3]);
+      } else {
+        for (int i = 0; i < 4 && nq + i < N; ++i) {
+          dst[3 * i + 0] = r[i];
+          dst[3 * i + 1] = g[i];
+          dst[3 * i + 2] = u[i];
+        }
+      }
+    }
+  }
+}
+
 // FP32 reference path: 16x16 threads, 64x64 tile, LDS staged.
 __global__ void __launch_bounds__(256) k_dcsky_f32(const float* A, const float* B, float* C, int M, int K, int N) {
   __shared__ float As[3][64][17];
@@ -220,11 +331,24 @@ void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mo
   unsigned short *Ah = buf, *Al = buf + na, *Bh = buf + 2 * na, *Bl = buf + 2 * na + nb;
   hipLaunchKernelGGL(k_split_a, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, st, A, M, K, Kp, Ah, Al);
   hipLaunchKernelGGL(k_split_b, dim3((N + 31) / 32, Kp / 32, 3), dim3(32, 8), 0, st, B, K, N, Kp, Bh, Bl);
-  const int Mt = (M + BM - 1) / BM, Nt = (N + BN - 1) / BN;
-  const long long tiles = (long long)Mt * Nt;
-  if (tiles > 0x7fffffffLL - 8) throw std::runtime_error("dcsky: matrix too large");
-  const unsigned nblk = (unsigned)((tiles + 7) / 8 * 8);
-  hipLaunchKernelGGL(k_dcsky_mfma, dim3(nblk), dim3(DB), 0, st, Ah, Al, Bh, Bl, C, M, N, Kp, Mt, Nt);
+  // Short K (a few K steps) is bound by latency and the output write: the
+  // per-wavefront kernel is as fast there; long K is MFMA work and needs the
+  // LDS-staged tiles.  mode 2 forces the former, mode 3 the latter.
+  const bool staged = mode == 3 || (mode == 1 && Kp > 256);
+  if (!staged) {
+    const int Mt = (M + BM - 1) / BM, Nt = (N + BN - 1) / BN;
+    const long long tiles = (long long)Mt * Nt;
+    if (tiles > 0x7fffffffLL - 8) throw std::runtime_error("dcsky: matrix too large");
+    const unsigned nblk = (unsigned)((tiles + 7) / 8 * 8);
+    hipLaunchKernelGGL(k_dcsky_mfma, dim3(nblk), dim3(DB), 0, st, Ah, Al, Bh, Bl, C, M, N, Kp, Mt, Nt);
+  } else {
+    const int Mt = (M + LBM - 1) / LBM, Nt = (N + LBN - 1) / LBN;
+    const long long tiles = (long long)Mt * Nt;
+    if (tiles > 0x7fffffffLL - 8) throw std::runtime_error("dcsky: matrix too large");
+    const unsigned nblk = (unsigned)((tiles + 7) / 8 * 8);
+    hipLaunchKernelGGL((k_dcsky_lds<4, 4, 2, 2>), dim3(nblk), dim3(1024), 0, st, Ah, Al, Bh, Bl, C, M, N, Kp, Mt,
+                       Nt);
+  }
   dck(hipFreeAsync(buf, st), "dcsky scratch free");
 }
 

@@ -102,7 +102,9 @@ int build_lbvh(const float* d_lo, const float* d_hi, const int32_t* d_ids, uint3
                const float scene_hi[3], BVHNode* d_nodes, int32_t* d_leaf_prim, DevArena& arena, hipStream_t st);
 
 // DC x sky product C[M,N,3] = A[M,K,3] x B[K,N,3], per channel.
-// mode 0: FP32 FMA, mode 1: BF16 hi/lo split on MFMA with FP32 accumulation.
+// mode 0: FP32 FMA, mode 1: BF16 hi/lo split on MFMA with FP32 accumulation
+// (LDS-staged 128 x 128 tiles for long K), 2 / 3: the split product forced to
+// per-wavefront operand loads / LDS-staged tiles.
 void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mode, hipStream_t st);
 
 }  // namespace rg

@@ -224,16 +224,17 @@ def test_klems_bins_match_reference(L):
 def test_dcsky_against_fp64(L):
     _say("dcsky")
     rng = np.random.default_rng(0)
-    for (M, K, N) in [(37, 19, 51), (250, 146, 877), (64, 2305, 96)]:
+    for (M, K, N) in [(37, 19, 51), (250, 146, 877), (64, 2305, 96), (300, 77, 130), (129, 33, 257)]:
         A = rng.random((M, K, 3), dtype=np.float32)
         B = rng.lognormal(0, 1, (K, N, 3)).astype(np.float32)
         ref = np.einsum("mkc,knc->mnc", A.astype(np.float64), B.astype(np.float64), optimize=True)
         c32 = L.dcsky(A, B, 0)
-        cbf = L.dcsky(A, B, 1)
         e32 = np.abs(c32 - ref).max() / np.abs(ref).max()
-        ebf = (np.abs(cbf - ref) / np.abs(ref)).max()
         assert e32 < 1e-5, (M, K, N, e32)
-        assert ebf < 1e-4, (M, K, N, ebf)
+        for mode in (1, 2, 3):       # automatic, direct and LDS-staged split-BF16 products
+            cbf = L.dcsky(A, B, mode)
+            ebf = (np.abs(cbf - ref) / np.abs(ref)).max()
+            assert ebf < 1e-4, (M, K, N, mode, ebf)
 
 
 def test_config2_full_size_properties(L):
