@@ -40,15 +40,18 @@ typedef struct rg_params {
 
 /* rcontrib modifier with its bin function (rt/rcontrib.c:92 addmodifier).
  * bin_kind: 0 constant bin `cbin`, 1 Reinhart rbin (util/reinhartb.cal),
- * 2/3/4 Klems full/half/quarter kbin (util/klems_*.cal). */
+ * 2/3/4 Klems full/half/quarter kbin (util/klems_*.cal), 5 rfluxmtx uniform
+ * surface receiver, bin 0 when -D.nrm > 0 (util/rfluxmtx.c:465-470),
+ * 6 Shirley-Chiu scbin with SCdim in `mf` (util/disk2square.cal),
+ * 7 CIE sky-scanner cbin (util/cieskyscan.cal). */
 typedef struct rg_modifier {
   const char *name;
   int bin_kind;
   int nbins;
   int cbin;
-  double mf;           /* Reinhart MF */
+  double mf;           /* Reinhart MF, Shirley-Chiu SCdim */
   double rhs;          /* RHS: +1 right-handed */
-  double nrm[3];       /* rNx,rNy,rNz (Reinhart) / Nx,Ny,Nz (Klems) */
+  double nrm[3];       /* rNx,rNy,rNz (Reinhart, SC, CIE, uniform) / Nx,Ny,Nz (Klems) */
   double up[3];        /* Ux,Uy,Uz */
 } rg_modifier;
 

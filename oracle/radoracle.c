@@ -27,6 +27,7 @@
  *   trace_contrib    rt/rcontrib.c:272-316, eval_irrad :321-338
  *   rayirrad         rt/rtrace.c:416-431
  *   rbin             util/reinhartb.cal:19-47; kbin util/klems_full.cal:10-43
+ *   scbin, cbin      util/disk2square.cal:34-90, util/cieskyscan.cal:21-53
  *
  * Deliberate differences from the reference, shared with the GPU path:
  *   - random numbers come from a counter-based generator keyed by the ray's
@@ -779,6 +780,43 @@ static int bin_of(const ro_mod *m, const double *D) {
   double bval;
   if (m->bin_kind == 0) {
     bval = m->cbin;
+  } else if (m->bin_kind == 5) {   /* rfluxmtx h=u surface: if(-D.n,0,-1), util/rfluxmtx.c:465-470 */
+    bval = (-D[0] * m->nrm[0] - D[1] * m->nrm[1] - D[2] * m->nrm[2] > 0) ? 0 : -1;
+  } else if (m->bin_kind == 6) {   /* scbin, util/disk2square.cal:67-90 and :34-64 */
+    const double *N = m->nrm, *U = m->up;
+    double inc_dz = -D[0] * N[0] - D[1] * N[1] - D[2] * N[2];
+    double inc_rx = -m->rhs * (D[0] * (U[1] * N[2] - U[2] * N[1]) + D[1] * (U[2] * N[0] - U[0] * N[2]) + D[2] * (U[0] * N[1] - U[1] * N[0]));
+    double inc_ry = D[0] * U[0] + D[1] * U[1] + D[2] * U[2] + inc_dz * (N[0] * U[0] + N[1] * U[1] + N[2] * U[2]);
+    double den2 = inc_rx * inc_rx + inc_ry * inc_ry;
+    double radf = (den2 - 1e-7 > 0) ? sqrt((1 - inc_dz * inc_dz) / den2) : 0;
+    double x = inc_rx * radf, y = -inc_ry * radf;
+    double r = sqrt(x * x + y * y), phi = atan2(y, x), a, b;
+    if (-phi - PI / 4 > 0) phi += 2 * PI;                    /* norm_radians */
+    switch ((int)floor((phi + PI / 4) / (PI / 2)) + 1) {    /* in_disk_rgn */
+      case 1: a = r; b = phi * r / (PI / 4); break;
+      case 2: a = (PI / 2 - phi) * r / (PI / 4); b = r; break;
+      case 3: a = -r; b = (PI - phi) * r / (PI / 4); break;
+      case 4: a = (phi - 3 * PI / 2) * r / (PI / 4); b = -r; break;
+      default: a = r; b = -r; break;
+    }
+    bval = (inc_dz > 0) ? floor((a + 1) / 2 * m->mf) * m->mf + floor((b + 1) / 2 * m->mf) : -1;
+  } else if (m->bin_kind == 7) {   /* cbin, util/cieskyscan.cal:21-53 */
+    static const double cnaz[8] = {30, 30, 24, 24, 18, 12, 6, 1};
+    const double *N = m->nrm, *U = m->up;
+    double inc_dz = -D[0] * N[0] - D[1] * N[1] - D[2] * N[2];
+    double inc_rx = -m->rhs * (D[0] * (U[1] * N[2] - U[2] * N[1]) + D[1] * (U[2] * N[0] - U[0] * N[2]) + D[2] * (U[0] * N[1] - U[1] * N[0]));
+    double inc_ry = D[0] * U[0] + D[1] * U[1] + D[2] * U[2] + inc_dz * (N[0] * U[0] + N[1] * U[1] + N[2] * U[2]);
+    double r_alt = asin_deg(inc_dz), c_azi = atan2_deg(-inc_rx, -inc_ry);
+    double row = floor(r_alt / (90 / 7.5)), inc, fwd, acc = 0;
+    int k, ri = (int)row;
+    if (!(r_alt > 0) || ri < 0 || ri > 7) {
+      bval = -1;
+    } else {
+      inc = 360 / cnaz[ri];
+      fwd = floor(row / 2 + 1.25) - row / 2 - .75;
+      for (k = 0; k < ri; k++) acc += cnaz[k];   /* caccum */
+      bval = acc + ((359.9999 - c_azi > 0) ? (fwd > 0 ? floor(c_azi / inc) : cnaz[ri] - 1 - floor(c_azi / inc)) : 0);
+    }
   } else if (m->bin_kind == 1) {
     const double *N = m->nrm, *U = m->up;
     double inc_dz = -D[0] * N[0] - D[1] * N[1] - D[2] * N[2];

@@ -7,7 +7,8 @@ The compute runs in libradgpu.so (HIP kernels + C ABI, include/radgpu.h);
 build it with `python -m pyradiance_amd.build`.
 """
 
+from .rflux import rfluxmtx
 from .rt import Rcontrib, rtrace
 from .util import dcsky, dctimestep, read_matrix, write_matrix
 
-__all__ = ["rtrace", "Rcontrib", "dctimestep", "dcsky", "read_matrix", "write_matrix"]
+__all__ = ["rtrace", "Rcontrib", "rfluxmtx", "dctimestep", "dcsky", "read_matrix", "write_matrix"]

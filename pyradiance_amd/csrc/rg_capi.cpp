@@ -395,7 +395,9 @@ static std::vector<BinSpec> bin_specs(const rg_modifier* mods, int nmods, std::u
     if (!strcmp(m.name, "void")) throw std::runtime_error("cannot track 'void' modifier");
     if (slot.count(m.name)) throw std::runtime_error(std::string("duplicate modifier '") + m.name + "'");
     if (m.nbins < 1) throw std::runtime_error("bad bin count");
-    if (m.bin_kind < 0 || m.bin_kind > 4) throw std::runtime_error("bad bin kind");
+    if (m.bin_kind < 0 || m.bin_kind > 7) throw std::runtime_error("bad bin kind");
+    if ((m.bin_kind == BK_REINHART || m.bin_kind == BK_SHIRLEY_CHIU) && !(m.mf > 0))
+      throw std::runtime_error(std::string("bad MF / SCdim for modifier '") + m.name + "'");
     slot[m.name] = i;
     BinSpec& b = bins[i];
     memset(&b, 0, sizeof b);

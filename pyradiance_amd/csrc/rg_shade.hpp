@@ -112,7 +112,8 @@ struct RunCfg {
 
 enum : uint32_t { ERR_STACK = 1, ERR_CAPACITY = 2, ERR_NOMAT = 4, ERR_GEN = 8, ERR_PATH = 16, ERR_ADJ_DIRECT = 32 };
 
-// ---- bins (util/reinhartb.cal:19-47, util/klems_full.cal:10-43) ------------
+// ---- bins (util/reinhartb.cal:19-47, util/klems_full.cal:10-43,
+//      util/disk2square.cal:34-90, util/cieskyscan.cal:21-53) ------------
 RG_HD double cal_asin_deg(double x) {   // Asin() of reinhartb.cal
   const double DEGREE = PI_D / 180;
   double v = (x - 1 > 0) ? PI_D / 2 : ((-1 - x > 0) ? -PI_D / 2 : asin(x));
@@ -186,6 +187,57 @@ __device__ inline double klems_bin(V3 D, const BinSpec& b, int which) {
   return acc + azn;
 }
 
+// Oriented axis values shared by rbin, scbin and cbin (reinhartb.cal:27-30)
+struct IncAxes { double dz, rx, ry; };
+__device__ inline IncAxes inc_axes(V3 D, const BinSpec& b) {
+  double rNx = b.nrm[0], rNy = b.nrm[1], rNz = b.nrm[2];
+  double Ux = b.up[0], Uy = b.up[1], Uz = b.up[2];
+  IncAxes a;
+  a.dz = -D.x * rNx - D.y * rNy - D.z * rNz;
+  a.rx = -b.rhs * (D.x * (Uy * rNz - Uz * rNy) + D.y * (Uz * rNx - Ux * rNz) + D.z * (Ux * rNy - Uy * rNx));
+  a.ry = D.x * Ux + D.y * Uy + D.z * Uz + a.dz * (rNx * Ux + rNy * Uy + rNz * Uz);
+  return a;
+}
+
+// scbin of util/disk2square.cal:67-90: the incident direction projected to
+// the unit disk, mapped to the square (:34-64), binned SCdim x SCdim
+__device__ inline double shirley_chiu_bin(V3 D, const BinSpec& b) {
+  IncAxes a = inc_axes(D, b);
+  if (!(a.dz > 0)) return -1;
+  double den2 = a.rx * a.rx + a.ry * a.ry;
+  double radf = (den2 - 1e-7 > 0) ? sqrt((1 - a.dz * a.dz) / den2) : 0;
+  double x = a.rx * radf, y = -a.ry * radf;
+  double r = sqrt(x * x + y * y), phi = atan2(y, x);
+  if (-phi - PI_D / 4 > 0) phi += 2 * PI_D;
+  int rgn = (int)floor((phi + PI_D / 4) / (PI_D / 2)) + 1;
+  double q = r / (PI_D / 4), sa, sb;
+  if (rgn == 1) { sa = r; sb = phi * q; }
+  else if (rgn == 2) { sa = (PI_D / 2 - phi) * q; sb = r; }
+  else if (rgn == 3) { sa = -r; sb = (PI_D - phi) * q; }
+  else if (rgn == 4) { sa = (phi - 3 * PI_D / 2) * q; sb = -r; }
+  else { sa = r; sb = -r; }
+  return floor((sa + 1) / 2 * b.mf) * b.mf + floor((sb + 1) / 2 * b.mf);
+}
+
+// cbin of util/cieskyscan.cal:21-53: 145 CIE sky-scanner patches, 12-degree
+// rows, azimuth from -U winding alternately by row
+__device__ inline double cie_bin(V3 D, const BinSpec& b) {
+  IncAxes a = inc_axes(D, b);
+  double r_alt = cal_asin_deg(a.dz);
+  if (!(r_alt > 0)) return -1;
+  double c_azi = cal_atan2_deg(-a.rx, -a.ry);
+  double row = floor(r_alt / (90 / 7.5));
+  int ri = (int)row;
+  if (ri < 0 || ri > 7) return -1;
+  double naz = ri < 7 ? (double)kTnaz[ri] : 1.0;   // cnaz() = tnaz() plus the zenith patch
+  double acc = 0;
+  for (int k = 0; k < ri; ++k) acc += kTnaz[k];
+  double fwd = floor(row / 2 + 1.25) - row / 2 - .75;
+  double inc = 360 / naz;
+  double azn = (359.9999 - c_azi > 0) ? (fwd > 0 ? floor(c_azi / inc) : naz - 1 - floor(c_azi / inc)) : 0;
+  return acc + azn;
+}
+
 __device__ inline int eval_bin(V3 D, const BinSpec& b) {
   double bval;
   switch (b.kind) {
@@ -193,7 +245,10 @@ __device__ inline int eval_bin(V3 D, const BinSpec& b) {
     case BK_REINHART: bval = reinhart_bin(D, b); break;
     case BK_KLEMS_FULL: bval = klems_bin(D, b, 0); break;
     case BK_KLEMS_HALF: bval = klems_bin(D, b, 1); break;
-    default: bval = klems_bin(D, b, 2); break;
+    case BK_KLEMS_QUARTER: bval = klems_bin(D, b, 2); break;
+    case BK_HALFSPACE: bval = (-D.x * b.nrm[0] - D.y * b.nrm[1] - D.z * b.nrm[2] > 0) ? 0.0 : -1.0; break;
+    case BK_SHIRLEY_CHIU: bval = shirley_chiu_bin(D, b); break;
+    default: bval = cie_bin(D, b); break;
   }
   if (bval <= -.5) return -1;
   int bn = (int)(bval + .5);

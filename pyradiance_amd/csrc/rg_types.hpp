@@ -99,16 +99,21 @@ struct Prim {
   int32_t srcidx;    // source index if this primitive is a local source, else -1
 };
 
-// Bin functions evaluated per binned ray (util/reinhartb.cal, util/klems_*.cal)
+// Bin functions evaluated per binned ray: util/reinhartb.cal, util/klems_*.cal,
+// util/disk2square.cal (Shirley-Chiu scbin, SCdim held in `mf`) and
+// util/cieskyscan.cal (cbin).  BK_HALFSPACE is rfluxmtx's uniform receiver
+// surface, bin 0 for rays that meet it from the front:
+// if(-Dx*nx-Dy*ny-Dz*nz,0,-1), util/rfluxmtx.c:465-470.
 enum BinKind : int32_t { BK_CONST = 0, BK_REINHART = 1, BK_KLEMS_FULL = 2,
-                         BK_KLEMS_HALF = 3, BK_KLEMS_QUARTER = 4 };
+                         BK_KLEMS_HALF = 3, BK_KLEMS_QUARTER = 4, BK_HALFSPACE = 5,
+                         BK_SHIRLEY_CHIU = 6, BK_CIE = 7 };
 
 struct BinSpec {
   int32_t kind;      // BinKind
   int32_t nbins;
   int32_t col0;      // first output column (in units of bins)
   int32_t cbin;      // constant bin for BK_CONST
-  double mf;         // Reinhart subdivision
+  double mf;         // Reinhart subdivision MF, or Shirley-Chiu SCdim
   double rhs;        // +1 right-handed, -1 left-handed
   double nrm[3];     // rNx,rNy,rNz (Reinhart) or Nx,Ny,Nz (Klems)
   double up[3];      // Ux,Uy,Uz

@@ -106,6 +106,15 @@ def parse(argv: list[str], rcontrib: bool = False) -> Options:
             o.yres = int(nums(1)[0])
         elif a.startswith("-f") and len(a) == 4 and a[2] in "afdc" and a[3] in "afdc":
             o.inform, o.outform = a[2], a[3]
+        elif a.startswith("-f") and len(a) == 3 and a[2] in "afd":
+            o.inform = o.outform = a[2]          # one letter: same format both ways (rt/rcmain.c:141-144)
+        elif rcontrib and a in ("-fo", "-fo+", "-fo-"):
+            o.ignored.append(a)                  # force overwrite of -o files; output goes to stdout only
+        elif a.startswith("-w") and a[2:] in ("", "+", "-"):
+            o.ignored.append(a)                  # warning messages on/off
+        elif a in ("-ld", "-ld+", "-ld-"):
+            if a != "-ld-":
+                raise ValueError(f"{a}: limited ray distance is not supported on the GPU path")
         elif a.startswith("-o") and len(a) > 2 and not rcontrib:
             o.outspec = a[2:]
         elif a in _IGNORED:

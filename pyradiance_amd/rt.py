@@ -31,21 +31,34 @@ def _rterr(msg: str, code: int = 1) -> RuntimeError:
 
 def _scene(octree: Path | str) -> "_lib.Context":
     """One committed context per scene file, reused while the file is unchanged."""
-    p = Path(octree)
-    try:
-        st = p.stat()
-    except OSError:
-        raise _rterr(f"cannot open octree file \"{octree}\"")
-    key = (str(p.resolve()), st.st_mtime_ns, st.st_size)
-    ctx = _contexts.get(key[0])
+    return _scene_files([octree])
+
+
+def _scene_files(paths: Sequence[Path | str]) -> "_lib.Context":
+    """One committed context per ordered list of octree / scene files (loaded
+    in order, as oconv reads its arguments), reused while none of them
+    changes."""
+    key = []
+    for f in paths:
+        p = Path(f)
+        try:
+            st = p.stat()
+        except OSError:
+            raise _rterr(f"cannot open octree file \"{f}\"" if len(paths) == 1 else f"cannot open scene file \"{f}\"")
+        key.append((str(p.resolve()), st.st_mtime_ns, st.st_size))
+    name = "\0".join(k[0] for k in key)
+    ctx = _contexts.get(name)
     if ctx is not None and ctx[0] == key:
         return ctx[1]
     c = _lib.Context(0)
     try:
-        c.load(str(p)).commit()
+        for f in paths:
+            c.load(str(f))
+        c.commit()
     except RuntimeError as e:
+        c.close()
         raise _rterr(str(e))
-    _contexts[key[0]] = (key, c)
+    _contexts[name] = (key, c)
     return c
 
 
@@ -77,10 +90,18 @@ def _octree_header(octree: Path | str) -> list[str]:
     return [ln for ln in lines if ln and not ln.startswith("FORMAT=")]
 
 
-def _header(prog: str, argv: list[str], octree, outform: str, extra: str = "") -> bytes:
+def _header(prog: str, argv: list[str], octree, outform: str, extra: str = "",
+            scene_lines: list[str] | None = None) -> bytes:
+    """Output header as rtrace / rcontrib write it (rt/rtmain.c:418-424,
+    rt/rc2.c:97-125): the scene's own header lines (or `scene_lines`), the
+    command without the octree argument, SOFTWARE, CAPDATE/GMT, NCOMP, `extra`,
+    byte order for binary output, FORMAT."""
     fmt = {"a": "ascii", "f": "float", "d": "double", "c": "32-bit_rle_rgbe"}[outform]
-    lines = ["#?RADIANCE", *_octree_header(octree), " ".join([prog, *argv, str(octree)]),
-             f"SOFTWARE= {VERSION}", time.strftime("CAPDATE= %Y:%m:%d %H:%M:%S"), "NCOMP=3"]
+    now = time.time()
+    lines = ["#?RADIANCE", *(_octree_header(octree) if scene_lines is None else scene_lines),
+             " ".join([prog, *argv]), f"SOFTWARE= {VERSION}",
+             time.strftime("CAPDATE= %Y:%m:%d %H:%M:%S", time.localtime(now)),
+             time.strftime("GMT= %Y:%m:%d %H:%M:%S", time.gmtime(now)), "NCOMP=3"]
     if extra:
         lines.append(extra.rstrip("\n"))
     if outform in "fd":
@@ -199,6 +220,23 @@ _KDIR = {"N": ((0, -1, 0), (0, 0, 1)), "E": ((-1, 0, 0), (0, 0, 1)), "S": ((0, 1
          "W": ((1, 0, 0), (0, 0, 1)), "D": ((0, 0, -1), (0, 1, 0))}
 
 
+_FLT = r"([-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)"
+# rfluxmtx's uniform surface receiver bin (util/rfluxmtx.c:465-470)
+_HALFSPACE = re.compile(r"if\(-Dx\*" + _FLT + r"-Dy\*" + _FLT + r"-Dz\*" + _FLT + r",0,-1\)")
+
+
+def _oriented(m, vars_: dict, defaults: tuple | None, cal: str) -> None:
+    """rNx..rNz, Ux..Uz of a binning cal file; without defaults they must be set."""
+    keys = ("rNx", "rNy", "rNz", "Ux", "Uy", "Uz")
+    if defaults is None:
+        miss = [k for k in keys if k not in vars_]
+        if miss:
+            raise ValueError(f"{cal}: undefined variable {miss[0]}")
+        defaults = (0.0,) * 6
+    v = [vars_.get(k, d) for k, d in zip(keys, defaults)]
+    m.nrm[:], m.up[:] = v[:3], v[3:]
+
+
 def _modifier_spec(name: str, calfile, expression, nbins, binv, param) -> "_lib.RgModifier":
     """Map a modifier's -f/-e/-p/-bn/-b arguments onto a GPU bin function."""
     m = _lib.RgModifier()
@@ -236,11 +274,37 @@ def _modifier_spec(name: str, calfile, expression, nbins, binv, param) -> "_lib.
             m.nrm[:], m.up[:] = v[:3], v[3:]
         m.nbins = n if nbins in (None, nname) else int(float(nbins))
         return m
+    if cal == "disk2square.cal":   # Shirley-Chiu square bins, util/disk2square.cal:67-90
+        if binv not in (None, "scbin"):
+            raise ValueError(f"bin expression {binv!r} is not supported with {cal}")
+        if "SCdim" not in vars_:
+            raise ValueError(f"{cal}: undefined variable SCdim")
+        m.bin_kind, m.mf = 6, vars_["SCdim"]
+        _oriented(m, vars_, None, cal)
+        n = int(m.mf) * int(m.mf)
+        m.nbins = n if nbins in (None, "SCdim*SCdim") else int(float(nbins))
+        return m
+    if cal == "cieskyscan.cal":   # CIE sky-scanner patches, util/cieskyscan.cal:21-53
+        if binv not in (None, "cbin"):
+            raise ValueError(f"bin expression {binv!r} is not supported with {cal}")
+        m.bin_kind = 7
+        _oriented(m, vars_, (0.0, 0.0, -1.0, 0.0, 1.0, 0.0), cal)
+        m.nbins = 145 if nbins in (None, "Ncbins") else int(float(nbins))
+        return m
     if cal is not None:
         raise ValueError(f"bin file {cal} is not supported on the GPU path")
-    m.bin_kind = 0
-    m.cbin = int(float(binv)) if binv is not None else 0
     m.nbins = int(float(nbins)) if nbins is not None else 1
+    b = (binv or "0").replace(" ", "")
+    mm = _HALFSPACE.fullmatch(b)
+    if mm:
+        m.bin_kind = 5
+        m.nrm[:] = [float(x) for x in mm.groups()]
+        return m
+    m.bin_kind = 0
+    try:
+        m.cbin = int(float(b))
+    except ValueError:
+        raise ValueError(f"bin expression {binv!r} is not supported on the GPU path")
     return m
 
 
@@ -301,32 +365,55 @@ class Rcontrib:
         return self._run()[0]
 
     def _run(self):
-        try:
-            opt = _params.parse(self.argv, rcontrib=True)
-            mods = [_modifier_spec(*m) for m in self.modifiers]
-        except ValueError as e:
-            raise _rterr(str(e))
-        if not mods:
-            raise _rterr("missing required modifier argument")
-        ctx = _scene(self.octree)
-        opt.params.seed = _seed(opt.uncorrelated)
-        inp = self.inp if isinstance(self.inp, bytes) else bytes(self.inp)
-        vals = _read_rays(inp, opt.inform)
-        acc = max(opt.accumulate, 1)
-        nrec = len(vals) // acc
-        if nrec * acc != len(vals):
-            vals = vals[:nrec * acc]
-        flags = (_lib.RT_IMM_IRRAD if opt.imm_irrad else 0) | (_lib.RT_LAMB_IRRAD if opt.lamb_irrad else 0)
-        try:
-            ctx.set_params(opt.params)
-            out = ctx.contrib(vals, mods, acc, flags)
-        except RuntimeError as e:
-            raise _rterr(str(e))
-        return out, opt
+        return _contrib_run(self.argv, self.modifiers, self.inp, lambda: _scene(self.octree))
 
     def __call__(self) -> bytes:
-        out, opt = self._run()
-        body = _format(out.ravel(), opt.outform, "%.6e", out.shape[1] if out.size else 3)
-        if not opt.header:
-            return body
-        return _header("rcontrib", self.argv, self.octree, opt.outform) + body
+        out, opt, mods = self._run()
+        return _contrib_bytes(out, opt, mods, "rcontrib", self.argv, self.octree)
+
+
+def _contrib_run(argv: list[str], modifiers: list, inp, scene):
+    """rcontrib's record loop on the GPU: parse `argv`, map each modifier
+    tuple (name, calfile, expression, nbins, binv, param) onto a bin function,
+    read the rays and accumulate -c records.  `scene()` returns the committed
+    context.  Returns (matrix, options, modifier specs)."""
+    try:
+        opt = _params.parse(argv, rcontrib=True)
+        mods = [_modifier_spec(*m) for m in modifiers]
+    except ValueError as e:
+        raise _rterr(str(e))
+    if not mods:
+        raise _rterr("missing required modifier argument")
+    ctx = scene()
+    opt.params.seed = _seed(opt.uncorrelated)
+    inp = inp if isinstance(inp, bytes) else bytes(inp)
+    vals = _read_rays(inp, opt.inform)
+    acc = max(opt.accumulate, 1)
+    nrec = len(vals) // acc
+    if nrec * acc != len(vals):
+        vals = vals[:nrec * acc]
+    flags = (_lib.RT_IMM_IRRAD if opt.imm_irrad else 0) | (_lib.RT_LAMB_IRRAD if opt.lamb_irrad else 0)
+    if nrec == 0:
+        return np.zeros((0, 3 * sum(m.nbins for m in mods))), opt, mods
+    try:
+        ctx.set_params(opt.params)
+        out = ctx.contrib(vals, mods, acc, flags)
+    except RuntimeError as e:
+        raise _rterr(str(e))
+    return out, opt, mods
+
+
+def _contrib_bytes(out: np.ndarray, opt, mods: list, prog: str, argv: list[str], octree,
+                   scene_lines: list[str] | None = None) -> bytes:
+    """Formatted rcontrib output to stdout with its header (rt/rc2.c:150-170):
+    NROWS when -y is set, NCOLS = total bins over the modifiers."""
+    body = _format(out.ravel(), opt.outform, "%.6e", out.shape[1] if out.size else 3)
+    if not opt.header:
+        return body
+    reclen = sum(m.nbins for m in mods)
+    info = []
+    if opt.yres > 0:
+        info.append(f"NROWS={opt.yres * max(opt.xres, 1)}")
+    if opt.xres <= 0 or reclen > 1:
+        info.append(f"NCOLS={reclen}")
+    return _header(prog, argv, octree, opt.outform, "\n".join(info), scene_lines) + body

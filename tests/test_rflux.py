@@ -1,0 +1,139 @@
+"""rfluxmtx front end on the CPU: receiver parsing and option screening
+against the reference rfluxmtx's own command lines and messages, and the new
+bin bases (Shirley-Chiu, CIE, uniform surface, tilted left-handed frames)
+through the oracle against the reference cal files."""
+
+import json
+import math
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from pyradiance_amd import rflux, rt
+
+CASES = json.loads((GOLDEN / "rflux_cases.json").read_text())
+RECEIVERS = {k: v for k, v in CASES.items() if not k.startswith("_")}
+
+
+@pytest.mark.parametrize("name", sorted(k for k, v in RECEIVERS.items() if v["status"] == 0))
+def test_receiver_args_match_reference(name):
+    case = RECEIVERS[name]
+    argv, recs, _ = rflux.command(f"{name}.rad", [], text=case["text"])
+    assert argv == case["args"]
+    assert [r.modifier for r in recs] == [case["args"][i + 1] for i, a in enumerate(case["args"]) if a == "-m"]
+
+
+@pytest.mark.parametrize("name", sorted(k for k, v in RECEIVERS.items() if v["status"] != 0))
+def test_receiver_errors_match_reference(name):
+    case = RECEIVERS[name]
+    with pytest.raises(RuntimeError) as e:
+        rflux.command(f"{name}.rad", [], text=case["text"])
+    assert str(e.value).endswith(case["messages"][-1])
+
+
+@pytest.mark.parametrize("key", sorted(CASES["_options"]))
+def test_option_screen_matches_reference(key):
+    case = CASES["_options"][key]
+    text = RECEIVERS["skyr4"]["text"]
+    if case["args"] is None:
+        with pytest.raises(RuntimeError):
+            rflux.command("skyr4.rad", case["options"], text=text)
+    else:
+        argv, _, _ = rflux.command("skyr4.rad", case["options"], text=text)
+        assert argv == case["args"]
+
+
+def test_receiver_geometry():
+    """Normals and areas from the surface definitions: vector area of a
+    polygon, ring axis and annulus area, a source's reversed direction and
+    cap area pi*sin^2(theta/2)."""
+    text = ("#@rfluxmtx h=u\nvoid glow a 0 0 4 1 1 1 0\n"
+            "a polygon p 0 0 12 0 0 0 2 0 0 2 3 0 0 3 0\n"
+            "#@rfluxmtx h=kf\nvoid glow b 0 0 4 1 1 1 0\n"
+            "b ring r 0 0 8 0 0 0 0 4 3 1 2\n"
+            "#@rfluxmtx h=u\nvoid glow c 0 0 4 1 1 1 0\n"
+            "c source s 0 0 4 0 0 2 60\n")
+    a, b, c = rflux.parse_receiver("t.rad", text)
+    assert a.surfaces[0].area == pytest.approx(6.0) and np.allclose(a.nrm, (0, 0, 1))
+    assert b.surfaces[0].area == pytest.approx(3 * math.pi) and np.allclose(b.nrm, (0, 0.8, 0.6))
+    assert c.surfaces[0].area == pytest.approx(math.pi * math.sin(math.radians(30)) ** 2)
+    assert np.allclose(c.nrm, (0, 0, -1))
+    # the default up vector: +Z unless the normal is within ~45 degrees of Z
+    assert b.vup == (0.0, 0.0, 1.0) and a.vup == (0.0, 1.0, 0.0)
+
+
+def test_sender_surface_is_refused():
+    with pytest.raises(RuntimeError, match="sender surfaces"):
+        rflux.rfluxmtx("x.rad", surface="window.rad", rays=b"")
+
+
+def _oracle_mod(oracle, m):
+    return oracle.mod(m.name.decode(), m.bin_kind, m.nbins, m.cbin, m.mf, m.rhs, tuple(m.nrm), tuple(m.up))
+
+
+FLUX = np.load(GOLDEN / "bins_flux.npz")
+FLUX_NAMES = sorted(k for k in FLUX.files if k != "dirs" and not k.endswith("_spec"))
+
+
+def flux_modifier(name):
+    cal, prm, nb, bv = (str(x) for x in FLUX[name + "_spec"])
+    return rt._modifier_spec("skyglow", cal or None, None, nb, bv, prm or None)
+
+
+@pytest.mark.parametrize("name", FLUX_NAMES)
+def test_oracle_flux_bins_match_reference(oracle, name):
+    m = flux_modifier(name)
+    assert np.array_equal(oracle.bins(_oracle_mod(oracle, m), FLUX["dirs"]), FLUX[name])
+
+
+def test_shirley_chiu_bins_are_equal_projected_solid_angle(oracle):
+    """The Shirley-Chiu map is area preserving from the disk to the square, so
+    cosine-weighted directions fill the SCdim^2 bins evenly."""
+    rng = np.random.default_rng(3)
+    u, v = rng.random(200000), rng.random(200000)
+    r, phi = np.sqrt(u), 2 * np.pi * v
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), np.sqrt(1 - u)], 1)   # towards +Z
+    m = rt._modifier_spec("s", "disk2square.cal", None, "SCdim*SCdim", "scbin",
+                          "SCdim=5,rNx=0,rNy=0,rNz=-1,Ux=0,Uy=1,Uz=0,RHS=+1")
+    b = oracle.bins(_oracle_mod(oracle, m), d)
+    assert (b >= 0).all()
+    cnt = np.bincount(b, minlength=25)
+    exp = len(d) / 25
+    assert ((cnt - exp) ** 2 / exp).sum() < 80          # chi-square, 24 dof
+
+
+def test_cie_bins_follow_patch_solid_angles(oracle):
+    """Uniform directions over the upper hemisphere land in each CIE row in
+    proportion to the row's solid angle (12-degree bands, zenith cap)."""
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=(200000, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    d[:, 2] = np.abs(d[:, 2])
+    m = rt._modifier_spec("s", "cieskyscan.cal", None, "Ncbins", "cbin", None)
+    b = oracle.bins(_oracle_mod(oracle, m), d)
+    naz = [30, 30, 24, 24, 18, 12, 6, 1]
+    start = np.cumsum([0] + naz)
+    row = np.searchsorted(start, b, side="right") - 1
+    frac = np.bincount(row, minlength=8) / len(d)
+    edges = np.radians([0, 12, 24, 36, 48, 60, 72, 84, 90])
+    want = np.sin(edges[1:]) - np.sin(edges[:-1])
+    assert np.allclose(frac, want, atol=0.005)
+
+
+def test_oracle_box_matches_reference_rfluxmtx(oracle):
+    """The box receivers of every basis and one occluder: the oracle run with
+    the parsed receivers equals the reference rfluxmtx -ab 0, ray by ray."""
+    g = np.load(GOLDEN / "rflux_box.npz")
+    recs = rflux.parse_receiver("box.rad", str(g["receiver"]))
+    mods = [_oracle_mod(oracle, rt._modifier_spec(*r.spec)) for r in recs]
+    scene = oracle.Scene(str(g["scene"]) + str(g["receiver"]))
+    p = oracle.params(ambounce=0)
+    out, _ = scene.contrib(g["rays"], p, mods)
+    v = out[:, 0::3]
+    assert v.shape[1] == int(g["ncols"])
+    nz = v != 0
+    assert (nz.sum(1) <= 1).all()
+    col = np.where(nz.any(1), nz.argmax(1), -1)
+    assert np.array_equal(col, g["col"])
+    assert np.array_equal(v[np.arange(len(v)), np.maximum(col, 0)] * (col >= 0), g["value"])
