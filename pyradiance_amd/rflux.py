@@ -577,6 +577,96 @@ def _oconv_line(files: list[str], receiver: str, nowarn: bool) -> str:
     return "oconv -f " + ("-w " if nowarn else "") + " ".join([*(q(f) for f in files), q(receiver)])
 
 
+class _SenderParser:
+    """add_send_object() of util/rfluxmtx.c:1153-1195: every polygon or ring
+    of the sender file is one sender element; the directives seen so far
+    apply to the whole sender."""
+
+    def __init__(self, nowarn: bool):
+        self.recv = _ReceiverParser(nowarn)     # add_surface() with the same rules
+        self.recv.cur = _Params(sign="")
+        self.curmod = ""
+        self.warnings: list[str] = []
+
+    def on_object(self, rd: _SceneReader) -> bool:
+        thismod, otype, oname = rd.token(), rd.token(), rd.token()
+        if oname is None:
+            rd.i = len(rd.t)
+            return False
+        if otype == "alias":
+            rd.token()
+            return False
+        if otype in _SURF_TYPES:
+            if otype == "source":
+                raise _err("cannot use source as a sender!")
+            if thismod != self.curmod:
+                if self.curmod:
+                    self.warnings.append("warning - multiple modifiers in sender")
+                self.curmod = thismod
+            _parse_params(self.recv.cur, rd.newparams)
+            rd.newparams = ""
+            self.recv.add_surface(otype, oname, rd)
+            return False
+        rd.skip_args()
+        return False
+
+
+@dataclass
+class Sender:
+    """Prepared sender basis (prepare_sampler(), util/rfluxmtx.c:928-1010)."""
+    hemis: str
+    hsiz: int
+    sign: str
+    nrm: list
+    vup: list
+    udir: list
+    vdir: list
+    surfaces: list
+    nbins: int
+
+
+def parse_sender(surface: str | Path, text: str | None = None, nowarn: bool = False) -> Sender:
+    """The sender surfaces and their hemisphere basis, checked and oriented
+    as prepare_sampler() does."""
+    if text is None:
+        try:
+            text = Path(surface).read_text()
+        except OSError:
+            raise _err(f"cannot load '{surface}'")
+    sp = _SenderParser(nowarn)
+    _SceneReader(text, str(surface)).run(sp.on_object)
+    p = sp.recv.cur
+    if not p.slist:
+        raise _err("no sender surface!")
+    if not p.hemis:
+        raise _err("missing sender sampling type!")
+    if _normalize(p.nrm) == 0:
+        raise _err("undefined normal for sender sampling")
+    if _normalize(p.vup) == 0:
+        p.vup = [0.0, 0.0, 1.0] if abs(p.nrm[2]) < .7 else [0.0, 1.0, 0.0]
+    n, up = p.nrm, p.vup
+    udir = [up[1] * n[2] - up[2] * n[1], up[2] * n[0] - up[0] * n[2], up[0] * n[1] - up[1] * n[0]]
+    if _normalize(udir) == 0:
+        raise _err("up vector coincides with sender normal")
+    vdir = [n[1] * udir[2] - n[2] * udir[1], n[2] * udir[0] - n[0] * udir[2], n[0] * udir[1] - n[1] * udir[0]]
+    if p.sign == "-":
+        udir = [-x for x in udir]
+    h = p.hemis
+    h0 = h[0].lower()
+    if h0 == "u" or h[0] == "1" or (h0 == "s" and h[1:2].lower() == "c") or h0 in ("r", "t"):
+        pass
+    elif h0 == "k":
+        k = {"1": "1", "2": "2", "4": "4", "f": "1", "F": "1", "": "1", "h": "2", "H": "2",
+             "q": "4", "Q": "4"}.get(h[1:2])
+        if k is None:
+            raise _err(f"unrecognized sender sampling: h={h}")
+        h = "k" + k
+    else:
+        raise _err(f"unrecognized sender sampling: h={h}")
+    from . import sender as _snd
+    return Sender(h, p.hsiz, p.sign, list(n), list(p.vup), udir, vdir, list(p.slist), _snd.nbins(h, p.hsiz))
+
+
 def rfluxmtx(
     receiver: str | Path,
     surface: None | str | Path = None,
@@ -584,43 +674,71 @@ def rfluxmtx(
     params: None | Sequence[str] = None,
     octree: None | Path | str = None,
     scene: None | Sequence[Path | str] = None,
+    seed: int | None = None,
 ) -> bytes:
     """rfluxmtx with the signature of pyradiance (src/pyradiance/util.py:833-871).
 
     Sender "-" (rays given, no `surface`): each input ray (or each -c group
     of rays) is one output record over the receivers' bins, as
     `rcontrib -fo+ [params] -faa -c N <receiver bins> <system + receiver>`.
-    Sender surfaces are not sampled on the GPU path and raise an error."""
-    if surface is not None:
-        raise _err("sender surfaces are not supported on the GPU path (use sender '-' with rays)")
-    argv, recs, s = command(receiver, params)
+
+    Sender surface: one record per bin of the sender's basis, each the mean
+    of -c N (default 10000) rays sampled over that bin and the sender
+    surfaces (util/rfluxmtx.c:1447-1474); `seed` seeds the sampling (fixed by
+    default, as the reference's unseeded generator)."""
+    argv, recs, s, snd = command(receiver, params, sender=surface)
     if any(r.args[0] == "-o" for r in recs):
         raise _err("receiver output files (o=) are not supported on the GPU path")
     files = (["-i", str(octree)] if octree is not None else []) + [str(f) for f in (scene or [])]
     oconv = _oconv_line(files, str(receiver), s.nowarn)
     paths = [f for f in files if f != "-i"] + [str(receiver)]
-    nopt = len(argv) - sum(len(r.args) for r in recs)
-    out, opt, mods = _rt._contrib_run(argv[:nopt], [r.spec for r in recs], rays if rays is not None else b"",
-                                      lambda: _rt._scene_files(paths))
+    nrecv = sum(len(r.args) for r in recs)
+    if snd is None:
+        opts = argv[:len(argv) - nrecv]
+        inp = rays if rays is not None else b""
+    else:
+        import numpy as np
+        from . import sender as _snd
+        opts = argv[:len(argv) - nrecv - 2] + argv[-2:]
+        rng = np.random.default_rng(0 if seed is None else seed)
+        try:
+            inp = _snd.sender_rays(snd, s.sampcnt, rng).astype("<f8").tobytes()
+        except ValueError as e:
+            raise _err(str(e))
+    out, opt, mods = _rt._contrib_run(opts, [r.spec for r in recs], inp, lambda: _rt._scene_files(paths))
     # rcontrib's octree is the "!oconv ..." command: its header is that line
     # (rt/rc2.c:101-105)
     return _rt._contrib_bytes(out, opt, mods, "rcontrib", argv, None, scene_lines=[oconv])
 
 
-def command(receiver: str | Path, params: None | Sequence[str] = None, text: str | None = None):
-    """The rcontrib argument list rfluxmtx forms in pass-through mode
-    (util/rfluxmtx.c:1401-1446), without the program name and the octree:
-    (argv, receivers, screened options)."""
+def command(receiver: str | Path, params: None | Sequence[str] = None, text: str | None = None,
+            sender: str | Path | None = None, sender_text: str | None = None):
+    """The rcontrib argument list rfluxmtx forms (util/rfluxmtx.c:1401-1459),
+    without the program name and the octree: (argv, receivers, screened
+    options, prepared sender or None for pass-through)."""
     s = _screen(params or [])
-    recs = parse_receiver(receiver, text, nowarn=s.nowarn, binjitter=s.binjitter)
+    passthru = sender is None and sender_text is None
+    if not passthru and s.iropt:
+        raise _err("-i, -I supported for pass-through only")
     argv = ["-fo+", *s.passthru]
-    if s.iropt:
-        argv.append(s.iropt)
-    if s.xrs:
-        argv += ["-x", s.xrs]
-    if s.yrs:
-        argv += ["-y", s.yrs]
-    if s.ldopt:
-        argv.append(s.ldopt)
-    argv += [s.fmt, "-c", str(s.sampcnt if s.sampcnt > 0 else 1)]
-    return argv + [a for r in recs for a in r.args], recs, s
+    if passthru:
+        if s.iropt:
+            argv.append(s.iropt)
+        if s.xrs:
+            argv += ["-x", s.xrs]
+        if s.yrs:
+            argv += ["-y", s.yrs]
+        if s.ldopt:
+            argv.append(s.ldopt)
+        s.sampcnt = s.sampcnt if s.sampcnt > 0 else 1
+    else:
+        s.fmt = s.fmt[:2] + "d" + s.fmt[3]      # rays go to rcontrib as binary doubles
+        s.sampcnt = s.sampcnt if s.sampcnt > 0 else 10000
+    argv += [s.fmt, "-c", str(s.sampcnt)]
+    recs = parse_receiver(receiver, text, nowarn=s.nowarn, binjitter=s.binjitter)
+    argv += [a for r in recs for a in r.args]
+    snd = None
+    if not passthru:
+        snd = parse_sender(sender, sender_text, nowarn=s.nowarn)
+        argv += ["-y", str(snd.nbins)]
+    return argv, recs, s, snd

@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden/ from the reference tools
 built into oracle/_ref (rtrace, rcontrib, oconv) from the reference sources.
 
-    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux]
+    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send]
 
 Fixtures (inputs and the reference's outputs only):
   c1_seed{0,1,2}.npz     config-1 room: sensors, rtrace -I -ab 0 -dj 0 -ds 0 values
@@ -20,6 +20,8 @@ Fixtures (inputs and the reference's outputs only):
                          forms from them, its exit status and messages
   rflux_box.npz          rays from the origin into receivers of every basis:
                          the non-zero output column and value of rfluxmtx -ab 0
+  rflux_send.npz         sender surfaces of every basis: rfluxmtx's rcontrib
+                         arguments and flux matrices (-ab 0, -c N samples)
 The config-1/2 scenes come from pyradiance_amd.scenes (seeded) and are not
 stored; the two scenes above are stored as text with their vectors.
 """
@@ -342,6 +344,56 @@ RFLUX_OPTS = [
 ]
 
 
+# Sender surfaces (rfluxmtx sampling mode) with their receivers; -ab 0 and
+# no other geometry, so every sample ray ends on a receiver with radiance 1.
+RFLUX_SEND = {
+    "window_kf": dict(
+        sender="#@rfluxmtx h=kf u=Z\nvoid polygon win 0 0 12 -.5 0 0  .5 0 0  .5 0 1.5  -.5 0 1.5\n",
+        receiver="#@rfluxmtx h=u\nvoid glow gg 0 0 4 1 1 1 0\ngg source ground 0 0 4 0 0 -1 180\n"
+                 "#@rfluxmtx h=r1 u=Y\nvoid glow sg 0 0 4 1 1 1 0\nsg source sky 0 0 4 0 0 1 180\n",
+        count=1000),
+    "ring_r1": dict(
+        sender="#@rfluxmtx h=r1 u=Y\nvoid ring lamp 0 0 8 0 0 2 0 0 1 0.1 0.4\n",
+        receiver="#@rfluxmtx h=kq u=Y\nvoid glow fl 0 0 4 1 1 1 0\n"
+                 "fl polygon floor 0 0 12 -10 -10 0 10 -10 0 10 10 0 -10 10 0\n"
+                 "#@rfluxmtx h=u\nvoid glow gg 0 0 4 1 1 1 0\ngg source ground 0 0 4 0 0 -1 180\n"
+                 "#@rfluxmtx h=u\nvoid glow sg 0 0 4 1 1 1 0\nsg source sky 0 0 4 0 0 1 180\n",
+        count=1000),
+    "pair_u": dict(
+        sender="#@rfluxmtx h=u\nvoid polygon a 0 0 12 0 0 0 0 0 1 1 0 1 1 0 0\n"
+               "void polygon b 0 0 12 1 0 0 1 0 1 1.8 0.6 1 1.8 0.6 0\n",
+        receiver="#@rfluxmtx h=sc4 u=Z\nvoid glow w 0 0 4 1 1 1 0\nw source half 0 0 4 0 -1 0 180\n"
+                 "#@rfluxmtx h=u\nvoid glow b 0 0 4 1 1 1 0\nb source back 0 0 4 0 1 0 180\n",
+        count=40000),
+    "sc3_cie": dict(
+        sender="#@rfluxmtx h=-sc3 u=Y\nvoid polygon s 0 0 9 0 0 0 0 1 0 1 0 0\n",
+        receiver="#@rfluxmtx h=cie u=Y\nvoid glow sg 0 0 4 1 1 1 0\nsg source sky 0 0 4 0 0 1 180\n"
+                 "#@rfluxmtx h=u\nvoid glow gg 0 0 4 1 1 1 0\ngg source ground 0 0 4 0 0 -1 180\n",
+        count=4000),
+}
+
+
+def rflux_send():
+    """The reference rfluxmtx in sampling mode (sender surfaces): argument
+    lists (-v) and the flux matrices, -ab 0, binary double output."""
+    import shlex
+    env = dict(ENV, PATH=f"{REF}:{os.environ.get('PATH', '')}")
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, c in RFLUX_SEND.items():
+            (Path(td) / "send.rad").write_text(c["sender"])
+            (Path(td) / "recv.rad").write_text(c["receiver"])
+            r = subprocess.run([str(REF / "rfluxmtx"), "-v", "-h", "-ab", "0", "-ffd", "-c", str(c["count"]),
+                                "send.rad", "recv.rad"], capture_output=True, env=env, cwd=td, check=True)
+            line = [ln for ln in r.stderr.decode().splitlines() if "opening pipe" in ln or ": running: " in ln]
+            args = shlex.split(line[-1].split(": ", 2)[2])[1:-1]
+            v = np.frombuffer(r.stdout, dtype="<f8")
+            out[name + "_args"] = np.array(args)
+            out[name] = v.reshape(int(args[args.index("-y") + 1]), -1)
+            out[name + "_text"] = np.array([c["sender"], c["receiver"], str(c["count"])])
+    np.savez_compressed(HERE / "rflux_send.npz", **out)
+
+
 def rflux():
     """The reference rfluxmtx (sender "-"): the rcontrib arguments it forms
     for each receiver file (-v), its error messages, and the output of the
@@ -410,7 +462,7 @@ if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
     jobs = {"config1": config1, "bins": bins, "stat_room": stat_room, "primitives": primitives,
-            "shading": shading, "bins_flux": bins_flux, "rflux": rflux}
+            "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send}
     for name in sys.argv[1:] or list(jobs):
         jobs[name]()
     print("golden vectors written to", HERE)

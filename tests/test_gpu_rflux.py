@@ -119,3 +119,34 @@ def test_rfluxmtx_equals_rcontrib_with_its_arguments(L, tmp_path):
         rc.add_modifier(modifier=name, calfile=cal, expression=expr, nbins=nbins, binv=binv, param=param)
     b = rc()
     assert len(a) > 0 and a == b
+
+
+SEND = np.load(GOLDEN / "rflux_send.npz")
+SEND_NAMES = sorted(k for k in SEND.files if not k.endswith(("_args", "_text")))
+
+
+@pytest.mark.parametrize("name", SEND_NAMES)
+def test_rfluxmtx_sender_matches_reference(L, oracle, tmp_path, name):
+    """Sampling mode: the GPU flux matrix for a sender surface agrees with
+    the reference rfluxmtx within Monte Carlo error, and equals the oracle
+    traced over the same sample rays."""
+    from pyradiance_amd import rflux, rfluxmtx, rt, sender
+    _say(f"rfluxmtx sender {name}")
+    snd, rcv, cnt = (str(x) for x in SEND[name + "_text"])
+    (tmp_path / "send.rad").write_text(snd)
+    (tmp_path / "recv.rad").write_text(rcv)
+    params = ["-h", "-ab", "0", "-ffd", "-c", cnt]
+    out = rfluxmtx(tmp_path / "recv.rad", surface=tmp_path / "send.rad", params=params, seed=3)
+    want = SEND[name]
+    got = np.frombuffer(out, dtype="<f8").reshape(want.shape)
+    cnt = int(cnt)
+    p = 0.5 * (got + want)
+    tol = 5 * np.sqrt(2 * p * (1 - np.minimum(p, 1)) / cnt) + 3.0 / cnt
+    assert (np.abs(got - want) <= tol).all()
+    assert np.allclose(got[:, 0::3].sum(1), 1.0)
+    _, recs, _, sp = rflux.command(tmp_path / "recv.rad", params, sender=tmp_path / "send.rad")
+    rays = sender.sender_rays(sp, cnt, np.random.default_rng(3))
+    mods = [oracle.mod(m.name.decode(), m.bin_kind, m.nbins, m.cbin, m.mf, m.rhs, tuple(m.nrm), tuple(m.up))
+            for m in (rt._modifier_spec(*r.spec) for r in recs)]
+    ref, _ = oracle.Scene(rcv).contrib(rays, oracle.params(ambounce=0), mods, accum=cnt)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-15)

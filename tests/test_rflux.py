@@ -19,7 +19,7 @@ RECEIVERS = {k: v for k, v in CASES.items() if not k.startswith("_")}
 @pytest.mark.parametrize("name", sorted(k for k, v in RECEIVERS.items() if v["status"] == 0))
 def test_receiver_args_match_reference(name):
     case = RECEIVERS[name]
-    argv, recs, _ = rflux.command(f"{name}.rad", [], text=case["text"])
+    argv, recs, _, _ = rflux.command(f"{name}.rad", [], text=case["text"])
     assert argv == case["args"]
     assert [r.modifier for r in recs] == [case["args"][i + 1] for i, a in enumerate(case["args"]) if a == "-m"]
 
@@ -40,7 +40,7 @@ def test_option_screen_matches_reference(key):
         with pytest.raises(RuntimeError):
             rflux.command("skyr4.rad", case["options"], text=text)
     else:
-        argv, _, _ = rflux.command("skyr4.rad", case["options"], text=text)
+        argv, _, _, _ = rflux.command("skyr4.rad", case["options"], text=text)
         assert argv == case["args"]
 
 
@@ -61,11 +61,6 @@ def test_receiver_geometry():
     assert np.allclose(c.nrm, (0, 0, -1))
     # the default up vector: +Z unless the normal is within ~45 degrees of Z
     assert b.vup == (0.0, 0.0, 1.0) and a.vup == (0.0, 1.0, 0.0)
-
-
-def test_sender_surface_is_refused():
-    with pytest.raises(RuntimeError, match="sender surfaces"):
-        rflux.rfluxmtx("x.rad", surface="window.rad", rays=b"")
 
 
 def _oracle_mod(oracle, m):
@@ -137,3 +132,54 @@ def test_oracle_box_matches_reference_rfluxmtx(oracle):
     col = np.where(nz.any(1), nz.argmax(1), -1)
     assert np.array_equal(col, g["col"])
     assert np.array_equal(v[np.arange(len(v)), np.maximum(col, 0)] * (col >= 0), g["value"])
+
+
+SEND = np.load(GOLDEN / "rflux_send.npz")
+SEND_NAMES = sorted(k for k in SEND.files if not k.endswith(("_args", "_text")))
+
+
+def test_hilbert_curve_is_a_bijection_of_adjacent_cells():
+    """Consecutive Hilbert indices land on cells one step apart, and every
+    cell is visited once (the defining properties of the curve)."""
+    from pyradiance_amd import sender
+    for n, bits in ((2, 4), (3, 3)):
+        c = sender.hilbert_i2c(n, bits, np.arange(1 << (n * bits))).astype(np.int64)
+        assert len({tuple(r) for r in c}) == 1 << (n * bits)
+        assert (np.abs(np.diff(c, axis=0)).sum(1) == 1).all()
+        assert (c >= 0).all() and (c < (1 << bits)).all()
+
+
+@pytest.mark.parametrize("name", SEND_NAMES)
+def test_sender_args_match_reference(name):
+    snd, rcv, cnt = (str(x) for x in SEND[name + "_text"])
+    argv, _, _, sp = rflux.command("recv.rad", ["-h", "-ab", "0", "-ffd", "-c", cnt], text=rcv,
+                                   sender="send.rad", sender_text=snd)
+    assert argv == list(SEND[name + "_args"])
+    assert sp.nbins == SEND[name].shape[0]
+
+
+def _flux_close(a, b, count):
+    """Element-wise agreement of two Monte Carlo flux matrices of `count`
+    samples per row: 5 sigma of a binomial fraction plus a count floor."""
+    p = 0.5 * (a + b)
+    tol = 5 * np.sqrt(2 * p * (1 - np.minimum(p, 1)) / count) + 3.0 / count
+    return np.abs(a - b) <= tol
+
+
+@pytest.mark.parametrize("name", SEND_NAMES)
+def test_oracle_sender_samples_match_reference(oracle, name):
+    """Our sender sampling traced by the oracle reproduces the reference
+    rfluxmtx flux matrix within Monte Carlo error; rows sum to 1 (every ray
+    ends on a unit-radiance receiver)."""
+    from pyradiance_amd import sender
+    snd, rcv, cnt = (str(x) for x in SEND[name + "_text"])
+    cnt = int(cnt)
+    _, recs, _, sp = rflux.command("recv.rad", ["-ab", "0", "-c", str(cnt)], text=rcv, sender="send.rad",
+                                   sender_text=snd)
+    rays = sender.sender_rays(sp, cnt, np.random.default_rng(11))
+    mods = [_oracle_mod(oracle, rt._modifier_spec(*r.spec)) for r in recs]
+    out, _ = oracle.Scene(rcv).contrib(rays, oracle.params(ambounce=0), mods, accum=cnt)
+    want = SEND[name]
+    assert out.shape == want.shape
+    assert np.allclose(out[:, 0::3].sum(1), 1.0)
+    assert _flux_close(out, want, cnt).all()
