@@ -116,11 +116,73 @@ RG_HD int quadratic(double r[2], double a, double b, double c) {
   return 2;
 }
 
+// Signed volume of the ray against one triangle edge (volume_sign() of
+// rt/o_mesh.c:60-95): computed with the edge's lower-id vertex first, so the
+// two triangles sharing an edge see opposite signs and a ray through the
+// edge is claimed by exactly one of them.  `rev`: the stored order runs
+// against the id order.  Never 0.
+RG_HD int tri_vsign(V3 o, V3 d, V3 p, V3 q, bool rev) {
+  if (rev) { V3 t = p; p = q; q = t; }
+  V3 a = p - o, b = q - o;
+  double vol = a.x * (b.y * d.z - b.z * d.y) + a.y * (b.z * d.x - b.x * d.z) + a.z * (b.x * d.y - b.y * d.x);
+  int sg = vol > .0 ? 1 : -1;
+  return rev ? -sg : sg;
+}
+
+// Barycentric weights of p in (v1, v2, v3): comp_baryc() / eval_baryc() of
+// common/tmesh.c:44-110 (projection onto the dominant plane).
+RG_HD bool tri_baryc(double wt[3], V3 p, V3 v1, V3 v2, V3 v3) {
+  V3 vab = v1 - v2, vcb = v3 - v2;
+  V3 va = cross(vab, vcb);
+  int ax = va.y * va.y > va.x * va.x;
+  if (va.z * va.z > comp(va, ax) * comp(va, ax)) ax = 2;
+  int ax0 = (ax + 1) % 3, ax1 = (ax + 2) % 3;
+  double tm[2][3];
+  for (int i = 0; i < 2; ++i) {
+    double ab0 = comp(v1, ax0) - comp(v2, ax0), cb0 = comp(v3, ax0) - comp(v2, ax0);
+    double ab1 = comp(v1, ax1) - comp(v2, ax1), cb1 = comp(v3, ax1) - comp(v2, ax1);
+    double dd = cb0 * cb0 + cb1 * cb1;
+    if (dd <= FTINY * FTINY) return false;
+    dd = (cb0 * ab0 + cb1 * ab1) / dd;
+    double a0 = ab0 - cb0 * dd, a1 = ab1 - cb1 * dd;
+    dd = a0 * a0 + a1 * a1;
+    if (dd <= FTINY * FTINY) return false;
+    dd = 1.0 / dd;
+    tm[i][0] = a0 * dd;
+    tm[i][1] = a1 * dd;
+    tm[i][2] = -(comp(v2, ax0) * tm[i][0] + comp(v2, ax1) * tm[i][1]);
+    V3 t = v1; v1 = v2; v2 = v3; v3 = t;
+  }
+  double u = comp(p, ax0), v = comp(p, ax1);
+  wt[0] = u * tm[0][0] + v * tm[0][1] + tm[0][2];
+  wt[1] = u * tm[1][0] + v * tm[1][1] + tm[1][2];
+  wt[2] = 1. - wt[1] - wt[0];
+  return true;
+}
+
 // Test one primitive; updates h when it wins.
 RG_HD void hit_prim(const DevScene& S, int pi, V3 o, V3 d, HitRec& h) {
   const Prim& P = S.prims[pi];
   const double* g = S.geom + P.goff;
   switch (P.kind) {
+    case PK_TRIANGLE: {
+      // mesh_hit() of rt/o_mesh.c:98-141: equal edge-volume signs, then the
+      // plane distance along (v0-v2)x(v1-v0)
+      V3 a = load3(g), b = load3(g + 3), c = load3(g + 6);
+      int fl = (int)g[12];
+      int s1 = tri_vsign(o, d, a, b, fl & TRI_REV0);
+      int s2 = tri_vsign(o, d, b, c, fl & TRI_REV1);
+      if (s1 != s2) return;
+      if (s2 != tri_vsign(o, d, c, a, fl & TRI_REV2)) return;
+      V3 nrm = cross(a - c, b - a);
+      double dn = dot(d, nrm);
+      if (dn == 0.0) return;
+      double t = dot(a - o, nrm) / dn;
+      double rod = -dot(d, load3(g + 9));
+      if (ray_reject(S, pi, t, rod, h)) return;
+      h.t = t; h.rod = rod; h.prim = pi;
+      return;
+    }
     case PK_POLYGON: {
       V3 n = load3(g);
       double rdot = -dot(d, n);
@@ -232,6 +294,7 @@ RG_HD V3 prim_normal(const DevScene& S, int pi, V3 p) {
   const double* g = S.geom + P.goff;
   switch (P.kind) {
     case PK_POLYGON: return load3(g);
+    case PK_TRIANGLE: return load3(g + 9);
     case PK_SPHERE:
     case PK_BUBBLE: {
       double rad = g[3] * (1 - 2 * (P.kind == PK_BUBBLE));
@@ -259,6 +322,29 @@ RG_HD V3 prim_normal(const DevScene& S, int pi, V3 p) {
       return nrm;
     }
   }
+}
+
+// Normal perturbation at a hit (the reference's pert): mesh triangles with
+// vertex normals interpolate them barycentrically (rt/o_mesh.c:192-204);
+// everything else is unperturbed.
+RG_HD V3 prim_pert(const DevScene& S, int pi, V3 p, V3 ron) {
+  const Prim& P = S.prims[pi];
+  V3 z = {0.0, 0.0, 0.0};
+  if (P.kind != PK_TRIANGLE) return z;
+  const double* g = S.geom + P.goff;
+  if (!((int)g[12] & TRI_NORMALS)) return z;
+  double wt[3];
+  if (!tri_baryc(wt, p, load3(g), load3(g + 3), load3(g + 6))) return z;
+  V3 pn = load3(g + 13) * wt[0] + load3(g + 16) * wt[1] + load3(g + 19) * wt[2];
+  double l2 = dot(pn, pn);
+  if (l2 == 0.0) return z;
+  pn = pn * (1.0 / sqrt(l2));
+  return pn - ron;
+}
+
+RG_HD bool prim_phong(const DevScene& S, int pi) {
+  const Prim& P = S.prims[pi];
+  return P.kind == PK_TRIANGLE && ((int)S.geom[P.goff + 12] & TRI_PHONG);
 }
 
 }  // namespace rg

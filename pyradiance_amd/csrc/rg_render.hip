@@ -451,11 +451,38 @@ __device__ __forceinline__ void trace_contrib(const DevScene& S, const RunCfg& c
 // ---- materials ------------------------------------------------------------
 struct HitS {
   V3 rop, ron;     // hit point, normal (flipped by the material when needed)
+  V3 pert;         // normal perturbation (mesh vertex normals), zero if none
   double rod;      // -DOT(rdir, ron)
   double rot;
   int prim;        // -1 for the irradiance pseudo-hit
   int flat;
+  int phong;       // modifier named "Phong": perturbation not applied to transmission
 };
+
+// flipsurface() of rt/raytrace.c:519-531
+__device__ __forceinline__ void flip_surface(HitS& hs) {
+  hs.rod = -hs.rod;
+  hs.ron = -hs.ron;
+  hs.pert = -hs.pert;
+}
+
+// raynormal() of rt/raytrace.c:446-480: perturbed normal, kept on the ray's
+// side of the surface
+__device__ __forceinline__ double ray_normal(V3& norm, const HitS& hs, V3 rdir) {
+  norm = hs.ron + hs.pert;
+  double l2 = dot(norm, norm);
+  if (l2 == 0.0) {
+    norm = hs.ron;
+    return hs.rod;
+  }
+  norm = norm * (1.0 / sqrt(l2));
+  double newdot = -dot(norm, rdir);
+  if ((newdot > 0.0) != (hs.rod > 0.0)) {
+    norm = norm + rdir * (2.0 * newdot);
+    newdot = -newdot;
+  }
+  return newdot;
+}
 
 enum { SP_REFL = 01, SP_TRAN = 02, SP_PURE = 04, SP_FLAT = 010, SP_RBLT = 020, SP_TBLT = 040 };
 
@@ -696,17 +723,21 @@ __device__ __forceinline__ void shade_normal(const DevScene& S, const RunCfg& c,
       emit(sk, s, r);
       return;
     }
-    hs.rod = -hs.rod;
-    hs.ron = -hs.ron;
+    flip_surface(hs);
   }
   NormDat nd;
   nd.mcolor = c3((float)m.f[0], (float)m.f[1], (float)m.f[2]);
   nd.specfl = 0;
   nd.alpha2 = m.f[4] * m.f[4];
   if (nd.alpha2 <= FTINY) nd.specfl |= SP_PURE;
-  nd.pnorm = hs.ron;
-  nd.pdot = hs.rod;
-  if (hs.prim >= 0 && hs.flat) nd.specfl |= SP_FLAT;
+  const bool hastexture = dot(hs.pert, hs.pert) > FTINY * FTINY;
+  if (hastexture) {
+    nd.pdot = ray_normal(nd.pnorm, hs, r.d);
+  } else {
+    nd.pnorm = hs.ron;
+    nd.pdot = hs.rod;
+  }
+  if (!hastexture && hs.prim >= 0 && hs.flat) nd.specfl |= SP_FLAT;
   if (nd.pdot < .001) nd.pdot = .001;
   nd.rspec = m.f[3];
   double fest = 0.;
@@ -722,6 +753,13 @@ __device__ __forceinline__ void shade_normal(const DevScene& S, const RunCfg& c,
     if (nd.tspec > FTINY) {
       nd.specfl |= SP_TRAN;
       if (!(nd.specfl & SP_PURE) && c.p.specthresh >= nd.tspec - FTINY) nd.specfl |= SP_TBLT;
+      if (hastexture && !(r.crtype & (SHADOW | AMBIENT)) && !hs.phong) {
+        V3 pd = r.d - hs.pert;
+        if (dot(pd, hs.ron) < -FTINY) {
+          normalize(pd);
+          nd.prdir = pd;
+        }
+      }
     }
   } else {
     nd.tdiff = nd.tspec = nd.trans = 0.0;
@@ -754,6 +792,7 @@ __device__ __forceinline__ void shade_normal(const DevScene& S, const RunCfg& c,
     C3 sc = nd.scolor;
     Spawn s = make_child(r, hs.rop, REFLECTED, &sc, SLOT_REFL, 0);
     V3 dir = r.d + nd.pnorm * (2. * nd.pdot);
+    if (hastexture && dot(dir, hs.ron) <= FTINY) dir = r.d + hs.ron * (2. * hs.rod);   // penetration
     normalize(dir);
     set_dir(s, dir);
     emit(sk, s, r, e_spec);
@@ -789,9 +828,11 @@ __device__ __forceinline__ void shade_glass(const RunCfg& c, const RayS& r, HitS
   } else if (r.crtype & SHADOW) {
     return;
   }
-  if (hs.rod < 0.0) { hs.rod = -hs.rod; hs.ron = -hs.ron; }
+  if (hs.rod < 0.0) flip_surface(hs);
+  const bool hastexture = dot(hs.pert, hs.pert) > FTINY * FTINY;
   V3 pnorm = hs.ron;
   double pdot = hs.rod;
+  if (hastexture) pdot = ray_normal(pnorm, hs, r.d);
   double cos2 = sqrt((1.0 - 1.0 / (rindex * rindex)) + pdot * pdot / (rindex * rindex));
   if (hastrans) for (int i = 0; i < 3; ++i) mc[i] = pow(mc[i], 1.0 / cos2);
   double r1e = (pdot - rindex * cos2) / (pdot + rindex * cos2);
@@ -807,7 +848,12 @@ __device__ __forceinline__ void shade_glass(const RunCfg& c, const RayS& r, HitS
     }
     C3 tc = c3((float)ct[0], (float)ct[1], (float)ct[2]);
     Spawn s = make_child(r, hs.rop, TRANS, &tc, SLOT_TRANS, 0);
-    set_dir(s, r.d);
+    V3 td = r.d;
+    if (hastexture && !(r.crtype & (SHADOW | AMBIENT)) && !hs.phong) {
+      V3 pd = r.d + hs.pert * (2. * (1. - rindex));
+      if (dot(pd, pd) > 0.0) { normalize(pd); td = pd; }
+    }
+    set_dir(s, td);
     emit(sk, s, r);
   }
   if (r.crtype & SHADOW) return;
@@ -834,8 +880,10 @@ __device__ __forceinline__ void shade_hit(const DevScene& S, const RunCfg& c, co
   hs.rop = r.o + r.d * h.t;
   hs.ron = prim_normal(S, h.prim, hs.rop);
   hs.rod = -dot(r.d, hs.ron);
+  hs.pert = prim_pert(S, h.prim, hs.rop, hs.ron);
   hs.prim = h.prim;
   hs.flat = P.flat;
+  hs.phong = prim_phong(S, h.prim);
   if (P.mat < 0) { atomicOr(c.err, ERR_NOMAT); return; }
   const Material& m = S.mats[P.mat];
   bool zeroed = false;
@@ -954,6 +1002,8 @@ __global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene Sarg, RunCfg c, c
   hs.rod = 1.0;
   hs.prim = -1;
   hs.flat = 0;
+  hs.pert = V3{0.0, 0.0, 0.0};
+  hs.phong = 0;
   shade_normal(S, c, r, hs, S.mats[S.lamb], -1, sk);
 }
 

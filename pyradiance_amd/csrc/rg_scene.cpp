@@ -6,6 +6,8 @@
 //   o_sphere() radius rule  rt/sphere.c:31-39
 //   marksources()           rt/source.c:47-142
 //   ssetsrc/fsetsrc/sphsetsrc/rsetsrc + setflatss  rt/srcsupp.c:60-260
+//   xf()                    common/xf.c:40-150
+//   mesh triangles          rt/o_mesh.c:146-217, common/mesh.c:174-278
 #include "rg_scene.hpp"
 
 #include <algorithm>
@@ -51,6 +53,108 @@ bool perpendicular3(double* vp, const double* v) {
   throw RadError(o.type + " \"" + o.name + "\": " + m);
 }
 
+// xf() of common/xf.c:40-150: transform options -> 4x4 matrix for row
+// vectors (p' = p M) and the scale factor (negative after an odd number of
+// mirrors).  Returns the number of arguments used.
+struct Xf {
+  double m[4][4];
+  double sca;
+};
+void ident4(double m[4][4]) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) m[i][j] = i == j;
+}
+void mult4(double r[4][4], const double a[4][4], const double b[4][4]) {
+  double t[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) t[i][j] = a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j] + a[i][3] * b[3][j];
+  memcpy(r, t, sizeof t);
+}
+bool is_number(const std::string& w, bool integer) {
+  if (w.empty()) return false;
+  char* e = nullptr;
+  if (integer) strtol(w.c_str(), &e, 10);
+  else strtod(w.c_str(), &e);
+  return e && *e == '\0';
+}
+int parse_xf(Xf& ret, const std::vector<std::string>& av, size_t start) {
+  const int ac = (int)(av.size() - start);
+  auto A = [&](int i) -> const std::string& { return av[start + i]; };
+  auto ok = [&](int i, size_t optlen, const char* types) {   // checkarg()
+    if (A(i).size() != optlen) return false;
+    int n = (int)strlen(types);
+    if (i + n >= ac) return false;
+    for (int k = 0; k < n; ++k)
+      if (!is_number(A(i + 1 + k), types[k] == 'i')) return false;
+    return true;
+  };
+  double xfmat[4][4], m4[4][4];
+  double xfsca = 1.0;
+  int icnt = 1;
+  ident4(ret.m);
+  ret.sca = 1.0;
+  ident4(xfmat);
+  int i;
+  for (i = 0; i < ac && A(i)[0] == '-' && A(i).size() > 1; ++i) {
+    ident4(m4);
+    const std::string& o = A(i);
+    bool done = false;
+    switch (o[1]) {
+      case 't':
+        if (!ok(i, 2, "fff")) { done = true; break; }
+        for (int k = 0; k < 3; ++k) m4[3][k] = atof(A(++i).c_str());
+        break;
+      case 'r': {
+        if (o.size() != 3 || (o[2] != 'x' && o[2] != 'y' && o[2] != 'z') || !ok(i, 3, "f")) { done = true; break; }
+        double a = (PI_D / 180.) * atof(A(++i).c_str()), cs = cos(a), sn = sin(a);
+        if (o[2] == 'x') { m4[1][1] = m4[2][2] = cs; m4[1][2] = sn; m4[2][1] = -sn; }
+        else if (o[2] == 'y') { m4[0][0] = m4[2][2] = cs; m4[2][0] = sn; m4[0][2] = -sn; }
+        else { m4[0][0] = m4[1][1] = cs; m4[0][1] = sn; m4[1][0] = -sn; }
+        break;
+      }
+      case 's': {
+        if (!ok(i, 2, "f")) { done = true; break; }
+        double v = atof(A(i + 1).c_str());
+        if (v == 0.0) { done = true; break; }
+        ++i;
+        xfsca *= v;
+        m4[0][0] = m4[1][1] = m4[2][2] = v;
+        break;
+      }
+      case 'm':
+        if (o.size() != 3 || (o[2] != 'x' && o[2] != 'y' && o[2] != 'z')) { done = true; break; }
+        xfsca *= -1.0;
+        m4[o[2] - 'x'][o[2] - 'x'] = -1.0;
+        break;
+      case 'i':
+        if (!ok(i, 2, "i")) { done = true; break; }
+        while (icnt-- > 0) {
+          mult4(ret.m, ret.m, xfmat);
+          ret.sca *= xfsca;
+        }
+        icnt = atoi(A(++i).c_str());
+        ident4(xfmat);
+        xfsca = 1.0;
+        continue;
+      default:
+        done = true;
+    }
+    if (done) break;
+    mult4(xfmat, xfmat, m4);
+  }
+  while (icnt-- > 0) {
+    mult4(ret.m, ret.m, xfmat);
+    ret.sca *= xfsca;
+  }
+  return i;
+}
+void xf_point(double r[3], const double p[3], const Xf& x) {   // multp3()
+  for (int k = 0; k < 3; ++k) r[k] = p[0] * x.m[0][k] + p[1] * x.m[1][k] + p[2] * x.m[2][k] + x.m[3][k];
+}
+void xf_vector(double r[3], const double v[3], const Xf& x) {  // multv3()
+  for (int k = 0; k < 3; ++k) r[k] = v[0] * x.m[0][k] + v[1] * x.m[1][k] + v[2] * x.m[2][k];
+}
+
 int material_kind(const std::string& t) {
   if (t == "plastic") return MK_PLASTIC;
   if (t == "metal") return MK_METAL;
@@ -70,10 +174,10 @@ CompiledScene compile_scene(const SceneDB& db) {
   const int nobj = (int)objs.size();
   std::unordered_map<int, int> matidx;   // material object -> table index
 
-  // findmaterial(): follow the modifier chain to the first material
-  auto findmaterial = [&](int oi) -> int {
+  // findmaterial(): follow the modifier chain from modifier m to the first
+  // material (for object oi, m is its own modifier)
+  auto findmaterial_from = [&](int oi, int m) -> int {
     int guard = 0;
-    int m = objs[oi].omod;
     while (m >= 0) {
       if (++guard > 10000) objerr(objs[oi], "modifier loop");
       const RObj& mo = objs[m];
@@ -93,6 +197,7 @@ CompiledScene compile_scene(const SceneDB& db) {
     }
     return -1;
   };
+  auto findmaterial = [&](int oi) -> int { return findmaterial_from(oi, objs[oi].omod); };
 
   std::function<int(int)> material_index = [&](int mo) -> int {
     if (mo < 0) return -1;
@@ -136,11 +241,89 @@ CompiledScene compile_scene(const SceneDB& db) {
   // pass 1: primitives
   std::vector<int> obj_prim(nobj, -1);
   std::vector<std::array<double, 6>> dbounds;
+  // mesh objects: one triangle primitive per mesh triangle, in world space
+  // (the reference transforms the ray into the mesh instead; o_mesh.c:146-217)
+  auto add_mesh = [&](int oi) {
+    const RObj& o = objs[oi];
+    auto itm = db.obj_mesh.find(oi);
+    if (itm == db.obj_mesh.end()) objerr(o, "mesh file not loaded");
+    const MeshData& M = db.meshes[itm->second];
+    Xf X;
+    if (parse_xf(X, o.sargs, 1) != (int)o.sargs.size() - 1) objerr(o, "bad transform");
+    const bool mirror = X.sca < 0;
+    const size_t nv = M.has_n.size();
+    std::vector<double> W(3 * nv), WN(3 * nv);
+    for (size_t v = 0; v < nv; ++v) {
+      xf_point(&W[3 * v], &M.xyz[3 * v], X);
+      xf_vector(&WN[3 * v], &M.nrm[3 * v], X);
+    }
+    int objmat = -2;
+    for (const auto& t : M.tris) {
+      // the triangle's own material when the mesh's modifier is void
+      const bool pseudo = o.omod < 0 && t.mat >= 0;
+      const int mod = pseudo ? t.mat : o.omod;
+      if (mod < 0) continue;                         // void: never shaded
+      int mo;
+      if (pseudo) {
+        mo = findmaterial_from(oi, mod);
+      } else {
+        if (objmat == -2) objmat = findmaterial(oi);
+        mo = objmat;
+      }
+      if (mo < 0) continue;
+      const double *a = &W[3 * t.v[0]], *b = &W[3 * t.v[1]], *c = &W[3 * t.v[2]];
+      double e1[3], e2[3], n[3];
+      for (int k = 0; k < 3; ++k) { e1[k] = a[k] - c[k]; e2[k] = b[k] - a[k]; }
+      cross3(n, e1, e2);
+      if (norm3(n) == 0.0) continue;                 // degenerate: always tangent
+      if (mirror) for (int k = 0; k < 3; ++k) n[k] = -n[k];
+      Prim P;
+      memset(&P, 0, sizeof P);
+      P.kind = PK_TRIANGLE;
+      P.mat = material_index(mo);
+      P.omod = mod;
+      P.modslot = -1;
+      P.srcidx = -1;
+      P.goff = (int)cs.geom.size();
+      P.nv = 3;
+      P.flat = pseudo ? 1 : 0;                       // the pseudo object is a face
+      const bool normals = M.has_n[t.v[0]] && M.has_n[t.v[1]] && M.has_n[t.v[2]];
+      int fl = (t.v[0] > t.v[1] ? TRI_REV0 : 0) | (t.v[1] > t.v[2] ? TRI_REV1 : 0) |
+               (t.v[2] > t.v[0] ? TRI_REV2 : 0) | (normals ? TRI_NORMALS : 0) |
+               (objs[mod].name == "Phong" ? TRI_PHONG : 0);
+      cs.geom.insert(cs.geom.end(), {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], n[0], n[1], n[2],
+                                     (double)fl});
+      if (normals)
+        for (int k = 0; k < 3; ++k) {
+          const double* vn = &WN[3 * t.v[k]];
+          cs.geom.insert(cs.geom.end(), {vn[0], vn[1], vn[2]});
+        }
+      double lo[3], hi[3];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min({a[k], b[k], c[k]});
+        hi[k] = std::max({a[k], b[k], c[k]});
+      }
+      cs.prims.push_back(P);
+      cs.prim_modname.push_back(objs[mod].name);
+      dbounds.push_back({lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
+      grow(lo, hi);
+    }
+  };
   for (int oi = 0; oi < nobj; ++oi) {
     const RObj& o = objs[oi];
     if (!SceneDB::is_surface(o.type)) {
-      if (o.type == "instance" || o.type == "mesh")
-        objerr(o, "instances and meshes are not supported on the GPU path");
+      if (o.type == "mesh") {
+        if (o.omod >= 0) {
+          int mo = findmaterial(oi);
+          if (mo >= 0) {
+            const std::string& mt = objs[mo].type;
+            if (mt == "light" || mt == "glow" || mt == "illum" || mt == "spotlight") objerr(o, "illegal material");
+          }
+        }
+        add_mesh(oi);
+      } else if (o.type == "instance") {
+        objerr(o, "instances are not supported on the GPU path");
+      }
       continue;
     }
     if (o.type == "source") continue;

@@ -46,11 +46,21 @@ enum : uint16_t {
   RAYREFL = RSHADOW | REFLECTED | RAMBIENT | RSPECULAR,
 };
 
-// primitive kinds
+// primitive kinds (PK_TRIANGLE: one triangle of a compiled mesh, rt/o_mesh.c)
 enum PrimKind : int32_t {
   PK_POLYGON = 0, PK_SPHERE = 1, PK_BUBBLE = 2, PK_RING = 3,
-  PK_CONE = 4, PK_CUP = 5, PK_CYLINDER = 6, PK_TUBE = 7,
+  PK_CONE = 4, PK_CUP = 5, PK_CYLINDER = 6, PK_TUBE = 7, PK_TRIANGLE = 8,
 };
+
+// Mesh triangle geometry block (world coordinates, FP64):
+//   v0[3] v1[3] v2[3]  vertices in stored order
+//   n[3]               unit normal as the reference reports it (the mesh-space
+//                      (v0-v2)x(v1-v0) carried through the transform)
+//   flags              TRI_REV0..2: edge (v0,v1) / (v1,v2) / (v2,v0) runs
+//                      against vertex-id order; TRI_NORMALS: vn follow;
+//                      TRI_PHONG: the modifier is named "Phong"
+//   vn0[3] vn1[3] vn2[3]  transformed vertex normals (unnormalised)
+enum : int32_t { TRI_REV0 = 1, TRI_REV1 = 2, TRI_REV2 = 4, TRI_NORMALS = 8, TRI_PHONG = 16 };
 
 // material kinds (only the set the GPU path shades)
 enum MatKind : int32_t {

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <array>
 #include <unordered_set>
 
 namespace rg {
@@ -179,6 +180,7 @@ void SceneDB::parse_stream(const std::string& text, const std::string& where, in
       }
     }
     if (o.omod == -2) throw RadError("(" + where + "): inappropriate use of 'inherit' modifier");
+    if (o.type == "mesh" || o.type == "instance") o.dir = cur_dir_;
     add(std::move(o));
   }
 }
@@ -195,17 +197,33 @@ static std::string slurp(const std::string& path) {
   return ss.str();
 }
 
-void SceneDB::load_rad_file(const std::string& path) { parse_stream(slurp(path), path, 0); }
+static std::string dir_of(const std::string& path) {
+  size_t sl = path.rfind('/');
+  return sl == std::string::npos ? std::string() : path.substr(0, sl + 1);
+}
+
+void SceneDB::load_rad_file(const std::string& path) {
+  std::string keep = cur_dir_;
+  cur_dir_ = dir_of(path);
+  try {
+    parse_stream(slurp(path), path, 0);
+  } catch (...) {
+    cur_dir_ = keep;
+    throw;
+  }
+  cur_dir_ = keep;
+}
 
 // ---- octree ---------------------------------------------------------------
 
-namespace {
 struct OctReader {
   const std::string& b;
   size_t p = 0;
   const std::string& fn;
-  OctReader(const std::string& buf, const std::string& name) : b(buf), fn(name) {}
-  [[noreturn]] void fail(const char* m) { throw RadError("octree \"" + fn + "\": " + m); }
+  const char* what;
+  OctReader(const std::string& buf, const std::string& name, const char* kind = "octree")
+      : b(buf), fn(name), what(kind) {}
+  [[noreturn]] void fail(const char* m) { throw RadError(std::string(what) + " \"" + fn + "\": " + m); }
   int getc() { return p < b.size() ? (unsigned char)b[p++] : -1; }
   // getint(): big-endian, sign-extended from the first byte (portio.c:112-133)
   long getint(int siz) {
@@ -253,7 +271,31 @@ struct OctReader {
     }
   }
 };
-}  // namespace
+
+// readscene(): type map, then objects until a -1 type byte (sceneio.c:90-109);
+// modifier numbers are relative to the first object of the block.
+void SceneDB::read_objects(OctReader& rd, int objsize, const std::string& dir) {
+  std::vector<std::string> tmap;
+  for (std::string s = rd.getstr(); !s.empty(); s = rd.getstr()) tmap.push_back(s);
+  size_t base = objs.size();
+  for (;;) {
+    long ti = rd.getint(1);
+    if (ti == -1) break;
+    if (ti < 0 || ti >= (long)tmap.size()) rd.fail("reference to unknown type");
+    RObj o;
+    o.type = tmap[ti];
+    long m = rd.getint(objsize);
+    o.omod = m == -1 ? -1 : (int)(m + (long)base);
+    o.name = rd.getstr();
+    long ns = rd.getint(2);
+    for (long i = 0; i < ns; ++i) o.sargs.push_back(rd.getstr());
+    long nf = rd.getint(2);
+    for (long i = 0; i < nf; ++i) o.fargs.push_back(rd.getflt());
+    if (!known_types().count(o.type)) rd.fail(("unknown object type \"" + o.type + "\"").c_str());
+    if (o.type == "mesh" || o.type == "instance") o.dir = dir;
+    add(std::move(o));
+  }
+}
 
 void SceneDB::load_oct(const std::string& path) {
   std::string buf = slurp(path);
@@ -306,28 +348,158 @@ void SceneDB::load_oct(const std::string& path) {
     return;
   }
   rd.skiptree((int)objsize, 0);
-  // readscene(): type map, then objects until a -1 type byte (sceneio.c:90-109)
-  std::vector<std::string> tmap;
-  for (std::string s = rd.getstr(); !s.empty(); s = rd.getstr()) tmap.push_back(s);
   size_t base = objs.size();
-  for (;;) {
-    long ti = rd.getint(1);
-    if (ti == -1) break;
-    if (ti < 0 || ti >= (long)tmap.size()) throw RadError("octree \"" + path + "\": reference to unknown type");
-    RObj o;
-    o.type = tmap[ti];
-    long m = rd.getint((int)objsize);
-    o.omod = m == -1 ? -1 : (int)(m + (long)base);
-    o.name = rd.getstr();
-    long ns = rd.getint(2);
-    for (long i = 0; i < ns; ++i) o.sargs.push_back(rd.getstr());
-    long nf = rd.getint(2);
-    for (long i = 0; i < nf; ++i) o.fargs.push_back(rd.getflt());
-    if (!known_types().count(o.type)) throw RadError("octree \"" + path + "\": unknown object type \"" + o.type + "\"");
-    add(std::move(o));
-  }
+  read_objects(rd, (int)objsize, dir_of(path));
   if ((long)(objs.size() - base) != nobj)
     throw RadError("octree \"" + path + "\": bad object count");
+}
+
+// ---- compiled meshes (common/readmesh.c:190-300) ------------------------------
+
+// decodedir() of common/dircode.c:83-106: two 13-bit components, the axis
+// they belong to and three sign bits; 0 is "no normal".
+static void decode_dir(double dv[3], int32_t dc) {
+  static const int itab[4][3] = {{1, 0, 2}, {0, 1, 2}, {1, 2, 0}, {0, 2, 1}};
+  const double DCSCALE = 11584.5;
+  const int ndx = ((dc & 020) != 0) << 1 | ((dc & 010) != 0);
+  double d1 = (dc >> 5 & 0x1fff) * (1. / DCSCALE);
+  double d2 = (dc >> 18 & 0x1fff) * (1. / DCSCALE);
+  double der = sqrt(1. - d1 * d1 - d2 * d2);
+  dv[itab[ndx][0]] = d1;
+  dv[itab[ndx][1]] = d2;
+  dv[itab[ndx][2]] = der;
+  if (dc & 01) dv[0] = -dv[0];
+  if (dc & 02) dv[1] = -dv[1];
+  if (dc & 04) dv[2] = -dv[2];
+}
+
+int SceneDB::load_mesh(const std::string& path) {
+  auto it = mesh_files_.find(path);
+  if (it != mesh_files_.end()) return it->second;
+  std::string buf = slurp(path);
+  OctReader rd(buf, path, "mesh");
+  size_t hend = buf.find("\n\n");
+  if (buf.compare(0, 10, "#?RADIANCE") != 0 || hend == std::string::npos ||
+      buf.substr(0, hend).find("FORMAT=Radiance_tmesh") == std::string::npos)
+    throw RadError("\"" + path + "\": not a compiled mesh");
+  rd.p = hend + 2;
+  const long MESHMAGIC = 1 * 8 + 311;         // common/mesh.h:128-129
+  long objsize = rd.getint(2) - MESHMAGIC;
+  if (objsize <= 0 || objsize > 8) throw RadError("\"" + path + "\": incompatible mesh format");
+  double org[3], size;
+  for (int i = 0; i < 3; ++i) org[i] = atof(rd.getstr().c_str());
+  size = atof(rd.getstr().c_str());
+  for (int i = 0; i < 4; ++i) rd.getflt();    // (u,v) limits: no patterns here
+  rd.skiptree((int)objsize, 0);
+  MeshData M;
+  M.path = path;
+  M.mat0 = (int)objs.size();
+  read_objects(rd, (int)objsize, dir_of(path));
+  M.nmats = (int)objs.size() - M.mat0;
+  long npatch = rd.getint(4);
+  if (npatch < 0 || npatch > (1L << 22)) rd.fail("bad patch count");
+  struct J1 { int32_t v1j; int v2, v3, mat; };
+  struct J2 { int32_t v1j, v2j; int v3, mat; };
+  std::vector<int64_t> base(npatch + 1, 0);
+  std::vector<std::vector<std::array<int, 4>>> local(npatch);
+  std::vector<std::vector<J1>> j1(npatch);
+  std::vector<std::vector<J2>> j2(npatch);
+  const double vres = (1. / 4294967296.) * size;
+  for (long pn = 0; pn < npatch; ++pn) {
+    int flags = (int)rd.getint(1);
+    if (!(flags & 1) || (flags & ~7)) rd.fail("bad patch flags");
+    int nv = (int)rd.getint(2);
+    if (nv <= 0 || nv > 256) rd.fail("bad number of patch vertices");
+    base[pn + 1] = base[pn] + nv;
+    size_t v0 = M.has_n.size();
+    M.xyz.resize(3 * (v0 + nv));
+    M.nrm.resize(3 * (v0 + nv), 0.0);
+    M.has_n.resize(v0 + nv, 0);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < 3; ++j)
+        M.xyz[3 * (v0 + i) + j] = org[j] + ((uint32_t)rd.getint(4) + .5) * vres;
+    if (flags & 2)
+      for (int i = 0; i < nv; ++i) {
+        int32_t code = (int32_t)rd.getint(4);
+        if (code) {
+          decode_dir(&M.nrm[3 * (v0 + i)], code);
+          M.has_n[v0 + i] = 1;
+        }
+      }
+    if (flags & 4)
+      for (int i = 0; i < 2 * nv; ++i) rd.getint(4);
+    int ntris = (int)rd.getint(2);
+    if (ntris < 0 || ntris > 512) rd.fail("bad number of local triangles");
+    auto& lt = local[pn];
+    lt.resize(ntris);
+    for (int i = 0; i < ntris; ++i)
+      for (int k = 0; k < 3; ++k) lt[i][k] = (int)(rd.getint(1) & 0xff);
+    if (rd.getint(2) > 1) {
+      for (int i = 0; i < ntris; ++i) lt[i][3] = (int)rd.getint(2);
+    } else {
+      int sole = (int)rd.getint(2);
+      for (int i = 0; i < ntris; ++i) lt[i][3] = sole;
+    }
+    int n1 = (int)rd.getint(2);
+    if (n1 < 0 || n1 > 256) rd.fail("bad number of joiner triangles");
+    for (int i = 0; i < n1; ++i) {
+      J1 t;
+      t.v1j = (int32_t)rd.getint(4);
+      t.v2 = (int)(rd.getint(1) & 0xff);
+      t.v3 = (int)(rd.getint(1) & 0xff);
+      t.mat = (int)rd.getint(2);
+      j1[pn].push_back(t);
+    }
+    int n2 = (int)rd.getint(2);
+    if (n2 < 0 || n2 > 256) rd.fail("bad number of double joiner triangles");
+    for (int i = 0; i < n2; ++i) {
+      J2 t;
+      t.v1j = (int32_t)rd.getint(4);
+      t.v2j = (int32_t)rd.getint(4);
+      t.v3 = (int)(rd.getint(1) & 0xff);
+      t.mat = (int)rd.getint(2);
+      j2[pn].push_back(t);
+    }
+  }
+  // vertex id (patch << 8 | local) -> flat index; flat order follows the ids
+  auto flat = [&](int64_t id) -> int32_t {
+    int64_t pn = id >> 8, k = id & 0xff;
+    if (pn < 0 || pn >= npatch || base[pn] + k >= base[pn + 1]) rd.fail("bad vertex reference");
+    return (int32_t)(base[pn] + k);
+  };
+  auto matof = [&](int m) { return m == -1 ? -1 : M.mat0 + m; };
+  for (long pn = 0; pn < npatch; ++pn) {
+    for (auto& t : local[pn])
+      M.tris.push_back({{flat(pn << 8 | t[0]), flat(pn << 8 | t[1]), flat(pn << 8 | t[2])}, matof(t[3])});
+    for (auto& t : j1[pn])
+      M.tris.push_back({{flat(t.v1j), flat(pn << 8 | t.v2), flat(pn << 8 | t.v3)}, matof(t.mat)});
+    for (auto& t : j2[pn])
+      M.tris.push_back({{flat(t.v1j), flat(t.v2j), flat(pn << 8 | t.v3)}, matof(t.mat)});
+  }
+  for (auto& t : M.tris)
+    if (t.mat >= 0 && t.mat >= M.mat0 + M.nmats) rd.fail("material index out of range");
+  int idx = (int)meshes.size();
+  meshes.push_back(std::move(M));
+  mesh_files_[path] = idx;
+  return idx;
+}
+
+void SceneDB::resolve_meshes() {
+  for (int i = 0; i < (int)objs.size(); ++i) {
+    if (objs[i].type != "mesh" || obj_mesh.count(i)) continue;
+    const RObj o = objs[i];
+    if (o.sargs.empty()) throw RadError("mesh \"" + o.name + "\": bad # of arguments");
+    const std::string& f = o.sargs[0];
+    std::vector<std::string> cands = {f};
+    if (f[0] != '/' && !o.dir.empty()) cands.push_back(o.dir + f);
+    std::string use;
+    for (auto& c : cands) {
+      std::ifstream probe(c);
+      if (probe.good()) { use = c; break; }
+    }
+    if (use.empty()) throw RadError("cannot find mesh file \"" + f + "\"");
+    obj_mesh[i] = load_mesh(use);
+  }
 }
 
 void SceneDB::load_any(const std::string& path) {

@@ -4,8 +4,11 @@
 #include <vector>
 #include <unordered_map>
 #include <stdexcept>
+#include <stdint.h>
 
 namespace rg {
+
+struct OctReader;
 
 struct RadError : std::runtime_error {
   explicit RadError(const std::string& m) : std::runtime_error(m) {}
@@ -18,6 +21,20 @@ struct RObj {
   std::string name;
   std::vector<std::string> sargs;
   std::vector<double> fargs;
+  std::string dir;                // mesh / instance objects: directory of the defining file
+};
+
+// A compiled triangle mesh (common/mesh.h MESH) with its vertices decoded:
+// positions from the 32-bit cube coordinates, normals from the 4-byte
+// direction code (common/mesh.c:236-278, common/dircode.c:83-106).
+struct MeshData {
+  std::string path;
+  std::vector<double> xyz;        // [vertex][3]
+  std::vector<double> nrm;        // [vertex][3], zero where the vertex has none
+  std::vector<uint8_t> has_n;     // vertex carries a normal
+  struct Tri { int32_t v[3]; int32_t mat; };   // flat vertex indices; material object or -1
+  std::vector<Tri> tris;
+  int mat0 = 0, nmats = 0;        // the mesh's own material objects in SceneDB::objs
 };
 
 class SceneDB {
@@ -26,6 +43,13 @@ class SceneDB {
   double cuorg[3] = {0, 0, 0};
   double cusize = 0;
   bool allow_exec = false;        // run "!command" lines (readobj.c:58-61)
+  std::vector<MeshData> meshes;
+  std::unordered_map<int, int> obj_mesh;    // mesh object -> meshes[]
+
+  // Loads the mesh file of every mesh object not loaded yet, appending each
+  // file's material objects after the scene's (as the renderer's preload
+  // does, rt/preload.c with common/mesh.c:74-139).  Files are shared.
+  void resolve_meshes();
 
   void load_rad_text(const std::string& text, const std::string& where);
   void load_rad_file(const std::string& path);
@@ -39,8 +63,12 @@ class SceneDB {
 
  private:
   std::unordered_map<std::string, int> modtab_;
+  std::unordered_map<std::string, int> mesh_files_;
+  std::string cur_dir_;           // directory of the file being read
   void add(RObj&& o);
   void parse_stream(const std::string& text, const std::string& where, int depth);
+  int load_mesh(const std::string& path);
+  void read_objects(OctReader& rd, int objsize, const std::string& dir);
 };
 
 }  // namespace rg

@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden/ from the reference tools
 built into oracle/_ref (rtrace, rcontrib, oconv) from the reference sources.
 
-    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send]
+    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send mesh]
 
 Fixtures (inputs and the reference's outputs only):
   c1_seed{0,1,2}.npz     config-1 room: sensors, rtrace -I -ab 0 -dj 0 -ds 0 values
@@ -22,6 +22,8 @@ Fixtures (inputs and the reference's outputs only):
                          the non-zero output column and value of rfluxmtx -ab 0
   rflux_send.npz         sender surfaces of every basis: rfluxmtx's rcontrib
                          arguments and flux matrices (-ab 0, -c N samples)
+  mesh/, mesh_rtrace.npz compiled meshes from the reference obj2mesh, a scene
+                         placing them with transforms, and rtrace -oLNnv values
 The config-1/2 scenes come from pyradiance_amd.scenes (seeded) and are not
 stored; the two scenes above are stored as text with their vectors.
 """
@@ -458,11 +460,114 @@ def rflux():
     (HERE / "rflux_cases.json").write_text(json.dumps(out, indent=1) + "\n")
 
 
+def _uv_sphere_obj(nu, nv, r, center, mats, normals=True, uv=False):
+    """OBJ text of a UV sphere; `mats` splits the latitude bands between
+    usemtl names (None: no usemtl)."""
+    cx, cy, cz = center
+    lines, vid = [], lambda i, j: i * nu + (j % nu) + 1
+    for i in range(nv + 1):
+        th = np.pi * i / nv
+        for j in range(nu):
+            ph = 2 * np.pi * j / nu
+            n = (np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th))
+            lines.append("v %.9f %.9f %.9f" % (cx + r * n[0], cy + r * n[1], cz + r * n[2]))
+            if normals:
+                lines.append("vn %.9f %.9f %.9f" % n)
+            if uv:
+                lines.append("vt %.6f %.6f" % (j / nu, i / nv))
+    bands = np.array_split(np.arange(nv), len(mats))
+    for band, mat in zip(bands, mats):
+        if mat:
+            lines.append("usemtl " + mat)
+        for i in band:
+            for j in range(nu):
+                a, b, c, d = vid(i, j), vid(i, j + 1), vid(i + 1, j + 1), vid(i + 1, j)
+                def f(*k):
+                    if normals and uv:
+                        return " ".join(f"{x}/{x}/{x}" for x in k)
+                    if normals:
+                        return " ".join(f"{x}//{x}" for x in k)
+                    return " ".join(str(x) for x in k)
+                if i > 0:
+                    lines.append("f " + f(a, d, b))
+                if i < nv - 1:
+                    lines.append("f " + f(b, d, c))
+    return "\n".join(lines) + "\n"
+
+
+def _torus_obj(R, r, nu, nv):
+    lines = []
+    for i in range(nu):
+        u = 2 * np.pi * i / nu
+        for j in range(nv):
+            v = 2 * np.pi * j / nv
+            lines.append("v %.9f %.9f %.9f" % ((R + r * np.cos(v)) * np.cos(u), (R + r * np.cos(v)) * np.sin(u),
+                                               r * np.sin(v)))
+    idx = lambda i, j: (i % nu) * nv + (j % nv) + 1
+    for i in range(nu):
+        for j in range(nv):
+            lines.append(f"f {idx(i, j)} {idx(i + 1, j)} {idx(i + 1, j + 1)}")
+            lines.append(f"f {idx(i, j)} {idx(i + 1, j + 1)} {idx(i, j + 1)}")
+    return "\n".join(lines) + "\n"
+
+
+MESH_SCENE = """void plastic floor_m 0 0 5 .4 .4 .4 0 0
+floor_m polygon floor 0 0 12 -5 -5 0  5 -5 0  5 5 0  -5 5 0
+void light bright 0 0 3 50 50 50
+bright sphere lamp 0 0 4 2 -3 5 0.5
+void glass glassy 0 0 3 .9 .9 .9
+void mesh ballA 7 ballA.rtm -rz 30 -t 0 0 1 0 0
+glassy mesh torus 8 torus.rtm -s 0.5 -mx -t 2 0 0.5 0 0
+void mesh bead 5 bead.rtm -t -2 0.5 1 0 0
+"""
+
+
+def mesh():
+    """Compiled meshes made by the reference obj2mesh (smooth sphere with two
+    materials over several patches; flat torus with the mesh object's glass,
+    scaled and mirrored; smooth bead with (u,v) and a pure trans material),
+    and the reference rtrace -oLNnv for rays
+    aimed at them, with the deterministic settings of shading() (mesh/*.rtm,
+    mesh/scene.rad, mesh_rtrace.npz)."""
+    md = HERE / "mesh"
+    md.mkdir(exist_ok=True)
+    with tempfile.TemporaryDirectory() as td:
+        t = Path(td)
+        (t / "a.obj").write_text(_uv_sphere_obj(48, 32, 1.0, (0, 0, 0), ["red", "chrome"]))
+        (t / "a.mat").write_text("void plastic red 0 0 5 .7 .2 .2 .03 .05\nvoid metal chrome 0 0 5 .8 .8 .8 .9 0\n")
+        (t / "torus.obj").write_text(_torus_obj(0.6, 0.2, 24, 12))
+        (t / "c.obj").write_text(_uv_sphere_obj(10, 8, 0.4, (0, 0, 0), ["frost"], uv=True))
+        (t / "c.mat").write_text("void trans frost 0 0 7 .6 .6 .9 .04 0 .5 .8\n")
+        subprocess.run([str(REF / "obj2mesh"), "-a", "a.mat", "a.obj", str(md / "ballA.rtm")], cwd=td, check=True)
+        subprocess.run([str(REF / "obj2mesh"), "torus.obj", str(md / "torus.rtm")], cwd=td, check=True)
+        subprocess.run([str(REF / "obj2mesh"), "-a", "c.mat", "c.obj", str(md / "bead.rtm")], cwd=td, check=True)
+    (md / "scene.rad").write_text(MESH_SCENE)
+    rng = np.random.default_rng(8)
+    centers = np.array([[0, 0, 1], [2, 0, 0.5], [-2, 0.5, 1]])
+    o = rng.normal(size=(3000, 3))
+    o = o / np.linalg.norm(o, axis=1)[:, None] * 6 + [0, 0, 1.5]
+    o[:, 2] = np.abs(o[:, 2] - 1.5) + 0.2
+    tg = centers[rng.integers(0, 3, len(o))] + rng.normal(size=(len(o), 3)) * 0.6
+    d = tg - o
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = np.hstack([o, d])
+    with tempfile.TemporaryDirectory() as td:
+        octf = Path(td) / "scene.oct"
+        with open(octf, "wb") as f:
+            subprocess.run([str(REF / "oconv"), "-f", "scene.rad"], stdout=f, cwd=md, check=True)
+        r = subprocess.run([str(REF / "rtrace"), "-h", *SHADING_ARGS, "-oLNnv", str(octf)], input=rays_text(rays),
+                           capture_output=True, env=ENV, cwd=md, check=True)
+    v = np.array(r.stdout.split(), dtype=np.float64).reshape(len(rays), 10)
+    np.savez_compressed(HERE / "mesh_rtrace.npz", rays=rays, dist=v[:, 0], normal=v[:, 1:4], pnormal=v[:, 4:7],
+                        value=v[:, 7:10])
+
+
 if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
     jobs = {"config1": config1, "bins": bins, "stat_room": stat_room, "primitives": primitives,
-            "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send}
+            "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
+            "mesh": mesh}
     for name in sys.argv[1:] or list(jobs):
         jobs[name]()
     print("golden vectors written to", HERE)

@@ -1,0 +1,117 @@
+"""Mesh ingestion on the CPU: compiled meshes written by the reference
+obj2mesh are read, transformed and intersected by this project's host code
+(scene_io.cpp, rg_scene.cpp and the host/device primitive tests of
+rg_geom.hpp, built here with g++), and compared with the reference rtrace's
+-oLNn output for the same rays: same hit or miss, same distance and both
+normals (unperturbed and interpolated from the vertex normals)."""
+
+import ctypes as C
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+CSRC = ROOT / "pyradiance_amd" / "csrc"
+
+
+@pytest.fixture(scope="module")
+def sc(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    out = tmp_path_factory.mktemp("scene") / "scenecheck.so"
+    subprocess.run(["g++", "-O2", "-shared", "-fPIC", "-std=c++17", f"-I{CSRC}",
+                    str(ROOT / "tests/native/scene_check.cpp"), str(CSRC / "scene_io.cpp"),
+                    str(CSRC / "rg_scene.cpp"), "-o", str(out)], check=True)
+    L = C.CDLL(str(out))
+    L.scene_compile.argtypes = [C.POINTER(C.c_char_p), C.c_int, C.c_char_p, C.c_int]
+    L.scene_trace.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+    L.scene_prims.argtypes = [C.POINTER(C.c_int)]
+    L.scene_mat_name.restype = C.c_char_p
+    return L
+
+
+def compile_files(L, files):
+    arr = (C.c_char_p * len(files))(*[str(f).encode() for f in files])
+    err = C.create_string_buffer(512)
+    n = L.scene_compile(arr, len(files), err, 512)
+    if n < 0:
+        raise RuntimeError(err.value.decode())
+    return n
+
+
+def trace(L, rays):
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    out = np.zeros((len(rays), 8))
+    L.scene_trace(rays.ctypes.data_as(C.POINTER(C.c_double)), len(rays), out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def test_mesh_hits_match_reference_rtrace(sc):
+    g = np.load(GOLDEN / "mesh_rtrace.npz")
+    n = compile_files(sc, [GOLDEN / "mesh" / "scene.rad"])
+    assert n > 3000                      # 3 meshes of triangles + floor + lamp
+    out = trace(sc, g["rays"])
+    hit_ref = g["dist"] < 1e9
+    hit = out[:, 7] >= 0
+    assert np.array_equal(hit, hit_ref)
+    m = hit
+    # the reference prints 7 significant digits
+    assert np.allclose(out[m, 0], g["dist"][m], rtol=1e-6, atol=1e-6)
+    assert np.allclose(out[m, 1:4], g["normal"][m], atol=1e-6)
+    assert np.allclose(out[m, 4:7], g["pnormal"][m], atol=1e-6)
+    # smooth shading really perturbs: the interpolated normals differ
+    assert np.abs(g["normal"][m] - g["pnormal"][m]).max() > 1e-2
+
+
+def test_mesh_materials_and_kinds(sc):
+    compile_files(sc, [GOLDEN / "mesh" / "scene.rad"])
+    n = compile_files(sc, [GOLDEN / "mesh" / "scene.rad"])
+    p = np.zeros((n, 4), dtype=np.int32)
+    sc.scene_prims(p.ctypes.data_as(C.POINTER(C.c_int)))
+    tri = p[:, 0] == 8
+    names = {sc.scene_mat_name(int(m)).decode() for m in p[tri, 1]}
+    # triangle materials from inside the meshes, and the torus's own modifier
+    assert names == {"red", "chrome", "frost", "glassy"}
+    # triangles whose material comes from the mesh are faces (flat); the torus's are not
+    flat = {sc.scene_mat_name(int(m)).decode(): bool(f) for m, f in zip(p[tri, 1], p[tri, 2])}
+    assert flat == {"red": True, "chrome": True, "frost": True, "glassy": False}
+
+
+def test_transform_options_follow_xf(sc, tmp_path):
+    """-t/-rx/-ry/-rz/-s/-mx/-i compose as row-vector matrices applied in
+    order (common/xf.c): a unit triangle placed by a transform lands where
+    the same operations applied by hand put it."""
+    import math
+    (tmp_path / "tri.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    refbin = ROOT / "oracle" / "_ref" / "obj2mesh"
+    if not refbin.exists():
+        pytest.skip("reference obj2mesh not built")
+    subprocess.run([str(refbin), "tri.obj", "tri.rtm"], cwd=tmp_path, check=True)
+    args = "-rz 90 -s 2 -t 1 0 0 -i 2 -ry 90"
+    (tmp_path / "s.rad").write_text(f"void plastic m 0 0 5 .5 .5 .5 0 0\nm mesh t {len(args.split()) + 1} tri.rtm "
+                                    f"{args} 0 0\n")
+    compile_files(sc, [tmp_path / "s.rad"])
+    geom = np.zeros(64)
+    sc.scene_geom(geom.ctypes.data_as(C.POINTER(C.c_double)), 64)
+
+    def rz(a):
+        c, s = math.cos(math.radians(a)), math.sin(math.radians(a))
+        return np.array([[c, s, 0, 0], [-s, c, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]])
+
+    def ry(a):
+        c, s = math.cos(math.radians(a)), math.sin(math.radians(a))
+        return np.array([[c, 0, -s, 0], [0, 1, 0, 0], [s, 0, c, 0], [0, 0, 0, 1]])
+
+    S = np.diag([2.0, 2, 2, 1])
+    T = np.eye(4)
+    T[3, :3] = (1, 0, 0)
+    M = rz(90) @ S @ T @ ry(90) @ ry(90)          # "-i 2" repeats what follows it
+    want = []
+    for v in ((0, 0, 0), (1, 0, 0), (0, 1, 0)):
+        want.append((np.array([*v, 1.0]) @ M)[:3])
+    got = geom[:9].reshape(3, 3)
+    # obj2mesh quantises positions to its bounding cube (32 bits)
+    assert np.allclose(got, np.array(want), atol=1e-6)
