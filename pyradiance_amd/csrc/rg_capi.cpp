@@ -154,6 +154,7 @@ int rg_scene_load_text(rg_ctx* c, const char* text, int allow_exec) {
 int rg_scene_commit(rg_ctx* c) {
   RG_TRY(c, {
     hipck(hipSetDevice(c->device), "hipSetDevice");
+    c->db.resolve_instances();
     c->db.resolve_meshes();
     c->cs = compile_scene(c->db);
     CompiledScene& cs = c->cs;

@@ -155,6 +155,49 @@ void xf_vector(double r[3], const double v[3], const Xf& x) {  // multv3()
   for (int k = 0; k < 3; ++k) r[k] = v[0] * x.m[0][k] + v[1] * x.m[1][k] + v[2] * x.m[2][k];
 }
 
+// X followed by the parent transform P (row vectors: M = Mx * Mp)
+void compose_xf(Xf& x, const Xf& p) {
+  mult4(x.m, x.m, p.m);
+  x.sca *= p.sca;
+}
+
+// A surface object's real arguments carried through a transform: points
+// transformed, axis vectors rotated, radii scaled; a mirrored polygon's
+// vertex order is reversed so that its normal stays the transformed normal.
+RObj xf_object(const RObj& o, const Xf& X) {
+  RObj t = o;
+  auto& f = t.fargs;
+  const double sca = fabs(X.sca);
+  auto pt = [&](size_t k) {
+    double r[3];
+    xf_point(r, &o.fargs[k], X);
+    f[k] = r[0]; f[k + 1] = r[1]; f[k + 2] = r[2];
+  };
+  if (o.type == "polygon") {
+    for (size_t k = 0; k + 2 < f.size(); k += 3) pt(k);
+    if (X.sca < 0) {
+      size_t nv = f.size() / 3;
+      for (size_t a = 1, b = nv - 1; a < b; ++a, --b)
+        for (int c = 0; c < 3; ++c) std::swap(f[3 * a + c], f[3 * b + c]);
+    }
+  } else if (o.type == "sphere" || o.type == "bubble") {
+    if (f.size() >= 4) { pt(0); f[3] *= sca; }
+  } else if (o.type == "ring") {
+    if (f.size() >= 8) {
+      pt(0);
+      double v[3];
+      xf_vector(v, &o.fargs[3], X);
+      f[3] = v[0]; f[4] = v[1]; f[5] = v[2];
+      f[6] *= sca; f[7] *= sca;
+    }
+  } else if (f.size() >= 7) {          // cone, cup, cylinder, tube
+    pt(0);
+    pt(3);
+    for (size_t k = 6; k < f.size(); ++k) f[k] *= sca;
+  }
+  return t;
+}
+
 int material_kind(const std::string& t) {
   if (t == "plastic") return MK_PLASTIC;
   if (t == "metal") return MK_METAL;
@@ -241,15 +284,152 @@ CompiledScene compile_scene(const SceneDB& db) {
   // pass 1: primitives
   std::vector<int> obj_prim(nobj, -1);
   std::vector<std::array<double, 6>> dbounds;
+  // One surface object -> one primitive (-1 if degenerate).  mo: material
+  // object, omod: modifier that ranks coincident hits and names the
+  // primitive for rcontrib; flat: isflat() of the object reported as hit.
+  auto add_surface = [&](const RObj& o, int mo, int omod, bool flat) -> int {
+    Prim P;
+    memset(&P, 0, sizeof P);
+    P.mat = material_index(mo);
+    P.omod = omod;
+    P.modslot = -1;
+    P.srcidx = -1;
+    P.goff = (int)cs.geom.size();
+    P.nv = 0;
+    P.flat = 0;
+    double lo[3], hi[3];
+    const auto& f = o.fargs;
+    if (o.type == "polygon") {
+      if (f.size() < 9 || f.size() % 3) objerr(o, "bad # arguments");
+      int nv = (int)f.size() / 3;
+      auto V = [&](int k) { return &f[3 * k]; };
+      if (nv > 3) {
+        double d2 = 0;
+        for (int a = 0; a < 3; ++a) d2 += (V(0)[a] - V(nv - 1)[a]) * (V(0)[a] - V(nv - 1)[a]);
+        if (d2 <= FTINY * FTINY) nv--;
+      }
+      double n[3] = {0, 0, 0}, v1[3], v2[3], v3[3];
+      for (int a = 0; a < 3; ++a) v1[a] = V(1)[a] - V(0)[a];
+      for (int k = 2; k < nv; ++k) {
+        for (int a = 0; a < 3; ++a) v2[a] = V(k)[a] - V(0)[a];
+        cross3(v3, v1, v2);
+        for (int a = 0; a < 3; ++a) n[a] += v3[a];
+        memcpy(v1, v2, sizeof v1);
+      }
+      double area = norm3(n);
+      if (area == 0.0) {
+        cs.warnings.push_back("polygon \"" + o.name + "\": zero area");
+        return -1;
+      }
+      double offset = dot3(n, V(0));
+      for (int k = 1; k < nv; ++k) offset += dot3(n, V(k));
+      offset /= (double)nv;
+      int ax = fabs(n[1]) > fabs(n[0]);
+      if (fabs(n[2]) > fabs(n[ax])) ax = 2;
+      P.kind = PK_POLYGON;
+      P.nv = nv;
+      P.flat = 1;
+      cs.geom.insert(cs.geom.end(), {n[0], n[1], n[2], offset, (double)ax});
+      for (int a = 0; a < 3; ++a) { lo[a] = 1e300; hi[a] = -1e300; }
+      for (int k = 0; k < nv; ++k)
+        for (int a = 0; a < 3; ++a) {
+          cs.geom.push_back(V(k)[a]);
+          lo[a] = std::min(lo[a], V(k)[a]);
+          hi[a] = std::max(hi[a], V(k)[a]);
+        }
+    } else if (o.type == "sphere" || o.type == "bubble") {
+      if (f.size() != 4) objerr(o, "bad # arguments");
+      bool bubble = o.type == "bubble";
+      double r = f[3];
+      if (r < -FTINY) { bubble = !bubble; r = -r; cs.warnings.push_back(o.name + ": negative radius"); }
+      else if (r <= FTINY) objerr(o, "zero radius");
+      P.kind = bubble ? PK_BUBBLE : PK_SPHERE;
+      cs.geom.insert(cs.geom.end(), {f[0], f[1], f[2], r});
+      for (int a = 0; a < 3; ++a) { lo[a] = f[a] - r; hi[a] = f[a] + r; }
+    } else {
+      // cone family (getcone)
+      bool ring = o.type == "ring";
+      bool cylfam = o.type == "cylinder" || o.type == "tube";
+      std::string t = o.type;
+      double P0[3], P1[3], R0, R1, ad[3];
+      if (cylfam) {
+        if (f.size() != 7) objerr(o, "bad # arguments");
+        double r = f[6];
+        if (r < -FTINY) { t = (t == "cylinder") ? "tube" : "cylinder"; r = -r; }
+        else if (r <= FTINY) { cs.warnings.push_back(o.name + ": illegal radii"); return -1; }
+        for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
+        R0 = R1 = r;
+      } else {
+        if (f.size() != 8) objerr(o, "bad # arguments");
+        double c6 = f[6], c7 = f[7];
+        int s0 = c6 < -FTINY ? -1 : (c6 > FTINY ? 1 : 0);
+        int s1 = c7 < -FTINY ? -1 : (c7 > FTINY ? 1 : 0);
+        if (s0 + s1 == 0) { cs.warnings.push_back(o.name + ": illegal radii"); return -1; }
+        if ((s0 < 0) | (s1 < 0)) {
+          if (ring) { cs.warnings.push_back(o.name + ": illegal radii"); return -1; }
+          t = (t == "cone") ? "cup" : "cone";
+        }
+        c6 *= s0;
+        c7 *= s1;
+        if (c7 - c6 > FTINY) {
+          for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
+          R0 = c6; R1 = c7;
+        } else if (c6 - c7 > FTINY) {
+          for (int a = 0; a < 3; ++a) { P0[a] = ring ? f[a] : f[3 + a]; P1[a] = ring ? f[3 + a] : f[a]; }
+          R0 = c7; R1 = c6;
+        } else {
+          if (ring) { cs.warnings.push_back(o.name + ": illegal radii"); return -1; }
+          t = (t == "cone") ? "cylinder" : "tube";
+          for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
+          R0 = R1 = c6;
+          cylfam = true;
+        }
+      }
+      double al;
+      if (ring) {
+        for (int a = 0; a < 3; ++a) ad[a] = f[3 + a];
+        al = norm3(ad);
+      } else {
+        for (int a = 0; a < 3; ++a) ad[a] = P1[a] - P0[a];
+        al = norm3(ad);
+      }
+      if (al == 0.0) { cs.warnings.push_back(o.name + ": unknown orientation"); return -1; }
+      if (ring) {
+        P.kind = PK_RING;
+        P.flat = 1;
+        cs.geom.insert(cs.geom.end(), {f[0], f[1], f[2], ad[0], ad[1], ad[2], R0, R1});
+        for (int a = 0; a < 3; ++a) { lo[a] = f[a] - R1; hi[a] = f[a] + R1; }
+      } else {
+        P.kind = t == "cone" ? PK_CONE : t == "cup" ? PK_CUP : t == "cylinder" ? PK_CYLINDER : PK_TUBE;
+        double sl = cylfam ? al : sqrt((R1 - R0) * (R1 - R0) + al * al);
+        cs.geom.insert(cs.geom.end(), {P0[0], P0[1], P0[2], ad[0], ad[1], ad[2], al, R0, R1, sl});
+        double rr = std::max(R0, R1);
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(P0[a], P1[a]) - rr;
+          hi[a] = std::max(P0[a], P1[a]) + rr;
+        }
+      }
+    }
+    if (!flat) P.flat = 0;
+    int idx = (int)cs.prims.size();
+    cs.prims.push_back(P);
+    cs.prim_modname.push_back(objs[omod].name);
+    dbounds.push_back({lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
+    grow(lo, hi);
+    return idx;
+  };
   // mesh objects: one triangle primitive per mesh triangle, in world space
   // (the reference transforms the ray into the mesh instead; o_mesh.c:146-217)
-  auto add_mesh = [&](int oi) {
+  // parent: transform of the enclosing instances (nullptr at top level);
+  // ovr: the instance object whose modifier replaces every material (-1 none)
+  auto add_mesh = [&](int oi, const Xf* parent, int ovr) {
     const RObj& o = objs[oi];
     auto itm = db.obj_mesh.find(oi);
     if (itm == db.obj_mesh.end()) objerr(o, "mesh file not loaded");
     const MeshData& M = db.meshes[itm->second];
     Xf X;
     if (parse_xf(X, o.sargs, 1) != (int)o.sargs.size() - 1) objerr(o, "bad transform");
+    if (parent) compose_xf(X, *parent);
     const bool mirror = X.sca < 0;
     const size_t nv = M.has_n.size();
     std::vector<double> W(3 * nv), WN(3 * nv);
@@ -260,14 +440,14 @@ CompiledScene compile_scene(const SceneDB& db) {
     int objmat = -2;
     for (const auto& t : M.tris) {
       // the triangle's own material when the mesh's modifier is void
-      const bool pseudo = o.omod < 0 && t.mat >= 0;
-      const int mod = pseudo ? t.mat : o.omod;
+      const bool pseudo = ovr < 0 && o.omod < 0 && t.mat >= 0;
+      const int mod = ovr >= 0 ? objs[ovr].omod : pseudo ? t.mat : o.omod;
       if (mod < 0) continue;                         // void: never shaded
       int mo;
       if (pseudo) {
         mo = findmaterial_from(oi, mod);
       } else {
-        if (objmat == -2) objmat = findmaterial(oi);
+        if (objmat == -2) objmat = findmaterial(ovr >= 0 ? ovr : oi);
         mo = objmat;
       }
       if (mo < 0) continue;
@@ -309,153 +489,58 @@ CompiledScene compile_scene(const SceneDB& db) {
       grow(lo, hi);
     }
   };
+  // instance objects: the instanced octree's surfaces, transformed into
+  // place (the reference transforms the ray instead, rt/o_instance.c:16-71);
+  // a non-void instance modifier replaces every material inside
+  std::function<void(int, const Xf*, int, int)> add_instance = [&](int oi, const Xf* parent, int ovr, int depth) {
+    const RObj& io = objs[oi];
+    if (depth > 32) objerr(io, "instances nested too deeply");
+    auto itb = db.obj_inst.find(oi);
+    if (itb == db.obj_inst.end()) objerr(io, "octree not loaded");
+    const SceneDB::Block blk = db.inst_blocks[itb->second];
+    Xf X;
+    if (parse_xf(X, io.sargs, 1) != (int)io.sargs.size() - 1) objerr(io, "bad transform");
+    if (parent) compose_xf(X, *parent);
+    const int over = ovr >= 0 ? ovr : (io.omod >= 0 ? oi : -1);
+    const int overmat = over >= 0 ? findmaterial(over) : -1;
+    for (int j = blk.first; j < blk.first + blk.count; ++j) {
+      const RObj& o = objs[j];
+      if (o.type == "mesh") {
+        add_mesh(j, &X, over);
+      } else if (o.type == "instance") {
+        add_instance(j, &X, over, depth + 1);
+      } else if (SceneDB::is_surface(o.type) && o.type != "source") {
+        if (over >= 0) {
+          if (overmat >= 0) add_surface(xf_object(o, X), overmat, objs[over].omod, false);
+        } else if (o.omod >= 0) {
+          int mo = findmaterial(j);
+          if (mo >= 0) add_surface(xf_object(o, X), mo, o.omod, true);
+        }
+      }
+    }
+  };
   for (int oi = 0; oi < nobj; ++oi) {
     const RObj& o = objs[oi];
-    if (!SceneDB::is_surface(o.type)) {
-      if (o.type == "mesh") {
-        if (o.omod >= 0) {
-          int mo = findmaterial(oi);
-          if (mo >= 0) {
-            const std::string& mt = objs[mo].type;
-            if (mt == "light" || mt == "glow" || mt == "illum" || mt == "spotlight") objerr(o, "illegal material");
-          }
+    if (!db.hidden.empty() && db.hidden[oi]) continue;   // inside an instanced octree
+    if (o.type == "mesh") {
+      if (o.omod >= 0) {
+        int mo = findmaterial(oi);
+        if (mo >= 0) {
+          const std::string& mt = objs[mo].type;
+          if (mt == "light" || mt == "glow" || mt == "illum" || mt == "spotlight") objerr(o, "illegal material");
         }
-        add_mesh(oi);
-      } else if (o.type == "instance") {
-        objerr(o, "instances are not supported on the GPU path");
       }
+      add_mesh(oi, nullptr, -1);
       continue;
     }
-    if (o.type == "source") continue;
+    if (o.type == "instance") {
+      add_instance(oi, nullptr, -1, 0);
+      continue;
+    }
+    if (!SceneDB::is_surface(o.type) || o.type == "source") continue;
     if (o.omod < 0) continue;   // no modifier: never shaded
     int mo = findmaterial(oi);
-    Prim P;
-    memset(&P, 0, sizeof P);
-    P.mat = material_index(mo);
-    P.omod = o.omod;
-    P.modslot = -1;
-    P.srcidx = -1;
-    P.goff = (int)cs.geom.size();
-    P.nv = 0;
-    P.flat = 0;
-    double lo[3], hi[3];
-    const auto& f = o.fargs;
-    if (o.type == "polygon") {
-      if (f.size() < 9 || f.size() % 3) objerr(o, "bad # arguments");
-      int nv = (int)f.size() / 3;
-      auto V = [&](int k) { return &f[3 * k]; };
-      if (nv > 3) {
-        double d2 = 0;
-        for (int a = 0; a < 3; ++a) d2 += (V(0)[a] - V(nv - 1)[a]) * (V(0)[a] - V(nv - 1)[a]);
-        if (d2 <= FTINY * FTINY) nv--;
-      }
-      double n[3] = {0, 0, 0}, v1[3], v2[3], v3[3];
-      for (int a = 0; a < 3; ++a) v1[a] = V(1)[a] - V(0)[a];
-      for (int k = 2; k < nv; ++k) {
-        for (int a = 0; a < 3; ++a) v2[a] = V(k)[a] - V(0)[a];
-        cross3(v3, v1, v2);
-        for (int a = 0; a < 3; ++a) n[a] += v3[a];
-        memcpy(v1, v2, sizeof v1);
-      }
-      double area = norm3(n);
-      if (area == 0.0) {
-        cs.warnings.push_back("polygon \"" + o.name + "\": zero area");
-        continue;
-      }
-      double offset = dot3(n, V(0));
-      for (int k = 1; k < nv; ++k) offset += dot3(n, V(k));
-      offset /= (double)nv;
-      int ax = fabs(n[1]) > fabs(n[0]);
-      if (fabs(n[2]) > fabs(n[ax])) ax = 2;
-      P.kind = PK_POLYGON;
-      P.nv = nv;
-      P.flat = 1;
-      cs.geom.insert(cs.geom.end(), {n[0], n[1], n[2], offset, (double)ax});
-      for (int a = 0; a < 3; ++a) { lo[a] = 1e300; hi[a] = -1e300; }
-      for (int k = 0; k < nv; ++k)
-        for (int a = 0; a < 3; ++a) {
-          cs.geom.push_back(V(k)[a]);
-          lo[a] = std::min(lo[a], V(k)[a]);
-          hi[a] = std::max(hi[a], V(k)[a]);
-        }
-    } else if (o.type == "sphere" || o.type == "bubble") {
-      if (f.size() != 4) objerr(o, "bad # arguments");
-      bool bubble = o.type == "bubble";
-      double r = f[3];
-      if (r < -FTINY) { bubble = !bubble; r = -r; cs.warnings.push_back(o.name + ": negative radius"); }
-      else if (r <= FTINY) objerr(o, "zero radius");
-      P.kind = bubble ? PK_BUBBLE : PK_SPHERE;
-      cs.geom.insert(cs.geom.end(), {f[0], f[1], f[2], r});
-      for (int a = 0; a < 3; ++a) { lo[a] = f[a] - r; hi[a] = f[a] + r; }
-    } else {
-      // cone family (getcone)
-      bool ring = o.type == "ring";
-      bool cylfam = o.type == "cylinder" || o.type == "tube";
-      std::string t = o.type;
-      double P0[3], P1[3], R0, R1, ad[3];
-      if (cylfam) {
-        if (f.size() != 7) objerr(o, "bad # arguments");
-        double r = f[6];
-        if (r < -FTINY) { t = (t == "cylinder") ? "tube" : "cylinder"; r = -r; }
-        else if (r <= FTINY) { cs.warnings.push_back(o.name + ": illegal radii"); continue; }
-        for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
-        R0 = R1 = r;
-      } else {
-        if (f.size() != 8) objerr(o, "bad # arguments");
-        double c6 = f[6], c7 = f[7];
-        int s0 = c6 < -FTINY ? -1 : (c6 > FTINY ? 1 : 0);
-        int s1 = c7 < -FTINY ? -1 : (c7 > FTINY ? 1 : 0);
-        if (s0 + s1 == 0) { cs.warnings.push_back(o.name + ": illegal radii"); continue; }
-        if ((s0 < 0) | (s1 < 0)) {
-          if (ring) { cs.warnings.push_back(o.name + ": illegal radii"); continue; }
-          t = (t == "cone") ? "cup" : "cone";
-        }
-        c6 *= s0;
-        c7 *= s1;
-        if (c7 - c6 > FTINY) {
-          for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
-          R0 = c6; R1 = c7;
-        } else if (c6 - c7 > FTINY) {
-          for (int a = 0; a < 3; ++a) { P0[a] = ring ? f[a] : f[3 + a]; P1[a] = ring ? f[3 + a] : f[a]; }
-          R0 = c7; R1 = c6;
-        } else {
-          if (ring) { cs.warnings.push_back(o.name + ": illegal radii"); continue; }
-          t = (t == "cone") ? "cylinder" : "tube";
-          for (int a = 0; a < 3; ++a) { P0[a] = f[a]; P1[a] = f[3 + a]; }
-          R0 = R1 = c6;
-          cylfam = true;
-        }
-      }
-      double al;
-      if (ring) {
-        for (int a = 0; a < 3; ++a) ad[a] = f[3 + a];
-        al = norm3(ad);
-      } else {
-        for (int a = 0; a < 3; ++a) ad[a] = P1[a] - P0[a];
-        al = norm3(ad);
-      }
-      if (al == 0.0) { cs.warnings.push_back(o.name + ": unknown orientation"); continue; }
-      if (ring) {
-        P.kind = PK_RING;
-        P.flat = 1;
-        cs.geom.insert(cs.geom.end(), {f[0], f[1], f[2], ad[0], ad[1], ad[2], R0, R1});
-        for (int a = 0; a < 3; ++a) { lo[a] = f[a] - R1; hi[a] = f[a] + R1; }
-      } else {
-        P.kind = t == "cone" ? PK_CONE : t == "cup" ? PK_CUP : t == "cylinder" ? PK_CYLINDER : PK_TUBE;
-        double sl = cylfam ? al : sqrt((R1 - R0) * (R1 - R0) + al * al);
-        cs.geom.insert(cs.geom.end(), {P0[0], P0[1], P0[2], ad[0], ad[1], ad[2], al, R0, R1, sl});
-        double rr = std::max(R0, R1);
-        for (int a = 0; a < 3; ++a) {
-          lo[a] = std::min(P0[a], P1[a]) - rr;
-          hi[a] = std::max(P0[a], P1[a]) + rr;
-        }
-      }
-    }
-    obj_prim[oi] = (int)cs.prims.size();
-    cs.prims.push_back(P);
-    cs.prim_modname.push_back(objs[o.omod].name);
-    dbounds.push_back({lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
-    grow(lo, hi);
+    obj_prim[oi] = add_surface(o, mo, o.omod, true);
   }
   if (cs.prims.empty()) {
     for (int i = 0; i < 3; ++i) { cs.bmin[i] = 0; cs.bmax[i] = 1; }
@@ -477,6 +562,7 @@ CompiledScene compile_scene(const SceneDB& db) {
   for (int oi = 0; oi < nobj; ++oi) {
     const RObj& o = objs[oi];
     if (!SceneDB::is_surface(o.type) || o.omod < 0) continue;
+    if (!db.hidden.empty() && db.hidden[oi]) continue;   // sources inside instances are not marked
     int mo = findmaterial(oi);
     if (mo < 0) continue;
     const RObj& m = objs[mo];

@@ -484,22 +484,54 @@ int SceneDB::load_mesh(const std::string& path) {
   return idx;
 }
 
+// The file a mesh or instance object names: as given (relative to the
+// current directory, as the reference's library path starts with "."), then
+// next to the file that defined the object.
+std::string SceneDB::find_file(const RObj& o, const char* what) const {
+  if (o.sargs.empty()) throw RadError(o.type + " \"" + o.name + "\": bad # of arguments");
+  const std::string& f = o.sargs[0];
+  std::vector<std::string> cands = {f};
+  if (f[0] != '/' && !o.dir.empty()) cands.push_back(o.dir + f);
+  for (auto& c : cands) {
+    std::ifstream probe(c);
+    if (probe.good()) return c;
+  }
+  throw RadError(std::string("cannot find ") + what + " file \"" + f + "\"");
+}
+
 void SceneDB::resolve_meshes() {
   for (int i = 0; i < (int)objs.size(); ++i) {
     if (objs[i].type != "mesh" || obj_mesh.count(i)) continue;
     const RObj o = objs[i];
-    if (o.sargs.empty()) throw RadError("mesh \"" + o.name + "\": bad # of arguments");
-    const std::string& f = o.sargs[0];
-    std::vector<std::string> cands = {f};
-    if (f[0] != '/' && !o.dir.empty()) cands.push_back(o.dir + f);
-    std::string use;
-    for (auto& c : cands) {
-      std::ifstream probe(c);
-      if (probe.good()) { use = c; break; }
-    }
-    if (use.empty()) throw RadError("cannot find mesh file \"" + f + "\"");
-    obj_mesh[i] = load_mesh(use);
+    obj_mesh[i] = load_mesh(find_file(o, "mesh"));
   }
+}
+
+void SceneDB::resolve_instances() {
+  // objs grows while we walk it: nested instances are reached too
+  for (int i = 0; i < (int)objs.size(); ++i) {
+    if (objs[i].type != "instance" || obj_inst.count(i)) continue;
+    const RObj o = objs[i];
+    std::string use = find_file(o, "octree");
+    auto it = inst_files_.find(use);
+    if (it != inst_files_.end()) {
+      obj_inst[i] = it->second;
+      continue;
+    }
+    if (inst_files_.size() > 4096) throw RadError("too many instanced octrees");
+    double keep_org[3] = {cuorg[0], cuorg[1], cuorg[2]}, keep_size = cusize;
+    int first = (int)objs.size();
+    load_oct(use);
+    for (int k = 0; k < 3; ++k) cuorg[k] = keep_org[k];
+    cusize = keep_size;
+    int b = (int)inst_blocks.size();
+    inst_blocks.push_back({first, (int)objs.size() - first});
+    inst_files_[use] = b;
+    obj_inst[i] = b;
+  }
+  hidden.assign(objs.size(), 0);
+  for (const auto& b : inst_blocks)
+    for (int k = b.first; k < b.first + b.count; ++k) hidden[k] = 1;
 }
 
 void SceneDB::load_any(const std::string& path) {

@@ -45,6 +45,18 @@ class SceneDB {
   bool allow_exec = false;        // run "!command" lines (readobj.c:58-61)
   std::vector<MeshData> meshes;
   std::unordered_map<int, int> obj_mesh;    // mesh object -> meshes[]
+  // Octrees placed by instance objects: their objects are appended to objs
+  // (common/instance.c:23-60) as one block per file and are not part of the
+  // top-level scene (hidden), so they are neither primitives nor sources
+  // except through their instances.
+  struct Block { int first, count; };
+  std::vector<Block> inst_blocks;
+  std::unordered_map<int, int> obj_inst;    // instance object -> inst_blocks[]
+  std::vector<uint8_t> hidden;              // per object: inside an instanced octree
+
+  // Loads the octree of every instance object not loaded yet (nested ones
+  // too); files are shared.
+  void resolve_instances();
 
   // Loads the mesh file of every mesh object not loaded yet, appending each
   // file's material objects after the scene's (as the renderer's preload
@@ -64,6 +76,8 @@ class SceneDB {
  private:
   std::unordered_map<std::string, int> modtab_;
   std::unordered_map<std::string, int> mesh_files_;
+  std::unordered_map<std::string, int> inst_files_;
+  std::string find_file(const RObj& o, const char* what) const;
   std::string cur_dir_;           // directory of the file being read
   void add(RObj&& o);
   void parse_stream(const std::string& text, const std::string& where, int depth);

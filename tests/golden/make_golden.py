@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden/ from the reference tools
 built into oracle/_ref (rtrace, rcontrib, oconv) from the reference sources.
 
-    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send mesh]
+    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send mesh instance]
 
 Fixtures (inputs and the reference's outputs only):
   c1_seed{0,1,2}.npz     config-1 room: sensors, rtrace -I -ab 0 -dj 0 -ds 0 values
@@ -24,6 +24,8 @@ Fixtures (inputs and the reference's outputs only):
                          arguments and flux matrices (-ab 0, -c N samples)
   mesh/, mesh_rtrace.npz compiled meshes from the reference obj2mesh, a scene
                          placing them with transforms, and rtrace -oLNnv values
+  inst/, inst_rtrace.npz octrees (frozen, file-list, nested) placed by instance
+                         objects, and rtrace -oLNnv values
 The config-1/2 scenes come from pyradiance_amd.scenes (seeded) and are not
 stored; the two scenes above are stored as text with their vectors.
 """
@@ -562,12 +564,77 @@ def mesh():
                         value=v[:, 7:10])
 
 
+INST_A = """void plastic wood 0 0 5 .5 .35 .2 0 0
+void metal steel 0 0 5 .6 .6 .65 .8 .02
+void trans frosted 0 0 7 .7 .8 .7 .03 0 .4 .3
+wood polygon seat 0 0 12 -.4 -.4 .5  .4 -.4 .5  .4 .4 .5  -.4 .4 .5
+wood polygon back 0 0 12 -.4 .4 .5  .4 .4 .5  .4 .4 1.2  -.4 .4 1.2
+steel cylinder leg1 0 0 7 -.35 -.35 0  -.35 -.35 .5  .04
+steel tube leg2 0 0 7 .35 -.35 0  .35 -.35 .5  .04
+steel cone knob 0 0 8 0 .4 1.2  0 .4 1.5  .1 0
+steel sphere ball 0 0 4 0 -.2 .8 .15
+frosted ring disc 0 0 8 .2 -.1 .51  0 0 1  0 .15
+"""
+INST_B = """void plastic paint 0 0 5 .2 .6 .3 0 0
+paint polygon panel 0 0 12 -.5 0 0  .5 0 0  .5 0 1  -.5 0 1
+paint cup bowl 0 0 8 0 -.5 .2  0 -.5 .6  .1 .4
+"""
+INST_C = """void instance inner 5 a.oct -t 0 0 .1 0 0
+void plastic base 0 0 5 .3 .3 .3 0 0
+base polygon pad 0 0 12 -.6 -.6 .1  .6 -.6 .1  .6 .6 .1  -.6 .6 .1
+"""
+INST_SCENE = """void plastic floor_m 0 0 5 .4 .4 .4 0 0
+floor_m polygon floor 0 0 12 -6 -6 0  6 -6 0  6 6 0  -6 6 0
+void light bright 0 0 3 50 50 50
+bright sphere lamp 0 0 4 2 -3 5 0.5
+void glass glassy 0 0 3 .9 .9 .9
+void instance a1 7 a.oct -rz 30 -t 1 1 0 0 0
+glassy instance a2 8 a.oct -s 0.8 -mx -t -2 0 0 0 0
+void instance c1 5 c.oct -t 0 -2.5 0 0 0
+void instance b1 5 b.oct -t 2.5 -1 0 0 0
+"""
+
+
+def instance():
+    """Octrees placed by instance objects: a frozen octree of every surface
+    type with its own materials (placed rotated, and scaled, mirrored and
+    overridden by a glass modifier), a file-list octree, and an octree that
+    itself instances the first; the reference rtrace -oLNnv for rays aimed
+    at them, with the deterministic settings of shading() (inst/*,
+    inst_rtrace.npz)."""
+    idir = HERE / "inst"
+    idir.mkdir(exist_ok=True)
+    for n, t in (("a.rad", INST_A), ("b.rad", INST_B), ("c.rad", INST_C), ("scene.rad", INST_SCENE)):
+        (idir / n).write_text(t)
+    for src, dst, frozen in (("a.rad", "a.oct", True), ("b.rad", "b.oct", False), ("c.rad", "c.oct", True)):
+        with open(idir / dst, "wb") as f:
+            subprocess.run([str(REF / "oconv"), *(["-f"] if frozen else []), src], stdout=f, cwd=idir, check=True)
+    rng = np.random.default_rng(9)
+    centers = np.array([[1, 1, 0.6], [-2, 0, 0.5], [0, -2.5, 0.7], [2.5, -1, 0.5]])
+    o = rng.normal(size=(3000, 3))
+    o = o / np.linalg.norm(o, axis=1)[:, None] * 6 + [0, 0, 1.0]
+    o[:, 2] = np.abs(o[:, 2] - 1.0) + 0.3
+    tg = centers[rng.integers(0, 4, len(o))] + rng.normal(size=(len(o), 3)) * 0.4
+    d = tg - o
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = np.hstack([o, d])
+    with tempfile.TemporaryDirectory() as td:
+        octf = Path(td) / "scene.oct"
+        with open(octf, "wb") as f:
+            subprocess.run([str(REF / "oconv"), "-f", "scene.rad"], stdout=f, cwd=idir, check=True)
+        r = subprocess.run([str(REF / "rtrace"), "-h", *SHADING_ARGS, "-oLNnv", str(octf)], input=rays_text(rays),
+                           capture_output=True, env=ENV, cwd=idir, check=True)
+    v = np.array(r.stdout.split(), dtype=np.float64).reshape(len(rays), 10)
+    np.savez_compressed(HERE / "inst_rtrace.npz", rays=rays, dist=v[:, 0], normal=v[:, 1:4], pnormal=v[:, 4:7],
+                        value=v[:, 7:10])
+
+
 if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
     jobs = {"config1": config1, "bins": bins, "stat_room": stat_room, "primitives": primitives,
             "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
-            "mesh": mesh}
+            "mesh": mesh, "instance": instance}
     for name in sys.argv[1:] or list(jobs):
         jobs[name]()
     print("golden vectors written to", HERE)

@@ -29,6 +29,7 @@ int scene_compile(const char** files, int nfiles, char* err, int errlen) {
   try {
     SceneDB db;
     for (int i = 0; i < nfiles; ++i) db.load_any(files[i]);
+    db.resolve_instances();
     db.resolve_meshes();
     g_cs = compile_scene(db);
     return (int)g_cs.prims.size();

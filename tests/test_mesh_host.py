@@ -115,3 +115,31 @@ def test_transform_options_follow_xf(sc, tmp_path):
     got = geom[:9].reshape(3, 3)
     # obj2mesh quantises positions to its bounding cube (32 bits)
     assert np.allclose(got, np.array(want), atol=1e-6)
+
+
+def test_instance_hits_match_reference_rtrace(sc):
+    """Instanced octrees (frozen with every surface type, file-list, nested;
+    rotated, scaled, mirrored, material override) against the reference
+    rtrace -oLN."""
+    g = np.load(GOLDEN / "inst_rtrace.npz")
+    n = compile_files(sc, [GOLDEN / "inst" / "scene.rad"])
+    # floor + lamp + 3 placements of a.oct (7 surfaces) + b.oct (2) + c's pad
+    assert n == 2 + 3 * 7 + 2 + 1
+    out = trace(sc, g["rays"])
+    hit_ref = g["dist"] < 1e9
+    hit = out[:, 7] >= 0
+    assert np.array_equal(hit, hit_ref)
+    m = hit
+    assert np.allclose(out[m, 0], g["dist"][m], rtol=1e-6, atol=1e-6)
+    assert np.allclose(out[m, 1:4], g["normal"][m], atol=1e-6)
+
+
+def test_instance_materials(sc):
+    n = compile_files(sc, [GOLDEN / "inst" / "scene.rad"])
+    p = np.zeros((n, 4), dtype=np.int32)
+    sc.scene_prims(p.ctypes.data_as(C.POINTER(C.c_int)))
+    names = [sc.scene_mat_name(int(m)).decode() for m in p[:, 1]]
+    # the glass-modified placement turns all seven of its surfaces to glass
+    assert names.count("glassy") == 7
+    assert names.count("wood") == 2 * 2 and names.count("steel") == 2 * 4 and names.count("frosted") == 2
+    assert names.count("paint") == 2 and names.count("base") == 1
