@@ -174,6 +174,42 @@ int rg_dcsky(const float *A, const float *B, float *C, int M, int K, int N, int 
 /* Same with device pointers on the current HIP device. */
 int rg_dcsky_device(const float *A, const float *B, float *C, int M, int K, int N, int mode);
 
+/* gendaymtx (gen/gendaymtx.c): annual Perez sky matrix from WEA or EPW
+ * weather text.  The host reads the tape and sets up each step; the GPU
+ * evaluates the sky patches.  Options as the reference's flags. */
+typedef struct rg_skyopts {
+  int mfactor;            /* -m: Reinhart subdivision (1 = 145 sky patches + ground) */
+  double sky_color[3];    /* -c; 0 0 0 with -d / -n (sun only) */
+  double ground_color[3]; /* -g; 0 0 0 with -d / -n */
+  double sun_color[3];    /* 1 1 1; 0 0 0 with -s (sky only) */
+  double rotation;        /* -r degrees */
+  int nsuns;              /* patches the sun is spread over: 4; 1 with -5 */
+  double fixed_sun_sa;    /* -5: solar solid angle (sr), <= 0 otherwise */
+  int output;             /* -O0 visible radiance, -O1 solar radiance */
+  int sun_hours_only;     /* -u */
+  int average;            /* -A */
+} rg_skyopts;
+typedef struct rg_sky rg_sky;
+
+/* Reads the weather data and prepares every time step (replaces the
+ * gendaymtx process up to its matrix).  On failure returns -1 with the
+ * message in rg_sky_error(NULL). */
+int rg_sky_create(rg_sky **out, const char *weather_text, const rg_skyopts *opts);
+void rg_sky_destroy(rg_sky *s);
+const char *rg_sky_error(const rg_sky *s);
+/* out[5]: sky patches (rows, ground first), stored columns (time steps, 1
+ * with -A), stored steps, steps counted, suns placed (for -D). */
+int rg_sky_shape(const rg_sky *s, int *out);
+/* latitude north, longitude east, degrees (LATLONG header) */
+int rg_sky_site(const rg_sky *s, double *latlong);
+/* per placed sun: id (solar minute) and good flag; altitude, azimuth
+ * (radians) and direct illuminance, as gendaymtx -D writes them */
+int rg_sky_suns(const rg_sky *s, int *id_good, double *alt_azi_illum);
+/* The matrix [patch][column][rgb] in FP32 into a host buffer, or into device
+ * memory of the current device (the B operand of rg_dcsky_device). */
+int rg_sky_matrix(rg_sky *s, float *out);
+int rg_sky_matrix_device(rg_sky *s, float *dev_out);
+
 #ifdef __cplusplus
 }
 #endif

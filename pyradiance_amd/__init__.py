@@ -7,8 +7,9 @@ The compute runs in libradgpu.so (HIP kernels + C ABI, include/radgpu.h);
 build it with `python -m pyradiance_amd.build`.
 """
 
+from .gen import gendaymtx
 from .rflux import rfluxmtx
 from .rt import Rcontrib, rtrace
 from .util import dcsky, dctimestep, read_matrix, write_matrix
 
-__all__ = ["rtrace", "Rcontrib", "rfluxmtx", "dctimestep", "dcsky", "read_matrix", "write_matrix"]
+__all__ = ["rtrace", "Rcontrib", "rfluxmtx", "gendaymtx", "dctimestep", "dcsky", "read_matrix", "write_matrix"]

@@ -22,6 +22,8 @@ EXPORTS = (
     "rg_default_params_rcontrib", "rg_trace", "rg_contrib", "rg_contrib_batch", "rg_trace_batch", "rg_last_nrays",
     "rg_set_profiling", "rg_set_bvh_builder", "rg_set_tuning", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
     "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint", "rg_first_hit",
+    "rg_sky_create", "rg_sky_destroy", "rg_sky_error", "rg_sky_shape", "rg_sky_site", "rg_sky_suns",
+    "rg_sky_matrix", "rg_sky_matrix_device",
 )
 
 BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
@@ -36,6 +38,14 @@ class RgParams(C.Structure):
         ("minweight", C.c_double), ("dstrsrc", C.c_double), ("srcsizerat", C.c_double),
         ("specthresh", C.c_double), ("specjitter", C.c_double), ("ambval", C.c_double * 3),
         ("directvis", C.c_int), ("contrib", C.c_int), ("seed", C.c_ulonglong),
+    ]
+
+
+class RgSkyOpts(C.Structure):
+    _fields_ = [
+        ("mfactor", C.c_int), ("sky_color", C.c_double * 3), ("ground_color", C.c_double * 3),
+        ("sun_color", C.c_double * 3), ("rotation", C.c_double), ("nsuns", C.c_int),
+        ("fixed_sun_sa", C.c_double), ("output", C.c_int), ("sun_hours_only", C.c_int), ("average", C.c_int),
     ]
 
 
@@ -113,6 +123,16 @@ def lib() -> C.CDLL:
     L.rg_material_set.argtypes = [vp, C.c_int, dp, C.c_int]
     L.rg_contrib_adjoint.argtypes = [vp, dp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, dp, dp,
                                      C.POINTER(C.c_int), C.c_int, dp]
+    L.rg_sky_create.argtypes = [C.POINTER(vp), C.c_char_p, C.POINTER(RgSkyOpts)]
+    L.rg_sky_destroy.argtypes = [vp]
+    L.rg_sky_destroy.restype = None
+    L.rg_sky_error.argtypes = [vp]
+    L.rg_sky_error.restype = C.c_char_p
+    L.rg_sky_shape.argtypes = [vp, C.POINTER(C.c_int)]
+    L.rg_sky_site.argtypes = [vp, dp]
+    L.rg_sky_suns.argtypes = [vp, C.POINTER(C.c_int), dp]
+    L.rg_sky_matrix.argtypes = [vp, C.POINTER(C.c_float)]
+    L.rg_sky_matrix_device.argtypes = [vp, vp]
     _lib = L
     return L
 
@@ -336,3 +356,52 @@ def dcsky_device(A_ptr: int, B_ptr: int, C_ptr: int, M: int, K: int, N: int, mod
     L = lib()
     if L.rg_dcsky_device(C.c_void_p(A_ptr), C.c_void_p(B_ptr), C.c_void_p(C_ptr), M, K, N, mode) != 0:
         raise RuntimeError(L.rg_last_error(None).decode())
+
+
+class Sky:
+    """A prepared gendaymtx run (rg_sky): weather data read, steps set up."""
+
+    def __init__(self, weather: str, opts: RgSkyOpts):
+        self._L = lib()
+        self._h = C.c_void_p()
+        if self._L.rg_sky_create(C.byref(self._h), weather.encode(), C.byref(opts)) != 0:
+            raise RuntimeError(self._L.rg_sky_error(None).decode())
+        v = (C.c_int * 5)()
+        self._L.rg_sky_shape(self._h, v)
+        self.npatch, self.ncols, self.nsteps, self.ntsteps, self.nsuns = list(v)
+
+    def site(self) -> tuple:
+        v = (C.c_double * 2)()
+        self._L.rg_sky_site(self._h, v)
+        return v[0], v[1]
+
+    def suns(self):
+        """(ids, good, altitude, azimuth, direct illuminance) of placed suns."""
+        ig = np.zeros((self.nsuns, 2), dtype=np.int32)
+        v = np.zeros((self.nsuns, 3))
+        if self.nsuns:
+            self._L.rg_sky_suns(self._h, ig.ctypes.data_as(C.POINTER(C.c_int)), _dptr(v))
+        return ig[:, 0], ig[:, 1], v[:, 0], v[:, 1], v[:, 2]
+
+    def matrix(self) -> np.ndarray:
+        """(npatch, ncols, 3) float32, the reference's row order."""
+        out = np.empty((self.npatch, self.ncols, 3), dtype=np.float32)
+        if self._L.rg_sky_matrix(self._h, out.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+            raise RuntimeError(self._L.rg_sky_error(self._h).decode())
+        return out
+
+    def matrix_device(self, ptr: int) -> None:
+        """Writes the matrix into device memory at `ptr` (npatch*ncols*3 floats)."""
+        if self._L.rg_sky_matrix_device(self._h, C.c_void_p(ptr)) != 0:
+            raise RuntimeError(self._L.rg_sky_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._L.rg_sky_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

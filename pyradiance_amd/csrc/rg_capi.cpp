@@ -11,6 +11,7 @@
 #include "rg_bvh.hpp"
 #include "rg_render.hpp"
 #include "rg_scene.hpp"
+#include "rg_sky.hpp"
 
 using namespace rg;
 
@@ -699,6 +700,107 @@ int rg_dcsky(const float* A, const float* B, float* C, int M, int K, int N, int 
     if (dA) (void)hipFree(dA);
     if (dB) (void)hipFree(dB);
     if (dC) (void)hipFree(dC);
+    return -1;
+  }
+}
+
+// ---- gendaymtx ---------------------------------------------------------------
+
+struct rg_sky {
+  rg::SkyPlan plan;
+  std::string err;
+};
+
+int rg_sky_create(rg_sky** out, const char* weather, const rg_skyopts* o) {
+  *out = nullptr;
+  try {
+    if (!weather || !o) throw std::runtime_error("missing weather data or options");
+    rg::SkyOpts so;
+    so.mfactor = o->mfactor;
+    for (int c = 0; c < 3; ++c) {
+      so.sky_color[c] = o->sky_color[c];
+      so.ground_color[c] = o->ground_color[c];
+      so.sun_color[c] = o->sun_color[c];
+    }
+    so.rotation = o->rotation;
+    so.nsuns = o->nsuns;
+    so.fixed_sun_sa = o->fixed_sun_sa;
+    so.output = o->output;
+    so.sun_hours_only = o->sun_hours_only;
+    so.average = o->average;
+    rg_sky* s = new rg_sky;
+    try {
+      s->plan = rg::sky_plan(weather, so);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+    return 0;
+  } catch (const std::exception& e) {
+    g_create_err = e.what();
+    return -1;
+  }
+}
+
+void rg_sky_destroy(rg_sky* s) { delete s; }
+
+const char* rg_sky_error(const rg_sky* s) { return s ? s->err.c_str() : g_create_err.c_str(); }
+
+int rg_sky_shape(const rg_sky* s, int* out) {
+  if (!s) return -1;
+  out[0] = s->plan.npatch;
+  out[1] = s->plan.ncols;
+  out[2] = s->plan.nsteps;
+  out[3] = s->plan.ntsteps;
+  out[4] = (int)s->plan.suns.size();
+  return 0;
+}
+
+int rg_sky_site(const rg_sky* s, double* latlong) {
+  if (!s) return -1;
+  latlong[0] = s->plan.latitude;
+  latlong[1] = s->plan.longitude;
+  return 0;
+}
+
+int rg_sky_suns(const rg_sky* s, int* id_good, double* alt_azi_illum) {
+  if (!s) return -1;
+  for (size_t i = 0; i < s->plan.suns.size(); ++i) {
+    const rg::SkySun& u = s->plan.suns[i];
+    id_good[2 * i] = u.id;
+    id_good[2 * i + 1] = u.good;
+    alt_azi_illum[3 * i] = u.altitude;
+    alt_azi_illum[3 * i + 1] = u.azimuth;
+    alt_azi_illum[3 * i + 2] = u.dir_illum;
+  }
+  return 0;
+}
+
+int rg_sky_matrix_device(rg_sky* s, float* dout) {
+  try {
+    if (!s) throw std::runtime_error("no sky");
+    rg::sky_matrix(s->plan, dout, nullptr);
+    return 0;
+  } catch (const std::exception& e) {
+    s->err = e.what();
+    return -1;
+  }
+}
+
+int rg_sky_matrix(rg_sky* s, float* out) {
+  float* d = nullptr;
+  try {
+    if (!s) throw std::runtime_error("no sky");
+    size_t n = (size_t)s->plan.npatch * s->plan.ncols * 3;
+    hipck(hipMalloc(&d, n * sizeof(float)), "hipMalloc");
+    rg::sky_matrix(s->plan, d, nullptr);
+    hipck(hipMemcpy(out, d, n * sizeof(float), hipMemcpyDeviceToHost), "download");
+    (void)hipFree(d);
+    return 0;
+  } catch (const std::exception& e) {
+    s->err = e.what();
+    if (d) (void)hipFree(d);
     return -1;
   }
 }

@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden/ from the reference tools
 built into oracle/_ref (rtrace, rcontrib, oconv) from the reference sources.
 
-    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send mesh instance]
+    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send mesh instance sky]
 
 Fixtures (inputs and the reference's outputs only):
   c1_seed{0,1,2}.npz     config-1 room: sensors, rtrace -I -ab 0 -dj 0 -ds 0 values
@@ -26,6 +26,9 @@ Fixtures (inputs and the reference's outputs only):
                          placing them with transforms, and rtrace -oLNnv values
   inst/, inst_rtrace.npz octrees (frozen, file-list, nested) placed by instance
                          objects, and rtrace -oLNnv values
+  sky.npz, sky_cases.json  synthetic weather tapes (WEA of each kind, EPW) and
+                         the reference gendaymtx matrices for a set of options,
+                         one ASCII output with header, -D/-M sun descriptions
 The config-1/2 scenes come from pyradiance_amd.scenes (seeded) and are not
 stored; the two scenes above are stored as text with their vectors.
 """
@@ -629,12 +632,101 @@ def instance():
                         value=v[:, 7:10])
 
 
+def _weather(kind: str, rng) -> str:
+    """Synthetic weather tapes (four days in four seasons, hourly): WEA with
+    direct normal (1), direct horizontal (2) or photometric (3) columns, or an
+    EPW file with a varying dew point."""
+    days = [(1, 15), (3, 21), (6, 21), (9, 30)]
+    lat, lon, tz = (37.62, 122.38, 120) if kind != "epw" else (-33.95, -151.18, -150)
+    recs = []
+    for mo, da in days:
+        jd = [0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334][mo - 1] + da
+        dec = 0.4093 * np.sin(2 * np.pi / 365 * (jd - 81))
+        for h in range(24):
+            hr = h + 0.5
+            st = hr + (lon - tz) / 15.0 * (1 if kind != "epw" else -1) * -1
+            alt = np.arcsin(np.sin(np.radians(lat)) * np.sin(dec) -
+                            np.cos(np.radians(lat)) * np.cos(dec) * np.cos(st * np.pi / 12))
+            sa = max(np.sin(alt), 0.0)
+            cloud = rng.random()
+            dn = 900 * sa ** 0.5 * cloud if sa > 0.02 else 0.0
+            dh = (60 * sa + 90 * sa * (1 - cloud) + 5 * rng.random()) if sa > 0.0 else 0.0
+            recs.append((mo, da, hr, dn, dh, sa))
+    if kind == "epw":
+        lines = ["LOCATION,Sydney,NSW,AUS,synthetic,947670,-33.95,151.18,10.0,3.0",
+                 "DESIGN CONDITIONS,0", "TYPICAL/EXTREME PERIODS,0", "GROUND TEMPERATURES,0",
+                 "HOLIDAYS/DAYLIGHT SAVINGS,No,0,0,0", "COMMENTS 1,synthetic", "COMMENTS 2,synthetic",
+                 "DATA PERIODS,1,1,Data,Sunday, 1/ 1,12/31"]
+        for k, (mo, da, hr, dn, dh, sa) in enumerate(recs):
+            dew = 8 + 6 * np.sin(k / 7.0)
+            lines.append(f"2001,{mo},{da},{int(hr + 0.5)},60,?9?9?9?9E0?9?9?9*9*9?9*9*9?9*9*9?9?9*9*9*9*9,"
+                         f"20.0,{dew:.1f},60,101300,0,1415,350,{dn * sa + dh:.0f},{dn:.0f},{dh:.0f},"
+                         "0,0,0,0,180,3.0,5,3,20.0,77777,9,999999999,20,0.1,0,88,0.2,0,0")
+        return "\n".join(lines) + "\n"
+    units = {"wea1": 1, "wea2": 2, "wea3": 3}[kind]
+    lines = ["place San_Francisco_USA", f"latitude {lat}", f"longitude {lon}", f"time_zone {tz}",
+             "site_elevation 5.0", f"weather_data_file_units {units}"]
+    for mo, da, hr, dn, dh, sa in recs:
+        a, b = dn, dh
+        if units == 2:
+            a = dn * sa
+        elif units == 3:
+            a, b = dn * 105.0, dh * 118.0
+        lines.append(f"{mo} {da} {hr:.3f} {a:.1f} {b:.1f}")
+    return "\n".join(lines) + "\n"
+
+
+SKY_CASES = {
+    "wea1_m1": ("wea1", ["-m", "1"]),
+    "wea1_m2_opts": ("wea1", ["-m", "2", "-c", ".9", "1", "1.1", "-g", ".3", ".25", ".2", "-r", "30"]),
+    "wea1_onesun": ("wea1", ["-m", "1", "-5", ".533"]),
+    "wea1_sun_only": ("wea1", ["-m", "1", "-d"]),
+    "wea1_sky_only": ("wea1", ["-m", "1", "-s"]),
+    "wea1_sun_hours": ("wea1", ["-m", "1", "-u"]),
+    "wea1_average": ("wea1", ["-m", "1", "-A"]),
+    "wea1_solar": ("wea1", ["-m", "1", "-O1"]),
+    "wea2_m1": ("wea2", ["-m", "1"]),
+    "wea3_m1": ("wea3", ["-m", "1"]),
+    "epw_m1": ("epw", ["-m", "1"]),
+    "epw_m4_hours": ("epw", ["-m", "4", "-u", "-A"]),
+}
+
+
+def sky():
+    """The reference gendaymtx on synthetic weather tapes: binary float
+    matrices for every case of SKY_CASES, the ASCII output and header of one
+    case, and the sun descriptions of -D (sky.npz, sky_cases.json)."""
+    import json
+    rng = np.random.default_rng(21)
+    tapes = {k: _weather(k, rng) for k in ("wea1", "wea2", "wea3", "epw")}
+    out, meta = {}, {"tapes": tapes, "cases": {}}
+    with tempfile.TemporaryDirectory() as td:
+        for k, t in tapes.items():
+            (Path(td) / f"{k}.txt").write_text(t)
+        for name, (tape, args) in SKY_CASES.items():
+            r = subprocess.run([str(REF / "gendaymtx"), "-h", *args, "-of", f"{tape}.txt"], capture_output=True,
+                               cwd=td, check=True)
+            out[name] = np.frombuffer(r.stdout, dtype="<f4").copy()
+            meta["cases"][name] = {"tape": tape, "args": args}
+        r = subprocess.run([str(REF / "gendaymtx"), "-m", "1", "wea1.txt"], capture_output=True, cwd=td, check=True)
+        meta["ascii_wea1_m1"] = r.stdout.decode()
+        r = subprocess.run([str(REF / "gendaymtx"), "-D", "-", "wea1.txt"], capture_output=True, cwd=td, check=True)
+        meta["suns_wea1"] = r.stdout.decode()
+        r = subprocess.run([str(REF / "gendaymtx"), "-m", "1", "-D", "suns.rad", "-M", "mods.txt", "-of", "-h",
+                            "epw.txt"], capture_output=True, cwd=td, check=True)
+        meta["suns_epw_file"] = (Path(td) / "suns.rad").read_text()
+        meta["mods_epw_file"] = (Path(td) / "mods.txt").read_text()
+        out["epw_with_suns"] = np.frombuffer(r.stdout, dtype="<f4").copy()
+    np.savez_compressed(HERE / "sky.npz", **out)
+    (HERE / "sky_cases.json").write_text(json.dumps(meta, indent=1) + "\n")
+
+
 if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
     jobs = {"config1": config1, "bins": bins, "stat_room": stat_room, "primitives": primitives,
             "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
-            "mesh": mesh, "instance": instance}
+            "mesh": mesh, "instance": instance, "sky": sky}
     for name in sys.argv[1:] or list(jobs):
         jobs[name]()
     print("golden vectors written to", HERE)
