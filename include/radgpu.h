@@ -174,6 +174,21 @@ int rg_dcsky(const float *A, const float *B, float *C, int M, int K, int N, int 
 /* Same with device pointers on the current HIP device. */
 int rg_dcsky_device(const float *A, const float *B, float *C, int M, int K, int N, int mode);
 
+/* A view after setview() (common/image.c:23-118): type 'v' 'l' 'c' 'a' 'h'
+ * 's', unit direction, horizontal and vertical vectors scaled by the view
+ * size, their squared half sizes, offsets and clipping distances. */
+typedef struct rg_view {
+  int type;
+  double vp[3], vdir[3], hvec[3], vvec[3];
+  double hn2, vn2, horiz, vert, hoff, voff, vfore, vaft;
+} rg_view;
+
+/* A picture: one primary ray per pixel generated on the GPU from the view
+ * (rpict -ps 1, rt/rpict.c:668-712, pixel jitter pj), traced as rg_trace
+ * does; out[yres][xres][3], top scanline first.  Pixels the view has no ray
+ * for are 0.  flags as rg_trace. */
+int rg_trace_view(rg_ctx *ctx, const rg_view *v, int xres, int yres, double pj, int flags, double *out);
+
 /* gendaymtx (gen/gendaymtx.c): annual Perez sky matrix from WEA or EPW
  * weather text.  The host reads the tape and sets up each step; the GPU
  * evaluates the sky patches.  Options as the reference's flags. */

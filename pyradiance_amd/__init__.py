@@ -9,7 +9,8 @@ build it with `python -m pyradiance_amd.build`.
 
 from .gen import gendaymtx
 from .rflux import rfluxmtx
-from .rt import Rcontrib, rtrace
+from .rt import Rcontrib, rpict, rtrace
 from .util import dcsky, dctimestep, read_matrix, write_matrix
+from .view import vwrays
 
-__all__ = ["rtrace", "Rcontrib", "rfluxmtx", "gendaymtx", "dctimestep", "dcsky", "read_matrix", "write_matrix"]
+__all__ = ["rtrace", "Rcontrib", "rpict", "vwrays", "rfluxmtx", "gendaymtx", "dctimestep", "dcsky", "read_matrix", "write_matrix"]

@@ -23,7 +23,7 @@ EXPORTS = (
     "rg_set_profiling", "rg_set_bvh_builder", "rg_set_tuning", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
     "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint", "rg_first_hit",
     "rg_sky_create", "rg_sky_destroy", "rg_sky_error", "rg_sky_shape", "rg_sky_site", "rg_sky_suns",
-    "rg_sky_matrix", "rg_sky_matrix_device",
+    "rg_sky_matrix", "rg_sky_matrix_device", "rg_trace_view",
 )
 
 BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
@@ -46,6 +46,15 @@ class RgSkyOpts(C.Structure):
         ("mfactor", C.c_int), ("sky_color", C.c_double * 3), ("ground_color", C.c_double * 3),
         ("sun_color", C.c_double * 3), ("rotation", C.c_double), ("nsuns", C.c_int),
         ("fixed_sun_sa", C.c_double), ("output", C.c_int), ("sun_hours_only", C.c_int), ("average", C.c_int),
+    ]
+
+
+class RgView(C.Structure):
+    _fields_ = [
+        ("type", C.c_int), ("vp", C.c_double * 3), ("vdir", C.c_double * 3), ("hvec", C.c_double * 3),
+        ("vvec", C.c_double * 3), ("hn2", C.c_double), ("vn2", C.c_double), ("horiz", C.c_double),
+        ("vert", C.c_double), ("hoff", C.c_double), ("voff", C.c_double), ("vfore", C.c_double),
+        ("vaft", C.c_double),
     ]
 
 
@@ -123,6 +132,7 @@ def lib() -> C.CDLL:
     L.rg_material_set.argtypes = [vp, C.c_int, dp, C.c_int]
     L.rg_contrib_adjoint.argtypes = [vp, dp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, dp, dp,
                                      C.POINTER(C.c_int), C.c_int, dp]
+    L.rg_trace_view.argtypes = [vp, C.POINTER(RgView), C.c_int, C.c_int, C.c_double, C.c_int, dp]
     L.rg_sky_create.argtypes = [C.POINTER(vp), C.c_char_p, C.POINTER(RgSkyOpts)]
     L.rg_sky_destroy.argtypes = [vp]
     L.rg_sky_destroy.restype = None
@@ -235,6 +245,19 @@ class Context:
         self._check(self._L.rg_contrib_adjoint(self._h, _dptr(rays), nrec, accum, flags, arr, len(mods), _dptr(dlde),
                                                _dptr(lcol), mi, len(mats), _dptr(grad)))
         return grad
+
+    def trace_view(self, view, xres: int, yres: int, pj: float = 0.0, flags: int = 0) -> np.ndarray:
+        """A picture of a set-up view (pyradiance_amd.view.View): (yres, xres, 3)
+        radiances, top scanline first, rays generated on the GPU."""
+        v = RgView()
+        v.type = ord(view.type)
+        v.vp[:], v.vdir[:], v.hvec[:], v.vvec[:] = view.vp, view.vdir, view.hvec, view.vvec
+        v.hn2, v.vn2, v.horiz, v.vert = view.hn2, view.vn2, view.horiz, view.vert
+        v.hoff, v.voff, v.vfore, v.vaft = view.hoff, view.voff, view.vfore, view.vaft
+        out = np.zeros((yres, xres, 3))
+        self._check(self._L.rg_trace_view(self._h, C.byref(v), int(xres), int(yres), float(pj), int(flags),
+                                          _dptr(out)))
+        return out
 
     def first_hit(self, rays: np.ndarray):
         """(t, normal, prim) of each ray's first intersection (rtrace -oLN)."""

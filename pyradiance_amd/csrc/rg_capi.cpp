@@ -12,6 +12,7 @@
 #include "rg_render.hpp"
 #include "rg_scene.hpp"
 #include "rg_sky.hpp"
+#include "rg_view.hpp"
 
 using namespace rg;
 
@@ -292,7 +293,7 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
     d_bins = (BinSpec*)c->arena.get("bins", sizeof(BinSpec) * bins.size(), c->st, false);
     hipck(hipMemcpyAsync(d_bins, bins.data(), sizeof(BinSpec) * bins.size(), hipMemcpyHostToDevice, c->st), "bins");
   }
-  if (nrays)
+  if (nrays && rays)      // rays == nullptr: already generated in the "rays" buffer on the device
     hipck(hipMemcpyAsync(d_rays, rays, sizeof(double) * 6 * nrays, hipMemcpyHostToDevice, c->st), "rays");
   RunCfg cfg;
   cfg.p = c->params;
@@ -384,6 +385,35 @@ int rg_trace(rg_ctx* c, const double* rays, long long n, int flags, double* out)
   RG_TRY(c, {
     std::vector<BinSpec> none;
     run(c, rays, n, 1, flags, 0, none, 3, out);
+  });
+}
+
+int rg_trace_view(rg_ctx* c, const rg_view* v, int xres, int yres, double pj, int flags, double* out) {
+  RG_TRY(c, {
+    if (!v || xres <= 0 || yres <= 0 || (long long)xres * yres > 0x7fffffffLL) throw std::runtime_error("bad resolution");
+    if (v->vaft > 1e-6) throw std::runtime_error("aft clipping plane (-va) is not supported on the GPU path");
+    if (!strchr("vlcahs", v->type) || !v->type) throw std::runtime_error("unknown view type");
+    if (!c->committed) throw std::runtime_error("scene not committed");
+    const long long n = (long long)xres * yres;
+    hipck(hipSetDevice(c->device), "hipSetDevice");
+    double* d_rays = (double*)c->arena.get("rays", sizeof(double) * 6 * n, c->st, false);
+    uint8_t* d_valid = (uint8_t*)c->arena.get("view_valid", (size_t)n, c->st, false);
+    ViewSpec vs;
+    memset(&vs, 0, sizeof vs);
+    vs.type = v->type;
+    for (int k = 0; k < 3; ++k) {
+      vs.vp[k] = v->vp[k]; vs.vdir[k] = v->vdir[k]; vs.hvec[k] = v->hvec[k]; vs.vvec[k] = v->vvec[k];
+    }
+    vs.hn2 = v->hn2; vs.vn2 = v->vn2; vs.horiz = v->horiz; vs.vert = v->vert;
+    vs.hoff = v->hoff; vs.voff = v->voff; vs.vfore = v->vfore; vs.vaft = v->vaft;
+    view_rays(vs, xres, yres, pj, c->params.seed, d_rays, d_valid, c->st);
+    hipck(hipGetLastError(), "view rays");
+    std::vector<BinSpec> none;
+    run(c, nullptr, n, 1, flags, 0, none, 3, out);
+    std::vector<uint8_t> valid((size_t)n);
+    hipck(hipMemcpy(valid.data(), d_valid, (size_t)n, hipMemcpyDeviceToHost), "view mask");
+    for (long long i = 0; i < n; ++i)
+      if (!valid[(size_t)i]) out[3 * i] = out[3 * i + 1] = out[3 * i + 2] = 0.0;
   });
 }
 

@@ -417,3 +417,95 @@ def _contrib_bytes(out: np.ndarray, opt, mods: list, prog: str, argv: list[str],
     if opt.xres <= 0 or reclen > 1:
         info.append(f"NCOLS={reclen}")
     return _header(prog, argv, octree, opt.outform, "\n".join(info), scene_lines) + body
+
+
+def rpict(
+    view: Sequence[str],
+    octree: Path | str,
+    xres: None | int = None,
+    yres: None | int = None,
+    report: float = 0,
+    report_file: None | Path = None,
+    params: None | Sequence[str] = None,
+) -> bytes:
+    """rpict on the GPU (reference wrapper: src/pyradiance/rt.py:238-275).
+
+    One primary ray per pixel, generated on the GPU from the view
+    (rt/rpict.c:668-712 with -pj jitter), shaded as rtrace does, written as a
+    Radiance picture.  Every pixel is sampled: -ps / -pt (adaptive sampling
+    and interpolation) are accepted and have no effect, as with -ps 1."""
+    from . import view as _view
+    argv = list(view)
+    if report:
+        argv += ["-t", str(report)]
+    if report_file:
+        argv += ["-e", str(report_file)]
+    if xres is not None:
+        argv += ["-x", str(xres)]
+    if yres is not None:
+        argv += ["-y", str(yres)]
+    if params:
+        argv += list(params)
+    v, rest = _view.parse_view(argv)
+    xr, yr, pa, pj = 512, 512, 1.0, 0.67
+    ropts = []
+    i = 0
+    try:
+        while i < len(rest):
+            a = rest[i]
+            if a in ("-x", "-y", "-pa", "-pj", "-ps", "-pt", "-pd", "-pm", "-t", "-e"):
+                if i + 1 >= len(rest):
+                    raise ValueError(f"missing argument for {a}")
+                val = rest[i + 1]
+                if a == "-x":
+                    xr = int(val)
+                elif a == "-y":
+                    yr = int(val)
+                elif a == "-pa":
+                    pa = float(val)
+                elif a == "-pj":
+                    pj = float(val)
+                elif a in ("-pd", "-pm") and float(val) > 1e-6:
+                    raise ValueError(f"{a}: depth of field and motion blur are not supported on the GPU path")
+                i += 2
+                continue
+            if a in ("-o", "-z", "-r", "-ro", "-S", "-P", "-PP", "-i"):
+                raise ValueError(f"rpict option {a} is not supported on the GPU path")
+            ropts.append(a)
+            i += 1
+        if xr <= 0 or yr <= 0:
+            raise ValueError("bad resolution")
+        opt = _params.parse(ropts, rcontrib=False)
+        # rpict's own defaults (rt/rpict.c:60-106) where they differ from rtrace's
+        given = set(ropts)
+        p = opt.params
+        if "-ds" not in given:
+            p.srcsizerat = 0.25
+        if "-lr" not in given:
+            p.maxdepth = 7
+        if "-lw" not in given:
+            p.minweight = 1e-4
+        if "-ad" not in given:
+            p.ambdiv = 512
+        if "-dj" not in given:
+            p.dstrsrc = 0.0
+        v.setup()
+    except ValueError as e:
+        raise _rterr(str(e))
+    pa, xr, yr = _view.normaspect(v.aspect(), pa, xr, yr)
+    ctx = _scene(octree)
+    opt.params.seed = _seed(opt.uncorrelated)
+    flags = (_lib.RT_IMM_IRRAD if opt.imm_irrad else 0) | (_lib.RT_LAMB_IRRAD if opt.lamb_irrad else 0)
+    try:
+        ctx.set_params(opt.params)
+        img = ctx.trace_view(v, xr, yr, pj, flags)
+    except RuntimeError as e:
+        raise _rterr(str(e))
+    now = time.time()
+    head = [*_octree_header(octree), " ".join(["rpict", *argv]), f"SOFTWARE= {VERSION}", "VIEW=" + v.header(),
+            time.strftime("CAPDATE= %Y:%m:%d %H:%M:%S", time.localtime(now)),
+            time.strftime("GMT= %Y:%m:%d %H:%M:%S", time.gmtime(now))]
+    if pa < .99 or pa > 1.01:
+        head.append("PIXASPECT=%f" % pa)
+    head.append("PRIMARIES= 0.6400 0.3300 0.2900 0.6000 0.1500 0.0600 0.3333 0.3333")
+    return _view.write_picture(img, head)

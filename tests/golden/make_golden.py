@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden/ from the reference tools
 built into oracle/_ref (rtrace, rcontrib, oconv) from the reference sources.
 
-    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send mesh instance sky]
+    python tests/golden/make_golden.py [config1 bins stat_room primitives shading bins_flux rflux rflux_send mesh instance sky image]
 
 Fixtures (inputs and the reference's outputs only):
   c1_seed{0,1,2}.npz     config-1 room: sensors, rtrace -I -ab 0 -dj 0 -ds 0 values
@@ -26,6 +26,9 @@ Fixtures (inputs and the reference's outputs only):
                          placing them with transforms, and rtrace -oLNnv values
   inst/, inst_rtrace.npz octrees (frozen, file-list, nested) placed by instance
                          objects, and rtrace -oLNnv values
+  vwrays.npz, image.json, image/*.hdr  reference vwrays rays and dimensions for
+                         every view type; reference rpict pictures of the
+                         mesh scene (-ps 1 -pj 0, deterministic shading)
   sky.npz, sky_cases.json  synthetic weather tapes (WEA of each kind, EPW) and
                          the reference gendaymtx matrices for a set of options,
                          one ASCII output with header, -D/-M sun descriptions
@@ -721,12 +724,60 @@ def sky():
     (HERE / "sky_cases.json").write_text(json.dumps(meta, indent=1) + "\n")
 
 
+VW_CASES = {
+    "per_offset": ["-vtv", "-vp", "0", "-5", "1.5", "-vd", "0", "1", "-0.1", "-vh", "50", "-vv", "40", "-vs", ".1",
+                   "-vl", "-.05", "-vo", ".5"],
+    "parallel": ["-vtl", "-vp", "0", "-5", "1", "-vd", "0", "1", "0", "-vh", "6", "-vv", "4"],
+    "cylinder": ["-vtc", "-vp", "0", "0", "1", "-vd", "1", "0", "0", "-vh", "180", "-vv", "60"],
+    "angular": ["-vta", "-vp", "0", "0", "3", "-vd", "0", "0", "-1", "-vu", "0", "1", "0", "-vh", "180", "-vv",
+                "180"],
+    "hemi": ["-vth", "-vp", "0", "0", "3", "-vd", "0", ".2", "-1", "-vu", "0", "1", "0", "-vh", "180", "-vv",
+             "180"],
+    "planisphere": ["-vts", "-vp", "0", "-4", "1", "-vd", "0", "1", "0", "-vh", "150", "-vv", "120"],
+}
+RP_VIEWS = {
+    "persp": ["-vtv", "-vp", "0", "-6", "2.5", "-vd", "0", "1", "-0.25", "-vh", "60", "-vv", "45"],
+    "fisheye": ["-vta", "-vp", "0.2", "-0.3", "4.5", "-vd", "0", "0", "-1", "-vu", "0", "1", "0", "-vh", "150",
+                "-vv", "150"],
+}
+
+
+def image():
+    """The image path: reference vwrays rays (double output, 16 x 12 where the
+    aspect allows) and -d dimensions for every view type, and reference rpict
+    pictures of the mesh scene at -ps 1 -pj 0 with the deterministic shading
+    settings (image/*.hdr, vwrays.npz, image.json)."""
+    import json
+    idir = HERE / "image"
+    idir.mkdir(exist_ok=True)
+    out, meta = {}, {"vwrays": VW_CASES, "rpict": RP_VIEWS, "dims": {}}
+    for name, v in VW_CASES.items():
+        r = subprocess.run([str(REF / "vwrays"), "-fd", "-x", "16", "-y", "12", *v], capture_output=True, check=True)
+        out[name] = np.frombuffer(r.stdout, dtype="<f8").reshape(-1, 6)
+        r = subprocess.run([str(REF / "vwrays"), "-d", "-x", "16", "-y", "12", *v], capture_output=True, check=True)
+        meta["dims"][name] = r.stdout.decode()
+    r = subprocess.run([str(REF / "vwrays"), "-fd", "-x", "16", "-y", "12", "-pa", "0", *VW_CASES["per_offset"]],
+                       capture_output=True, check=True)
+    out["per_offset_pa0"] = np.frombuffer(r.stdout, dtype="<f8").reshape(-1, 6)
+    np.savez_compressed(HERE / "vwrays.npz", **out)
+    md = HERE / "mesh"
+    with tempfile.TemporaryDirectory() as td:
+        octf = Path(td) / "scene.oct"
+        with open(octf, "wb") as f:
+            subprocess.run([str(REF / "oconv"), "-f", "scene.rad"], stdout=f, cwd=md, check=True)
+        for name, v in RP_VIEWS.items():
+            r = subprocess.run([str(REF / "rpict"), *v, "-x", "48", "-y", "48", "-ps", "1", "-pj", "0",
+                                *SHADING_ARGS, str(octf)], capture_output=True, env=ENV, cwd=md, check=True)
+            (idir / f"{name}.hdr").write_bytes(r.stdout)
+    (HERE / "image.json").write_text(json.dumps(meta, indent=1) + "\n")
+
+
 if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
     jobs = {"config1": config1, "bins": bins, "stat_room": stat_room, "primitives": primitives,
             "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
-            "mesh": mesh, "instance": instance, "sky": sky}
+            "mesh": mesh, "instance": instance, "sky": sky, "image": image}
     for name in sys.argv[1:] or list(jobs):
         jobs[name]()
     print("golden vectors written to", HERE)
