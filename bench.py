@@ -407,6 +407,197 @@ def grad_bench(steps: int = 3):
             "grad_norm": gnorm}
 
 
+REF_ENV_PATH = ROOT / "oracle" / "_ref"
+
+
+def _ref_env():
+    ref = REF_ENV_PATH
+    return dict(os.environ, RAYPATH=f".:{ref / 'lib'}:{ROOT / 'oracle'}", PATH=f"{ref}:{os.environ.get('PATH', '')}")
+
+
+def sky_bench(reps: int = 5, cpu: bool = True):
+    """gendaymtx on a synthetic annual hourly weather tape (8,760 steps) at
+    MF 1 (146 patches) and MF 4 (2,306 patches): host set-up, the device
+    matrix (sky patches evaluated and written into HBM), whole gendaymtx()
+    calls with float and ASCII output, and the reference gendaymtx on the
+    same tape."""
+    import torch
+    from pyradiance_amd import gen
+    wea = scenes.annual_wea(0)
+    ref = REF_ENV_PATH / "gendaymtx"
+    res = {"workload": "gendaymtx -m N, annual hourly WEA (8760 steps, synthetic, seed 0)"}
+    with tempfile.TemporaryDirectory() as td:
+        wf = Path(td) / "annual.wea"
+        wf.write_text(wea)
+        for mf in (1, 4):
+            o, _, _ = gen._parse_args(["-m", str(mf)])
+            t0 = time.perf_counter()
+            sky = _lib.Sky(wea, o)
+            plan_ms = (time.perf_counter() - t0) * 1e3
+            n = sky.npatch * sky.ncols * 3
+            npatch, ncols = sky.npatch, sky.ncols
+            out = torch.empty(n, dtype=torch.float32, device="cuda")
+            sky.matrix_device(out.data_ptr())
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                sky.matrix_device(out.data_ptr())
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            sky.close()
+            dev_ms = 1e3 * float(np.median(ts))
+            t0 = time.perf_counter()
+            mine = gen.gendaymtx(str(wf), mfactor=mf, outform="f", header=False)
+            tf = (time.perf_counter() - t0) * 1e3
+            e = {"patches": npatch, "steps": ncols, "host_setup_ms": plan_ms, "device_call_ms": dev_ms,
+                 "device_GB_written_per_s": n * 4 / (dev_ms * 1e-3) / 1e9, "gendaymtx_float_ms": tf,
+                 "output_MB": len(mine) / 1e6}
+            if mf == 1:
+                t0 = time.perf_counter()
+                asc = gen.gendaymtx(str(wf), mfactor=mf)
+                e["gendaymtx_ascii_ms"] = (time.perf_counter() - t0) * 1e3
+            if cpu and ref.exists():
+                t0 = time.perf_counter()
+                r = subprocess.run([str(ref), "-h", "-m", str(mf), "-of", str(wf)], capture_output=True)
+                e["reference_float_ms"] = (time.perf_counter() - t0) * 1e3
+                if r.returncode == 0 and len(r.stdout) == len(mine):
+                    a = np.frombuffer(mine, dtype=np.float32)
+                    b = np.frombuffer(r.stdout, dtype=np.float32)
+                    e["max_abs_diff_vs_reference"] = float(np.max(np.abs(a - b)))
+                    e["max_rel_diff_vs_reference"] = float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)))
+                if mf == 1:
+                    t0 = time.perf_counter()
+                    r = subprocess.run([str(ref), "-m", str(mf), str(wf)], capture_output=True)
+                    e["reference_ascii_ms"] = (time.perf_counter() - t0) * 1e3
+                    e["ascii_body_identical"] = r.stdout.split(b"\n\n", 1)[-1] == asc.split(b"\n\n", 1)[-1]
+            res[f"mf{mf}"] = e
+    return res
+
+
+RPICT_VIEW = ["-vtv", "-vp", "3", "3.8", "1.5", "-vd", "0", "-1", "0", "-vu", "0", "0", "1", "-vh", "80", "-vv", "60"]
+RPICT_OPTS = ["-ab", "1", "-ad", "1024", "-aa", "0"]
+
+
+def rpict_bench(xres: int = 1024, yres: int = 1024, reps: int = 3, cpu: bool = True, cpu_res: int = 128):
+    """rpict of the rcontrib room (config-2 scene: room, window, 200 boxes,
+    sky and ground glow) from its north side toward the window, one primary
+    ray per pixel (-ps 1) with -ab 1 -ad 1024 -aa 0 and rpict's other
+    defaults; the reference rpict on a smaller picture of the same view."""
+    from pyradiance_amd import params as _params
+    from pyradiance_amd import rt as _rt
+    from pyradiance_amd import view as _view
+    rad, _ = scenes.config2(0)
+    with tempfile.TemporaryDirectory() as td:
+        radf = Path(td) / "room.rad"
+        radf.write_text(rad)
+        ctx = _lib.Context(0)
+        ctx.load_text(rad).commit()
+        opt = _params.parse(RPICT_OPTS, rcontrib=False)
+        p = opt.params
+        p.srcsizerat, p.maxdepth, p.minweight, p.dstrsrc, p.seed = 0.25, 7, 1e-4, 0.0, 1
+        ctx.set_params(p)
+        v, _ = _view.parse_view(RPICT_VIEW)
+        v.setup()
+        _, xr, yr = _view.normaspect(v.aspect(), 1.0, xres, yres)
+        ctx.trace_view(v, xr, yr, 0.67)
+        ts, rays = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            img = ctx.trace_view(v, xr, yr, 0.67)
+            ts.append(time.perf_counter() - t0)
+            rays.append(ctx.last_nrays())
+        sec = float(np.median(ts))
+        e = {"workload": f"rpict {' '.join(RPICT_VIEW + RPICT_OPTS)} -ps 1 -x {xres} -y {yres} (config-2 room)",
+             "pixels": xr * yr, "picture_ms": sec * 1e3, "Mrays_per_s": float(np.mean(rays)) / sec / 1e6,
+             "rays_per_pixel": float(np.mean(rays)) / (xr * yr), "Mpixels_per_s": xr * yr / sec / 1e6,
+             "mean_radiance": float(img.mean())}
+        # the whole rpict() call: options, view, rays, trace, RGBE picture
+        _rt.rpict(RPICT_VIEW, radf, xres=xres, yres=yres, params=RPICT_OPTS + ["-ps", "1"])
+        t0 = time.perf_counter()
+        pic = _rt.rpict(RPICT_VIEW, radf, xres=xres, yres=yres, params=RPICT_OPTS + ["-ps", "1"])
+        e["rpict_call_ms"] = (time.perf_counter() - t0) * 1e3
+        e["picture_bytes"] = len(pic)
+        ref = REF_ENV_PATH
+        if cpu and (ref / "rpict").exists() and (ref / "oconv").exists():
+            octf = Path(td) / "room.oct"
+            with open(octf, "wb") as f:
+                subprocess.run([str(ref / "oconv"), "-f", str(radf)], stdout=f, check=True)
+
+            def run(x, y):
+                t0 = time.perf_counter()
+                r = subprocess.run([str(ref / "rpict"), *RPICT_VIEW, *RPICT_OPTS, "-ps", "1", "-x", str(x), "-y",
+                                    str(y), str(octf)], capture_output=True, env=_ref_env())
+                return time.perf_counter() - t0, r.returncode
+            w, rc = run(cpu_res, cpu_res)
+            load, rc1 = run(1, 1)
+            if rc == 0 and rc1 == 0:
+                _, cx, cy = _view.normaspect(v.aspect(), 1.0, cpu_res, cpu_res)
+                e["cpu_baseline"] = {"value": cx * cy / max(w - load, 1e-9) / 1e6, "unit": "Mpixels/s", "cores": 1,
+                                     "kind": "reference",
+                                     "sample": f"{cx}x{cy} picture of the same view and options, wall {w:.2f}s "
+                                               f"minus a 1x1 run {load:.2f}s (octree load)"}
+    return e
+
+
+RFLUX_OPTS = ["-I+", "-ab", "4", "-ad", "4096", "-lw", "1e-4", "-aa", "0"]
+
+
+def _window_receiver(room) -> str:
+    """Klems full receiver just inside the window, facing the room."""
+    x0 = 0.5 * (room.width - room.win_w)
+    x1, z0 = x0 + room.win_w, room.win_sill
+    z1 = z0 + room.win_h
+    return ("#@rfluxmtx h=kf u=Z\nvoid glow winglow\n0\n0\n4 1 1 1 0\n\n"
+            f"winglow polygon window_recv\n0\n0\n12 {x0} .01 {z0} {x0} .01 {z1} {x1} .01 {z1} {x1} .01 {z0}\n")
+
+
+def rflux_bench(reps: int = 3, cpu: bool = True, cpu_sample: int = 1000, cores: int = 16):
+    """rfluxmtx view matrix: the 2,500 sensors of the rcontrib room (room and
+    200 boxes, no sky) to a Klems full (145 bin) receiver on the window,
+    -I+ -ab 4 -ad 4096 -lw 1e-4 -aa 0, ASCII output; the whole rfluxmtx()
+    call (receiver parsing, rcontrib options, trace, formatting) and the
+    reference rfluxmtx (forking the reference rcontrib) on a sample."""
+    from pyradiance_amd import rflux
+    rng = np.random.default_rng(0)
+    room = scenes.Room()
+    rad = scenes.room_rad(room) + scenes.furniture_rad(rng, 200, room, 3, 7)
+    _, sensors = scenes.config2(0)
+    text = "".join(" ".join(f"{v:.17g}" for v in r) + "\n" for r in sensors).encode()
+    with tempfile.TemporaryDirectory() as td:
+        radf, recf = Path(td) / "room.rad", Path(td) / "window.rad"
+        radf.write_text(rad)
+        recf.write_text(_window_receiver(room))
+        rflux.rfluxmtx(recf, rays=text, params=RFLUX_OPTS, scene=[radf])
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = rflux.rfluxmtx(recf, rays=text, params=RFLUX_OPTS, scene=[radf])
+            ts.append(time.perf_counter() - t0)
+        sec = float(np.median(ts))
+        body = out.split(b"\n\n", 1)[-1]
+        vals = np.array(body.split(), dtype=np.float64)
+        e = {"workload": f"rfluxmtx {' '.join(RFLUX_OPTS)} - window.rad(h=kf) room.rad, 2500 sensors",
+             "records": len(sensors), "call_ms": sec * 1e3, "records_per_s": len(sensors) / sec,
+             "values": int(vals.size), "mean_value": float(vals.mean()) if vals.size else None}
+        ref = REF_ENV_PATH
+        if cpu and (ref / "rfluxmtx").exists() and (ref / "rcontrib").exists():
+            def run(k):
+                t0 = time.perf_counter()
+                r = subprocess.run([str(ref / "rfluxmtx"), "-n", str(cores), *RFLUX_OPTS, "-", str(recf), str(radf)],
+                                   input=b"".join(text.splitlines(keepends=True)[:k]), capture_output=True,
+                                   env=_ref_env(), cwd=td)
+                return time.perf_counter() - t0, r.returncode
+            w, rc = run(cpu_sample)
+            load, rc1 = run(1)
+            if rc == 0 and rc1 == 0:
+                e["cpu_baseline"] = {"value": cpu_sample / max(w - load, 1e-9), "unit": "records/s",
+                                     "cores": cores, "kind": "reference",
+                                     "sample": f"{cpu_sample} of the 2500 sensors, rfluxmtx -n {cores} wall {w:.2f}s "
+                                               f"minus a one-sensor run {load:.2f}s (oconv and octree load)"}
+    return e
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,6 +610,9 @@ def main():
     ap.add_argument("--shape", default="", metavar="K=V,...",
                     help="launch shape to use instead of autotuning, e.g. trace_kernel=1,refill=16")
     ap.add_argument("--no-config5", action="store_true", help="skip the 20000x2306x8760 DC x sky product")
+    ap.add_argument("--no-sky", action="store_true", help="skip the gendaymtx measurement")
+    ap.add_argument("--no-rpict", action="store_true", help="skip the rpict measurement")
+    ap.add_argument("--no-rflux", action="store_true", help="skip the rfluxmtx measurement")
     ap.add_argument("--cpu-sample", type=int, default=480)
     ap.add_argument("--sensors", type=int, default=2500)
     ap.add_argument("--bvh", choices=["auto", "sah", "lbvh"], default="auto")
@@ -509,6 +703,19 @@ def main():
         grad = grad_bench()
         log(f"grad step: {grad}")
 
+    sky = None
+    if not args.no_sky:
+        sky = sky_bench(cpu=not args.no_cpu_baseline)
+        log(f"gendaymtx: {sky}")
+    pict = None
+    if not args.no_rpict:
+        pict = rpict_bench(cpu=not args.no_cpu_baseline)
+        log(f"rpict: {pict}")
+    rfx = None
+    if not args.no_rflux:
+        rfx = rflux_bench(cpu=not args.no_cpu_baseline, cores=min(os.cpu_count() or 1, 16))
+        log(f"rfluxmtx: {rfx}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cores = min(os.cpu_count() or 1, 16)
@@ -550,6 +757,9 @@ def main():
         "dcsky_config5": dc5,
         "batched": batched,
         "grad_step": grad,
+        "gendaymtx": sky,
+        "rpict": pict,
+        "rfluxmtx": rfx,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

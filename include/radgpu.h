@@ -225,6 +225,19 @@ int rg_sky_suns(const rg_sky *s, int *id_good, double *alt_azi_illum);
 int rg_sky_matrix(rg_sky *s, float *out);
 int rg_sky_matrix_device(rg_sky *s, float *dev_out);
 
+/* ASCII records on the host cores: nrecords records of per_record values,
+ * each value printed with fmt ("%e" or "%g", optional ".N", N <= 17),
+ * values separated by sep, then tail and '\n', and one more '\n' after every
+ * `group` records (group 0: none).  Replaces the per-value printf of the
+ * reference's writers: rcontrib's "%.6e\t" records (rt/rc2.c:306),
+ * rtrace's "%e\t" values (rt/rtrace.c:912-916), gendaymtx's
+ * "%.3g %.3g %.3g\n" rows with a blank line after each patch
+ * (gen/gendaymtx.c:652-663).  Returns the
+ * bytes written to out, -1 for bad arguments, or -(2 + bytes needed) when
+ * cap is too small.  nthreads <= 0: one per host core. */
+long long rg_format_ascii(const double *values, long long nrecords, int per_record, long long group,
+                          const char *fmt, const char *sep, const char *tail, char *out, long long cap, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

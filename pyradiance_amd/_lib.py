@@ -23,7 +23,7 @@ EXPORTS = (
     "rg_set_profiling", "rg_set_bvh_builder", "rg_set_tuning", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
     "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint", "rg_first_hit",
     "rg_sky_create", "rg_sky_destroy", "rg_sky_error", "rg_sky_shape", "rg_sky_site", "rg_sky_suns",
-    "rg_sky_matrix", "rg_sky_matrix_device", "rg_trace_view",
+    "rg_sky_matrix", "rg_sky_matrix_device", "rg_trace_view", "rg_format_ascii",
 )
 
 BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
@@ -143,6 +143,9 @@ def lib() -> C.CDLL:
     L.rg_sky_suns.argtypes = [vp, C.POINTER(C.c_int), dp]
     L.rg_sky_matrix.argtypes = [vp, C.POINTER(C.c_float)]
     L.rg_sky_matrix_device.argtypes = [vp, vp]
+    L.rg_format_ascii.argtypes = [dp, C.c_longlong, C.c_int, C.c_longlong, C.c_char_p, C.c_char_p, C.c_char_p,
+                                  C.c_char_p, C.c_longlong, C.c_int]
+    L.rg_format_ascii.restype = C.c_longlong
     _lib = L
     return L
 
@@ -158,6 +161,26 @@ def default_params(rcontrib: bool = False) -> RgParams:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def format_ascii(values: np.ndarray, per_record: int, fmt: str, sep: str = "\t", tail: str = "\t",
+                 group: int = 0, nthreads: int = 0) -> bytes:
+    """ASCII records formatted on the host cores (rg_format_ascii): each of the
+    len(values) / per_record records is its values printed with `fmt`, joined
+    by `sep`, then `tail` and a newline; a blank line after every `group`
+    records."""
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    if v.size % per_record:
+        raise ValueError("values do not fill whole records")
+    nrec = v.size // per_record
+    width = 26 + len(sep.encode())
+    cap = v.size * width + nrec * (len(tail.encode()) + 1) + (nrec // group + 1 if group > 0 else 0)
+    buf = np.empty(max(cap, 1), dtype=np.uint8)
+    n = lib().rg_format_ascii(_dptr(v), nrec, per_record, group, fmt.encode(), sep.encode(), tail.encode(),
+                              buf.ctypes.data_as(C.c_char_p), cap, nthreads)
+    if n < 0:
+        raise ValueError(f"rg_format_ascii failed ({n}) for format {fmt!r}")
+    return buf[:n].tobytes()
 
 
 class Context:

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RADGPU_ARCH", "gfx950")
 
 SOURCES = ["scene_io.cpp", "rg_scene.cpp", "rg_bvh.cpp", "rg_capi.cpp", "rg_render.hip", "rg_build.hip",
-           "rg_dcsky.hip", "rg_sky.hip", "rg_view.hip"]
+           "rg_dcsky.hip", "rg_sky.hip", "rg_view.hip", "rg_format.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
           "-Wno-unused-but-set-variable"]

@@ -252,10 +252,7 @@ def _output(mtx, sky, f, o, argv) -> bytes:
     elif fmt == "d":
         out.append(mtx.astype("=f8").tobytes())
     else:
-        rows = []
-        for p in range(mtx.shape[0]):
-            rows.append("".join("%.3g %.3g %.3g\n" % (float(a), float(b), float(c)) for a, b, c in mtx[p]))
-            if mtx.shape[1] > 1:
-                rows.append("\n")
-        out.append("".join(rows).encode())
+        # "%.3g %.3g %.3g\n" per column, a blank line after each patch row
+        # when there is more than one column (gen/gendaymtx.c:652-663)
+        out.append(_lib.format_ascii(mtx, 3, "%.3g", " ", "", group=mtx.shape[1] if mtx.shape[1] > 1 else 0))
     return b"".join(out)

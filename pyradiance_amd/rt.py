@@ -115,9 +115,7 @@ def _format(values: np.ndarray, outform: str, fmt: str, per_line: int) -> bytes:
     if outform == "a":
         if values.size == 0:
             return b""
-        lines = values.reshape(-1, per_line)
-        row = (fmt + "\t") * per_line + "\n"
-        return "".join(row % tuple(r) for r in lines).encode()
+        return _lib.format_ascii(values, per_line, fmt, "\t", "\t")
     if outform == "f":
         return values.astype("<f4").tobytes()
     if outform == "d":

@@ -32,6 +32,7 @@ __all__ = [
     "reinhart_nbins",
     "config3",
     "boxes_rad",
+    "annual_wea",
 ]
 
 
@@ -237,6 +238,37 @@ def sky_matrix(seed: int, nbins: int, nhours: int = 8760, latitude: float = 37.8
     sin_alt = np.sin(lat) * np.sin(decl) + np.cos(lat) * np.cos(decl) * np.cos(ha)
     sky[:, sin_alt <= 0.0, :] = 0.0
     return sky
+
+
+def annual_wea(seed: int = 0, latitude: float = 37.62, longitude: float = 122.38, time_zone: float = 120.0) -> str:
+    """Synthetic hourly weather tape for a whole year (8,760 records) in the
+    WEA format gendaymtx reads (gen/gendaymtx.c:524-560): direct normal and
+    diffuse horizontal irradiance (weather_data_file_units 1, W/m^2).
+
+    Longitude and time zone are degrees west, as in WEA headers.  A cloud
+    fraction drawn per hour scales direct normal irradiance; the diffuse part
+    grows with cloud and sun altitude; night hours are zero.
+    """
+    rng = np.random.default_rng(seed)
+    mdays = [31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31]
+    lines = ["place synthetic", f"latitude {latitude}", f"longitude {longitude}", f"time_zone {time_zone}",
+             "site_elevation 5.0", "weather_data_file_units 1"]
+    lat = np.radians(latitude)
+    jd = 0
+    for mo, nd in enumerate(mdays, 1):
+        for da in range(1, nd + 1):
+            jd += 1
+            dec = 0.4093 * np.sin(2 * np.pi / 365 * (jd - 81))
+            for h in range(24):
+                hr = h + 0.5
+                st = hr - (longitude - time_zone) / 15.0
+                alt = np.arcsin(np.sin(lat) * np.sin(dec) - np.cos(lat) * np.cos(dec) * np.cos(st * np.pi / 12))
+                sa = max(np.sin(alt), 0.0)
+                cloud = rng.random()
+                dn = 900 * sa ** 0.5 * cloud if sa > 0.02 else 0.0
+                dh = (60 * sa + 90 * sa * (1 - cloud) + 5 * rng.random()) if sa > 0.0 else 0.0
+                lines.append(f"{mo} {da} {hr:.3f} {dn:.1f} {dh:.1f}")
+    return "\n".join(lines) + "\n"
 
 
 def boxes_rad(rng: np.random.Generator, nboxes: int, room: Room, mod_prefix: str = "obj",
