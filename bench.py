@@ -7,9 +7,13 @@ its BVH resident on the GPU.  Rays are ray segments (closest-hit queries),
 counted on the device.  Also reported: the DC x sky product
 (2500x146 by 146x8760, RGB) on MFMA, the dominant kernel's roofline figures,
 and the reference rcontrib timed on the host cores over a bounded sample.
+The line also carries gendaymtx (annual sky matrix), rpict (a picture of the
+same room) and rfluxmtx (a view matrix of the same room) measurements, each
+with its reference program timed beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
                     [--no-batched] [--bvh auto|sah|lbvh]
+    python bench.py --only sky|rpict|rflux [--no-cpu-baseline]
 
 roofline.traffic (HBM bytes per k_trace launch) is read from the committed
 rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload,
@@ -216,7 +220,7 @@ def copy_calibration(nbytes: int) -> None:
     """One device-to-device copy of `nbytes` (the runtime's plain copy kernel),
     used to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
     import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
+    hip = _lib.hip_runtime()
     a, b = C.c_void_p(), C.c_void_p()
     if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
         raise RuntimeError("hipMalloc failed")
@@ -613,6 +617,8 @@ def main():
     ap.add_argument("--no-sky", action="store_true", help="skip the gendaymtx measurement")
     ap.add_argument("--no-rpict", action="store_true", help="skip the rpict measurement")
     ap.add_argument("--no-rflux", action="store_true", help="skip the rfluxmtx measurement")
+    ap.add_argument("--only", choices=["sky", "rpict", "rflux"], default=None,
+                    help="run one of the gendaymtx / rpict / rfluxmtx measurements alone and print its JSON")
     ap.add_argument("--cpu-sample", type=int, default=480)
     ap.add_argument("--sensors", type=int, default=2500)
     ap.add_argument("--bvh", choices=["auto", "sah", "lbvh"], default="auto")
@@ -624,6 +630,15 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the forward pass shards "
               "by sensor rows with no communication, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.only:
+        if args.copy_calibration:
+            copy_calibration(args.copy_calibration)
+        cpu_leg = not args.no_cpu_baseline
+        fn = {"sky": lambda: sky_bench(cpu=cpu_leg), "rpict": lambda: rpict_bench(cpu=cpu_leg),
+              "rflux": lambda: rflux_bench(cpu=cpu_leg, cores=min(os.cpu_count() or 1, 16))}[args.only]
+        print(json.dumps(fn()))
+        return
 
     rad, sensors, mods = workload(0)
     sensors = sensors[: args.sensors]

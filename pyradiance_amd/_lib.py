@@ -66,6 +66,7 @@ class RgModifier(C.Structure):
 
 
 _lib = None
+_hip = None
 
 
 def _share_hip_runtime() -> None:
@@ -76,6 +77,7 @@ def _share_hip_runtime() -> None:
     device ("No HIP GPUs are available").  Preloaded here, this library binds
     to it by soname and torch's later import finds the same file.  torch
     itself is not imported."""
+    global _hip
     if os.environ.get("RADGPU_HIP_RUNTIME", "") == "system":
         return
     spec = importlib.util.find_spec("torch")
@@ -84,8 +86,19 @@ def _share_hip_runtime() -> None:
     for d in spec.submodule_search_locations:
         rt = Path(d) / "lib" / "libamdhip64.so"
         if rt.exists():
-            C.CDLL(str(rt), mode=C.RTLD_GLOBAL)
+            _hip = C.CDLL(str(rt), mode=C.RTLD_GLOBAL)
             return
+
+
+def hip_runtime() -> C.CDLL:
+    """The HIP runtime this library is bound to (the one shared with torch
+    when torch is installed).  Opening libamdhip64 by bare name instead can
+    map a second runtime into the process."""
+    lib()
+    if _hip is not None:
+        return _hip
+    # already mapped as libradgpu.so's dependency: dlopen returns that object
+    return C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL | os.RTLD_NOLOAD)
 
 
 def lib() -> C.CDLL:

@@ -437,13 +437,24 @@ struct SkyStepOut {
   float sunv[MAXSUN];
 };
 
+// Per-patch terms of the sun-sky angle and of the horizontal-illuminance
+// weight, from the patch's (float) altitude and azimuth as rh_init() stores
+// them (gen/gendaymtx.c:1221-1264).
+struct SkyPatchTab {
+  double czp, szp;     // cos, sin of the patch zenith angle
+  double sinp, domp;   // sin(altitude), solid angle
+  double azi;
+  double dir[3];       // rh_vector() direction
+};
+
 // CalcRelLuminance() with the sun-sky angle of CalcSkyPatchLumin()
-__device__ __forceinline__ float patch_lum(const SkyStep& k, float palt, float pazi) {
-  const double aas = fabs((double)pazi - k.azimuth);
-  const double zsa = kPI * 0.5 - (double)palt;
-  const double sspa = acos(cos(k.sun_zenith) * cos(zsa) + sin(k.sun_zenith) * sin(zsa) * cos(aas));
+// (gen/gendaymtx.c:1292-1322, :1172-1191); czs, szs: cos and sin of the sun
+// zenith angle.  The arithmetic is the reference's, term for term, in double.
+__device__ __forceinline__ float patch_lum(const SkyStep& k, double czs, double szs, const SkyPatchTab& t) {
+  const double aas = fabs(t.azi - k.azimuth);
+  const double sspa = acos(czs * t.czp + szs * t.szp * cos(aas));
   const double* pp = k.perez;
-  double l = (1.0 + pp[0] * exp(pp[1] / cos(zsa))) * (1.0 + pp[2] * exp(pp[3] * sspa) + pp[4] * cos(sspa) * cos(sspa));
+  double l = (1.0 + pp[0] * exp(pp[1] / t.czp)) * (1.0 + pp[2] * exp(pp[3] * sspa) + pp[4] * cos(sspa) * cos(sspa));
   float f = (float)l;
   return f < 0 ? 0.f : f;
 }
@@ -456,22 +467,54 @@ __device__ __forceinline__ void patch_dir(double v[3], float alt, float azi) {
   v[2] = sin((double)alt);
 }
 
-// insert (d, p) into a descending list; equal products keep the lower patch first
-__device__ __forceinline__ void top_insert(double* d, int* p, int n, double dv, int pv) {
-  for (int i = 0; i < n; ++i)
-    if (dv > d[i] || (dv == d[i] && pv < p[i] && p[i] >= 0)) {
-      for (int j = n - 1; j > i; --j) { d[j] = d[j - 1]; p[j] = p[j - 1]; }
-      d[i] = dv;
-      p[i] = pv;
-      return;
-    }
+// Insert (d, p) into the first n entries of a descending list; equal
+// products keep the lower patch first.  Fixed-size loops with the bound as a
+// condition, so the list stays in registers.
+__device__ __forceinline__ void top_insert(double (&d)[MAXSUN], int (&p)[MAXSUN], int n, double dv, int pv) {
+  int pos = MAXSUN;
+#pragma unroll
+  for (int i = MAXSUN - 1; i >= 0; --i)
+    if (i < n && (dv > d[i] || (dv == d[i] && pv < p[i] && p[i] >= 0))) pos = i;
+#pragma unroll
+  for (int j = MAXSUN - 1; j > 0; --j)
+    if (j > pos && j < n) { d[j] = d[j - 1]; p[j] = p[j - 1]; }
+#pragma unroll
+  for (int i = 0; i < MAXSUN; ++i)
+    if (i == pos) { d[i] = dv; p[i] = pv; }
 }
 
-// One block per stored step: normalisation and the sun's patches.
+// x precedes y in the nearest-patch order: larger product, then lower patch;
+// empty entries (patch -1) last.
+__device__ __forceinline__ bool sun_before(double xd, int xp, double yd, int yp) {
+  return xp >= 0 && (yp < 0 || xd > yd || (xd == yd && xp < yp));
+}
+
+__global__ void k_sky_patch(const float* __restrict__ palt, const float* __restrict__ pazi,
+                            const float* __restrict__ dom, int np, SkyPatchTab* __restrict__ tab) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double zsa = kPI * 0.5 - (double)palt[p];
+  SkyPatchTab t;
+  t.czp = cos(zsa);
+  t.szp = sin(zsa);
+  t.sinp = sin((double)palt[p]);
+  t.domp = (double)dom[p];
+  t.azi = (double)pazi[p];
+  patch_dir(t.dir, palt[p], pazi[p]);
+  tab[p] = t;
+}
+
+// One block per stored step: every patch's relative luminance (written
+// step-major to `lum` for the matrix pass), the normalisation sum, and the
+// sun's nearest patches.  The sum is per-thread partial sums over a strided
+// patch range, then a tree; the nearest patches are per-thread sorted lists
+// merged pairwise up a tree, which selects the same patches as the
+// reference's one sequential pass (largest dot products, ties to the lower
+// patch).
 __global__ void __launch_bounds__(SKY_BLOCK) k_sky_step(const SkyStep* __restrict__ steps, int nsteps,
-                                                        const float* __restrict__ palt, const float* __restrict__ pazi,
-                                                        const float* __restrict__ dom, int np, int nsuns,
-                                                        double fixed_sa, SkyStepOut* __restrict__ out) {
+                                                        const SkyPatchTab* __restrict__ tab, int np, int nsuns,
+                                                        double fixed_sa, float* __restrict__ lum,
+                                                        SkyStepOut* __restrict__ out) {
   const int s = blockIdx.x;
   if (s >= nsteps) return;
   const SkyStep k = steps[s];
@@ -483,25 +526,55 @@ __global__ void __launch_bounds__(SKY_BLOCK) k_sky_step(const SkyStep* __restric
   double nd[MAXSUN];
   int npch[MAXSUN];
   for (int i = 0; i < MAXSUN; ++i) { nd[i] = -1.0; npch[i] = -1; }
+  const double czs = cos(k.sun_zenith), szs = sin(k.sun_zenith);
   double sv[3];
-  patch_dir(sv, 0.f, 0.f);
   sv[1] = cos(k.altitude);
   sv[0] = sv[1] * sin(k.azimuth);
   sv[1] *= cos(k.azimuth);
   sv[2] = sin(k.altitude);
+  float* lrow = lum + (size_t)s * np;
   for (int p = 1 + tid; p < np; p += SKY_BLOCK) {
-    if (k.kind == 2) acc += (double)patch_lum(k, palt[p], pazi[p]) * sin((double)palt[p]) * (double)dom[p];
-    if (k.add_sun) {
-      double pv[3];
-      patch_dir(pv, palt[p], pazi[p]);
-      top_insert(nd, npch, nsuns, pv[0] * sv[0] + pv[1] * sv[1] + pv[2] * sv[2], p);
+    const SkyPatchTab& t = tab[p];
+    if (k.kind == 2) {
+      const float f = patch_lum(k, czs, szs, t);
+      lrow[p] = f;
+      acc += (double)f * t.sinp * t.domp;
     }
+    if (k.add_sun) top_insert(nd, npch, nsuns, t.dir[0] * sv[0] + t.dir[1] * sv[1] + t.dir[2] * sv[2], p);
   }
   red[tid] = acc;
   for (int i = 0; i < MAXSUN; ++i) { cd[tid * MAXSUN + i] = nd[i]; cp[tid * MAXSUN + i] = npch[i]; }
   __syncthreads();
   for (int w = SKY_BLOCK / 2; w > 0; w >>= 1) {
-    if (tid < w) red[tid] += red[tid + w];
+    if (tid < w) {
+      red[tid] += red[tid + w];
+      if (k.add_sun) {
+        // merge the sorted lists of threads tid and tid + w into tid's: each
+        // entry's merged rank is its own index plus the entries of the other
+        // list that precede it (ranks of empty entries may coincide; they
+        // are all alike)
+        double ad[MAXSUN], bd[MAXSUN];
+        int ap[MAXSUN], bp[MAXSUN];
+#pragma unroll
+        for (int i = 0; i < MAXSUN; ++i) {
+          ad[i] = cd[tid * MAXSUN + i];
+          ap[i] = cp[tid * MAXSUN + i];
+          bd[i] = cd[(tid + w) * MAXSUN + i];
+          bp[i] = cp[(tid + w) * MAXSUN + i];
+        }
+#pragma unroll
+        for (int i = 0; i < MAXSUN; ++i) {
+          int ra = i, rb = i;
+#pragma unroll
+          for (int j = 0; j < MAXSUN; ++j) {
+            if (j < nsuns && sun_before(bd[j], bp[j], ad[i], ap[i])) ++ra;
+            if (j < nsuns && sun_before(ad[j], ap[j], bd[i], bp[i])) ++rb;
+          }
+          if (i < nsuns && ra < nsuns) { cd[tid * MAXSUN + ra] = ad[i]; cp[tid * MAXSUN + ra] = ap[i]; }
+          if (i < nsuns && rb < nsuns) { cd[tid * MAXSUN + rb] = bd[i]; cp[tid * MAXSUN + rb] = bp[i]; }
+        }
+      }
+    }
     __syncthreads();
   }
   if (tid == 0) {
@@ -516,59 +589,75 @@ __global__ void __launch_bounds__(SKY_BLOCK) k_sky_step(const SkyStep* __restric
     o.scale = k.kind == 2 ? (k.diff_illum / norm) * (1. / WHTEFFICACY) : 0.0;
     o.nsun = 0;
     if (k.add_sun) {
-      double bd[MAXSUN];
-      int bp[MAXSUN];
-      for (int i = 0; i < MAXSUN; ++i) { bd[i] = -1.0; bp[i] = -1; }
-      for (int t = 0; t < SKY_BLOCK * MAXSUN; ++t)
-        if (cp[t] >= 0) top_insert(bd, bp, nsuns, cd[t], cp[t]);
       double wta[MAXSUN], wtot = 0;
-      for (int i = nsuns; i--;) wtot += wta[i] = 1. / (1.002 - bd[i]);
-      for (int i = 0; i < nsuns; ++i) {
-        if (bp[i] < 0) continue;
+#pragma unroll
+      for (int i = MAXSUN - 1; i >= 0; --i)
+        if (i < nsuns) wtot += wta[i] = 1. / (1.002 - cd[i]);
+#pragma unroll
+      // the list holds its patches first and empty entries after them, so
+      // the i-th placed sun patch is entry i
+      for (int i = 0; i < MAXSUN; ++i) {
+        if (i >= nsuns || cp[i] < 0) continue;
         float va = (float)(wta[i] * k.dir_illum / (WHTEFFICACY * wtot));
-        va = (float)((double)va / (fixed_sa > 0 ? fixed_sa : (double)dom[bp[i]]));
-        o.sunp[o.nsun] = bp[i];
-        o.sunv[o.nsun] = va;
-        o.nsun++;
+        va = (float)((double)va / (fixed_sa > 0 ? fixed_sa : (double)tab[cp[i]].domp));
+        o.sunp[i] = cp[i];
+        o.sunv[i] = va;
+        o.nsun = i + 1;
       }
     }
     out[s] = o;
   }
 }
 
-// The matrix, [patch][step][rgb]: threads along steps so stores coalesce.
+// The matrix, [patch][step][rgb], from the step-major luminances: a tile of
+// FILL_TS steps x FILL_TP patches is read along patches into LDS and written
+// along steps, so both sides move whole contiguous runs.
+constexpr int FILL_TS = 64, FILL_TP = 16;
+
 __global__ void __launch_bounds__(SKY_BLOCK) k_sky_fill(const SkyStep* __restrict__ steps, const SkyStepOut* __restrict__ so,
-                                                        int nsteps, const float* __restrict__ palt,
-                                                        const float* __restrict__ pazi, int np, float3 skyc, float3 sunc,
-                                                        float* __restrict__ out) {
-  const int s = blockIdx.x * SKY_BLOCK + threadIdx.x;
-  const int p = blockIdx.y;
-  if (s >= nsteps || p >= np) return;
-  const SkyStep& k = steps[s];
-  float v[3] = {0.f, 0.f, 0.f};
-  if (p == 0) {
-    for (int c = 0; c < 3; ++c) v[c] = k.ground[c];
-  } else if (k.kind == 2) {
-    const SkyStepOut& o = so[s];
-    float l = o.uniform ? 1.f : patch_lum(k, palt[p], pazi[p]);
-    float sc = (float)((double)l * o.scale);
-    v[0] = sc * skyc.x;
-    v[1] = sc * skyc.y;
-    v[2] = sc * skyc.z;
+                                                        const float* __restrict__ lum, int nsteps, int np, float3 skyc,
+                                                        float3 sunc, float* __restrict__ out) {
+  __shared__ float lt[FILL_TP][FILL_TS + 1];
+  const int s0 = blockIdx.x * FILL_TS, p0 = blockIdx.y * FILL_TP;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < FILL_TS * FILL_TP; i += SKY_BLOCK) {
+    const int sl = i / FILL_TP, pl = i % FILL_TP;
+    const int s = s0 + sl, p = p0 + pl;
+    float l = 0.f;
+    if (s < nsteps && p >= 1 && p < np && steps[s].kind == 2) l = lum[(size_t)s * np + p];
+    lt[pl][sl] = l;
   }
-  if (p > 0 && k.kind > 0) {
-    const SkyStepOut& o = so[s];
-    for (int i = 0; i < o.nsun; ++i)
-      if (o.sunp[i] == p) {
-        v[0] += o.sunv[i] * sunc.x;
-        v[1] += o.sunv[i] * sunc.y;
-        v[2] += o.sunv[i] * sunc.z;
-      }
+  __syncthreads();
+  const int sl = tid % FILL_TS;
+  const int s = s0 + sl;
+  if (s >= nsteps) return;
+  const int kind = steps[s].kind;
+  const SkyStepOut o = so[s];
+  for (int pl = tid / FILL_TS; pl < FILL_TP; pl += SKY_BLOCK / FILL_TS) {
+    const int p = p0 + pl;
+    if (p >= np) break;
+    float v[3] = {0.f, 0.f, 0.f};
+    if (p == 0) {
+      for (int c = 0; c < 3; ++c) v[c] = steps[s].ground[c];
+    } else if (kind == 2) {
+      const float l = o.uniform ? 1.f : lt[pl][sl];
+      const float sc = (float)((double)l * o.scale);
+      v[0] = sc * skyc.x;
+      v[1] = sc * skyc.y;
+      v[2] = sc * skyc.z;
+    }
+    if (p > 0 && kind > 0)
+      for (int i = 0; i < o.nsun; ++i)
+        if (o.sunp[i] == p) {
+          v[0] += o.sunv[i] * sunc.x;
+          v[1] += o.sunv[i] * sunc.y;
+          v[2] += o.sunv[i] * sunc.z;
+        }
+    float* d = out + ((size_t)p * nsteps + s) * 3;
+    d[0] = v[0];
+    d[1] = v[1];
+    d[2] = v[2];
   }
-  float* d = out + ((size_t)p * nsteps + s) * 3;
-  d[0] = v[0];
-  d[1] = v[1];
-  d[2] = v[2];
 }
 
 // -A: the rows summed in step order in FP32, then scaled by 1/ntsteps.
@@ -593,19 +682,29 @@ void sky_matrix(const SkyPlan& P, float* out, hipStream_t st) {
   if (ns <= 0) return;
   SkyStep* dsteps = nullptr;
   SkyStepOut* dso = nullptr;
-  float *dalt = nullptr, *dazi = nullptr, *ddom = nullptr, *dfull = nullptr;
+  SkyPatchTab* dtab = nullptr;
+  float *dalt = nullptr, *dazi = nullptr, *ddom = nullptr, *dfull = nullptr, *dlum = nullptr;
+  auto release = [&] {
+    for (void* q : {(void*)dsteps, (void*)dso, (void*)dtab, (void*)dalt, (void*)dazi, (void*)ddom, (void*)dfull,
+                    (void*)dlum})
+      (void)hipFree(q);
+  };
   try {
     ck(hipMalloc((void**)&dsteps, sizeof(SkyStep) * ns), "hipMalloc");
     ck(hipMalloc((void**)&dso, sizeof(SkyStepOut) * ns), "hipMalloc");
+    ck(hipMalloc((void**)&dtab, sizeof(SkyPatchTab) * np), "hipMalloc");
     ck(hipMalloc((void**)&dalt, sizeof(float) * np), "hipMalloc");
     ck(hipMalloc((void**)&dazi, sizeof(float) * np), "hipMalloc");
     ck(hipMalloc((void**)&ddom, sizeof(float) * np), "hipMalloc");
+    ck(hipMalloc((void**)&dlum, sizeof(float) * (size_t)np * ns), "hipMalloc");
     ck(hipMemcpyAsync(dsteps, P.steps.data(), sizeof(SkyStep) * ns, hipMemcpyHostToDevice, st), "upload");
     ck(hipMemcpyAsync(dalt, P.palt.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
     ck(hipMemcpyAsync(dazi, P.pazi.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
     ck(hipMemcpyAsync(ddom, P.dom.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
+    k_sky_patch<<<(np + 255) / 256, 256, 0, st>>>(dalt, dazi, ddom, np, dtab);
+    ck(hipGetLastError(), "k_sky_patch");
     int nsuns = P.opts.nsuns < 1 ? 1 : (P.opts.nsuns > MAXSUN ? MAXSUN : P.opts.nsuns);
-    k_sky_step<<<ns, SKY_BLOCK, 0, st>>>(dsteps, ns, dalt, dazi, ddom, np, nsuns, P.opts.fixed_sun_sa, dso);
+    k_sky_step<<<ns, SKY_BLOCK, 0, st>>>(dsteps, ns, dtab, np, nsuns, P.opts.fixed_sun_sa, dlum, dso);
     ck(hipGetLastError(), "k_sky_step");
     float* full = out;
     if (P.opts.average) {
@@ -615,8 +714,8 @@ void sky_matrix(const SkyPlan& P, float* out, hipStream_t st) {
     const SkyOpts& o = P.opts;
     float3 skyc = make_float3((float)o.sky_color[0], (float)o.sky_color[1], (float)o.sky_color[2]);
     float3 sunc = make_float3((float)o.sun_color[0], (float)o.sun_color[1], (float)o.sun_color[2]);
-    dim3 grid((ns + SKY_BLOCK - 1) / SKY_BLOCK, np);
-    k_sky_fill<<<grid, SKY_BLOCK, 0, st>>>(dsteps, dso, ns, dalt, dazi, np, skyc, sunc, full);
+    dim3 grid((ns + FILL_TS - 1) / FILL_TS, (np + FILL_TP - 1) / FILL_TP);
+    k_sky_fill<<<grid, SKY_BLOCK, 0, st>>>(dsteps, dso, dlum, ns, np, skyc, sunc, full);
     ck(hipGetLastError(), "k_sky_fill");
     if (P.opts.average) {
       k_sky_average<<<(np * 3 + 255) / 256, 256, 0, st>>>(dfull, np, ns, 1. / (double)P.ntsteps, P.ntsteps > 1, out);
@@ -624,11 +723,11 @@ void sky_matrix(const SkyPlan& P, float* out, hipStream_t st) {
     }
     ck(hipStreamSynchronize(st), "gendaymtx");
   } catch (...) {
-    hipStreamSynchronize(st);
-    hipFree(dsteps); hipFree(dso); hipFree(dalt); hipFree(dazi); hipFree(ddom); hipFree(dfull);
+    (void)hipStreamSynchronize(st);
+    release();
     throw;
   }
-  hipFree(dsteps); hipFree(dso); hipFree(dalt); hipFree(dazi); hipFree(ddom); hipFree(dfull);
+  release();
 }
 
 }  // namespace rg
