@@ -183,6 +183,9 @@ def format_ascii(values: np.ndarray, per_record: int, fmt: str, sep: str = "\t",
     by `sep`, then `tail` and a newline; a blank line after every `group`
     records."""
     v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    if nthreads <= 0:
+        # the cores this process may run on, not the machine's
+        nthreads = min(len(os.sched_getaffinity(0)), 32) if hasattr(os, "sched_getaffinity") else 0
     if v.size % per_record:
         raise ValueError("values do not fill whole records")
     nrec = v.size // per_record

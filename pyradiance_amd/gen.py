@@ -248,7 +248,7 @@ def _output(mtx, sky, f, o, argv) -> bytes:
         h.append("FORMAT=" + {"a": "ascii", "f": "float", "d": "double"}[fmt])
         out.append(("\n".join(h) + "\n\n").encode())
     if fmt == "f":
-        out.append(mtx.astype("=f4").tobytes())
+        out.append(np.ascontiguousarray(mtx, dtype="=f4").tobytes())
     elif fmt == "d":
         out.append(mtx.astype("=f8").tobytes())
     else:
