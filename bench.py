@@ -42,9 +42,10 @@ METRIC = "rcontrib Mrays/s on 1 MI355X, plain vs batched, at 3\u03c3 match; s pe
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
 
-# Algorithmic bytes per ray segment: ray record in + hit out (48 B), one
-# 64-byte node per BVH internal node visited, one 48-byte primitive record
-# per primitive tested, one 32-byte path-state read-modify-write.
+# Algorithmic bytes per ray segment: ray record in + hit out (48 B), one node
+# per BVH node visited (64 B binary, 128 B four-wide, as the renderer
+# reports), one 48-byte primitive record per primitive tested, one 32-byte
+# path-state read-modify-write.
 B_RAY, B_NODE, B_PRIM, B_STATE = 48.0, 64.0, 48.0, 32.0
 
 
@@ -177,10 +178,10 @@ def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, nsa
             "sampled_entries": int(nz.sum())}
 
 
-def committed_traffic(bvh: str):
+def committed_traffic(bvh: str, walk: str):
     """HBM bytes per traced ray of the traversal kernel from the committed
-    PMC summary, if it was measured with the same BVH builder (its
-    machine-readable lines are `bvh: <name>` and
+    PMC summary, if it was measured with the same BVH builder and walk (its
+    machine-readable lines are `bvh: <name>`, `walk: wide|binary` and
     `traffic_bytes_per_ray: <bytes>`).  Per ray, so that it applies to any
     generation window the launch shape uses."""
     f = ROOT / "profiles" / "k_trace_config2_hbm_traffic.md"
@@ -191,7 +192,7 @@ def committed_traffic(bvh: str):
         if ":" in line and not line.startswith(("|", " ", "#")):
             k, _, v = line.partition(":")
             kv[k.strip()] = v.strip()
-    if kv.get("bvh") != bvh or "traffic_bytes_per_ray" not in kv:
+    if kv.get("bvh") != bvh or kv.get("walk", "binary") != walk or "traffic_bytes_per_ray" not in kv:
         return None
     return float(kv["traffic_bytes_per_ray"])
 
@@ -687,7 +688,8 @@ def main():
     st_c = ctx.last_stats()
     ctx.set_profiling(0)
     nr = st_c["rays"]
-    per_ray = B_RAY + B_NODE * st_c["nodes"] / nr + B_PRIM * st_c["prims"] / nr + B_STATE
+    b_node = st_c.get("node_bytes") or B_NODE   # 64 B binary node, 128 B four-wide node
+    per_ray = B_RAY + b_node * st_c["nodes"] / nr + B_PRIM * st_c["prims"] / nr + B_STATE
     avg_launch_ms = st_t["trace_ms"] / max(st_t["launches"], 1)
     bytes_per_launch = per_ray * st_t["rays"] / max(st_t["launches"], 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
@@ -740,7 +742,8 @@ def main():
             log(f"cpu baseline failed: {e}")
 
     bvh_name = args.bvh if args.bvh != "auto" else ("sah" if info["primitives"] <= (4 << 20) else "lbvh")
-    traffic_ray = committed_traffic(bvh_name)
+    walk = "wide" if st_c.get("node_bytes") == 128 else "binary"
+    traffic_ray = committed_traffic(bvh_name, walk)
     # HBM bytes per launch = measured bytes per ray x rays per launch of this run
     traffic = traffic_ray * st_t["rays"] / max(st_t["launches"], 1) if traffic_ray else None
     line = {
@@ -766,7 +769,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_trace", "avg_launch_ms": avg_launch_ms,
                      "shade_kernel_ms_per_step": st_t["shade_ms"], "trace_kernel_ms_per_step": st_t["trace_ms"],
-                     "bytes_per_ray": per_ray, "nodes_per_ray": st_c["nodes"] / nr,
+                     "bytes_per_ray": per_ray, "nodes_per_ray": st_c["nodes"] / nr, "walk": walk,
                      "prims_per_ray": st_c["prims"] / nr},
         "dcsky": dcs,
         "dcsky_config5": dc5,

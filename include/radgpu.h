@@ -151,17 +151,24 @@ int rg_set_profiling(rg_ctx *ctx, int level);
  * persistent wavefront fetch new rays (default 16); "window" children per
  * generation launch (default 2M); "blocks_per_cu" persistent grid size;
  * "max_big" (0..32, applies at the next commit) room-sized primitives kept
- * out of the BVH and tested before the walk.
- * Results do not depend on these; only the time does. */
+ * out of the BVH and tested before the walk; "wide" 1 (default) the
+ * persistent walk fetches four child boxes per node when the scene has a
+ * four-wide tree (host SAH trees shallow enough for the traversal stack),
+ * 0 the binary walk.
+ * Results do not depend on these; only the time does (the walks visit
+ * primitives in different orders, which only matters for two surfaces
+ * within FTINY of each other along a ray, where the reference's own choice
+ * depends on its octree order). */
 int rg_set_tuning(rg_ctx *ctx, const char *key, long long value);
 
 #define RG_BVH_AUTO 0
 #define RG_BVH_SAH 1
 #define RG_BVH_LBVH 2
 int rg_set_bvh_builder(rg_ctx *ctx, int kind);
-/* out[0..7] = rays, generation launches, threads launched, summed traversal-
+/* out[0..8] = rays, generation launches, threads launched, summed traversal-
  * kernel milliseconds, nodes visited, primitives tested, batches, summed
- * shading-kernel milliseconds */
+ * shading-kernel milliseconds, bytes per node visited (64 binary, 128
+ * four-wide) */
 int rg_last_stats(rg_ctx *ctx, double *out, int n);
 
 /* DC x sky product, replaces cm_multiply() (util/cmatrix.c:421-474):

@@ -341,9 +341,9 @@ class Context:
         return self
 
     def last_stats(self) -> dict:
-        v = (C.c_double * 8)()
-        self._check(self._L.rg_last_stats(self._h, v, 8))
-        keys = ("rays", "launches", "threads", "trace_ms", "nodes", "prims", "batches", "shade_ms")
+        v = (C.c_double * 9)()
+        self._check(self._L.rg_last_stats(self._h, v, 9))
+        keys = ("rays", "launches", "threads", "trace_ms", "nodes", "prims", "batches", "shade_ms", "node_bytes")
         return dict(zip(keys, list(v)))
 
     def last_nrays(self) -> int:

@@ -168,4 +168,46 @@ int32_t build_sah_bvh(const float* lo, const float* hi, const int32_t* ids, uint
   return 0;
 }
 
+int collapse_bvh4(const std::vector<BVHNode>& nodes, int32_t root, std::vector<BVH4Node>& out) {
+  out.assign(nodes.size(), BVH4Node{});
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    const BVHNode& N = nodes[i];
+    BVH4Node& q = out[i];
+    int k = 0;
+    auto put = [&](const float* lo, const float* hi, int32_t ref) {
+      q.lox[k] = lo[0]; q.loy[k] = lo[1]; q.loz[k] = lo[2];
+      q.hix[k] = hi[0]; q.hiy[k] = hi[1]; q.hiz[k] = hi[2];
+      q.child[k] = ref;
+      ++k;
+    };
+    for (int side = 0; side < 2; ++side) {
+      const int32_t ref = side ? N.right : N.left;
+      if (ref >= 0) {
+        const BVHNode& C = nodes[(size_t)ref];
+        put(C.lmin, C.lmax, C.left);
+        put(C.rmin, C.rmax, C.right);
+      } else {
+        put(side ? N.rmin : N.lmin, side ? N.rmax : N.lmax, ref);
+      }
+    }
+    for (; k < 4; ++k) {
+      q.lox[k] = q.loy[k] = q.loz[k] = 0.f;
+      q.hix[k] = q.hiy[k] = q.hiz[k] = 0.f;
+      q.child[k] = INT32_MIN;
+    }
+  }
+  if (root < 0) return 0;
+  // depth over the four-wide nodes reached from the root
+  int depth = 0;
+  std::vector<std::pair<int32_t, int>> st{{root, 1}};
+  while (!st.empty()) {
+    auto [v, d] = st.back();
+    st.pop_back();
+    depth = std::max(depth, d);
+    for (int k = 0; k < 4; ++k)
+      if (out[(size_t)v].child[k] >= 0) st.push_back({out[(size_t)v].child[k], d + 1});
+  }
+  return depth;
+}
+
 }  // namespace rg

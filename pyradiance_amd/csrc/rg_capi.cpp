@@ -210,6 +210,7 @@ int rg_scene_commit(rg_ctx* c) {
     S.big = to_global(upload(c->scene_mem, "big", big, c->st));
     S.nbig = (int)big.size();
     S.nbvh = (int)n;
+    S.nodes4 = nullptr;
     BVHNode* nodes = (BVHNode*)c->scene_mem.get("nodes", sizeof(BVHNode) * std::max<uint32_t>(n, 1), c->st, true);
     int32_t* leaf = (int32_t*)c->scene_mem.get("leaf", sizeof(int32_t) * std::max<uint32_t>(n, 1), c->st, false);
     bool sah = c->bvh_builder == RG_BVH_SAH || (c->bvh_builder == RG_BVH_AUTO && n <= kSahMaxPrims);
@@ -222,6 +223,15 @@ int rg_scene_commit(rg_ctx* c) {
       if (!hn.empty())
         hipck(hipMemcpyAsync(nodes, hn.data(), sizeof(BVHNode) * hn.size(), hipMemcpyHostToDevice, c->st), "nodes");
       hipck(hipMemcpyAsync(leaf, small_ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st), "leaves");
+      // four-wide form for the persistent walk, when every path's pushes
+      // (at most three per four-wide node) fit the traversal stack
+      std::vector<BVH4Node> h4;
+      const int d4 = collapse_bvh4(hn, S.root, h4);
+      if (S.root >= 0 && !h4.empty() && 3 * d4 <= kTraceStackEntries) {
+        BVH4Node* n4 = (BVH4Node*)c->scene_mem.get("nodes4", sizeof(BVH4Node) * h4.size(), c->st, false);
+        hipck(hipMemcpyAsync(n4, h4.data(), sizeof(BVH4Node) * h4.size(), hipMemcpyHostToDevice, c->st), "nodes4");
+        S.nodes4 = to_global(n4);
+      }
       hipck(hipStreamSynchronize(c->st), "nodes");
     } else {
       S.root = build_lbvh(dlo, dhi, dids, n, slo, shi, nodes, leaf, c->arena, c->st);
@@ -354,10 +364,16 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   int max_srcs = c->dscene.nsrcs;
   if (batch)
     for (const DevScene& b : *batch) max_srcs = std::max(max_srcs, b.nsrcs);
+  // the four-wide walk needs the four-wide tree of every scene traced
+  Tuning tn = c->tuning;
+  if (!c->dscene.nodes4) tn.wide = 0;
+  if (batch)
+    for (const DevScene& b : *batch)
+      if (!b.nodes4) tn.wide = 0;
   if (nrays) {
     std::lock_guard<std::mutex> lock(g_render_mu);
-    render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, c->tuning, render_arena(c->device),
-           c->st, &c->last_stats);
+    render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, tn, render_arena(c->device), c->st,
+           &c->last_stats);
   }
   if (d_vis) hipck(hipMemcpyAsync(c->last_visits, d_vis, 16, hipMemcpyDeviceToHost, c->st), "visits");
   else c->last_visits[0] = c->last_visits[1] = 0;
@@ -664,6 +680,9 @@ int rg_set_tuning(rg_ctx* c, const char* key, long long value) {
     } else if (k == "max_big") {
       if (value < 0 || value > 32) throw std::runtime_error("max_big is 0..32");
       t.max_big = (uint32_t)value;
+    } else if (k == "wide") {
+      if (value != 0 && value != 1) throw std::runtime_error("wide is 0 or 1");
+      t.wide = (int)value;
     } else if (k == "blocks_per_cu") {
       if (value < 1 || value > 64) throw std::runtime_error("blocks_per_cu is 1..64");
       t.blocks_per_cu = (uint32_t)value;
@@ -690,10 +709,10 @@ int rg_set_profiling(rg_ctx* c, int level) {
 
 int rg_last_stats(rg_ctx* c, double* out, int n) {
   RG_TRY(c, {
-    double v[8] = {(double)c->last_nrays, (double)c->last_stats.launches, (double)c->last_stats.threads,
+    double v[9] = {(double)c->last_nrays, (double)c->last_stats.launches, (double)c->last_stats.threads,
                    c->last_stats.bounce_ms, (double)c->last_visits[0], (double)c->last_visits[1],
-                   (double)c->last_stats.generations, c->last_stats.shade_ms};
-    for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
+                   (double)c->last_stats.generations, c->last_stats.shade_ms, (double)c->last_stats.node_bytes};
+    for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
   });
 }
 

@@ -17,6 +17,7 @@ constexpr int BLOCK = 128;       // shading and seed kernels
 constexpr int TBLOCK = 256;      // traversal kernel
 constexpr int LSTACK = 16;       // per-lane traversal stack entries held in LDS
 constexpr int PSTACK = 48;       // further entries in private (scratch) memory
+static_assert(LSTACK + PSTACK == kTraceStackEntries, "stack size shared with the tree builders");
 
 // ---- wave-level helpers -----------------------------------------------------
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
@@ -1154,8 +1155,10 @@ k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t
 // kernel (its VALU lane utilisation is about a third).  Same admission,
 // closest-hit walk and output record as k_trace.  BATCH: rays of several
 // scenes (each lane keeps its ray's scene tables); otherwise the kernel
-// argument's scene, held in scalar registers.
-template <int BS, bool BATCH>
+// argument's scene, held in scalar registers.  WIDE: the walk fetches the
+// four-wide tree (four child boxes per node; the children hit are taken
+// nearest first, the others pushed farthest first).
+template <int BS, bool BATCH, bool WIDE>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(RG_TRACE_WAVES)))
 k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t nspawn, uint64_t c0,
            uint32_t nthreads, TraceOut* outp, uint32_t* work, uint32_t refill) {
@@ -1257,8 +1260,69 @@ k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint3
       }
       return DONE;
     };
+    auto push = [&](int32_t ref, float t) -> bool {
+      if (sp < LSTACK) { stk[sp * BS + lane] = ref; stn[sp * BS + lane] = t; }
+      else if (sp < LSTACK + PSTACK) deep[sp - LSTACK] = pack_entry(ref, t);
+      else return false;
+      ++sp;
+      return true;
+    };
+    if (WIDE) {
+      const BVH4Node* __restrict__ n4 = S.nodes4;
+      while (node >= 0) {
+        if (++iter > S.nbvh + 8) { atomicOr(c.err, ERR_STACK); node = DONE; leaf = -1; break; }
+        const BVH4Node& N = n4[node];
+        ++nvis;
+        const float tmax = (float)(h.t + 2 * FTINY) * 1.000001f;
+        const float kInf = __int_as_float(0x7f800000);
+        float t0, t1, t2, t3;
+        int32_t c0r = N.child[0], c1r = N.child[1], c2r = N.child[2], c3r = N.child[3];
+        {
+          float tk, lo[3], hi[3];
+          lo[0] = N.lox[0]; lo[1] = N.loy[0]; lo[2] = N.loz[0]; hi[0] = N.hix[0]; hi[1] = N.hiy[0]; hi[2] = N.hiz[0];
+          t0 = (slab(lo, hi, ox, oy, oz, ix, iy, iz, tmax, tk) & (c0r != DONE)) ? tk : kInf;
+          lo[0] = N.lox[1]; lo[1] = N.loy[1]; lo[2] = N.loz[1]; hi[0] = N.hix[1]; hi[1] = N.hiy[1]; hi[2] = N.hiz[1];
+          t1 = (slab(lo, hi, ox, oy, oz, ix, iy, iz, tmax, tk) & (c1r != DONE)) ? tk : kInf;
+          lo[0] = N.lox[2]; lo[1] = N.loy[2]; lo[2] = N.loz[2]; hi[0] = N.hix[2]; hi[1] = N.hiy[2]; hi[2] = N.hiz[2];
+          t2 = (slab(lo, hi, ox, oy, oz, ix, iy, iz, tmax, tk) & (c2r != DONE)) ? tk : kInf;
+          lo[0] = N.lox[3]; lo[1] = N.loy[3]; lo[2] = N.loz[3]; hi[0] = N.hix[3]; hi[1] = N.hiy[3]; hi[2] = N.hiz[3];
+          t3 = (slab(lo, hi, ox, oy, oz, ix, iy, iz, tmax, tk) & (c3r != DONE)) ? tk : kInf;
+        }
+        // ascending entry distance, equal distances in slot order: five
+        // compare-exchanges on (distance, slot)
+        int s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+        auto cx = [](float& ta, int32_t& ca, int& sa, float& tb, int32_t& cb, int& sb) {
+          const bool sw = (ta > tb) | ((ta == tb) & (sa > sb));
+          if (sw) {
+            float tt = ta; ta = tb; tb = tt;
+            int32_t cc = ca; ca = cb; cb = cc;
+            int ss = sa; sa = sb; sb = ss;
+          }
+        };
+        cx(t0, c0r, s0, t1, c1r, s1);
+        cx(t2, c2r, s2, t3, c3r, s3);
+        cx(t0, c0r, s0, t2, c2r, s2);
+        cx(t1, c1r, s1, t3, c3r, s3);
+        cx(t1, c1r, s1, t2, c2r, s2);
+        if (t0 == kInf) {
+          node = pop(tmax);
+        } else {
+          bool ok = true;
+          if (t3 != kInf) ok &= push(c3r, t3);
+          if (t2 != kInf) ok &= push(c2r, t2);
+          if (t1 != kInf) ok &= push(c1r, t1);
+          if (!ok) { atomicOr(c.err, ERR_STACK); node = DONE; leaf = -1; break; }
+          node = c0r;
+        }
+        if ((node < 0) & (node != DONE) & (leaf < 0)) {
+          leaf = ~node;
+          node = pop((float)(h.t + 2 * FTINY) * 1.000001f);
+        }
+        if (__ballot(leaf < 0) == 0ull) break;
+      }
+    }
     const BVHNode* __restrict__ nodes = S.nodes;
-    while (node >= 0) {
+    while (!WIDE && node >= 0) {
       if (++iter > S.nbvh + 8) { atomicOr(c.err, ERR_STACK); node = DONE; leaf = -1; break; }
       const BVHNode N = nodes[node];
       ++nvis;
@@ -1612,11 +1676,17 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       const uint32_t tblocks = (nth + TBLOCK - 1) / TBLOCK;
       if (tn.trace_kernel == 1) {
         const uint32_t nb = std::min(tblocks, pf_blocks);
-        if (cfg.nscenes)
-          hipLaunchKernelGGL((k_trace_pf<TBLOCK, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n,
-                             c0, nth, tout, d_cnt + 1, tn.refill);
+        if (cfg.nscenes && tn.wide)
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+        else if (cfg.nscenes)
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+        else if (tn.wide)
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
         else
-          hipLaunchKernelGGL((k_trace_pf<TBLOCK, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
                              bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
       } else {
         hipLaunchKernelGGL(k_trace, dim3(tblocks), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n, c0, nth, tout);
@@ -1661,7 +1731,10 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       else free_slots.push_back(s);
     }
   }
-  if (stats) stats->generations = gens;
+  if (stats) {
+    stats->generations = gens;
+    stats->node_bytes = (tn.trace_kernel == 1 && tn.wide) ? (int)sizeof(BVH4Node) : (int)sizeof(BVHNode);
+  }
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (ev2) (void)hipEventDestroy(ev2);

@@ -71,7 +71,11 @@ struct RenderStats {
   bool timing = false;       // time every generation's kernels with HIP events
   double bounce_ms = 0;      // summed traversal-kernel (k_trace) durations
   double shade_ms = 0;       // summed shading-kernel (k_shade) durations
+  int node_bytes = 64;       // bytes of one tree node the traversal fetched (64 binary, 128 four-wide)
 };
+
+// Per-lane traversal stack entries (LDS and private memory together).
+constexpr int kTraceStackEntries = 64;
 
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level);
 
@@ -83,6 +87,7 @@ struct Tuning {
   uint32_t window = 1u << 21;    // children per generation launch
   uint32_t blocks_per_cu = 4;    // persistent grid: blocks of 256 per compute unit
   uint32_t max_big = 32;         // room-sized primitives kept out of the BVH (applies at commit)
+  int wide = 1;                  // persistent traversal walks the four-wide tree when the scene has one
 };
 
 // max_srcs: the largest source count over the scenes of a batch (sizes the

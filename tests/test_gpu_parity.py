@@ -277,16 +277,17 @@ def test_sah_and_lbvh_trees_give_the_same_contributions(L):
 
 
 def test_persistent_and_per_thread_traversal_agree(L):
-    """The persistent dynamic-fetch kernel and the one-ray-per-thread kernel
-    trace the same rays: same ray count, same contributions (up to the order
-    of the FP64 additions), also with small refill thresholds and windows
-    that split the generations."""
+    """The persistent dynamic-fetch kernel (four-wide and binary walks) and
+    the one-ray-per-thread kernel trace the same rays: same ray count, same
+    contributions (up to the order of the FP64 additions), also with small
+    refill thresholds and windows that split the generations."""
     _say("persistent vs per-thread traversal")
     rad, sens = scenes.config2(1)
     sens = sens[::5]
     res = []
-    for kw in ({"trace_kernel": 0}, {"trace_kernel": 1}, {"trace_kernel": 1, "refill": 1, "window": 50000},
-               {"trace_kernel": 1, "refill": 64, "blocks_per_cu": 1}):
+    for kw in ({"trace_kernel": 0}, {"trace_kernel": 1}, {"trace_kernel": 1, "wide": 0},
+               {"trace_kernel": 1, "refill": 1, "window": 50000},
+               {"trace_kernel": 1, "refill": 64, "blocks_per_cu": 1, "wide": 0}):
         c = _ctx(L, rad)
         c.set_tuning(**kw)
         c.set_params(_rc_params(L, ambounce=2, ambdiv=512, minweight=1e-3, seed=7))
