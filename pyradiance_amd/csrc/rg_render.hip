@@ -15,9 +15,11 @@ namespace rg {
 
 constexpr int BLOCK = 128;       // shading and seed kernels
 constexpr int TBLOCK = 256;      // traversal kernel
-constexpr int LSTACK = 16;       // per-lane traversal stack entries held in LDS
-constexpr int PSTACK = 48;       // further entries in private (scratch) memory
-static_assert(LSTACK + PSTACK == kTraceStackEntries, "stack size shared with the tree builders");
+#ifndef RG_LSTACK
+#define RG_LSTACK 16
+#endif
+constexpr int LSTACK = RG_LSTACK;                    // per-lane traversal stack entries held in LDS
+constexpr int PSTACK = kTraceStackEntries - LSTACK;  // further entries in private (scratch) memory
 
 // ---- wave-level helpers -----------------------------------------------------
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
