@@ -177,9 +177,14 @@ __global__ void __launch_bounds__(DB) k_dcsky_mfma(const unsigned short* Ah, con
 // global loads are issued before the current stage's MFMAs, into the second
 // LDS buffer.  Rows are padded to 40 bf16 so the 16 lanes that read one
 // fragment column hit different banks.
-constexpr int LBM = 128, LBN = 128, LKS = 32;
-constexpr int LROW = 40;                      // padded row length (bf16)
-constexpr int LTILE = 128 * LROW;             // one 128 x 32 plane tile in LDS (bf16)
+constexpr int LBM = 128, LBN = 128;
+#ifndef RG_DCSKY_LKS
+#define RG_DCSKY_LKS 64
+#endif
+constexpr int LKS = RG_DCSKY_LKS;             // K per stage (32 or 64; 64 measured 22-28 % faster at K = 2306)
+constexpr int LROW = LKS + 8;                 // padded row length (bf16)
+constexpr int LTILE = 128 * LROW;             // one 128 x LKS plane tile in LDS (bf16)
+constexpr int LCPR = LKS / 8;                 // 16-byte chunks per tile row
 
 // WM x WN wavefronts, each holding TA x TB 16 x 16 MFMA tiles of all three
 // channels (16 * TA * WM = 16 * TB * WN = 128).
@@ -189,7 +194,7 @@ __global__ void __launch_bounds__(64 * WM * WN) k_dcsky_lds(const unsigned short
                                                             float* C, int M, int N, int Kp, int Mt, int Nt) {
   static_assert(16 * TA * WM == LBM && 16 * TB * WN == LBN, "block tile");
   constexpr int NT = 64 * WM * WN;
-  constexpr int NCH = 2048 / NT;              // 16-byte chunks each thread stages per stage
+  constexpr int NCH = 4 * 128 * LCPR / NT;    // 16-byte chunks each thread stages per stage
   __shared__ __attribute__((aligned(16))) unsigned short lds[2][4][LTILE];   // [buffer][Ah, Al, Bh, Bl]
   int tm, tn;
   if (!tile_of(Mt, Nt, tm, tn)) return;
@@ -198,13 +203,14 @@ __global__ void __launch_bounds__(64 * WM * WN) k_dcsky_lds(const unsigned short
   const int wm = wave / WN, wn = wave % WN;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int mb = tm * LBM, nb = tn * LBN;
-  // staging: chunk q (16 B = 8 bf16) of the stage: plane q >> 9, row (q & 511) >> 2, part q & 3
+  // staging: chunk q (16 B = 8 bf16) of the stage: plane q / (128 LCPR),
+  // row (q mod 128 LCPR) / LCPR, part q mod LCPR
   int soff[NCH], splane[NCH];
   size_t sg[NCH];
   const size_t csz[4] = {(size_t)M * Kp, (size_t)M * Kp, (size_t)N * Kp, (size_t)N * Kp};
   for (int j = 0; j < NCH; ++j) {
     const int q = t + j * NT;
-    const int pl = q >> 9, row = (q & 511) >> 2, part = q & 3;
+    const int pl = q / (128 * LCPR), row = (q % (128 * LCPR)) / LCPR, part = q % LCPR;
     splane[j] = pl;
     soff[j] = row * LROW + part * 8;
     sg[j] = (pl < 2 ? (size_t)min(mb + row, M - 1) : (size_t)min(nb + row, N - 1)) * Kp + part * 8;
@@ -230,27 +236,30 @@ __global__ void __launch_bounds__(64 * WM * WN) k_dcsky_lds(const unsigned short
     const int buf = s & 1;
     if (s + 1 < nstage) gload(s + 1, reg);
     const int c = s % 3;
-    bf16x8 fah[TA], fal[TA], fbh[TB], fbl[TB];
-    for (int a = 0; a < TA; ++a) {
-      const int row = wm * 16 * TA + 16 * a + r16;
-      fah[a] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][0][row * LROW + kq]);
-      fal[a] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][1][row * LROW + kq]);
-    }
-    for (int b = 0; b < TB; ++b) {
-      const int row = wn * 16 * TB + 16 * b + r16;
-      fbh[b] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][2][row * LROW + kq]);
-      fbl[b] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][3][row * LROW + kq]);
-    }
-    for (int a = 0; a < TA; ++a)
-      for (int b = 0; b < TB; ++b) {
-        f32x4 v = c == 0 ? acc[0][a][b] : (c == 1 ? acc[1][a][b] : acc[2][a][b]);
-        v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbh[b], fah[a], v, 0, 0, 0);
-        v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbl[b], fah[a], v, 0, 0, 0);
-        v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbh[b], fal[a], v, 0, 0, 0);
-        if (c == 0) acc[0][a][b] = v;
-        else if (c == 1) acc[1][a][b] = v;
-        else acc[2][a][b] = v;
+#pragma unroll
+    for (int kk = 0; kk < LKS; kk += 32) {
+      bf16x8 fah[TA], fal[TA], fbh[TB], fbl[TB];
+      for (int a = 0; a < TA; ++a) {
+        const int row = wm * 16 * TA + 16 * a + r16;
+        fah[a] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][0][row * LROW + kk + kq]);
+        fal[a] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][1][row * LROW + kk + kq]);
       }
+      for (int b = 0; b < TB; ++b) {
+        const int row = wn * 16 * TB + 16 * b + r16;
+        fbh[b] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][2][row * LROW + kk + kq]);
+        fbl[b] = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][3][row * LROW + kk + kq]);
+      }
+      for (int a = 0; a < TA; ++a)
+        for (int b = 0; b < TB; ++b) {
+          f32x4 v = c == 0 ? acc[0][a][b] : (c == 1 ? acc[1][a][b] : acc[2][a][b]);
+          v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbh[b], fah[a], v, 0, 0, 0);
+          v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbl[b], fah[a], v, 0, 0, 0);
+          v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fbh[b], fal[a], v, 0, 0, 0);
+          if (c == 0) acc[0][a][b] = v;
+          else if (c == 1) acc[1][a][b] = v;
+          else acc[2][a][b] = v;
+        }
+    }
     if (s + 1 < nstage) lstore(buf ^ 1, reg);
     __syncthreads();
   }
@@ -324,7 +333,10 @@ void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mo
     hipLaunchKernelGGL(k_dcsky_f32, dim3((N + 63) / 64, (M + 63) / 64), dim3(256), 0, st, A, B, C, M, K, N);
     return;
   }
-  int Kp = (K + 31) / 32 * 32;
+  // K padded to whole stages of the kernel that will run (zeros beyond K)
+  const bool staged = mode == 3 || (mode == 1 && K > 256);
+  const int kstep = staged ? LKS : 32;
+  int Kp = (K + kstep - 1) / kstep * kstep;
   size_t na = (size_t)3 * M * Kp, nb = (size_t)3 * N * Kp;
   unsigned short* buf = nullptr;
   dck(hipMallocAsync((void**)&buf, 2 * (na + nb) * sizeof(unsigned short), st), "dcsky scratch");
@@ -334,7 +346,6 @@ void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mo
   // Short K (a few K steps) is bound by latency and the output write: the
   // per-wavefront kernel is as fast there; long K is MFMA work and needs the
   // LDS-staged tiles.  mode 2 forces the former, mode 3 the latter.
-  const bool staged = mode == 3 || (mode == 1 && Kp > 256);
   if (!staged) {
     const int Mt = (M + BM - 1) / BM, Nt = (N + BN - 1) / BN;
     const long long tiles = (long long)Mt * Nt;
