@@ -33,6 +33,7 @@ __all__ = [
     "config3",
     "boxes_rad",
     "annual_wea",
+    "config5",
 ]
 
 
@@ -238,6 +239,110 @@ def sky_matrix(seed: int, nbins: int, nhours: int = 8760, latitude: float = 37.8
     sin_alt = np.sin(lat) * np.sin(decl) + np.cos(lat) * np.cos(decl) * np.cos(ha)
     sky[:, sin_alt <= 0.0, :] = 0.0
     return sky
+
+
+def _box_quads(lo: np.ndarray, hi: np.ndarray, nu: int, nv: int) -> np.ndarray:
+    """Vertices (nbox, 6*nu*nv, 4, 3) of boxes split as box_quads_rad() does
+    (same faces, order and outward vertex order), for many boxes at once."""
+    n = lo.shape[0]
+    ti, tj = np.arange(nu + 1) / nu, np.arange(nv + 1) / nv
+    corners = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    out = []
+    for ax, side, ua, va, sgn in ((0, 0, 1, 2, -1), (0, 1, 1, 2, 1), (1, 0, 0, 2, -1),
+                                  (1, 1, 0, 2, 1), (2, 0, 0, 1, -1), (2, 1, 0, 1, 1)):
+        cyc = (ua, va, ax) in ((0, 1, 2), (1, 2, 0), (2, 0, 1))
+        order = corners if (sgn > 0) == cyc else corners[::-1]
+        val = hi[:, ax] if side else lo[:, ax]
+        us = lo[:, ua, None] + (hi[:, ua] - lo[:, ua])[:, None] * ti
+        vs = lo[:, va, None] + (hi[:, va] - lo[:, va])[:, None] * tj
+        q = np.empty((n, nu, nv, 4, 3))
+        for c, (a, b) in enumerate(order):
+            q[:, :, :, c, ax] = val[:, None, None]
+            q[:, :, :, c, ua] = us[:, a:a + nu][:, :, None]
+            q[:, :, :, c, va] = vs[:, b:b + nv][:, None, :]
+        out.append(q.reshape(n, nu * nv, 4, 3))
+    return np.concatenate(out, axis=1)
+
+
+def _quads_text(mods, quads: np.ndarray) -> bytes:
+    """`mod polygon q` entries for quads (n, 4, 3) with modifier names mods[i]."""
+    from . import _lib
+    lines = _lib.format_ascii(quads.reshape(-1, 12), 12, "%.7g", " ", "").split(b"\n")
+    heads = {m: f"{m} polygon q\n0\n0\n12 ".encode() for m in set(mods)}
+    return b"".join([heads[m] + ln + b"\n" for m, ln in zip(mods, lines)])
+
+
+def config5(seed: int = 0, storeys: int = 10, boxes_per_storey: int = 1980, nu: int = 3, nv: int = 7,
+            sensors_per_storey: tuple = (50, 40)):
+    """rcontrib -ab 5 -ad 8192 -lw 5e-5 workload: a 10-storey office of about
+    5 M triangles (quads counted as two, as for config 2) under a binned sky.
+
+    Floor plate 60 x 30 m, storey height 3.5 m (0.3 m slabs).  Per storey: a
+    floor and a ceiling, facade bands (wall 0-0.9 m, glass 0.9-2.3 m = 40 % of
+    the facade, wall above), a 4 m corridor along x with two-sided partition
+    walls every 6 m, and `boxes_per_storey` seeded furniture boxes (sides
+    0.4-1.6 m, heights 0.4-2.0 m) each split into nu x nv quads per face, with
+    16 seeded gray plastics.  Sensors: a 50 x 40 grid per storey at 0.8 m
+    above the floor, facing +Z (20,000 in all).  Sky and ground glow sources
+    for Reinhart binning.  Returns (scene bytes, sensors (n, 6)).
+    """
+    rng = np.random.default_rng(seed)
+    W, D, H, slab = 60.0, 30.0, 3.5, 0.3
+    head = ["void plastic c5_floor\n0\n0\n5 .2 .2 .2 0 0\n", "void plastic c5_ceiling\n0\n0\n5 .7 .7 .7 0 0\n",
+            "void plastic c5_wall\n0\n0\n5 .5 .5 .5 0 0\n", "void glass c5_glass\n0\n0\n3 .6 .6 .6\n"]
+    for m in range(16):
+        r = rng.uniform(0.1, 0.8)
+        head.append(f"void plastic c5_box{m}\n0\n0\n5 {r:.6g} {r:.6g} {r:.6g} 0 0\n")
+    quads, mods = [], []
+
+    def quad(m, verts):
+        quads.append(np.asarray(verts, float))
+        mods.append(m)
+
+    def wall2(m, p0, p1, z0, z1):      # vertical two-sided wall from p0 to p1 (x, y)
+        a = [(p0[0], p0[1], z0), (p1[0], p1[1], z0), (p1[0], p1[1], z1), (p0[0], p0[1], z1)]
+        quad(m, a)
+        quad(m, a[::-1])
+
+    for k in range(storeys):
+        z0 = H * k
+        zc = z0 + H - slab
+        quad("c5_floor", [(0, 0, z0), (W, 0, z0), (W, D, z0), (0, D, z0)])
+        quad("c5_ceiling", [(0, 0, zc), (0, D, zc), (W, D, zc), (W, 0, zc)])
+        # facade, inward normals: south (y=0), east (x=W), north (y=D), west (x=0)
+        ring = [((0, 0), (W, 0)), ((W, 0), (W, D)), ((W, D), (0, D)), ((0, D), (0, 0))]
+        for (x0, y0), (x1, y1) in ring:
+            for m, za, zb in (("c5_wall", z0, z0 + 0.9), ("c5_glass", z0 + 0.9, z0 + 2.3), ("c5_wall", z0 + 2.3, zc)):
+                quad(m, [(x0, y0, za), (x0, y0, zb), (x1, y1, zb), (x1, y1, za)])
+        # corridor walls (door gaps of 1 m every 6 m) and office partitions
+        for y in (13.0, 17.0):
+            for x in np.arange(0.0, W, 6.0):
+                wall2("c5_wall", (x, y), (x + 5.0, y), z0, zc)
+        for x in np.arange(6.0, W, 6.0):
+            wall2("c5_wall", (x, 0.0), (x, 13.0), z0, zc)
+            wall2("c5_wall", (x, 17.0), (x, D), z0, zc)
+    env = np.stack(quads)
+    # furniture: boxes resting 2-10 mm above each floor, inside the plate
+    nb = storeys * boxes_per_storey
+    size = np.stack([rng.uniform(0.4, 1.6, nb), rng.uniform(0.4, 1.6, nb), rng.uniform(0.4, 2.0, nb)], axis=1)
+    lo = np.stack([rng.uniform(0.05, W - 0.05 - size[:, 0]), rng.uniform(0.05, D - 0.05 - size[:, 1]),
+                   rng.uniform(0.002, 0.01, nb)], axis=1)
+    lo[:, 2] += H * np.repeat(np.arange(storeys), boxes_per_storey)
+    bq = _box_quads(lo, lo + size, nu, nv).reshape(-1, 4, 3)
+    bmods = np.repeat(rng.integers(0, 16, nb), 6 * nu * nv)
+    text = ("".join(head) + sky_rad((1, 1, 1), ground=(1, 1, 1))).encode()
+    text += _quads_text(mods, env)
+    text += _quads_text([f"c5_box{m}" for m in bmods], bq)
+    nx, ny = sensors_per_storey
+    xs = (np.arange(nx) + 0.5) * (W / nx)
+    ys = (np.arange(ny) + 0.5) * (D / ny)
+    gx, gy = np.meshgrid(xs, ys, indexing="ij")
+    sens = []
+    for k in range(storeys):
+        s = np.zeros((nx * ny, 6))
+        s[:, 0], s[:, 1], s[:, 2], s[:, 5] = gx.ravel(), gy.ravel(), H * k + 0.8, 1.0
+        sens.append(s)
+    return text, np.concatenate(sens)
 
 
 def annual_wea(seed: int = 0, latitude: float = 37.62, longitude: float = 122.38, time_zone: float = 120.0) -> str:

@@ -125,3 +125,110 @@ extern "C" int check_sah(int n, int nrays, unsigned seed, int max_depth, int clu
   }
   return bad;
 }
+
+// Four-wide form of the same tree (rg::collapse_bvh4): every slot of node i
+// is a grandchild of binary node i, or its leaf child, with that child's box;
+// a walk of the four-wide tree reaches every primitive once and finds the
+// same nearest box entry as brute force; the four-wide depth is at most
+// half the binary depth, rounded up, plus one.  Returns 0 or a bit mask.
+extern "C" int check_bvh4(int n, int nrays, unsigned seed, int max_depth, int clustered, int* depth4_out) {
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<float> U(0.0f, 1.0f);
+  std::vector<float> lo(3 * n), hi(3 * n);
+  std::vector<int32_t> ids(n);
+  for (int i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      float c = clustered ? (float)(i % 7) : U(rng) * 10.0f;
+      float h = 0.001f + U(rng) * 0.2f;
+      lo[3 * i + a] = c - h;
+      hi[3 * i + a] = c + h;
+    }
+    ids[i] = i;
+  }
+  std::vector<BVHNode> nodes;
+  int32_t root = rg::build_sah_bvh(lo.data(), hi.data(), ids.data(), (uint32_t)n, nodes, max_depth);
+  std::vector<rg::BVH4Node> q;
+  int d4 = rg::collapse_bvh4(nodes, root, q);
+  if (depth4_out) *depth4_out = d4;
+  int bad = 0;
+  if (q.size() != nodes.size()) return 1;
+  if (n < 2) return d4 == 0 ? 0 : 2;
+  // slot contents
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    const BVHNode& N = nodes[i];
+    std::vector<std::pair<int32_t, const float*>> want;   // (ref, lo box) with hi box after it
+    std::vector<const float*> wanthi;
+    for (int side = 0; side < 2; ++side) {
+      int32_t ref = side ? N.right : N.left;
+      if (ref >= 0) {
+        const BVHNode& Cn = nodes[(size_t)ref];
+        want.push_back({Cn.left, Cn.lmin}); wanthi.push_back(Cn.lmax);
+        want.push_back({Cn.right, Cn.rmin}); wanthi.push_back(Cn.rmax);
+      } else {
+        want.push_back({ref, side ? N.rmin : N.lmin}); wanthi.push_back(side ? N.rmax : N.lmax);
+      }
+    }
+    const rg::BVH4Node& Q = q[i];
+    for (int k = 0; k < 4; ++k) {
+      if (k >= (int)want.size()) { if (Q.child[k] != INT32_MIN) bad |= 4; continue; }
+      if (Q.child[k] != want[k].first) bad |= 4;
+      const float got_lo[3] = {Q.lox[k], Q.loy[k], Q.loz[k]}, got_hi[3] = {Q.hix[k], Q.hiy[k], Q.hiz[k]};
+      for (int a = 0; a < 3; ++a)
+        if (got_lo[a] != want[k].second[a] || got_hi[a] != wanthi[k][a]) bad |= 8;
+    }
+  }
+  // every primitive once; depth bound
+  std::vector<int> seen(n, 0);
+  std::vector<std::pair<int32_t, int>> st{{root, 1}};
+  int deep = 0;
+  while (!st.empty()) {
+    auto [v, d] = st.back();
+    st.pop_back();
+    deep = std::max(deep, d);
+    for (int k = 0; k < 4; ++k) {
+      int32_t c = q[(size_t)v].child[k];
+      if (c == INT32_MIN) continue;
+      if (c >= 0) st.push_back({c, d + 1});
+      else if (~c < n) seen[~c]++;
+      else bad |= 16;
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (seen[i] != 1) bad |= 16;
+  if (deep != d4) bad |= 32;
+  // nearest box entry: four-wide walk vs brute force
+  for (int r = 0; r < nrays; ++r) {
+    float o[3], d[3], inv[3];
+    for (int a = 0; a < 3; ++a) { o[a] = U(rng) * 12.0f - 1.0f; d[a] = U(rng) * 2.0f - 1.0f; }
+    for (int a = 0; a < 3; ++a) inv[a] = 1.0f / (std::fabs(d[a]) < 1e-20f ? 1e-20f : d[a]);
+    float best = 1e30f, tb = 1e30f;
+    int bi = -1, ti = -1;
+    for (int i = 0; i < n; ++i) {
+      float tn;
+      if (slab(&lo[3 * i], &hi[3 * i], o, inv, tn)) {
+        tn = std::fmax(tn, 0.0f);
+        if (tn < best || (tn == best && i < bi)) { best = tn; bi = i; }
+      }
+    }
+    std::vector<int> stk{root};
+    while (!stk.empty()) {
+      const rg::BVH4Node& Q = q[(size_t)stk.back()];
+      stk.pop_back();
+      for (int k = 0; k < 4; ++k) {
+        if (Q.child[k] == INT32_MIN) continue;
+        const float mn[3] = {Q.lox[k], Q.loy[k], Q.loz[k]}, mx[3] = {Q.hix[k], Q.hiy[k], Q.hiz[k]};
+        float tn;
+        if (!slab(mn, mx, o, inv, tn)) continue;
+        if (Q.child[k] >= 0) { stk.push_back(Q.child[k]); continue; }
+        int p = ~Q.child[k];
+        float tp;
+        if (slab(&lo[3 * p], &hi[3 * p], o, inv, tp)) {
+          tp = std::fmax(tp, 0.0f);
+          if (tp < tb || (tp == tb && p < ti)) { tb = tp; ti = p; }
+        }
+      }
+    }
+    if (ti != bi) bad |= 64;
+  }
+  return bad;
+}

@@ -25,6 +25,7 @@ def bvhcheck(tmp_path_factory):
                     "-o", str(out)], check=True)
     L = C.CDLL(str(out))
     L.check_sah.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.check_bvh4.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_int)]
     return L
 
 
@@ -42,3 +43,16 @@ def test_sah_depth_limit_forces_median_splits(bvhcheck):
     bad = bvhcheck.check_sah(50000, 300, 3, 30, 0, C.byref(depth))
     assert bad == 0, f"failed checks mask {bad:#x}"
     assert depth.value <= 30
+
+
+@pytest.mark.parametrize("n,clustered", [(1, 0), (2, 0), (3, 0), (5, 0), (257, 0), (20000, 0), (20000, 1)])
+def test_four_wide_tree_holds_the_grandchildren_and_walks_like_brute_force(bvhcheck, n, clustered):
+    """The four-wide nodes the persistent traversal walks (rg_bvh.cpp
+    collapse_bvh4): slots, boxes, coverage, depth and nearest-box walks."""
+    d4, d2 = C.c_int(), C.c_int()
+    bad = bvhcheck.check_bvh4(n, 1000, 11, 60, clustered, C.byref(d4))
+    assert bad == 0, f"failed checks mask {bad:#x}"
+    assert bvhcheck.check_sah(n, 0, 11, 60, clustered, C.byref(d2)) == 0
+    if n >= 2:
+        # binary depth counts edges below the root, four-wide depth counts nodes
+        assert d4.value <= (d2.value + 2) // 2 + 1
