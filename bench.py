@@ -13,7 +13,7 @@ with its reference program timed beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
                     [--no-batched] [--bvh auto|sah|lbvh]
-    python bench.py --only sky|rpict|rflux [--no-cpu-baseline]
+    python bench.py --only sky|rpict|rflux|config5 [--no-cpu-baseline]
 
 roofline.traffic (HBM bytes per k_trace launch) is read from the committed
 rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload,
@@ -412,6 +412,61 @@ def grad_bench(steps: int = 3):
             "grad_norm": gnorm}
 
 
+def config5_bench(seed: int = 0):
+    """Config 5 end to end on one GPU: the 10-storey office (2,495,700 quads
+    = 5 M triangles, 20,000 sensors) traced with rcontrib -I+ -ab 5 -ad 8192
+    -lw 5e-5 -aa 0 into Reinhart MF:4 + ground bins, then that 20000 x 2306
+    RGB DC matrix times a synthetic 2306 x 8760 sky on the split-BF16 MFMA
+    product."""
+    import torch
+    t0 = time.perf_counter()
+    rad, sens = scenes.config5(seed)
+    t_gen = time.perf_counter() - t0
+    ctx = _lib.Context(0)
+    t0 = time.perf_counter()
+    ctx.load_text(rad).commit()
+    t_commit = time.perf_counter() - t0
+    del rad
+    info = ctx.info()
+    p = gpu_params()
+    p.ambounce, p.ambdiv, p.minweight = 5, 8192, 5e-5
+    ctx.set_params(p)
+    m = _lib.RgModifier()
+    m.name, m.bin_kind, m.nbins, m.mf, m.rhs = b"skyglow", 1, scenes.reinhart_nbins(4), 4.0, 1.0
+    m.nrm[:] = (0.0, 0.0, -1.0)
+    m.up[:] = (0.0, 1.0, 0.0)
+    g = _lib.RgModifier()
+    g.name, g.bin_kind, g.nbins, g.cbin = b"groundglow", 0, 1, 0
+    mods = [m, g]
+    ctx.contrib(sens[::100], mods, 1, _lib.RT_IMM_IRRAD)
+    t0 = time.perf_counter()
+    out = ctx.contrib(sens, mods, 1, _lib.RT_IMM_IRRAD)
+    t_trace = time.perf_counter() - t0
+    rays = ctx.last_nrays()
+    st = ctx.last_stats()
+    ctx.close()
+    nb = scenes.reinhart_nbins(4) + 1
+    A = np.ascontiguousarray(out.reshape(len(sens), nb, 3), dtype=np.float32)
+    del out
+    B = scenes.sky_matrix(seed, nb, nhours=8760)
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    dC = torch.empty((len(sens), 8760, 3), dtype=torch.float32, device="cuda")
+    _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), len(sens), nb, 8760, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), len(sens), nb, 8760, 1)
+    torch.cuda.synchronize()
+    t_dc = time.perf_counter() - t0
+    finite = bool(torch.isfinite(dC).all().item())
+    return {"workload": "config 5: rcontrib -I+ -ab 5 -ad 8192 -lw 5e-5 -aa 0, Reinhart MF:4 + ground, "
+                        "10-storey office, then DC x sky 20000x2306x8760",
+            "primitives": int(info["primitives"]), "sensors": int(len(sens)), "scene_generate_s": t_gen,
+            "scene_commit_s": t_commit, "rays": int(rays), "trace_s": t_trace, "Mrays_per_s": rays / t_trace / 1e6,
+            "rays_per_sensor": rays / len(sens), "walk": "wide" if st.get("node_bytes") == 128 else "binary",
+            "dcsky_s": t_dc, "dcsky_output_finite": finite,
+            "dc_nonzero_fraction": float((A > 0).mean())}
+
+
 REF_ENV_PATH = ROOT / "oracle" / "_ref"
 
 
@@ -614,11 +669,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--shape", default="", metavar="K=V,...",
                     help="launch shape to use instead of autotuning, e.g. trace_kernel=1,refill=16")
-    ap.add_argument("--no-config5", action="store_true", help="skip the 20000x2306x8760 DC x sky product")
+    ap.add_argument("--no-config5", action="store_true",
+                    help="skip config 5 (the 20000x2306x8760 DC x sky product and the 5 M-triangle rcontrib pass)")
     ap.add_argument("--no-sky", action="store_true", help="skip the gendaymtx measurement")
     ap.add_argument("--no-rpict", action="store_true", help="skip the rpict measurement")
     ap.add_argument("--no-rflux", action="store_true", help="skip the rfluxmtx measurement")
-    ap.add_argument("--only", choices=["sky", "rpict", "rflux"], default=None,
+    ap.add_argument("--only", choices=["sky", "rpict", "rflux", "config5"], default=None,
                     help="run one of the gendaymtx / rpict / rfluxmtx measurements alone and print its JSON")
     ap.add_argument("--cpu-sample", type=int, default=480)
     ap.add_argument("--sensors", type=int, default=2500)
@@ -637,7 +693,8 @@ def main():
             copy_calibration(args.copy_calibration)
         cpu_leg = not args.no_cpu_baseline
         fn = {"sky": lambda: sky_bench(cpu=cpu_leg), "rpict": lambda: rpict_bench(cpu=cpu_leg),
-              "rflux": lambda: rflux_bench(cpu=cpu_leg, cores=min(os.cpu_count() or 1, 16))}[args.only]
+              "rflux": lambda: rflux_bench(cpu=cpu_leg, cores=min(os.cpu_count() or 1, 16)),
+              "config5": config5_bench}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -702,11 +759,13 @@ def main():
     dcs["device"] = dcsky_device_bench(2500, 146, 8760)
     log(f"dcsky: {dcs}")
     log(f"dcsky with LDS-staged tiles: {dcsky_device_bench(2500, 146, 8760, mode=3)}")
-    dc5 = None
+    dc5 = c5 = None
     if not args.no_config5:
         dc5 = dcsky_device_bench(20000, 2306, 8760)
         log(f"dcsky config 5 size: {dc5}")
         log(f"dcsky config 5 size without LDS staging: {dcsky_device_bench(20000, 2306, 8760, mode=2)}")
+        c5 = config5_bench()
+        log(f"config 5 end to end: {c5}")
 
     batched = None
     if not args.no_batched:
@@ -773,6 +832,7 @@ def main():
                      "prims_per_ray": st_c["prims"] / nr},
         "dcsky": dcs,
         "dcsky_config5": dc5,
+        "config5": c5,
         "batched": batched,
         "grad_step": grad,
         "gendaymtx": sky,

@@ -234,8 +234,9 @@ class Context:
         self._check(self._L.rg_scene_load(self._h, str(path).encode(), int(allow_exec)))
         return self
 
-    def load_text(self, text: str, allow_exec: bool = False) -> "Context":
-        self._check(self._L.rg_scene_load_text(self._h, text.encode(), int(allow_exec)))
+    def load_text(self, text: str | bytes, allow_exec: bool = False) -> "Context":
+        data = text if isinstance(text, bytes) else text.encode()
+        self._check(self._L.rg_scene_load_text(self._h, data, int(allow_exec)))
         return self
 
     def commit(self) -> "Context":

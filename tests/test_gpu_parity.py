@@ -257,6 +257,34 @@ def test_config2_full_size_properties(L):
     assert n1 > 2500 * 4096
 
 
+def test_config5_full_size_properties(L):
+    """Full config-5 scene (10-storey office, 2,495,700 quads = 5 M
+    triangles) under a Reinhart MF:4 sky (2305 + 1 bins), -ab 5 -ad 8192
+    -lw 5e-5, on every 100th of its 20,000 sensors: every record a valid
+    irradiance split (non-negative, total <= pi per unit sky), the four-wide
+    and the binary walks agree, and the rays exceed the first bounce."""
+    _say("config5 full size")
+    rad, sens = scenes.config5(0)
+    c = _ctx(L, rad)
+    del rad
+    assert c.info()["primitives"] == 2_495_700
+    sub = sens[::100]
+    mods = _sky_mods(L, 4)
+    c.set_params(_rc_params(L, ambounce=5, ambdiv=8192, minweight=5e-5, seed=3))
+    out = c.contrib(sub, mods, 1, L.RT_IMM_IRRAD)
+    n1 = c.last_nrays()
+    assert out.shape == (200, 3 * 2306)
+    assert (out >= 0).all()
+    tot = out.sum(1) / 3
+    assert (tot <= math.pi * 1.02).all() and tot.max() > 0.01
+    assert n1 > 200 * 8192
+    c.set_tuning(wide=0)
+    out2 = c.contrib(sub, mods, 1, L.RT_IMM_IRRAD)
+    assert c.last_nrays() == n1
+    assert np.allclose(out, out2, rtol=1e-9, atol=1e-15)
+    c.close()
+
+
 def test_sah_and_lbvh_trees_give_the_same_contributions(L):
     """The two spatial indexes change only the traversal order: closest hits,
     and so every sample, agree (up to coincident-surface ties)."""
