@@ -150,8 +150,9 @@ int rg_set_profiling(rg_ctx *ctx, int level);
  * dynamic ray fetch (default); "refill" idle lanes (1..64) that make a
  * persistent wavefront fetch new rays (default 16); "window" children per
  * generation launch (default 2M); "blocks_per_cu" persistent grid size;
- * "max_big" (0..32, applies at the next commit) room-sized primitives kept
- * out of the BVH and tested before the walk; "wide" 1 (default) the
+ * "max_big" (0..32, default 8, applies at the next commit): room-sized
+ * primitives are kept out of the BVH and tested before the walk when there
+ * are at most this many, else all go into the BVH; "wide" 1 (default) the
  * persistent walk fetches four child boxes per node when the scene has a
  * four-wide tree (host SAH trees shallow enough for the traversal stack),
  * 0 the binary walk.

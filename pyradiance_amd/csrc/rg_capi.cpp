@@ -170,9 +170,12 @@ int rg_scene_commit(rg_ctx* c) {
     S.nsrcs = (int)cs.srcs.size();
     S.lamb = cs.lamb;
     S.cusize = cs.cusize;
-    // Room-sized primitives (box diagonal > 1/8 of the scene's, at most 32,
-    // largest first) stay out of the BVH: inside the tree their boxes would
-    // inflate every ancestor; each ray tests them first instead.
+    // Room-sized primitives (box diagonal > 1/8 of the scene's) stay out of
+    // the BVH when there are at most max_big of them (a room's walls): inside
+    // the tree their boxes would inflate every ancestor, and each ray tests
+    // them first, nearest plane first.  More of them than that (the floors
+    // and facades of a building) all go into the tree: tested on every ray
+    // they would cost more than the walk they shorten.
     std::vector<int32_t> big, small_ids;
     {
       double sd2 = 0;
@@ -184,7 +187,7 @@ int rg_scene_commit(rg_ctx* c) {
         if (d2 * 64.0 > sd2) cand.push_back({d2, (int32_t)k});
       }
       std::sort(cand.begin(), cand.end(), [](auto& a, auto& b) { return a.first > b.first; });
-      if (cand.size() > c->tuning.max_big) cand.resize(c->tuning.max_big);
+      if (cand.size() > c->tuning.max_big) cand.clear();
       if (cand.size() == cs.prims.size() && !cand.empty()) cand.pop_back();   // keep the tree non-empty
       std::vector<char> isbig(cs.prims.size(), 0);
       for (auto& pr : cand) { isbig[pr.second] = 1; big.push_back(pr.second); }

@@ -86,7 +86,7 @@ struct Tuning {
   uint32_t refill = 16;          // idle lanes that make a persistent wavefront fetch new rays
   uint32_t window = 1u << 21;    // children per generation launch
   uint32_t blocks_per_cu = 4;    // persistent grid: blocks of 256 per compute unit
-  uint32_t max_big = 32;         // room-sized primitives kept out of the BVH (applies at commit)
+  uint32_t max_big = 8;          // room-sized primitives kept out of the BVH when there are at most this many (applies at commit)
   int wide = 1;                  // persistent traversal walks the four-wide tree when the scene has one
 };
 
