@@ -169,45 +169,61 @@ int32_t build_sah_bvh(const float* lo, const float* hi, const int32_t* ids, uint
 }
 
 int collapse_bvh4(const std::vector<BVHNode>& nodes, int32_t root, std::vector<BVH4Node>& out) {
-  out.assign(nodes.size(), BVH4Node{});
-  for (size_t i = 0; i < nodes.size(); ++i) {
-    const BVHNode& N = nodes[i];
-    BVH4Node& q = out[i];
+  out.clear();
+  if (root < 0 || nodes.empty()) return 0;
+  // the binary nodes a four-wide walk visits (the root, then grandchildren
+  // of visited nodes), numbered in depth-first preorder with the first slot's
+  // subtree first, so a node's first child follows it in memory
+  auto slots = [&](int32_t b, int32_t* ref, const float** lo, const float** hi) {
+    const BVHNode& N = nodes[(size_t)b];
     int k = 0;
-    auto put = [&](const float* lo, const float* hi, int32_t ref) {
-      q.lox[k] = lo[0]; q.loy[k] = lo[1]; q.loz[k] = lo[2];
-      q.hix[k] = hi[0]; q.hiy[k] = hi[1]; q.hiz[k] = hi[2];
-      q.child[k] = ref;
-      ++k;
-    };
     for (int side = 0; side < 2; ++side) {
-      const int32_t ref = side ? N.right : N.left;
-      if (ref >= 0) {
-        const BVHNode& C = nodes[(size_t)ref];
-        put(C.lmin, C.lmax, C.left);
-        put(C.rmin, C.rmax, C.right);
+      const int32_t r = side ? N.right : N.left;
+      if (r >= 0) {
+        const BVHNode& C = nodes[(size_t)r];
+        ref[k] = C.left; lo[k] = C.lmin; hi[k] = C.lmax; ++k;
+        ref[k] = C.right; lo[k] = C.rmin; hi[k] = C.rmax; ++k;
       } else {
-        put(side ? N.rmin : N.lmin, side ? N.rmax : N.lmax, ref);
+        ref[k] = r; lo[k] = side ? N.rmin : N.lmin; hi[k] = side ? N.rmax : N.lmax; ++k;
       }
     }
-    for (; k < 4; ++k) {
-      q.lox[k] = q.loy[k] = q.loz[k] = 0.f;
-      q.hix[k] = q.hiy[k] = q.hiz[k] = 0.f;
-      q.child[k] = INT32_MIN;
+    return k;
+  };
+  std::vector<int32_t> order, newidx(nodes.size(), -1), st{root};
+  while (!st.empty()) {
+    const int32_t b = st.back();
+    st.pop_back();
+    newidx[(size_t)b] = (int32_t)order.size();
+    order.push_back(b);
+    int32_t ref[4];
+    const float *lo[4], *hi[4];
+    const int n = slots(b, ref, lo, hi);
+    for (int k = n; k--;)
+      if (ref[k] >= 0) st.push_back(ref[k]);
+  }
+  out.assign(order.size(), BVH4Node{});
+  std::vector<int> depth(order.size(), 1);
+  int dmax = 0;
+  for (size_t i = 0; i < order.size(); ++i) {
+    int32_t ref[4];
+    const float *lo[4], *hi[4];
+    const int n = slots(order[i], ref, lo, hi);
+    BVH4Node& q = out[i];
+    dmax = std::max(dmax, depth[i]);
+    for (int k = 0; k < 4; ++k) {
+      if (k < n) {
+        q.lox[k] = lo[k][0]; q.loy[k] = lo[k][1]; q.loz[k] = lo[k][2];
+        q.hix[k] = hi[k][0]; q.hiy[k] = hi[k][1]; q.hiz[k] = hi[k][2];
+        q.child[k] = ref[k] >= 0 ? newidx[(size_t)ref[k]] : ref[k];
+        if (ref[k] >= 0) depth[(size_t)q.child[k]] = depth[i] + 1;   // preorder: children come later
+      } else {
+        q.lox[k] = q.loy[k] = q.loz[k] = 0.f;
+        q.hix[k] = q.hiy[k] = q.hiz[k] = 0.f;
+        q.child[k] = INT32_MIN;
+      }
     }
   }
-  if (root < 0) return 0;
-  // depth over the four-wide nodes reached from the root
-  int depth = 0;
-  std::vector<std::pair<int32_t, int>> st{{root, 1}};
-  while (!st.empty()) {
-    auto [v, d] = st.back();
-    st.pop_back();
-    depth = std::max(depth, d);
-    for (int k = 0; k < 4; ++k)
-      if (out[(size_t)v].child[k] >= 0) st.push_back({out[(size_t)v].child[k], d + 1});
-  }
-  return depth;
+  return dmax;
 }
 
 }  // namespace rg

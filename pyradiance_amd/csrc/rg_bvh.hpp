@@ -21,9 +21,11 @@ namespace rg {
 int32_t build_sah_bvh(const float* lo, const float* hi, const int32_t* ids, uint32_t n, std::vector<BVHNode>& nodes,
                       int max_depth);
 
-// Four-wide form of a binary tree: out[i] holds the grandchildren of binary
-// node i (a leaf child kept in place), for every internal node.  Returns the
-// largest number of four-wide nodes on a path from `root` to a leaf.
+// Four-wide form of a binary tree: one node per binary node a four-wide
+// walk visits (the root, then the grandchildren of visited nodes), holding
+// that node's grandchildren (a leaf child kept in place), numbered in
+// depth-first preorder from 0 (the root).  Returns the largest number of
+// four-wide nodes on a path from `root` to a leaf.
 int collapse_bvh4(const std::vector<BVHNode>& nodes, int32_t root, std::vector<BVH4Node>& out);
 
 }  // namespace rg

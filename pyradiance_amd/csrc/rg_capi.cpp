@@ -230,7 +230,7 @@ int rg_scene_commit(rg_ctx* c) {
       // (at most three per four-wide node) fit the traversal stack
       std::vector<BVH4Node> h4;
       const int d4 = collapse_bvh4(hn, S.root, h4);
-      if (S.root >= 0 && !h4.empty() && 3 * d4 <= kTraceStackEntries) {
+      if (S.root == 0 && !h4.empty() && 3 * d4 <= kTraceStackEntries) {   // both walks start at node 0
         BVH4Node* n4 = (BVH4Node*)c->scene_mem.get("nodes4", sizeof(BVH4Node) * h4.size(), c->st, false);
         hipck(hipMemcpyAsync(n4, h4.data(), sizeof(BVH4Node) * h4.size(), hipMemcpyHostToDevice, c->st), "nodes4");
         S.nodes4 = to_global(n4);

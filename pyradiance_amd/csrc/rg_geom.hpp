@@ -22,8 +22,8 @@ struct alignas(16) BVHNode {
 
 // Four child boxes per node, coordinates grouped by axis: one 128-byte fetch
 // tests four children.  Built from the binary tree by pulling each child's
-// children up (a leaf child stays as it is), so its child refs are binary
-// node indices or leaf refs; unused slots hold INT32_MIN.
+// children up (a leaf child stays as it is); child refs are indices into the
+// four-wide array (root 0) or leaf refs; unused slots hold INT32_MIN.
 struct alignas(16) BVH4Node {
   float lox[4], loy[4], loz[4];
   float hix[4], hiy[4], hiz[4];
@@ -37,7 +37,7 @@ struct DevScene {
   const RG_GLOBAL Material* mats;
   const RG_GLOBAL Source* srcs;
   const RG_GLOBAL BVHNode* nodes;
-  const RG_GLOBAL BVH4Node* nodes4;    // four-wide form of `nodes`, indexed alike; null: none
+  const RG_GLOBAL BVH4Node* nodes4;    // four-wide form of `nodes` (root 0); null: none
   const RG_GLOBAL int32_t* leaf_prim;   // sorted leaf position -> primitive index
   const RG_GLOBAL int32_t* big;         // primitives kept out of the BVH, tested first
   int32_t nbig;

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <random>
+#include <tuple>
 #include <vector>
 
 #include "rg_bvh.hpp"
@@ -126,11 +127,12 @@ extern "C" int check_sah(int n, int nrays, unsigned seed, int max_depth, int clu
   return bad;
 }
 
-// Four-wide form of the same tree (rg::collapse_bvh4): every slot of node i
-// is a grandchild of binary node i, or its leaf child, with that child's box;
-// a walk of the four-wide tree reaches every primitive once and finds the
-// same nearest box entry as brute force; the four-wide depth is at most
-// half the binary depth, rounded up, plus one.  Returns 0 or a bit mask.
+// Four-wide form of the same tree (rg::collapse_bvh4): each four-wide node,
+// met in a walk from the root alongside the binary tree, holds the binary
+// node's grandchildren (or its leaf child) with their boxes in slot order;
+// every four-wide node is reached exactly once and numbered after its
+// parent; the walk reaches every primitive once and finds the same nearest
+// box entry as brute force.  Returns 0 or a bit mask.
 extern "C" int check_bvh4(int n, int nrays, unsigned seed, int max_depth, int clustered, int* depth4_out) {
   std::mt19937 rng(seed);
   std::uniform_real_distribution<float> U(0.0f, 1.0f);
@@ -150,49 +152,50 @@ extern "C" int check_bvh4(int n, int nrays, unsigned seed, int max_depth, int cl
   std::vector<rg::BVH4Node> q;
   int d4 = rg::collapse_bvh4(nodes, root, q);
   if (depth4_out) *depth4_out = d4;
+  if (n < 2) return (d4 == 0 && q.empty()) ? 0 : 2;
   int bad = 0;
-  if (q.size() != nodes.size()) return 1;
-  if (n < 2) return d4 == 0 ? 0 : 2;
-  // slot contents
-  for (size_t i = 0; i < nodes.size(); ++i) {
-    const BVHNode& N = nodes[i];
-    std::vector<std::pair<int32_t, const float*>> want;   // (ref, lo box) with hi box after it
-    std::vector<const float*> wanthi;
+  // parallel walk: binary node b <-> four-wide node j
+  std::vector<int> hits(q.size(), 0), seen(n, 0);
+  int deep = 0;
+  std::vector<std::tuple<int32_t, int32_t, int>> st{{root, 0, 1}};
+  while (!st.empty()) {
+    auto [b, j, d] = st.back();
+    st.pop_back();
+    if (j < 0 || j >= (int)q.size()) { bad |= 1; continue; }
+    hits[(size_t)j]++;
+    deep = std::max(deep, d);
+    const BVHNode& N = nodes[(size_t)b];
+    int32_t wref[4];
+    const float *wlo[4], *whi[4];
+    int k = 0;
     for (int side = 0; side < 2; ++side) {
       int32_t ref = side ? N.right : N.left;
       if (ref >= 0) {
         const BVHNode& Cn = nodes[(size_t)ref];
-        want.push_back({Cn.left, Cn.lmin}); wanthi.push_back(Cn.lmax);
-        want.push_back({Cn.right, Cn.rmin}); wanthi.push_back(Cn.rmax);
+        wref[k] = Cn.left; wlo[k] = Cn.lmin; whi[k] = Cn.lmax; ++k;
+        wref[k] = Cn.right; wlo[k] = Cn.rmin; whi[k] = Cn.rmax; ++k;
       } else {
-        want.push_back({ref, side ? N.rmin : N.lmin}); wanthi.push_back(side ? N.rmax : N.lmax);
+        wref[k] = ref; wlo[k] = side ? N.rmin : N.lmin; whi[k] = side ? N.rmax : N.lmax; ++k;
       }
     }
-    const rg::BVH4Node& Q = q[i];
-    for (int k = 0; k < 4; ++k) {
-      if (k >= (int)want.size()) { if (Q.child[k] != INT32_MIN) bad |= 4; continue; }
-      if (Q.child[k] != want[k].first) bad |= 4;
-      const float got_lo[3] = {Q.lox[k], Q.loy[k], Q.loz[k]}, got_hi[3] = {Q.hix[k], Q.hiy[k], Q.hiz[k]};
+    const rg::BVH4Node& Q = q[(size_t)j];
+    for (int s2 = 0; s2 < 4; ++s2) {
+      if (s2 >= k) { if (Q.child[s2] != INT32_MIN) bad |= 4; continue; }
+      const float got_lo[3] = {Q.lox[s2], Q.loy[s2], Q.loz[s2]}, got_hi[3] = {Q.hix[s2], Q.hiy[s2], Q.hiz[s2]};
       for (int a = 0; a < 3; ++a)
-        if (got_lo[a] != want[k].second[a] || got_hi[a] != wanthi[k][a]) bad |= 8;
+        if (got_lo[a] != wlo[s2][a] || got_hi[a] != whi[s2][a]) bad |= 8;
+      if (wref[s2] < 0) {
+        if (Q.child[s2] != wref[s2]) bad |= 4;
+        else if (~wref[s2] < n) seen[~wref[s2]]++;
+        else bad |= 16;
+      } else {
+        if (Q.child[s2] <= j) bad |= 4;                 // children numbered after their parent
+        st.push_back({wref[s2], Q.child[s2], d + 1});
+      }
     }
   }
-  // every primitive once; depth bound
-  std::vector<int> seen(n, 0);
-  std::vector<std::pair<int32_t, int>> st{{root, 1}};
-  int deep = 0;
-  while (!st.empty()) {
-    auto [v, d] = st.back();
-    st.pop_back();
-    deep = std::max(deep, d);
-    for (int k = 0; k < 4; ++k) {
-      int32_t c = q[(size_t)v].child[k];
-      if (c == INT32_MIN) continue;
-      if (c >= 0) st.push_back({c, d + 1});
-      else if (~c < n) seen[~c]++;
-      else bad |= 16;
-    }
-  }
+  for (size_t j = 0; j < q.size(); ++j)
+    if (hits[j] != 1) bad |= 2;
   for (int i = 0; i < n; ++i)
     if (seen[i] != 1) bad |= 16;
   if (deep != d4) bad |= 32;
@@ -210,7 +213,7 @@ extern "C" int check_bvh4(int n, int nrays, unsigned seed, int max_depth, int cl
         if (tn < best || (tn == best && i < bi)) { best = tn; bi = i; }
       }
     }
-    std::vector<int> stk{root};
+    std::vector<int> stk{0};
     while (!stk.empty()) {
       const rg::BVH4Node& Q = q[(size_t)stk.back()];
       stk.pop_back();
