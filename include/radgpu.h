@@ -92,6 +92,13 @@ int rg_trace(rg_ctx *ctx, const double *rays, long long n, int flags, double *ou
 int rg_contrib(rg_ctx *ctx, const double *rays, long long nrec, int accum, int flags,
                const rg_modifier *mods, int nmods, double *out);
 
+/* rg_contrib with the rays and the output in device memory of the context's
+ * device (the in-process manager's resident buffers, rt/RcontribSimulManager
+ * get_output_array, src/binding/radiance_ext.cpp:346-362): no host copies;
+ * the call returns when the output is written. */
+int rg_contrib_device(rg_ctx *ctx, const double *d_rays, long long nrec, int accum, int flags,
+                      const rg_modifier *mods, int nmods, double *d_out);
+
 /* Batched rcontrib: the records of nctx scenes (one committed context each,
  * same device) traced in one wavefront.  rays holds scene 0's nrec[0]*accum
  * rays, then scene 1's, ...; out holds the records in the same order.

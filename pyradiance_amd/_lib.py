@@ -23,7 +23,7 @@ EXPORTS = (
     "rg_set_profiling", "rg_set_bvh_builder", "rg_set_tuning", "rg_last_stats", "rg_dcsky", "rg_dcsky_device",
     "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint", "rg_first_hit",
     "rg_sky_create", "rg_sky_destroy", "rg_sky_error", "rg_sky_shape", "rg_sky_site", "rg_sky_suns",
-    "rg_sky_matrix", "rg_sky_matrix_device", "rg_trace_view", "rg_format_ascii",
+    "rg_sky_matrix", "rg_sky_matrix_device", "rg_trace_view", "rg_format_ascii", "rg_contrib_device",
 )
 
 BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
@@ -129,6 +129,7 @@ def lib() -> C.CDLL:
     L.rg_default_params_rcontrib.restype = None
     L.rg_trace.argtypes = [vp, dp, C.c_longlong, C.c_int, dp]
     L.rg_contrib.argtypes = [vp, dp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, dp]
+    L.rg_contrib_device.argtypes = [vp, vp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, vp]
     L.rg_contrib_batch.argtypes = [C.POINTER(vp), C.c_int, dp, C.POINTER(C.c_longlong), C.c_int, C.c_int,
                                    C.POINTER(RgModifier), C.c_int, dp]
     L.rg_trace_batch.argtypes = [C.POINTER(vp), C.c_int, dp, C.POINTER(C.c_longlong), C.c_int, dp]
@@ -266,6 +267,15 @@ class Context:
         out = np.zeros((nrec, ncol))
         self._check(self._L.rg_contrib(self._h, _dptr(rays), nrec, accum, flags, arr, len(mods), _dptr(out)))
         return out
+
+    def contrib_device(self, rays_ptr: int, nrec: int, mods: list, out_ptr: int, accum: int = 1,
+                       flags: int = 0) -> None:
+        """rg_contrib on device buffers: rays_ptr holds nrec*accum x 6 doubles,
+        out_ptr receives nrec x (3 * sum nbins) doubles (device addresses of
+        this context's device, e.g. torch tensors' data_ptr())."""
+        arr = (RgModifier * len(mods))(*mods)
+        self._check(self._L.rg_contrib_device(self._h, C.c_void_p(rays_ptr), nrec, accum, flags, arr, len(mods),
+                                              C.c_void_p(out_ptr)))
 
     def contrib_adjoint(self, rays: np.ndarray, mods: list, dlde: np.ndarray, lcol: np.ndarray, mats: list,
                         accum: int = 1, flags: int = 0) -> np.ndarray:

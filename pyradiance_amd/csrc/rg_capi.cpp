@@ -286,10 +286,16 @@ struct AdjSpec {
   double* grad;
 };
 
+__global__ void k_scale_out(double* v, size_t n, double s) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= s;
+}
+
+// device_io: `rays` and `out` are device memory of the context's device
 static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int flags, int mode,
                 const std::vector<BinSpec>& bins, int ncols, double* out,
                 const std::vector<DevScene>* batch = nullptr, const std::vector<uint32_t>* batch_start = nullptr,
-                const AdjSpec* adj = nullptr) {
+                const AdjSpec* adj = nullptr, bool device_io = false) {
   if (!c->committed) throw std::runtime_error("scene not committed");
   if (nrays < 0 || nrays > 0x7fffffffLL) throw std::runtime_error("bad ray count");
   if (accum < 1) throw std::runtime_error("bad accumulation count");
@@ -307,7 +313,8 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
     hipck(hipMemcpyAsync(d_bins, bins.data(), sizeof(BinSpec) * bins.size(), hipMemcpyHostToDevice, c->st), "bins");
   }
   if (nrays && rays)      // rays == nullptr: already generated in the "rays" buffer on the device
-    hipck(hipMemcpyAsync(d_rays, rays, sizeof(double) * 6 * nrays, hipMemcpyHostToDevice, c->st), "rays");
+    hipck(hipMemcpyAsync(d_rays, rays, sizeof(double) * 6 * nrays,
+                         device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st), "rays");
   RunCfg cfg;
   cfg.p = c->params;
   cfg.mode = mode;
@@ -383,7 +390,13 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   uint32_t herr = 0;
   hipck(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c->st), "err");
   hipck(hipMemcpyAsync(&c->last_nrays, d_nr, 8, hipMemcpyDeviceToHost, c->st), "nrays");
-  if (outn && out) hipck(hipMemcpyAsync(out, d_out, sizeof(double) * outn, hipMemcpyDeviceToHost, c->st), "out");
+  if (outn && out && device_io) {
+    if (accum > 1)
+      k_scale_out<<<(unsigned)((outn + 255) / 256), 256, 0, c->st>>>(d_out, outn, 1.0 / accum);
+    hipck(hipMemcpyAsync(out, d_out, sizeof(double) * outn, hipMemcpyDeviceToDevice, c->st), "out");
+  } else if (outn && out) {
+    hipck(hipMemcpyAsync(out, d_out, sizeof(double) * outn, hipMemcpyDeviceToHost, c->st), "out");
+  }
   if (adj)
     hipck(hipMemcpyAsync(adj->grad, d_grad, sizeof(double) * 5 * adj->nrows, hipMemcpyDeviceToHost, c->st), "grad");
   hipck(hipStreamSynchronize(c->st), "render");
@@ -393,7 +406,7 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   if (herr & ERR_PATH) throw std::runtime_error("adjoint: more than 7 differentiable vertices on one path");
   if (herr & ERR_ADJ_DIRECT)
     throw std::runtime_error("adjoint: a tracked modifier is reached by shadow rays (direct light is not differentiated)");
-  if (!out) return;
+  if (!out || device_io) return;
   if (accum > 1) {
     double s = 1.0 / accum;
     for (size_t i = 0; i < outn; ++i) out[i] *= s;
@@ -506,6 +519,26 @@ int rg_contrib(rg_ctx* c, const double* rays, long long nrec, int accum, int fla
     c->dscene = S;
     try {
       run(c, rays, nrec * (long long)accum, accum, flags, 1, bins, ncols, out);
+    } catch (...) {
+      c->dscene = saved;
+      throw;
+    }
+    c->dscene = saved;
+  });
+}
+
+int rg_contrib_device(rg_ctx* c, const double* d_rays, long long nrec, int accum, int flags, const rg_modifier* mods,
+                      int nmods, double* d_out) {
+  RG_TRY(c, {
+    if (nrec > 0 && (!d_rays || !d_out)) throw std::runtime_error("null device buffer");
+    std::unordered_map<std::string, int> slot;
+    int ncols = 0;
+    std::vector<BinSpec> bins = bin_specs(mods, nmods, slot, ncols);
+    DevScene S = contrib_scene(c, slot, c->st);
+    DevScene saved = c->dscene;
+    c->dscene = S;
+    try {
+      run(c, d_rays, nrec * (long long)accum, accum, flags, 1, bins, ncols, d_out, nullptr, nullptr, nullptr, true);
     } catch (...) {
       c->dscene = saved;
       throw;

@@ -257,6 +257,28 @@ def test_config2_full_size_properties(L):
     assert n1 > 2500 * 4096
 
 
+def test_contrib_on_device_buffers_matches_host_buffers(L):
+    """rg_contrib_device (rays and output resident on the GPU) gives the same
+    records as rg_contrib for the same seed, also when records average
+    several rays."""
+    import torch
+    _say("contrib on device buffers")
+    rad, sens = scenes.config2(2)
+    c = _ctx(L, rad)
+    c.set_params(_rc_params(L, ambounce=2, ambdiv=256, minweight=1e-3, seed=5))
+    mods = _sky_mods(L)
+    for accum in (1, 3):
+        rays = np.repeat(sens[:300], accum, axis=0)
+        host = c.contrib(rays, mods, accum, L.RT_IMM_IRRAD)
+        n_host = c.last_nrays()
+        d_rays = torch.from_numpy(rays).cuda()
+        d_out = torch.full(host.shape, -1.0, dtype=torch.float64, device="cuda")
+        c.contrib_device(d_rays.data_ptr(), rays.shape[0] // accum, mods, d_out.data_ptr(), accum, L.RT_IMM_IRRAD)
+        assert c.last_nrays() == n_host
+        assert np.allclose(d_out.cpu().numpy(), host, rtol=1e-12, atol=1e-15)
+    c.close()
+
+
 def test_config5_full_size_properties(L):
     """Full config-5 scene (10-storey office, 2,495,700 quads = 5 M
     triangles) under a Reinhart MF:4 sky (2305 + 1 bins), -ab 5 -ad 8192
