@@ -737,6 +737,11 @@ def main():
         step()
         log(f"warmup {i} done: {ctx.last_nrays()} rays")
     times, rays = [], []
+    # HIP events around every traversal and shading launch, on the context's
+    # stream, during the timed steps (the renderer synchronises once per
+    # window anyway, to size the next one, so the events add no waits)
+    ctx.set_profiling(1)
+    st_t = {"trace_ms": 0.0, "shade_ms": 0.0, "launches": 0.0, "rays": 0.0}
     torch.cuda.synchronize()
     for i in range(args.steps):
         t0 = time.perf_counter()
@@ -744,7 +749,12 @@ def main():
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         rays.append(ctx.last_nrays())
+        st = ctx.last_stats()
+        for k in st_t:
+            st_t[k] += st[k]
         log(f"step {i}: {times[-1] * 1e3:.1f} ms, {rays[-1]} rays")
+    ctx.set_profiling(0)
+    nsteps = max(args.steps, 1)
     ms = 1e3 * float(np.mean(times))
     value = float(np.sum(rays)) / float(np.sum(times)) / 1e6
     out = d_out.cpu().numpy()
@@ -754,10 +764,7 @@ def main():
     ctx.contrib(sensors, mods, 1, flags)
     host_rate = ctx.last_nrays() / (time.perf_counter() - t0) / 1e6
 
-    # roofline of the generation kernel: event timing run, then a counting run
-    ctx.set_profiling(1)
-    step()
-    st_t = ctx.last_stats()
+    # node and primitive counts per ray: a separate counting run
     ctx.set_profiling(2)
     step()
     st_c = ctx.last_stats()
@@ -768,7 +775,8 @@ def main():
     avg_launch_ms = st_t["trace_ms"] / max(st_t["launches"], 1)
     bytes_per_launch = per_ray * st_t["rays"] / max(st_t["launches"], 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
-    log(f"k_trace: {st_t['launches']:.0f} launches, {st_t['trace_ms']:.1f} ms total; k_shade {st_t['shade_ms']:.1f} ms; "
+    log(f"k_trace: {st_t['launches'] / nsteps:.0f} launches, {st_t['trace_ms'] / nsteps:.1f} ms per step; "
+        f"k_shade {st_t['shade_ms'] / nsteps:.1f} ms; "
         f"{st_c['nodes'] / nr:.1f} nodes, {st_c['prims'] / nr:.1f} prims per ray; {per_ray:.0f} B/ray")
 
     if args.copy_calibration:
@@ -847,7 +855,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_trace", "avg_launch_ms": avg_launch_ms,
-                     "shade_kernel_ms_per_step": st_t["shade_ms"], "trace_kernel_ms_per_step": st_t["trace_ms"],
+                     "shade_kernel_ms_per_step": st_t["shade_ms"] / nsteps,
+                     "trace_kernel_ms_per_step": st_t["trace_ms"] / nsteps,
+                     "launches_timed": int(st_t["launches"]),
                      "bytes_per_ray": per_ray, "nodes_per_ray": st_c["nodes"] / nr, "walk": walk,
                      "prims_per_ray": st_c["prims"] / nr},
         "dcsky": dcs,
