@@ -423,3 +423,43 @@ def test_batched_rtrace_config1_sun_positions(L):
     for seed in range(3):
         ref = np.load(GOLDEN / f"c1_seed{seed}.npz")["values"]
         assert np.abs(res[seed] - ref).max() <= 1e-4 * ref.max()
+
+
+def test_empty_and_smallest_inputs(L, tmp_path):
+    """Zero rays, zero records and 1 x 1 x 1 products are valid calls: empty
+    results, no error (the reference programs print only their header for an
+    empty ray stream)."""
+    import torch
+    from pyradiance_amd import rt
+    _say("empty and smallest inputs")
+    rad, _ = scenes.config2(0, nboxes=10)
+    c = _ctx(L, rad)
+    assert c.trace(np.zeros((0, 6)), L.RT_IMM_IRRAD).shape == (0, 3)
+    t, nrm, prim = c.first_hit(np.zeros((0, 6)))
+    assert t.shape == (0,) and prim.shape == (0,)
+    mods = _sky_mods(L)
+    c.set_params(_rc_params(L, ambounce=1, ambdiv=64, seed=1))
+    assert c.contrib(np.zeros((0, 6)), mods, 1, L.RT_IMM_IRRAD).shape == (0, 438)
+    assert c.last_nrays() == 0
+    c.contrib_device(0, 0, mods, 0, 1, L.RT_IMM_IRRAD)
+    assert c.last_nrays() == 0
+    # one ray, one record, records of several rays
+    one = np.array([[3.0, 2.0, 0.8, 0.0, 0.0, 1.0]])
+    assert c.trace(one, L.RT_IMM_IRRAD).shape == (1, 3)
+    r3 = c.contrib(np.repeat(one, 3, axis=0), mods, 3, L.RT_IMM_IRRAD)
+    assert r3.shape == (1, 438) and (r3 >= 0).all()
+    c.close()
+    # the Python entry points on an empty ray stream
+    (tmp_path / "room.rad").write_text(rad)
+    out = rt.rtrace(b"", tmp_path / "room.rad", params=["-ab", "0"])
+    assert out.endswith(b"\n\n") and out.startswith(b"#?RADIANCE")
+    assert rt.rtrace(b"", tmp_path / "room.rad", header=False) == b""
+    # 1 x 1 x 1 and 1 x K x 1 products
+    A = np.full((1, 1, 3), 2.0, dtype=np.float32)
+    B = np.full((1, 1, 3), 3.0, dtype=np.float32)
+    for mode in (0, 1, 2, 3):
+        assert np.allclose(L.dcsky(A, B, mode), 6.0, rtol=1e-5)
+    A = np.ones((1, 300, 3), dtype=np.float32)
+    B = np.ones((300, 1, 3), dtype=np.float32)
+    for mode in (1, 3):
+        assert np.allclose(L.dcsky(A, B, mode), 300.0, rtol=1e-5)
