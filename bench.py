@@ -687,7 +687,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the forward pass shards "
-              "by sensor rows with no communication, see DESIGN.md)", file=sys.stderr)
+              "by sensor rows with no communication; this measurement runs on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     if args.only:
