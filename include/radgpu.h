@@ -162,7 +162,12 @@ int rg_set_profiling(rg_ctx *ctx, int level);
  * are at most this many, else all go into the BVH; "wide" 1 (default) the
  * persistent walk fetches four child boxes per node when the scene has a
  * four-wide tree (host SAH trees shallow enough for the traversal stack),
- * 0 the binary walk.
+ * 0 the binary walk; "shadow_cache" 1 (default) shadow rays are first
+ * checked against the blocker last recorded for their source and cell, as
+ * the reference's shadow cache does (rt/srcobstr.c:28-339, filled by cast
+ * rays at the first call after a commit; those rays count once, in the call
+ * that first shadow-tests their source, as in the reference's counter), 0
+ * every shadow ray is traced.
  * Results do not depend on these; only the time does (the walks visit
  * primitives in different orders, which only matters for two surfaces
  * within FTINY of each other along a ray, where the reference's own choice

@@ -19,6 +19,11 @@
  *   m_light + macros rt/source.c:683-793
  *   direct           rt/source.c:398-557 (every source tested: -dt 0)
  *   srcray/nextssamp rt/source.c:219-257, rt/srcsamp.c:36-144 (one partition)
+ *   shadow cache     rt/srcobstr.c:28-339 (castshadow, initobscache,
+ *                    srcobstructp, srcblocker, srcblocked; SHADCACHE 20 of
+ *                    rt/source.h:19-20), with the cube of ot/oconv.c:120-139
+ *                    and ot/bbox.c:23-140; it decides which shadow rays are
+ *                    traced (and so the ray count), never a value
  *   sourcehit        rt/source.c:317-377
  *   marksources      rt/source.c:47-142, ssetsrc etc. rt/srcsupp.c:60-260
  *   multambient      rt/ambient.c:229-330 (-aa 0 branch)
@@ -100,6 +105,7 @@ struct ro_scene {
   OSrc *s;
   int ns;
   double *vpool;
+  double cuorg[3], cusize;   /* the octree's cube as oconv sets it */
 };
 
 /* ---- counter-based random numbers (same generator as the GPU path) ---- */
@@ -318,6 +324,55 @@ static void setflatss(OSrc *S) {
   vcross(S->ss[1], S->snorm, S->ss[0]);
 }
 
+/* bounding cube of the surfaces as oconv computes it (ot/oconv.c:120-139,
+ * add2bbox ot/bbox.c:23-95, circle2bbox :117-140), margin 10*FTINY */
+static void box_point(double *lo, double *hi, const double *p) {
+  int k;
+  for (k = 0; k < 3; k++) {
+    if (p[k] < lo[k]) lo[k] = p[k];
+    if (p[k] > hi[k]) hi[k] = p[k];
+  }
+}
+static void box_circle(double *lo, double *hi, const double *c, const double *n, double rad) {
+  int k;
+  for (k = 0; k < 3; k++) {
+    double r = sqrt(1. - n[k] * n[k]), d = c[k] + r * rad;
+    if (d > hi[k]) hi[k] = d;
+    d = c[k] - r * rad;
+    if (d < lo[k]) lo[k] = d;
+  }
+}
+static void scene_cube(ro_scene *s) {
+  double lo[3] = {FHUGE, FHUGE, FHUGE}, hi[3] = {-FHUGE, -FHUGE, -FHUGE};
+  int i, k;
+  for (i = 0; i < s->n; i++) {
+    const OObj *o = &s->o[i];
+    if (!is_surface(o->type) || o->type == T_SOURCE || !o->ok) continue;
+    if (o->type == T_POLY) {
+      for (k = 0; k < o->nv; k++) box_point(lo, hi, o->va + 3 * k);
+    } else if (o->type == T_SPHERE || o->type == T_BUBBLE) {
+      for (k = 0; k < 3; k++) {
+        double v[3] = {o->f[0], o->f[1], o->f[2]};
+        v[k] -= o->f[3];
+        box_point(lo, hi, v);
+        v[k] += 2.0 * o->f[3];
+        box_point(lo, hi, v);
+      }
+    } else {
+      double p1[3];
+      for (k = 0; k < 3; k++) p1[k] = o->type == T_RING ? o->p0[k] : o->p0[k] + o->ad[k] * o->al;
+      if (o->type != T_RING) box_circle(lo, hi, o->p0, o->ad, o->r0);
+      box_circle(lo, hi, p1, o->ad, o->r1);
+    }
+  }
+  s->cusize = 0;
+  if (lo[0] > hi[0]) return;
+  for (k = 0; k < 3; k++) { lo[k] -= 10 * FTINY; hi[k] += 10 * FTINY; }
+  for (k = 0; k < 3; k++)
+    if (hi[k] - lo[k] > s->cusize) s->cusize = hi[k] - lo[k];
+  for (k = 0; k < 3; k++) s->cuorg[k] = (hi[k] + lo[k] - s->cusize) * .5;
+}
+
 ro_scene *ro_scene_parse(const char *text, char *err, int errlen) {
   ro_scene *s = (ro_scene *)calloc(1, sizeof(ro_scene));
   const char *p = text;
@@ -470,6 +525,7 @@ ro_scene *ro_scene_parse(const char *text, char *err, int errlen) {
     s->s = (OSrc *)realloc(s->s, sizeof(OSrc) * (s->ns + 1));
     s->s[s->ns++] = S;
   }
+  scene_cube(s);
   return s;
 fail:
   ro_scene_free(s);
@@ -519,6 +575,7 @@ typedef struct {
   double *out;         /* current record row */
   unsigned long long nrays;
   int lamb;            /* -1: use built-in Lambertian */
+  struct OCache *cache;  /* shadow cache per source (one run = one process) */
 } Ctx;
 
 static const double LAMBF[5] = {PI, PI, PI, 0, 0};
@@ -739,6 +796,225 @@ static int localhit(Ctx *c, ORay *r) {
     hit_obj(c->s, i, r);
   }
   return r->ro >= 0;
+}
+
+
+/* ---- shadow cache (rt/srcobstr.c) ----
+ * The reference keeps, per light source, the last opaque object found to
+ * block it in each of SHADCACHE^2-ish cells (distant sources: cells over
+ * the cube face the source shines on, indexed by where the ray origin
+ * projects along the source direction; local sources: cells over the
+ * directions of approach).  At a source's first shadow test it fills the
+ * cells by casting a grid of rays (castshadow), which the ray counter
+ * counts; afterwards a shadow ray whose cell's blocker intersects it is not
+ * traced, and a traced shadow ray found blocked records its blocker. */
+#define SHADCACHE 20
+typedef struct OCache {
+  int init, len, ax;
+  double o[3], e1, e2, u[3], v[3];
+  int *obs;            /* object index per cell, -1: none */
+} OCache;
+
+static int src_cacheable(const OSrc *S) { return !(S->flags & (SSKIP | SPROX)); }
+static void cache_init(Ctx *c, int sn);
+
+/* srcobstructp(): cell of ray r aimed at source sn, -1 outside */
+static int cache_cell(const Ctx *c, int sn, const ORay *r) {
+  const OSrc *S = &c->s->s[sn];
+  const OCache *C = &c->cache[sn];
+  int ondx;
+  if (S->flags & SDISTANT) {
+    int ax = C->ax, ax1 = (ax + 1) % 3, ax2 = (ax + 2) % 3;
+    double t = (C->o[ax] - r->rorg[ax]) / S->sloc[ax];
+    if (t <= FTINY) return -1;
+    ondx = 2 * SHADCACHE * (int)(2 * SHADCACHE * C->e1 * (r->rorg[ax1] + t * S->sloc[ax1] - C->o[ax1]));
+    ondx += (int)(2 * SHADCACHE * C->e2 * (r->rorg[ax2] + t * S->sloc[ax2] - C->o[ax2]));
+    if ((ondx < 0) | (ondx >= 4 * SHADCACHE * SHADCACHE)) return -1;
+  } else if (S->flags & SFLAT) {
+    double sd[3], sd0m, sd1m;
+    sd[0] = -DOT(r->rdir, C->u);
+    sd[1] = -DOT(r->rdir, C->v);
+    sd[2] = -DOT(r->rdir, S->snorm);
+    if (sd[2] < 0) return -1;
+    sd0m = fabs(sd[0]);
+    sd1m = fabs(sd[1]);
+    if (sd[2] >= sd0m && sd[2] >= sd1m) {
+      ondx = SHADCACHE * (int)(SHADCACHE * (.5 - FTINY) * (1. + sd[0] / sd[2]));
+      ondx += (int)(SHADCACHE * (.5 - FTINY) * (1. + sd[1] / sd[2]));
+    } else if (sd0m >= sd1m) {
+      ondx = SHADCACHE * SHADCACHE;
+      if (sd[0] < 0) ondx += ((SHADCACHE + 1) >> 1) * SHADCACHE;
+      ondx += SHADCACHE * (int)(SHADCACHE * (.5 - FTINY) * (1. - sd[2] / sd0m));
+      ondx += (int)(SHADCACHE * (.5 - FTINY) * (1. + sd[1] / sd0m));
+    } else {
+      ondx = SHADCACHE * SHADCACHE + ((SHADCACHE + 1) >> 1) * SHADCACHE * 2;
+      if (sd[1] < 0) ondx += ((SHADCACHE + 1) >> 1) * SHADCACHE;
+      ondx += SHADCACHE * (int)(SHADCACHE * (.5 - FTINY) * (1. - sd[2] / sd1m));
+      ondx += (int)(SHADCACHE * (.5 - FTINY) * (1. + sd[0] / sd1m));
+    }
+  } else {
+    int ax = 0, ax1, ax2, k;
+    double amax = 0;
+    for (k = 3; k--;)
+      if (fabs(r->rdir[k]) > amax) { amax = fabs(r->rdir[k]); ax = k; }
+    ax1 = (ax + 1) % 3;
+    ax2 = (ax + 2) % 3;
+    ondx = 2 * SHADCACHE * SHADCACHE * ax;
+    if (r->rdir[ax] < 0) ondx += SHADCACHE * SHADCACHE;
+    ondx += SHADCACHE * (int)(SHADCACHE * (.5 - FTINY) * (1. + r->rdir[ax1] / amax));
+    ondx += (int)(SHADCACHE * (.5 - FTINY) * (1. + r->rdir[ax2] / amax));
+  }
+  return ondx;
+}
+
+static int mat_opaque(const ro_scene *s, int oi) {
+  int m = s->o[oi].mat, t;
+  if (m < 0) return 0;
+  t = s->o[m].type;
+  return t == M_PLASTIC || t == M_METAL || t == M_LIGHT || t == M_GLOW;
+}
+
+/* srcblocker(): record what blocked ray r (aimed at r->rsrc); 1 if recorded */
+static int src_blocker(Ctx *c, const ORay *r) {
+  int cell;
+  if (r->ro < 0 || r->rsrc < 0 || c->s->s[r->rsrc].obj == r->ro) return 0;
+  if (!mat_opaque(c->s, r->ro)) return 0;
+  if (!src_cacheable(&c->s->s[r->rsrc])) return 1;   /* written to a dummy cell */
+  if (!c->cache[r->rsrc].init) cache_init(c, r->rsrc);
+  cell = cache_cell(c, r->rsrc, r);
+  if (cell >= 0) c->cache[r->rsrc].obs[cell] = r->ro;
+  return 1;
+}
+
+/* castshadow() */
+static void cast_shadow(Ctx *c, int sn, const double *org, const double *dir) {
+  ORay rt;
+  int k;
+  memset(&rt, 0, sizeof rt);
+  memcpy(rt.rorg, org, sizeof rt.rorg);
+  memcpy(rt.rdir, dir, sizeof rt.rdir);
+  rt.crtype = rt.rtype = PRIMARY;
+  rt.rsrc = -1;
+  while (localhit(c, &rt)) {
+    ORay rt1 = rt;
+    rt1.crtype |= rt1.rtype = RSHADOW;
+    for (k = 0; k < 3; k++) { rt1.rdir[k] = -rt.rdir[k]; rt1.rorg[k] = rt.rop[k] - rt.rdir[k]; }
+    rt1.rod = -rt.rod;
+    rt1.rot = 1.;
+    rt1.rsrc = sn;
+    if (src_blocker(c, &rt1)) return;
+    for (k = 0; k < 3; k++) rt.rorg[k] = rt.rop[k] + rt.rdir[k] * 10. * FTINY;
+  }
+}
+
+/* initobscache() */
+static void cache_init(Ctx *c, int sn) {
+  const OSrc *S = &c->s->s[sn];
+  OCache *C = &c->cache[sn];
+  double rorg[3], rdir[3], d;
+  int i, j, k;
+  C->init = 1;
+  if (S->flags & SDISTANT) C->len = 4 * SHADCACHE * SHADCACHE;
+  else if (S->flags & SFLAT) C->len = SHADCACHE * SHADCACHE * 3 + (SHADCACHE & 1) * SHADCACHE * 4;
+  else C->len = SHADCACHE * SHADCACHE * 6;
+  C->obs = (int *)malloc(sizeof(int) * C->len);
+  for (i = 0; i < C->len; i++) C->obs[i] = -1;
+  if (S->flags & SDISTANT) {
+    double amax = 0;
+    int ax = 0, ax1, ax2;
+    for (k = 3; k--;)
+      if (fabs(S->sloc[k]) > amax) { amax = fabs(S->sloc[k]); ax = k; }
+    C->ax = ax;
+    ax1 = (ax + 1) % 3;
+    ax2 = (ax + 2) % 3;
+    memcpy(C->o, c->s->cuorg, sizeof C->o);
+    if (S->sloc[ax] > 0) C->o[ax] += c->s->cusize;
+    if (S->sloc[ax1] < 0) C->o[ax1] += c->s->cusize * S->sloc[ax1] / amax;
+    if (S->sloc[ax2] < 0) C->o[ax2] += c->s->cusize * S->sloc[ax2] / amax;
+    C->e1 = 1. / (c->s->cusize * (1. + fabs(S->sloc[ax1]) / amax));
+    C->e2 = 1. / (c->s->cusize * (1. + fabs(S->sloc[ax2]) / amax));
+  } else if (S->flags & SFLAT) {
+    memcpy(C->u, S->ss[0], sizeof C->u);
+    vnorm(C->u);
+    vcross(C->v, S->snorm, C->u);
+  }
+  if (sn >= 200) return;   /* MAX2SHADE */
+  if (S->flags & SDISTANT) {
+    int ax1 = (C->ax + 1) % 3, ax2 = (C->ax + 2) % 3;
+    for (k = 0; k < 3; k++) rdir[k] = -S->sloc[k];
+    for (i = 2 * SHADCACHE; i--;)
+      for (j = 2 * SHADCACHE; j--;) {
+        memcpy(rorg, C->o, sizeof rorg);
+        rorg[ax1] += (i + .5) / (2 * SHADCACHE * C->e1);
+        rorg[ax2] += (j + .5) / (2 * SHADCACHE * C->e2);
+        cast_shadow(c, sn, rorg, rdir);
+      }
+  } else if (S->flags & SFLAT) {
+    d = 0.01 * S->srad;
+    for (k = 0; k < 3; k++) rorg[k] = S->sloc[k] + S->snorm[k] * d;
+    for (i = SHADCACHE; i--;)
+      for (j = SHADCACHE; j--;) {
+        d = 2. / SHADCACHE * (i + .5) - 1.;
+        for (k = 0; k < 3; k++) rdir[k] = S->snorm[k] + C->u[k] * d;
+        d = 2. / SHADCACHE * (j + .5) - 1.;
+        for (k = 0; k < 3; k++) rdir[k] += C->v[k] * d;
+        vnorm(rdir);
+        cast_shadow(c, sn, rorg, rdir);
+      }
+    for (int kk = 2; kk--;)
+      for (i = SHADCACHE; i--;)
+        for (j = SHADCACHE >> 1; j--;) {
+          d = 2. / SHADCACHE * (i + .5) - 1.;
+          for (k = 0; k < 3; k++) rdir[k] = kk ? C->u[k] + C->v[k] * d : C->v[k] + C->u[k] * d;
+          d = 1. - 2. / SHADCACHE * (j + .5);
+          for (k = 0; k < 3; k++) rdir[k] += S->snorm[k] * d;
+          vnorm(rdir);
+          cast_shadow(c, sn, rorg, rdir);
+          d = 2. * DOT(rdir, S->snorm);
+          for (k = 0; k < 3; k++) rdir[k] = d * S->snorm[k] - rdir[k];
+          cast_shadow(c, sn, rorg, rdir);
+        }
+  } else {
+    for (int kk = 6; kk--;) {
+      int ax = kk % 3, ax1 = (kk + 1) % 3, ax2 = (kk + 2) % 3;
+      for (i = SHADCACHE; i--;)
+        for (j = SHADCACHE; j--;) {
+          rdir[0] = rdir[1] = rdir[2] = 0.;
+          rdir[ax] = kk < 3 ? 1. : -1.;
+          rdir[ax1] = 2. / SHADCACHE * (i + .5) - 1.;
+          rdir[ax2] = 2. / SHADCACHE * (j + .5) - 1.;
+          vnorm(rdir);
+          d = 1.05 * S->srad;
+          for (k = 0; k < 3; k++) rorg[k] = S->sloc[k] + rdir[k] * d;
+          cast_shadow(c, sn, rorg, rdir);
+        }
+    }
+  }
+}
+
+/* srcblocked(): 1 when the cached blocker of the ray's cell intersects it
+ * (in front of a local source) */
+static int src_blocked(Ctx *c, const ORay *sr) {
+  const OSrc *S = &c->s->s[sr->rsrc];
+  ORay t = *sr;
+  int cell, ob;
+  if (!src_cacheable(S)) return 0;
+  if (!c->cache[sr->rsrc].init) cache_init(c, sr->rsrc);
+  cell = cache_cell(c, sr->rsrc, sr);
+  if (cell < 0 || (ob = c->cache[sr->rsrc].obs[cell]) < 0) return 0;
+  t.ro = -1;
+  t.rot = FHUGE;
+  t.rod = 1.0;
+  if (!hit_obj(c->s, ob, &t)) return 0;
+  if (S->flags & SDISTANT) return 1;
+  return !hit_obj(c->s, S->obj, &t);
+}
+
+static void cache_free(Ctx *c) {
+  int i;
+  if (!c->cache) return;
+  for (i = 0; i < c->s->ns; i++) free(c->cache[i].obs);
+  free(c->cache);
 }
 
 void ro_closest(const ro_scene *s, const double *rays, long n, int *prim, double *t) {
@@ -1046,6 +1322,11 @@ static void direct(Ctx *c, ORay *r, const ND *nd) {
       dom *= S->ss2 / (dist * dist);
       if ((S->flags & SPROX) && dist > S->prox) continue;
     }
+    /* shadow cache check of the source ray (rt/source.c:437-446) */
+    rayorigin(c, &sr, RSHADOW, r, NULL, SLOT_SHADOW + sn);
+    memcpy(sr.rdir, dir, sizeof dir);
+    sr.rsrc = sn;
+    if (src_blocked(c, &sr)) continue;
     dirnorm(c, coef, nd, dir, dom);
     if (!(cmaxd(coef) > 0.0)) continue;
     /* potential: srcvalue() */
@@ -1071,9 +1352,15 @@ static void direct(Ctx *c, ORay *r, const ND *nd) {
     memcpy(sr.rdir, dir, sizeof dir);
     sr.rsrc = sn;
     if (localhit(c, &sr) && (sr.ro != S->obj || c->mode == 1)) {
+      const int hit = sr.ro;
       if (!rayshade(c, &sr, c->s->o[sr.ro].mat)) raytrans(c, &sr);
       trace_contrib(c, &sr);
-      if (cmeand(sr.rcol) <= FTINY) continue;
+      if (cmeand(sr.rcol) <= FTINY) {
+        ORay b = sr;
+        b.ro = hit;
+        src_blocker(c, &b);   /* rt/source.c:512-518 */
+        continue;
+      }
       for (k = 0; k < 3; k++) val[k] = sr.rcol[k] * sr.rcoef[k];
     } else if (c->mode == 1 && (S->flags & SDISTANT)) {
       int si = sourcehit(c, &sr);
@@ -1375,12 +1662,14 @@ int ro_trace(const ro_scene *s, const ro_params *p, const double *rays, long n, 
   c.mode = 0;
   c.irrad = (flags & 16) != 0;
   c.modslot = (int *)calloc(s->n + 1, sizeof(int));
+  c.cache = (OCache *)calloc(s->ns + 1, sizeof(OCache));
   for (i = 0; i < n; i++) {
     c.out = out + 3 * i;
     out[3 * i] = out[3 * i + 1] = out[3 * i + 2] = 0;
     eval_ray(&c, rays + 6 * i, i, i, (flags & 8) != 0);
   }
   free(c.modslot);
+  cache_free(&c);
   if (nrays) *nrays = c.nrays;
   (void)err; (void)errlen;
   return 0;
@@ -1401,6 +1690,7 @@ int ro_contrib(const ro_scene *s, const ro_params *p, const double *rays, long n
   c.col0 = (int *)calloc(nmods + 1, sizeof(int));
   for (k = 0; k < nmods; k++) { c.col0[k] = col; col += mods[k].nbins; }
   c.ncols = 3 * col;
+  c.cache = (OCache *)calloc(s->ns + 1, sizeof(OCache));
   c.modslot = (int *)malloc(sizeof(int) * (s->n + 1));
   for (i = 0; i < s->n; i++) {
     c.modslot[i] = -1;
@@ -1417,6 +1707,7 @@ int ro_contrib(const ro_scene *s, const ro_params *p, const double *rays, long n
   }
   free(c.modslot);
   free(c.col0);
+  cache_free(&c);
   if (nrays) *nrays = c.nrays;
   (void)err; (void)errlen;
   return 0;

@@ -39,6 +39,12 @@ struct rg_ctx {
   int profile = 0;              // 0 off, 1 time k_bounce launches, 2 also count visits
   RenderStats last_stats;
   unsigned long long last_visits[2] = {0, 0};
+  // shadow cache: filled at the first call after a commit (the reference
+  // fills a source's cache at its first shadow test in the process); the
+  // cast rays of a source count once, in the call that first tests it
+  bool shadow_filled = false;
+  std::vector<unsigned long long> cast_count;   // closest-hit queries per source's cast rays
+  std::vector<char> cast_counted;
 };
 
 static thread_local std::string g_create_err;
@@ -170,6 +176,17 @@ int rg_scene_commit(rg_ctx* c) {
     S.nsrcs = (int)cs.srcs.size();
     S.lamb = cs.lamb;
     S.cusize = cs.cusize;
+    S.scache = to_global(upload(c->scene_mem, "scache", cs.scache, c->st));
+    S.shadow_obs = to_global((unsigned long long*)c->scene_mem.get(
+        "shadow_obs", sizeof(unsigned long long) * std::max<int32_t>(cs.ncells, 1), c->st, true));
+    S.src_tested = (RG_GLOBAL uint32_t*)c->scene_mem.get("src_tested", sizeof(uint32_t) * std::max<size_t>(cs.srcs.size(), 1),
+                                               c->st, true);
+    S.shadow_on = 0;
+    upload(c->scene_mem, "cast_rays", cs.cast_rays, c->st);
+    upload(c->scene_mem, "cast_src", cs.cast_src, c->st);
+    c->shadow_filled = false;
+    c->cast_count.assign(cs.srcs.size(), 0);
+    c->cast_counted.assign(cs.srcs.size(), 0);
     // Room-sized primitives (box diagonal > 1/8 of the scene's) stay out of
     // the BVH when there are at most max_big of them (a room's walls): inside
     // the tree their boxes would inflate every ancestor, and each ray tests
@@ -291,11 +308,61 @@ __global__ void k_scale_out(double* v, size_t n, double s) {
   if (i < n) v[i] *= s;
 }
 
+// Fills the context's shadow cache once per commit (rt/srcobstr.c:63-183).
+static void shadow_fill(rg_ctx* c) {
+  if (c->shadow_filled) return;
+  const CompiledScene& cs = c->cs;
+  const uint32_t n = (uint32_t)cs.cast_src.size();
+  unsigned long long* d_cnt =
+      (unsigned long long*)c->arena.get("cast_count", sizeof(unsigned long long) * std::max<size_t>(cs.srcs.size(), 1),
+                                        c->st, true);
+  uint32_t* d_err = (uint32_t*)c->arena.get("cast_err", 8, c->st, true);
+  shadow_cast(c->dscene, (const double*)c->scene_mem.get("cast_rays", sizeof(double) * std::max<size_t>(cs.cast_rays.size(), 1), c->st, false),
+              (const int32_t*)c->scene_mem.get("cast_src", sizeof(int32_t) * std::max<size_t>(n, 1), c->st, false), n,
+              (unsigned long long*)c->dscene.shadow_obs, d_cnt, d_err, c->st);
+  uint32_t herr = 0;
+  if (!cs.srcs.empty())
+    hipck(hipMemcpyAsync(c->cast_count.data(), d_cnt, sizeof(unsigned long long) * cs.srcs.size(), hipMemcpyDeviceToHost,
+                         c->st), "cast counts");
+  hipck(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c->st), "cast err");
+  hipck(hipStreamSynchronize(c->st), "shadow cache");
+  if (herr & ERR_STACK) throw std::runtime_error("BVH traversal stack overflow");
+  c->shadow_filled = true;
+}
+
+// Before a call: the cache in use (or not) and no source tested yet.
+static void shadow_begin(rg_ctx* c, DevScene& S, bool on) {
+  S.shadow_on = 0;
+  if (!on || c->cs.scache.empty()) return;
+  shadow_fill(c);
+  S.shadow_on = 1;
+  hipck(hipMemsetAsync(S.src_tested, 0, sizeof(uint32_t) * c->cs.srcs.size(), c->st), "tested flags");
+}
+
+// After a call: the cast rays of every source tested for the first time
+// since the commit, as the reference's counter counts them.
+static unsigned long long shadow_end(rg_ctx* c, const DevScene& S) {
+  if (!S.shadow_on) return 0;
+  const size_t ns = c->cs.srcs.size();
+  std::vector<uint32_t> tested(ns);
+  hipck(hipMemcpyAsync(tested.data(), S.src_tested, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost, c->st), "tested");
+  hipck(hipStreamSynchronize(c->st), "tested");
+  unsigned long long add = 0;
+  for (size_t k = 0; k < ns; ++k)
+    if (tested[k] && !c->cast_counted[k]) {
+      c->cast_counted[k] = 1;
+      add += c->cast_count[k];
+    }
+  return add;
+}
+
 // device_io: `rays` and `out` are device memory of the context's device
+// batch_ctx: the contexts of the batch's scenes (their shadow caches).
 static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int flags, int mode,
                 const std::vector<BinSpec>& bins, int ncols, double* out,
-                const std::vector<DevScene>* batch = nullptr, const std::vector<uint32_t>* batch_start = nullptr,
-                const AdjSpec* adj = nullptr, bool device_io = false) {
+                std::vector<DevScene>* batch = nullptr, const std::vector<uint32_t>* batch_start = nullptr,
+                const AdjSpec* adj = nullptr, bool device_io = false,
+                const std::vector<rg_ctx*>* batch_ctx = nullptr) {
   if (!c->committed) throw std::runtime_error("scene not committed");
   if (nrays < 0 || nrays > 0x7fffffffLL) throw std::runtime_error("bad ray count");
   if (accum < 1) throw std::runtime_error("bad accumulation count");
@@ -329,6 +396,16 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   cfg.nscenes = 0;
   cfg.scenes = nullptr;
   cfg.scene_start = nullptr;
+  // shadow caches of the scenes traced (filled at the first call)
+  const bool shadow = c->tuning.shadow_cache != 0;
+  if (batch) {
+    for (size_t i = 0; i < batch->size(); ++i) {
+      shadow_begin((*batch_ctx)[i], (*batch)[i], shadow);
+      hipck(hipStreamSynchronize((*batch_ctx)[i]->st), "shadow cache");
+    }
+  } else {
+    shadow_begin(c, c->dscene, shadow);
+  }
   if (batch) {
     DevScene* d_sc = (DevScene*)c->arena.get("batch_scenes", sizeof(DevScene) * batch->size(), c->st, false);
     uint32_t* d_ss = (uint32_t*)c->arena.get("batch_start", sizeof(uint32_t) * batch_start->size(), c->st, false);
@@ -400,6 +477,11 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   if (adj)
     hipck(hipMemcpyAsync(adj->grad, d_grad, sizeof(double) * 5 * adj->nrows, hipMemcpyDeviceToHost, c->st), "grad");
   hipck(hipStreamSynchronize(c->st), "render");
+  if (batch) {
+    for (size_t i = 0; i < batch->size(); ++i) c->last_nrays += shadow_end((*batch_ctx)[i], (*batch)[i]);
+  } else {
+    c->last_nrays += shadow_end(c, c->dscene);
+  }
   if (herr & ERR_STACK) throw std::runtime_error("BVH traversal stack overflow");
   if (herr & ERR_CAPACITY) throw std::runtime_error("spawn buffer overflow");
   if (herr & ERR_NOMAT) throw std::runtime_error("surface without material");
@@ -643,7 +725,8 @@ int rg_contrib_batch(rg_ctx* const* ctxs, int nctx, const double* rays, const lo
       if (total > 0x7fffffffLL) throw std::runtime_error("bad ray count");
       start.push_back((uint32_t)total);
     }
-    run(c0, rays, total, accum, flags, 1, bins, ncols, out, &scenes, &start);
+    std::vector<rg_ctx*> cl(ctxs, ctxs + nctx);
+    run(c0, rays, total, accum, flags, 1, bins, ncols, out, &scenes, &start, nullptr, false, &cl);
   });
 }
 
@@ -668,7 +751,8 @@ int rg_trace_batch(rg_ctx* const* ctxs, int nctx, const double* rays, const long
       start.push_back((uint32_t)total);
     }
     std::vector<BinSpec> none;
-    run(c0, rays, total, 1, flags, 0, none, 3, out, &scenes, &start);
+    std::vector<rg_ctx*> cl(ctxs, ctxs + nctx);
+    run(c0, rays, total, 1, flags, 0, none, 3, out, &scenes, &start, nullptr, false, &cl);
   });
 }
 
@@ -719,6 +803,9 @@ int rg_set_tuning(rg_ctx* c, const char* key, long long value) {
     } else if (k == "wide") {
       if (value != 0 && value != 1) throw std::runtime_error("wide is 0 or 1");
       t.wide = (int)value;
+    } else if (k == "shadow_cache") {
+      if (value != 0 && value != 1) throw std::runtime_error("shadow_cache is 0 or 1");
+      t.shadow_cache = (int)value;
     } else if (k == "blocks_per_cu") {
       if (value < 1 || value > 64) throw std::runtime_error("blocks_per_cu is 1..64");
       t.blocks_per_cu = (uint32_t)value;

@@ -47,8 +47,11 @@ struct DevScene {
   int32_t nsrcs;
   int32_t root;                 // root node (<0: single leaf)
   int32_t lamb;                 // index of the built-in Lambertian material
-  int32_t pad;
+  int32_t shadow_on;            // shadow cache in use (caches filled)
   double cusize;                // scene cube size (ambient distance clamp)
+  const RG_GLOBAL SrcCache* scache;             // per source
+  const RG_GLOBAL unsigned long long* shadow_obs;  // blocker cells: (sequence+1) << 32 | primitive; 0 empty
+  RG_GLOBAL uint32_t* src_tested;               // per source: shadow-tested during the current call
 };
 
 struct HitRec {

@@ -528,6 +528,78 @@ __device__ __forceinline__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdi
   return scval;
 }
 
+// ---- shadow cache (rt/srcobstr.c) ---------------------------------------------
+// srcobstructp() (rt/srcobstr.c:196-278): the cell of a ray from o along d
+// aimed at source src; -1 when the ray falls outside the grid.
+__device__ __forceinline__ int shadow_cell(const Source& src, const SrcCache& C, V3 o, V3 d) {
+  constexpr int S = kShadCache;
+  int ondx;
+  if (C.kind == SC_DISTANT) {
+    const int ax = C.ax, ax1 = ax == 2 ? 0 : ax + 1, ax2 = ax == 0 ? 2 : ax - 1;
+    const double sl[3] = {src.sloc[0], src.sloc[1], src.sloc[2]};
+    const double ro[3] = {o.x, o.y, o.z};
+    const double t = (C.o[ax] - ro[ax]) / sl[ax];
+    if (t <= FTINY) return -1;
+    ondx = 2 * S * (int)(2 * S * C.e1 * (ro[ax1] + t * sl[ax1] - C.o[ax1]));
+    ondx += (int)(2 * S * C.e2 * (ro[ax2] + t * sl[ax2] - C.o[ax2]));
+    if ((ondx < 0) | (ondx >= 4 * S * S)) return -1;
+  } else if (C.kind == SC_FLAT) {
+    const double sd0 = -dot(d, load3(C.u)), sd1 = -dot(d, load3(C.v)), sd2 = -dot(d, load3(src.snorm));
+    if (sd2 < 0) return -1;
+    const double sd0m = fabs(sd0), sd1m = fabs(sd1);
+    if (sd2 >= sd0m && sd2 >= sd1m) {
+      ondx = S * (int)(S * (.5 - FTINY) * (1. + sd0 / sd2));
+      ondx += (int)(S * (.5 - FTINY) * (1. + sd1 / sd2));
+    } else if (sd0m >= sd1m) {
+      ondx = S * S;
+      if (sd0 < 0) ondx += ((S + 1) >> 1) * S;
+      ondx += S * (int)(S * (.5 - FTINY) * (1. - sd2 / sd0m));
+      ondx += (int)(S * (.5 - FTINY) * (1. + sd1 / sd0m));
+    } else {
+      ondx = S * S + ((S + 1) >> 1) * S * 2;
+      if (sd1 < 0) ondx += ((S + 1) >> 1) * S;
+      ondx += S * (int)(S * (.5 - FTINY) * (1. - sd2 / sd1m));
+      ondx += (int)(S * (.5 - FTINY) * (1. + sd0 / sd1m));
+    }
+    if ((ondx < 0) | (ondx >= C.ncell)) return -1;
+  } else {
+    const double dd[3] = {d.x, d.y, d.z};
+    int ax = 0;
+    double amax = 0;
+    for (int k = 3; k--;)
+      if (fabs(dd[k]) > amax) { amax = fabs(dd[k]); ax = k; }
+    const int ax1 = ax == 2 ? 0 : ax + 1, ax2 = ax == 0 ? 2 : ax - 1;
+    ondx = 2 * S * S * ax;
+    if (dd[ax] < 0) ondx += S * S;
+    ondx += S * (int)(S * (.5 - FTINY) * (1. + dd[ax1] / amax));
+    ondx += (int)(S * (.5 - FTINY) * (1. + dd[ax2] / amax));
+    if ((ondx < 0) | (ondx >= C.ncell)) return -1;
+  }
+  return ondx;
+}
+
+// srcblocked() (rt/srcobstr.c:323-339): the cached blocker of the shadow
+// ray's cell intersects it (and, for a local source, before the source).
+// Marks the source as shadow-tested in this call.
+__device__ __forceinline__ bool shadow_blocked(const DevScene& S, int sn, V3 o, V3 d) {
+  if (!S.shadow_on) return false;
+  const SrcCache& C = S.scache[sn];
+  if (C.kind == SC_NONE) return false;
+  if (!S.src_tested[sn]) S.src_tested[sn] = 1u;
+  const Source& src = S.srcs[sn];
+  const int cell = shadow_cell(src, C, o, d);
+  if (cell < 0) return false;
+  const unsigned long long e = S.shadow_obs[C.cell0 + cell];
+  if (!(e >> 32)) return false;
+  HitRec h;
+  h.t = FHUGE; h.rod = 1.0; h.prim = -1;
+  hit_prim(S, (int)(uint32_t)e, o, d, h);
+  if (h.prim < 0) return false;
+  if (src.flags & SDISTANT) return true;
+  hit_prim(S, src.prim, o, d, h);
+  return h.prim != src.prim;
+}
+
 // direct() of rt/source.c:398-557 with every source tested (no shadow
 // threshold / untested-source statistics): one shadow spawn per source with
 // non-zero potential.
@@ -593,6 +665,8 @@ __device__ __forceinline__ void direct_light(const DevScene& S, const RunCfg& c,
       dom *= src.ss2 / (dist * dist);
       if ((src.flags & SPROX) && dist > src.prox) continue;
     }
+    // the shadow cache decides before anything else (rt/source.c:437-446)
+    if (shadow_blocked(S, sn, hs.rop, dir)) continue;
     C3 coef = dirnorm(nd, c, r.d, dir, dom);
     if (!(cmax(coef) > 0.0)) continue;
     // potential: the ray must reach the source (srcvalue, rt/source.c:261-299)
@@ -885,7 +959,7 @@ __device__ __forceinline__ void shade_hit(const DevScene& S, const RunCfg& c, co
   hs.rod = -dot(r.d, hs.ron);
   hs.pert = prim_pert(S, h.prim, hs.rop, hs.ron);
   hs.prim = h.prim;
-  hs.flat = P.flat;
+  hs.flat = P.flat & PF_FLAT;
   hs.phong = prim_phong(S, h.prim);
   if (P.mat < 0) { atomicOr(c.err, ERR_NOMAT); return; }
   const Material& m = S.mats[P.mat];
@@ -1412,6 +1486,56 @@ void first_hit(const DevScene& S, const double* d_rays, uint32_t n, double* d_t,
   hipLaunchKernelGGL(k_first_hit, dim3((n + TBLOCK - 1) / TBLOCK), dim3(TBLOCK), 0, st, S, d_rays, n, d_t, d_n, d_p,
                      d_err);
   check(hipGetLastError(), "k_first_hit launch");
+}
+
+// Shadow-cache fill (castshadow() and initobscache() of rt/srcobstr.c:28-183):
+// each cast ray walks from its start point until it meets an object that
+// srcblocker() (:298-320) would record - opaque material, not the source
+// itself, not part of a mesh or an instance - and files that object under
+// the cell its reversed ray falls in.  Passing a transparent or unrecorded
+// object costs another closest-hit query, as in the reference.  Cells of
+// two rays: the later ray of the reference's order wins (the sequence
+// number sits in the high word).  counts[source] += closest-hit queries.
+__global__ void __launch_bounds__(TBLOCK) k_shadow_cast(DevScene S, const double* rays, const int32_t* rsrc, uint32_t n,
+                                                        unsigned long long* obs, unsigned long long* counts,
+                                                        uint32_t* err) {
+  __shared__ int32_t stk[LSTACK * TBLOCK];
+  __shared__ float stn[LSTACK * TBLOCK];
+  __shared__ float tes[NBIG_ORDER * TBLOCK];
+  const uint32_t i = blockIdx.x * TBLOCK + threadIdx.x;
+  if (i >= n) return;
+  const int sn = rsrc[i];
+  const Source& src = S.srcs[sn];
+  const SrcCache& C = S.scache[sn];
+  const uint32_t seq = i;
+  V3 o = load3(rays + 6 * (size_t)i), d = load3(rays + 6 * (size_t)i + 3);
+  unsigned long long nq = 0;
+  for (int pass = 0; pass < 4096; ++pass) {
+    HitRec h;
+    closest_hit<TBLOCK>(S, o, d, h, stk, stn, tes, err, nullptr);
+    ++nq;
+    if (h.prim < 0) break;
+    const V3 rop = o + d * h.t;
+    const Prim& P = S.prims[h.prim];
+    int mk = P.mat >= 0 ? S.mats[P.mat].kind : MK_NONE;
+    const bool opaque = mk == MK_PLASTIC || mk == MK_METAL || mk == MK_LIGHT || mk == MK_GLOW;
+    if (opaque && !(P.flat & PF_COMPLEX) && h.prim != src.prim) {
+      const int cell = shadow_cell(src, C, rop - d, -d);
+      if (cell >= 0)
+        atomicMax(obs + C.cell0 + cell, ((unsigned long long)(seq + 1) << 32) | (uint32_t)h.prim);
+      break;
+    }
+    o = rop + d * (10. * FTINY);
+  }
+  atomicAdd(counts + sn, nq);
+}
+
+void shadow_cast(const DevScene& S, const double* d_rays, const int32_t* d_src, uint32_t n, unsigned long long* d_obs,
+                 unsigned long long* d_counts, uint32_t* d_err, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_shadow_cast, dim3((n + TBLOCK - 1) / TBLOCK), dim3(TBLOCK), 0, st, S, d_rays, d_src, n, d_obs,
+                     d_counts, d_err);
+  check(hipGetLastError(), "k_shadow_cast launch");
 }
 
 // One generation, part 2: shade the hits and append the next spawn records.

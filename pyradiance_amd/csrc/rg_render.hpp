@@ -88,6 +88,7 @@ struct Tuning {
   uint32_t blocks_per_cu = 4;    // persistent grid: blocks of 256 per compute unit
   uint32_t max_big = 8;          // room-sized primitives kept out of the BVH when there are at most this many (applies at commit)
   int wide = 1;                  // persistent traversal walks the four-wide tree when the scene has one
+  int shadow_cache = 1;          // pre-empt shadow rays by the reference's shadow cache (rt/srcobstr.c)
 };
 
 // max_srcs: the largest source count over the scenes of a batch (sizes the
@@ -99,6 +100,12 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
 // unflipped normal (n x 3), primitive index (-1 on a miss).
 void first_hit(const DevScene& S, const double* d_rays, uint32_t n, double* d_t, double* d_n, int32_t* d_p,
                uint32_t* d_err, hipStream_t st);
+
+// Shadow-cache fill: n cast rays (n x 6 doubles, d_src their sources) file
+// their first recordable blocker into d_obs; d_counts[source] += the
+// closest-hit queries they took.
+void shadow_cast(const DevScene& S, const double* d_rays, const int32_t* d_src, uint32_t n, unsigned long long* d_obs,
+                 unsigned long long* d_counts, uint32_t* d_err, hipStream_t st);
 
 // BVH over the primitives' FP32 bounds (lo/hi: 3*n floats, already widened).
 // Fills nodes (n-1 entries) and leaf_prim (n entries); returns the root ref.

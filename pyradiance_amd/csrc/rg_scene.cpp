@@ -211,6 +211,112 @@ int material_kind(const std::string& t) {
 
 }  // namespace
 
+// Shadow-cache layout and cast rays of every source, as initobscache() sets
+// them up (rt/srcobstr.c:63-183): proximity (glow) and skipped sources get
+// no cache; sources from the 200th on (MAX2SHADE, :20-21) get cells but no
+// cast rays.  Cast rays are listed in the reference's loop order, which
+// decides which of two rays aimed at one cell records its blocker.
+static void shadow_cache_setup(CompiledScene& cs) {
+  constexpr int S = kShadCache;
+  cs.scache.assign(cs.srcs.size(), SrcCache{});
+  cs.ncells = 0;
+  cs.cast_rays.clear();
+  cs.cast_src.clear();
+  auto cast = [&](int sn, const double* o, const double* d) {
+    cs.cast_rays.insert(cs.cast_rays.end(), {o[0], o[1], o[2], d[0], d[1], d[2]});
+    cs.cast_src.push_back(sn);
+  };
+  for (size_t sn = 0; sn < cs.srcs.size(); ++sn) {
+    const Source& src = cs.srcs[sn];
+    SrcCache& C = cs.scache[sn];
+    memset(&C, 0, sizeof C);
+    if (src.flags & (SSKIP | SPROX | SSPOT | SVIRTUAL)) continue;
+    if (src.flags & SDISTANT) {
+      C.kind = SC_DISTANT;
+      C.ncell = 4 * S * S;
+    } else if (src.flags & SFLAT) {
+      C.kind = SC_FLAT;
+      C.ncell = S * S * 3 + (S & 1) * S * 4;
+    } else {
+      C.kind = SC_SPHERE;
+      C.ncell = S * S * 6;
+    }
+    C.cell0 = cs.ncells;
+    cs.ncells += C.ncell;
+    if (C.kind == SC_DISTANT) {
+      double amax = 0;
+      int ax = 0;
+      for (int a = 3; a--;)
+        if (fabs(src.sloc[a]) > amax) { amax = fabs(src.sloc[a]); ax = a; }
+      const int ax1 = (ax + 1) % 3, ax2 = (ax + 2) % 3;
+      C.ax = ax;
+      for (int a = 0; a < 3; ++a) C.o[a] = cs.cuorg[a];
+      if (src.sloc[ax] > 0) C.o[ax] += cs.cube_size;
+      if (src.sloc[ax1] < 0) C.o[ax1] += cs.cube_size * src.sloc[ax1] / amax;
+      if (src.sloc[ax2] < 0) C.o[ax2] += cs.cube_size * src.sloc[ax2] / amax;
+      C.e1 = 1. / (cs.cube_size * (1. + fabs(src.sloc[ax1]) / amax));
+      C.e2 = 1. / (cs.cube_size * (1. + fabs(src.sloc[ax2]) / amax));
+    } else if (C.kind == SC_FLAT) {
+      for (int a = 0; a < 3; ++a) C.u[a] = src.ss[0][a];
+      norm3(C.u);
+      cross3(C.v, src.snorm, C.u);
+    }
+    if (sn >= 200) continue;
+    double org[3], dir[3], d;
+    if (C.kind == SC_DISTANT) {
+      const int ax1 = (C.ax + 1) % 3, ax2 = (C.ax + 2) % 3;
+      for (int a = 0; a < 3; ++a) dir[a] = -src.sloc[a];
+      for (int i = 2 * S; i--;)
+        for (int j = 2 * S; j--;) {
+          for (int a = 0; a < 3; ++a) org[a] = C.o[a];
+          org[ax1] += (i + .5) / (2 * S * C.e1);
+          org[ax2] += (j + .5) / (2 * S * C.e2);
+          cast((int)sn, org, dir);
+        }
+    } else if (C.kind == SC_FLAT) {
+      d = 0.01 * src.srad;
+      for (int a = 0; a < 3; ++a) org[a] = src.sloc[a] + src.snorm[a] * d;
+      for (int i = S; i--;)
+        for (int j = S; j--;) {
+          d = 2. / S * (i + .5) - 1.;
+          for (int a = 0; a < 3; ++a) dir[a] = src.snorm[a] + C.u[a] * d;
+          d = 2. / S * (j + .5) - 1.;
+          for (int a = 0; a < 3; ++a) dir[a] += C.v[a] * d;
+          norm3(dir);
+          cast((int)sn, org, dir);
+        }
+      for (int k = 2; k--;)
+        for (int i = S; i--;)
+          for (int j = S >> 1; j--;) {
+            d = 2. / S * (i + .5) - 1.;
+            for (int a = 0; a < 3; ++a) dir[a] = k ? C.u[a] + C.v[a] * d : C.v[a] + C.u[a] * d;
+            d = 1. - 2. / S * (j + .5);
+            for (int a = 0; a < 3; ++a) dir[a] += src.snorm[a] * d;
+            norm3(dir);
+            cast((int)sn, org, dir);
+            d = 2. * dot3(dir, src.snorm);
+            for (int a = 0; a < 3; ++a) dir[a] = d * src.snorm[a] - dir[a];
+            cast((int)sn, org, dir);
+          }
+    } else {
+      for (int k = 6; k--;) {
+        const int ax = k % 3, ax1 = (k + 1) % 3, ax2 = (k + 2) % 3;
+        for (int i = S; i--;)
+          for (int j = S; j--;) {
+            dir[0] = dir[1] = dir[2] = 0.;
+            dir[ax] = k < 3 ? 1. : -1.;
+            dir[ax1] = 2. / S * (i + .5) - 1.;
+            dir[ax2] = 2. / S * (j + .5) - 1.;
+            norm3(dir);
+            d = 1.05 * src.srad;
+            for (int a = 0; a < 3; ++a) org[a] = src.sloc[a] + dir[a] * d;
+            cast((int)sn, org, dir);
+          }
+      }
+    }
+  }
+}
+
 CompiledScene compile_scene(const SceneDB& db) {
   CompiledScene cs;
   const auto& objs = db.objs;
@@ -281,6 +387,24 @@ CompiledScene compile_scene(const SceneDB& db) {
     }
   };
 
+  // the octree cube's box, grown as oconv's add2bbox does (ot/bbox.c:23-140):
+  // polygon vertices, a sphere's six axis extremes, the end circles of the
+  // cone family (only the outer circle of a ring)
+  double cb_lo[3] = {1e300, 1e300, 1e300}, cb_hi[3] = {-1e300, -1e300, -1e300};
+  auto cube_point = [&](const double* p) {
+    for (int a = 0; a < 3; ++a) {
+      cb_lo[a] = std::min(cb_lo[a], p[a]);
+      cb_hi[a] = std::max(cb_hi[a], p[a]);
+    }
+  };
+  auto cube_circle = [&](const double* c, const double* n, double rad) {
+    for (int a = 0; a < 3; ++a) {
+      double r = sqrt(1. - n[a] * n[a]);
+      cb_hi[a] = std::max(cb_hi[a], c[a] + r * rad);
+      cb_lo[a] = std::min(cb_lo[a], c[a] - r * rad);
+    }
+  };
+
   // pass 1: primitives
   std::vector<int> obj_prim(nobj, -1);
   std::vector<std::array<double, 6>> dbounds;
@@ -328,15 +452,17 @@ CompiledScene compile_scene(const SceneDB& db) {
       if (fabs(n[2]) > fabs(n[ax])) ax = 2;
       P.kind = PK_POLYGON;
       P.nv = nv;
-      P.flat = 1;
+      P.flat = PF_FLAT;
       cs.geom.insert(cs.geom.end(), {n[0], n[1], n[2], offset, (double)ax});
       for (int a = 0; a < 3; ++a) { lo[a] = 1e300; hi[a] = -1e300; }
-      for (int k = 0; k < nv; ++k)
+      for (int k = 0; k < nv; ++k) {
         for (int a = 0; a < 3; ++a) {
           cs.geom.push_back(V(k)[a]);
           lo[a] = std::min(lo[a], V(k)[a]);
           hi[a] = std::max(hi[a], V(k)[a]);
         }
+        cube_point(V(k));
+      }
     } else if (o.type == "sphere" || o.type == "bubble") {
       if (f.size() != 4) objerr(o, "bad # arguments");
       bool bubble = o.type == "bubble";
@@ -346,6 +472,13 @@ CompiledScene compile_scene(const SceneDB& db) {
       P.kind = bubble ? PK_BUBBLE : PK_SPHERE;
       cs.geom.insert(cs.geom.end(), {f[0], f[1], f[2], r});
       for (int a = 0; a < 3; ++a) { lo[a] = f[a] - r; hi[a] = f[a] + r; }
+      for (int a = 0; a < 3; ++a) {
+        double v[3] = {f[0], f[1], f[2]};
+        v[a] -= r;
+        cube_point(v);
+        v[a] += 2.0 * r;
+        cube_point(v);
+      }
     } else {
       // cone family (getcone)
       bool ring = o.type == "ring";
@@ -396,10 +529,13 @@ CompiledScene compile_scene(const SceneDB& db) {
       if (al == 0.0) { cs.warnings.push_back(o.name + ": unknown orientation"); return -1; }
       if (ring) {
         P.kind = PK_RING;
-        P.flat = 1;
+        P.flat = PF_FLAT;
         cs.geom.insert(cs.geom.end(), {f[0], f[1], f[2], ad[0], ad[1], ad[2], R0, R1});
         for (int a = 0; a < 3; ++a) { lo[a] = f[a] - R1; hi[a] = f[a] + R1; }
+        cube_circle(f.data(), ad, R1);
       } else {
+        cube_circle(P0, ad, R0);
+        cube_circle(P1, ad, R1);
         P.kind = t == "cone" ? PK_CONE : t == "cup" ? PK_CUP : t == "cylinder" ? PK_CYLINDER : PK_TUBE;
         double sl = cylfam ? al : sqrt((R1 - R0) * (R1 - R0) + al * al);
         cs.geom.insert(cs.geom.end(), {P0[0], P0[1], P0[2], ad[0], ad[1], ad[2], al, R0, R1, sl});
@@ -466,7 +602,7 @@ CompiledScene compile_scene(const SceneDB& db) {
       P.srcidx = -1;
       P.goff = (int)cs.geom.size();
       P.nv = 3;
-      P.flat = pseudo ? 1 : 0;                       // the pseudo object is a face
+      P.flat = (pseudo ? PF_FLAT : 0) | PF_COMPLEX;   // the pseudo object is a face
       const bool normals = M.has_n[t.v[0]] && M.has_n[t.v[1]] && M.has_n[t.v[2]];
       int fl = (t.v[0] > t.v[1] ? TRI_REV0 : 0) | (t.v[1] > t.v[2] ? TRI_REV1 : 0) |
                (t.v[2] > t.v[0] ? TRI_REV2 : 0) | (normals ? TRI_NORMALS : 0) |
@@ -483,6 +619,9 @@ CompiledScene compile_scene(const SceneDB& db) {
         lo[k] = std::min({a[k], b[k], c[k]});
         hi[k] = std::max({a[k], b[k], c[k]});
       }
+      cube_point(a);
+      cube_point(b);
+      cube_point(c);
       cs.prims.push_back(P);
       cs.prim_modname.push_back(objs[mod].name);
       dbounds.push_back({lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
@@ -510,12 +649,14 @@ CompiledScene compile_scene(const SceneDB& db) {
       } else if (o.type == "instance") {
         add_instance(j, &X, over, depth + 1);
       } else if (SceneDB::is_surface(o.type) && o.type != "source") {
+        int pi = -1;
         if (over >= 0) {
-          if (overmat >= 0) add_surface(xf_object(o, X), overmat, objs[over].omod, false);
+          if (overmat >= 0) pi = add_surface(xf_object(o, X), overmat, objs[over].omod, false);
         } else if (o.omod >= 0) {
           int mo = findmaterial(j);
-          if (mo >= 0) add_surface(xf_object(o, X), mo, o.omod, true);
+          if (mo >= 0) pi = add_surface(xf_object(o, X), mo, o.omod, true);
         }
+        if (pi >= 0) cs.prims[pi].flat |= PF_COMPLEX;
       }
     }
   };
@@ -547,6 +688,20 @@ CompiledScene compile_scene(const SceneDB& db) {
   }
   double ext = std::max({cs.bmax[0] - cs.bmin[0], cs.bmax[1] - cs.bmin[1], cs.bmax[2] - cs.bmin[2], 1e-3});
   cs.cusize = ext;
+  // the octree cube: the file's for octree input, else as oconv sets it
+  // (ot/oconv.c:126-139: box widened by 10 FTINY, largest side, centred)
+  if (db.cusize > 0) {
+    for (int a = 0; a < 3; ++a) cs.cuorg[a] = db.cuorg[a];
+    cs.cube_size = db.cusize;
+  } else if (cb_lo[0] <= cb_hi[0]) {
+    cs.cube_size = 0;
+    for (int a = 0; a < 3; ++a) {
+      cb_lo[a] -= 10 * FTINY;
+      cb_hi[a] += 10 * FTINY;
+      cs.cube_size = std::max(cs.cube_size, cb_hi[a] - cb_lo[a]);
+    }
+    for (int a = 0; a < 3; ++a) cs.cuorg[a] = (cb_hi[a] + cb_lo[a] - cs.cube_size) * .5;
+  }
   // FP32 bounds, widened so that the FP32 slab test stays conservative
   double eps = 1e-5 + 4e-6 * ext;
   cs.lo.resize(3 * cs.prims.size());
@@ -687,6 +842,7 @@ CompiledScene compile_scene(const SceneDB& db) {
     cs.src_modname.push_back(objs[o.omod].name);
   }
   if (cs.srcs.size() > 32000) throw RadError("too many light sources for the GPU path");
+  shadow_cache_setup(cs);
 
   // built-in Lambertian used by -I / -i (ρ = π, rt/raytrace.c:24-28)
   Material L;

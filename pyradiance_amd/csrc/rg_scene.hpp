@@ -19,6 +19,12 @@ struct CompiledScene {
   std::vector<std::string> warnings;
   double bmin[3], bmax[3];
   double cusize = 0;
+  double cuorg[3] = {0, 0, 0};               // octree cube as oconv sets it (ot/oconv.c:120-139)
+  double cube_size = 0;
+  std::vector<SrcCache> scache;              // shadow cache layout per source
+  int32_t ncells = 0;                        // blocker cells of all sources
+  std::vector<double> cast_rays;             // shadow-cache rays (origin, direction), in the reference's order
+  std::vector<int32_t> cast_src;             // source of each cast ray
   int lamb = -1;                             // built-in Lambertian (rt/raytrace.c:24-28)
 };
 

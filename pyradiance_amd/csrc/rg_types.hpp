@@ -97,6 +97,29 @@ struct Source {
   int32_t pad;
 };
 
+// Shadow cache of one light source (rt/srcobstr.c): for each cell, the
+// opaque object last found to block the source (SHADCACHE of rt/source.h:19-20).
+// Distant sources: 2S x 2S cells over the cube face the source shines on,
+// indexed by where a ray origin projects along the source direction; flat
+// sources: 3 S^2 cells over directions of approach (front and four half
+// sides); others: 6 S^2 cells of a direction cube.
+constexpr int kShadCache = 20;
+enum : int32_t { SC_NONE = 0, SC_DISTANT = 1, SC_FLAT = 2, SC_SPHERE = 3 };
+struct SrcCache {
+  int32_t kind;      // SC_*
+  int32_t ax;        // distant: dominant axis of the source direction
+  int32_t cell0;     // first cell of this source in the blocker array
+  int32_t ncell;
+  double o[3];       // distant: origin of the cell grid on the cube face
+  double e1, e2;     // distant: inverse extents of the grid
+  double u[3], v[3]; // flat: in-plane axes
+};
+
+// Prim.flat bits: PF_FLAT isflat() of the object reported as hit;
+// PF_COMPLEX part of a mesh or an instance, whose hits the reference never
+// records as shadow blockers (rt/srcobstr.c:298-305).
+enum : int32_t { PF_FLAT = 1, PF_COMPLEX = 2 };
+
 // Per-primitive record (all kinds); geometry lives in a shared FP64 pool.
 struct Prim {
   int32_t kind;      // PrimKind
@@ -105,7 +128,7 @@ struct Prim {
   int32_t modslot;   // tracked-modifier slot (rcontrib), -1 if untracked
   int32_t goff;      // offset into geometry pool (doubles)
   int32_t nv;        // polygon vertex count
-  int32_t flat;      // isflat(): polygon / ring
+  int32_t flat;      // PF_FLAT | PF_COMPLEX bits
   int32_t srcidx;    // source index if this primitive is a local source, else -1
 };
 

@@ -75,12 +75,77 @@ def run(cmd, rays):
 
 
 def config1():
+    """Config-1 irradiance and the reference's ray count (nrays of
+    rt/raytrace.c:606, printed by the counting build) per sun position."""
     for seed in range(3):
         rad, sens, sun = scenes.config1(seed)
         with tempfile.TemporaryDirectory() as td:
             octf = oconv(rad, td)
-            v = run([str(REF / "rtrace"), "-h", "-I", "-ab", "0", "-dj", "0", "-ds", "0", octf], sens).reshape(-1, 3)
-        np.savez_compressed(HERE / f"c1_seed{seed}.npz", sensors=sens, sun=sun, values=v)
+            args = ["-h", "-I", "-ab", "0", "-dj", "0", "-ds", "0", octf]
+            v = run([str(REF / "rtrace"), *args], sens).reshape(-1, 3)
+            r = subprocess.run([str(REF / "rtrace_count"), *args], input=rays_text(sens), capture_output=True,
+                               env=ENV, check=True)
+            assert np.array_equal(np.array(r.stdout.split(), dtype=np.float64).reshape(-1, 3), v)
+            nrays = [int(ln.split()[1]) for ln in r.stderr.decode().splitlines() if ln.startswith("NRAYS")][0]
+        np.savez_compressed(HERE / f"c1_seed{seed}.npz", sensors=sens, sun=sun, values=v, nrays=nrays)
+
+
+def _stat_runs(octf: str, pts: np.ndarray, R: int, args: list, nproc: int | None = None):
+    """R in-stream repeats of every point through the counting rcontrib
+    (-n 1), the points split over parallel processes.  The reference seeds
+    its generator from time(0) under -u+ (rt/rcmain.c:331-333), so the
+    processes are started in different seconds to draw independent streams.
+    Returns the values (npts, R, ncols), the process boundaries and each
+    process's ray count."""
+    import time
+    nproc = nproc or os.cpu_count() or 1
+    chunks = [c for c in np.array_split(np.arange(len(pts)), nproc) if len(c)]
+    procs = []
+    with tempfile.TemporaryDirectory() as td:
+        for k, ch in enumerate(chunks):
+            pf = Path(td) / f"in{k}.txt"
+            pf.write_bytes(rays_text(np.repeat(pts[ch], R, axis=0)))
+            fin = open(pf, "rb")
+            fout = open(Path(td) / f"out{k}.txt", "wb")
+            procs.append((subprocess.Popen([str(REF / "rcontrib_count"), "-n", "1", *args, octf], stdin=fin,
+                                           stdout=fout, stderr=subprocess.PIPE, env=ENV), fin, fout))
+            time.sleep(1.05)
+        vals, nrays = [], []
+        for k, ((pr, fin, fout), ch) in enumerate(zip(procs, chunks)):
+            _, err = pr.communicate()
+            fin.close()
+            fout.close()
+            assert pr.returncode == 0, err.decode()
+            out = (Path(td) / f"out{k}.txt").read_bytes()
+            vals.append(np.array(out.split(), dtype=np.float64).reshape(len(ch), R, -1))
+            nrays.append([int(ln.split()[1]) for ln in err.decode().splitlines() if ln.startswith("NRAYS")][0])
+    bounds = np.array([0] + [int(c[-1]) + 1 for c in chunks])
+    return np.concatenate(vals), bounds, np.array(nrays, dtype=np.int64)
+
+
+def _save_stat(name: str, pts, R, v, bounds, nrays, args, **extra):
+    """Per-(point, column) mean and variance (FP64) of R estimates, the
+    per-estimate totals of each channel, and the ray counts."""
+    np.savez_compressed(HERE / name, points=pts, repeats=R, mean=v.mean(1), var=v.var(1, ddof=1),
+                        totals=np.stack([v[..., c::3].sum(2) for c in range(3)], -1), bounds=bounds, nrays=nrays,
+                        args=np.array(args), **extra)
+
+
+C2_ARGS = ["-h", "-I+", "-ab", "4", "-ad", "4096", "-lw", "1e-4", "-aa", "0", "-c", "1",
+           "-f", "reinhartb.cal", "-p", "MF=1", "-bn", "Nrbins", "-b", "rbin", "-m", "skyglow",
+           "-b", "0", "-bn", "1", "-m", "groundglow"]
+
+
+def c2_stat():
+    """Config 2 at the benchmark's options (rcontrib -I+ -ab 4 -ad 4096
+    -lw 1e-4 -aa 0 -c 1, Reinhart MF:1 + ground): every 10th of the 2,500
+    sensors, 16 in-stream estimates each."""
+    rad, sens = scenes.config2(0)
+    pts = sens[::10]
+    with tempfile.TemporaryDirectory() as td:
+        octf = oconv(rad, td)
+        v, bounds, nrays = _stat_runs(octf, pts, 16, C2_ARGS)
+    _save_stat("c2_stat.npz", pts, 16, v, bounds, nrays, C2_ARGS, sensor_stride=10)
 
 
 def bins():
@@ -204,14 +269,20 @@ def shading():
     sensors = np.hstack([sens, sdir])
     with tempfile.TemporaryDirectory() as td:
         octf = oconv(SHADING, td)
-        r = subprocess.run([str(REF / "rtrace"), "-h", "-fad", *SHADING_ARGS, octf], input=rays_text(rays),
+        # the counting build prints the reference's nrays (rt/raytrace.c:606);
+        # -u- because under -u+ a flat source's sampling axes are drawn at
+        # random (rt/srcsupp.c:79), and they orient its shadow-cache cells,
+        # so the count (never a value) changes from run to run
+        r = subprocess.run([str(REF / "rtrace_count"), "-h", "-fad", "-u-", *SHADING_ARGS, octf], input=rays_text(rays),
                            capture_output=True, env=ENV, check=True)
         val = np.frombuffer(r.stdout, dtype=np.float64).reshape(n, 3)
-        r = subprocess.run([str(REF / "rtrace"), "-h", "-fad", "-I", *SHADING_ARGS, octf],
+        nr_val = [int(ln.split()[1]) for ln in r.stderr.decode().splitlines() if ln.startswith("NRAYS")][0]
+        r = subprocess.run([str(REF / "rtrace_count"), "-h", "-fad", "-u-", "-I", *SHADING_ARGS, octf],
                            input=rays_text(sensors), capture_output=True, env=ENV, check=True)
         irr = np.frombuffer(r.stdout, dtype=np.float64).reshape(ns, 3)
+        nr_irr = [int(ln.split()[1]) for ln in r.stderr.decode().splitlines() if ln.startswith("NRAYS")][0]
     np.savez_compressed(HERE / "shading.npz", scene=np.array(SHADING), args=np.array(SHADING_ARGS), rays=rays,
-                        values=val, sensors=sensors, irradiance=irr)
+                        values=val, sensors=sensors, irradiance=irr, nrays_values=nr_val, nrays_irradiance=nr_irr)
 
 
 def stat_room():
@@ -775,7 +846,7 @@ def image():
 if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
-    jobs = {"config1": config1, "bins": bins, "stat_room": stat_room, "primitives": primitives,
+    jobs = {"config1": config1, "c2_stat": c2_stat, "bins": bins, "stat_room": stat_room, "primitives": primitives,
             "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
             "mesh": mesh, "instance": instance, "sky": sky, "image": image}
     for name in sys.argv[1:] or list(jobs):

@@ -44,7 +44,7 @@ void scene_prims(int* out) {
     const Prim& P = g_cs.prims[i];
     out[4 * i] = P.kind;
     out[4 * i + 1] = P.mat;
-    out[4 * i + 2] = P.flat;
+    out[4 * i + 2] = P.flat & PF_FLAT;
     out[4 * i + 3] = P.goff;
   }
 }

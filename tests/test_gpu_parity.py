@@ -64,6 +64,7 @@ def test_config1_irradiance_matches_reference(L, oracle, seed):
     orc, nr = oracle.Scene(rad).trace(sens, oracle.params(ambounce=0, dstrsrc=0.0, srcsizerat=0.0), 8)
     assert np.abs(out - orc).max() <= 1e-6 * orc.max()
     assert c.last_nrays() == nr          # ray counts: bit-exact
+    assert c.last_nrays() == int(g["nrays"])   # the reference's own counter (rt/raytrace.c:606)
 
 
 def _mixed_scene():
@@ -397,6 +398,9 @@ def test_batched_scenes_match_individual_runs(L):
     res = L.contrib_batch(ctxs, [s for _, s in sc], mods, 1, L.RT_IMM_IRRAD)
     nb = ctxs[0].last_nrays()
     total = 0
+    # fresh contexts: each first call counts its scene's shadow-cache rays,
+    # as the batch did
+    ctxs = [_ctx(L, rad) for rad, _ in sc]
     for k, (c, (_, sens)) in enumerate(zip(ctxs, sc)):
         p.seed = 21 + k
         c.set_params(p)

@@ -1,0 +1,108 @@
+"""GPU against the reference at the benchmark's own options: per-(sensor,
+bin, channel) means of independent estimates within 3 sigma, the aggregate
+within 1 %, ray counts against the reference's counter (nrays,
+rt/raytrace.c:606), through the C ABI."""
+
+import numpy as np
+import pytest
+
+import stat3s
+from conftest import GOLDEN
+from pyradiance_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def L():
+    from pyradiance_amd import _lib
+    _lib.lib()
+    return _lib
+
+
+def _sky_mods(L, mf=1):
+    m = L.RgModifier()
+    m.name, m.bin_kind, m.nbins, m.mf, m.rhs = b"skyglow", 1, scenes.reinhart_nbins(mf), float(mf), 1.0
+    m.nrm[:] = (0, 0, -1)
+    m.up[:] = (0, 1, 0)
+    g = L.RgModifier()
+    g.name, g.bin_kind, g.nbins, g.cbin = b"groundglow", 0, 1, 0
+    return [m, g]
+
+
+def test_config2_matches_reference_at_3_sigma(L):
+    """Config 2 (room + 200 tessellated boxes, rcontrib -I+ -ab 4 -ad 4096
+    -lw 1e-4 -aa 0, Reinhart MF:1 + ground): 250 sensors (every 10th),
+    16 independent GPU estimates each against 16 reference estimates
+    (c2_stat.npz).  Two reference runs against each other give 0.44 % of
+    entries above 3 sigma and mean z^2 1.10 (stat3s)."""
+    _say("config2 vs reference, 250 sensors x 16 estimates")
+    g = np.load(GOLDEN / "c2_stat.npz")
+    R = int(g["repeats"])
+    rad, sens = scenes.config2(0)
+    assert np.array_equal(sens[::int(g["sensor_stride"])], g["points"])
+    c = L.Context(0)
+    c.load_text(rad).commit()
+    p = L.default_params(True)
+    p.ambounce, p.ambdiv, p.minweight, p.seed = 4, 4096, 1e-4, 11
+    c.set_params(p)
+    rays = np.repeat(g["points"], R, axis=0)          # every ray draws its own samples
+    out = c.contrib(rays, _sky_mods(L), 1, L.RT_IMM_IRRAD).reshape(len(g["points"]), R, -1)
+    nr = c.last_nrays()
+    c.close()
+    s = stat3s.summary(g["mean"], g["var"], R, out.mean(1), out.var(1, ddof=1), R)
+    _say(f"  {s}")
+    stat3s.check(s)
+    # per-sensor totals (the irradiance split over all bins)
+    tot = out.reshape(len(g["points"]), R, -1, 3).sum(2)
+    st = stat3s.summary(g["totals"].mean(1), g["totals"].var(1, ddof=1), R, tot.mean(1), tot.var(1, ddof=1), R)
+    assert st["frac_over_3"] <= 0.02 and st["aggregate"] < 0.01, st
+    # ray counts: the reference's counter over the same 4000 estimates
+    ref_n = int(g["nrays"].sum())
+    _say(f"  rays {nr} vs reference {ref_n}")
+    assert abs(nr - ref_n) <= 0.01 * ref_n
+
+
+def test_shadow_cache_ray_counts(L):
+    """The shadow cache (rt/srcobstr.c) on the local-source fixture: the GPU
+    counts the cache's cast rays and skips the shadow rays it settles, so
+    its counter follows the reference's (-u-): within 0.3 % (the reference
+    also records blockers of traced shadow rays in its own sequential order,
+    which the GPU does not), the values unchanged, and without the cache
+    every shadow ray is traced."""
+    _say("shadow cache counts, local sources")
+    g = np.load(GOLDEN / "shading.npz")
+    c = L.Context(0)
+    c.load_text(str(g["scene"])).commit()
+    p = L.default_params(False)
+    p.ambounce, p.dstrsrc, p.srcsizerat, p.specthresh, p.maxdepth, p.minweight = 0, 0.0, 0.0, 1.0, 8, 1e-4
+    p.ambval[:] = (0.0, 0.0, 0.0)
+    c.set_params(p)
+    v = c.trace(g["rays"], 0)
+    nv = c.last_nrays()
+    e = c.trace(g["sensors"], L.RT_IMM_IRRAD)
+    ni = c.last_nrays()          # sources already counted by the first call
+    c.close()
+    _say(f"  rays {nv} vs reference {int(g['nrays_values'])}")
+    assert abs(nv - int(g["nrays_values"])) <= 0.003 * int(g["nrays_values"])
+    assert np.allclose(v, g["values"], rtol=1e-5, atol=1e-9)
+    assert np.allclose(e, g["irradiance"], rtol=1e-5, atol=1e-9)
+    # a fresh context counts the cast rays in its first call, as a process does
+    c = L.Context(0)
+    c.load_text(str(g["scene"])).commit()
+    c.set_params(p)
+    e2 = c.trace(g["sensors"], L.RT_IMM_IRRAD)
+    ni2 = c.last_nrays()
+    assert np.array_equal(e, e2)
+    _say(f"  -I rays {ni2} vs reference {int(g['nrays_irradiance'])}; second call {ni}")
+    assert abs(ni2 - int(g["nrays_irradiance"])) <= 0.003 * int(g["nrays_irradiance"])
+    assert ni < ni2 - 1000
+    c.set_tuning(shadow_cache=0)
+    e3 = c.trace(g["sensors"], L.RT_IMM_IRRAD)
+    assert np.allclose(e3, e, rtol=1e-12, atol=1e-15)
+    assert c.last_nrays() > ni
+    c.close()
