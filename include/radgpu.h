@@ -94,10 +94,18 @@ int rg_contrib(rg_ctx *ctx, const double *rays, long long nrec, int accum, int f
 
 /* rg_contrib with the rays and the output in device memory of the context's
  * device (the in-process manager's resident buffers, rt/RcontribSimulManager
- * get_output_array, src/binding/radiance_ext.cpp:346-362): no host copies;
- * the call returns when the output is written. */
+ * get_output_array, src/binding/radiance_ext.cpp:346-362): no host copies.
+ * The rays are read only after all work queued before the call on the
+ * caller's stream (rg_set_stream; the null stream by default) has finished,
+ * so a kernel that has just written them on that stream is complete; work
+ * on other streams must be synchronized by the caller.  The call returns
+ * when the output is written. */
 int rg_contrib_device(rg_ctx *ctx, const double *d_rays, long long nrec, int accum, int flags,
                       const rg_modifier *mods, int nmods, double *d_out);
+/* The caller's HIP stream (a hipStream_t; NULL: the null stream) that the
+ * device-memory entry points order their input reads after (no reference
+ * counterpart: the reference has no device buffers). */
+int rg_set_stream(rg_ctx *ctx, void *stream);
 
 /* Batched rcontrib: the records of nctx scenes (one committed context each,
  * same device) traced in one wavefront.  rays holds scene 0's nrec[0]*accum

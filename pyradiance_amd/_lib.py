@@ -24,6 +24,7 @@ EXPORTS = (
     "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint", "rg_first_hit",
     "rg_sky_create", "rg_sky_destroy", "rg_sky_error", "rg_sky_shape", "rg_sky_site", "rg_sky_suns",
     "rg_sky_matrix", "rg_sky_matrix_device", "rg_trace_view", "rg_format_ascii", "rg_contrib_device",
+    "rg_set_stream",
 )
 
 BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
@@ -130,6 +131,7 @@ def lib() -> C.CDLL:
     L.rg_trace.argtypes = [vp, dp, C.c_longlong, C.c_int, dp]
     L.rg_contrib.argtypes = [vp, dp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, dp]
     L.rg_contrib_device.argtypes = [vp, vp, C.c_longlong, C.c_int, C.c_int, C.POINTER(RgModifier), C.c_int, vp]
+    L.rg_set_stream.argtypes = [vp, vp]
     L.rg_contrib_batch.argtypes = [C.POINTER(vp), C.c_int, dp, C.POINTER(C.c_longlong), C.c_int, C.c_int,
                                    C.POINTER(RgModifier), C.c_int, dp]
     L.rg_trace_batch.argtypes = [C.POINTER(vp), C.c_int, dp, C.POINTER(C.c_longlong), C.c_int, dp]
@@ -276,6 +278,12 @@ class Context:
         arr = (RgModifier * len(mods))(*mods)
         self._check(self._L.rg_contrib_device(self._h, C.c_void_p(rays_ptr), nrec, accum, flags, arr, len(mods),
                                               C.c_void_p(out_ptr)))
+
+    def set_stream(self, stream: int) -> None:
+        """Order device-buffer inputs after the work queued on this HIP stream
+        (a raw hipStream_t handle, e.g. torch.cuda.current_stream().cuda_stream;
+        0: the null stream)."""
+        self._check(self._L.rg_set_stream(self._h, C.c_void_p(stream)))
 
     def contrib_adjoint(self, rays: np.ndarray, mods: list, dlde: np.ndarray, lcol: np.ndarray, mats: list,
                         accum: int = 1, flags: int = 0) -> np.ndarray:

@@ -45,6 +45,8 @@ struct rg_ctx {
   bool shadow_filled = false;
   std::vector<unsigned long long> cast_count;   // closest-hit queries per source's cast rays
   std::vector<char> cast_counted;
+  hipStream_t caller = nullptr;   // device-memory inputs are read after the work queued here
+  hipEvent_t caller_ev = nullptr;
 };
 
 static thread_local std::string g_create_err;
@@ -135,6 +137,7 @@ void rg_ctx_destroy(rg_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->caller_ev) (void)hipEventDestroy(c->caller_ev);
   c->arena.release();
   c->scene_mem.release();
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -609,10 +612,24 @@ int rg_contrib(rg_ctx* c, const double* rays, long long nrec, int accum, int fla
   });
 }
 
+// The context's stream waits for the work queued so far on the caller's
+// stream, so device inputs written there are complete before they are read.
+static void order_after_caller(rg_ctx* c) {
+  hipck(hipSetDevice(c->device), "hipSetDevice");
+  if (!c->caller_ev) hipck(hipEventCreateWithFlags(&c->caller_ev, hipEventDisableTiming), "event");
+  hipck(hipEventRecord(c->caller_ev, c->caller), "caller event");
+  hipck(hipStreamWaitEvent(c->st, c->caller_ev, 0), "caller wait");
+}
+
+int rg_set_stream(rg_ctx* c, void* stream) {
+  RG_TRY(c, { c->caller = (hipStream_t)stream; });
+}
+
 int rg_contrib_device(rg_ctx* c, const double* d_rays, long long nrec, int accum, int flags, const rg_modifier* mods,
                       int nmods, double* d_out) {
   RG_TRY(c, {
     if (nrec > 0 && (!d_rays || !d_out)) throw std::runtime_error("null device buffer");
+    order_after_caller(c);
     std::unordered_map<std::string, int> slot;
     int ncols = 0;
     std::vector<BinSpec> bins = bin_specs(mods, nmods, slot, ncols);

@@ -1155,11 +1155,11 @@ __device__ __forceinline__ bool build_ray(const RunCfg& c, const Spawn& sp, uint
   return admit(c, r, sp.flags);
 }
 
-__device__ __forceinline__ uint32_t find_spawn(const uint32_t* offs, uint32_t nspawn, uint64_t gid) {
+__device__ __forceinline__ uint32_t find_spawn(const uint64_t* offs, uint32_t nspawn, uint64_t gid) {
   uint32_t lo = 0, hi = nspawn;      // last s with offs[s] <= gid
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
-    if ((uint64_t)offs[mid] <= gid) lo = mid; else hi = mid;
+    if (offs[mid] <= gid) lo = mid; else hi = mid;
   }
   return lo;
 }
@@ -1178,7 +1178,7 @@ struct alignas(16) TraceOut {
 #define RG_TRACE_WAVES 4     // waves per SIMD the register budget of k_trace targets
 #endif
 __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(RG_TRACE_WAVES)))
-k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t nspawn, uint64_t c0,
+k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint64_t* offs, uint32_t nspawn, uint64_t c0,
         uint32_t nthreads, TraceOut* outp) {
   __shared__ int32_t stk[LSTACK * TBLOCK];
   __shared__ float stn[LSTACK * TBLOCK];
@@ -1199,7 +1199,7 @@ k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t
   uint32_t lo = base, hi = (uint32_t)min((uint64_t)nspawn, (uint64_t)base + (gid - gid0) + 1);
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
-    if ((uint64_t)offs[mid] <= gid) lo = mid; else hi = mid;
+    if (offs[mid] <= gid) lo = mid; else hi = mid;
   }
   uint32_t si = lo;
   RayS r;
@@ -1236,7 +1236,7 @@ k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t
 // nearest first, the others pushed farthest first).
 template <int BS, bool BATCH, bool WIDE>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(RG_TRACE_WAVES)))
-k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint32_t nspawn, uint64_t c0,
+k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint64_t* offs, uint32_t nspawn, uint64_t c0,
            uint32_t nthreads, TraceOut* outp, uint32_t* work, uint32_t refill) {
   __shared__ int32_t stk[LSTACK * BS];
   __shared__ float stn[LSTACK * BS];
@@ -1278,7 +1278,7 @@ k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint32_t* offs, uint3
           uint32_t lo = s0, hi = (uint32_t)min((uint64_t)nspawn, (uint64_t)s0 + (gid - gid0) + 1);
           while (hi - lo > 1) {
             uint32_t mid = (lo + hi) >> 1;
-            if ((uint64_t)offs[mid] <= gid) lo = mid; else hi = mid;
+            if (offs[mid] <= gid) lo = mid; else hi = mid;
           }
           si = lo;
           RayS r;
@@ -1545,7 +1545,7 @@ void shadow_cast(const DevScene& S, const double* d_rays, const int32_t* d_src, 
 #endif
 template <bool ADJ>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RG_SHADE_WAVES))) k_shade(DevScene Sarg, RunCfg c, const Spawn* in, const PathRec* pin,
-                                                 const uint32_t* offs, uint64_t c0, uint32_t nthreads,
+                                                 const uint64_t* offs, uint64_t c0, uint32_t nthreads,
                                                  const TraceOut* tr, Sink sk) {
   uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
   if (t >= nthreads) return;
@@ -1673,6 +1673,47 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(uint32_t* out, uint32_t
     if (base + k < n) out[base + k] += a;
 }
 
+// The same in 64 bits, for child counts whose totals can pass 2^32 (a
+// million -I sensors at -ad 4096 already spawn 2^32 children).
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* sh, uint64_t& total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+    uint64_t a = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  total = sh[SCAN_BLOCK - 1];
+  uint64_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+template <class TI>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_tiles64(const TI* in, uint64_t* out, uint32_t n,
+                                                             uint64_t* tile_sums) {
+  __shared__ uint64_t sh[SCAN_BLOCK];
+  size_t base = (size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint64_t v[SCAN_ITEMS], s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = base + k < n ? (uint64_t)in[base + k] : 0; s += v[k]; }
+  uint64_t total;
+  uint64_t ex = block_excl_scan64(s, sh, total);
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add64(uint64_t* out, uint32_t n, const uint64_t* tile_offs) {
+  size_t base = (size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint64_t a = tile_offs[blockIdx.x];
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    if (base + k < n) out[base + k] += a;
+}
+
 }  // namespace rg
 
 // ---- host side -----------------------------------------------------------------
@@ -1698,6 +1739,26 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& are
   }
 }
 
+// out[i] = sum(in[0..i)) in 64 bits, out[n] = total.  `out` holds n+1 entries.
+template <class TI>
+static void exclusive_scan64(const TI* in, uint64_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level) {
+  if (n == 0) {
+    check(hipMemsetAsync(out, 0, sizeof(uint64_t), st), "scan memset");
+    return;
+  }
+  uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  uint64_t* sums = (uint64_t*)arena.get("scan64_sums" + std::to_string(level), sizeof(uint64_t) * (ntiles + 1), st, true);
+  hipLaunchKernelGGL(k_scan_tiles64<TI>, dim3(ntiles), dim3(SCAN_BLOCK), 0, st, in, out, n, sums);
+  if (ntiles > 1) {
+    uint64_t* soffs = (uint64_t*)arena.get("scan64_offs" + std::to_string(level), sizeof(uint64_t) * (ntiles + 1), st, true);
+    exclusive_scan64<uint64_t>(sums, soffs, ntiles, arena, st, level + 1);
+    hipLaunchKernelGGL(k_scan_add64, dim3(ntiles), dim3(SCAN_BLOCK), 0, st, out, n, soffs);
+    check(hipMemcpyAsync(out + n, soffs + ntiles, sizeof(uint64_t), hipMemcpyDeviceToDevice, st), "scan total");
+  } else {
+    check(hipMemcpyAsync(out + n, sums, sizeof(uint64_t), hipMemcpyDeviceToDevice, st), "scan total");
+  }
+}
+
 void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens, bool imm_irrad, int max_srcs,
             const Tuning& tn, DevArena& arena, hipStream_t st, RenderStats* stats) {
   // Work proceeds in waves of at most `window` children; each child appends
@@ -1714,7 +1775,6 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
 
   const bool adj = cfg.adj != 0;
   struct Batch { Spawn* buf; PathRec* pbuf; uint32_t n; int slot; };
-  std::vector<Batch> stack;
   int nslots = 0;
   std::vector<int> free_slots;
   auto new_slot = [&]() -> int {
@@ -1728,30 +1788,12 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   auto slot_path = [&](int s, size_t cap) -> PathRec* {
     return adj ? (PathRec*)arena.get("path" + std::to_string(s), sizeof(PathRec) * cap, st, false) : nullptr;
   };
-  // Buffers are sized to the work at hand (and kept by the arena for later
-  // calls), so a context that only ever traces a few sensors stays small.
-  size_t batch_cap = (size_t)std::max<uint32_t>(nsens, 1u) * per_child;
-
-  // generation 0
-  {
-    int s = new_slot();
-    Spawn* b = slot_buf(s, batch_cap);
-    PathRec* pb = slot_path(s, batch_cap);
-    check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
-    uint32_t grid = (nsens + BLOCK - 1) / BLOCK;
-    if (imm_irrad) {
-      Sink sk{b, d_cnt, (uint32_t)batch_cap, cfg.err, pb};
-      if (adj) hipLaunchKernelGGL(k_irrad_seed<true>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, d_rays, nsens, 0u, sk);
-      else hipLaunchKernelGGL(k_irrad_seed<false>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, d_rays, nsens, 0u, sk);
-      check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
-      check(hipStreamSynchronize(st), "seed");
-      stack.push_back({b, pb, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s});
-    } else {
-      hipLaunchKernelGGL(k_primary_seed, dim3(grid), dim3(BLOCK), 0, st, cfg, d_rays, nsens, 0u, b);
-      if (pb) check(hipMemsetAsync(pb, 0, sizeof(PathRec) * nsens, st), "memset");   // empty paths
-      stack.push_back({b, pb, nsens, s});
-    }
-  }
+  // Spawn buffers: at most kSlotBytes each (a window's children each append
+  // at most per_child records), so capacities fit 32 bits and scenes with
+  // many light sources shrink their windows instead of failing to allocate.
+  constexpr size_t kSlotBytes = size_t(4) << 30;
+  const uint32_t slot_children = (uint32_t)std::max<size_t>(64, kSlotBytes / (sizeof(Spawn) * per_child));
+  const uint32_t win = std::min(window, slot_children);
   long gens = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   if (stats && stats->timing) {
@@ -1763,98 +1805,126 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   // opened and walked before the next window of the same batch, so live
   // spawn buffers are bounded by the depth of the ray tree, not by the size
   // of a generation.
-  struct Frame { Batch bt; uint32_t* offs; uint64_t total; uint64_t c0; };
+  struct Frame { Batch bt; uint64_t* offs; uint64_t total; uint64_t c0; };
   std::vector<Frame> frames;
   auto open_frame = [&](const Batch& b) {
     if (b.n == 0) { free_slots.push_back(b.slot); return; }
     if (++gens > max_batches) throw std::runtime_error("ray tree did not terminate");
     uint32_t* counts = (uint32_t*)arena.get("counts" + std::to_string(b.slot), sizeof(uint32_t) * (b.n + 1), st, false);
-    uint32_t* offs = (uint32_t*)arena.get("offs" + std::to_string(b.slot), sizeof(uint32_t) * (b.n + 1), st, false);
+    uint64_t* offs = (uint64_t*)arena.get("offs" + std::to_string(b.slot), sizeof(uint64_t) * (b.n + 1), st, false);
     hipLaunchKernelGGL(k_counts, dim3((b.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, b.buf, b.n, counts);
-    exclusive_scan(counts, offs, b.n, arena, st, 0);
-    check(hipMemcpyAsync(h_cnt, offs + b.n, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "total");
+    exclusive_scan64<uint32_t>(counts, offs, b.n, arena, st, 0);
+    check(hipMemcpyAsync(h_cnt, offs + b.n, sizeof(uint64_t), hipMemcpyDeviceToHost, st), "total");
     check(hipStreamSynchronize(st), "scan");
-    frames.push_back({b, offs, (uint64_t)h_cnt[0], 0});
+    const uint64_t total = (uint64_t)h_cnt[0] | ((uint64_t)h_cnt[1] << 32);   // little-endian 64-bit total
+    frames.push_back({b, offs, total, 0});
   };
-  for (const Batch& b : stack) open_frame(b);
-  stack.clear();
-  while (!frames.empty()) {
-    if (frames.back().c0 >= frames.back().total) {
-      free_slots.push_back(frames.back().bt.slot);
-      frames.pop_back();
-      continue;
-    }
-    const Batch bt = frames.back().bt;
-    uint32_t* offs = frames.back().offs;
-    const uint64_t c0 = frames.back().c0;
-    const uint64_t total = frames.back().total;
+  // Generation 0 per chunk of sensors (buffers are sized to the work at hand
+  // and kept by the arena for later calls, so a context that only ever
+  // traces a few sensors stays small); each chunk's tree is walked to the
+  // end before the next chunk is seeded.
+  const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(nsens, slot_children));
+  for (uint32_t s0 = 0; s0 < nsens; s0 += chunk) {
+    const uint32_t ns = std::min(chunk, nsens - s0);
+    const size_t batch_cap = (size_t)ns * per_child;
+    const double* rays0 = d_rays + 6 * (size_t)s0;
     {
-      uint32_t nth = (uint32_t)std::min<uint64_t>(window, total - c0);
-      frames.back().c0 += nth;
-      size_t cap = (size_t)nth * per_child;
       int s = new_slot();
-      Spawn* ob = slot_buf(s, cap);
-      PathRec* opb = slot_path(s, cap);
-      TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * nth, st, false);
-      check(hipMemsetAsync(d_cnt, 0, 2 * sizeof(uint32_t), st), "memset");   // spawn count, work counter
-      Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err, opb};
-      if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
-      const uint32_t tblocks = (nth + TBLOCK - 1) / TBLOCK;
-      if (tn.trace_kernel == 1) {
-        const uint32_t nb = std::min(tblocks, pf_blocks);
-        if (cfg.nscenes && tn.wide)
-          hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
-        else if (cfg.nscenes)
-          hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
-        else if (tn.wide)
-          hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
-        else
-          hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+      Spawn* b = slot_buf(s, batch_cap);
+      PathRec* pb = slot_path(s, batch_cap);
+      check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
+      uint32_t grid = (ns + BLOCK - 1) / BLOCK;
+      if (imm_irrad) {
+        Sink sk{b, d_cnt, (uint32_t)batch_cap, cfg.err, pb};
+        if (adj) hipLaunchKernelGGL(k_irrad_seed<true>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, rays0, ns, s0, sk);
+        else hipLaunchKernelGGL(k_irrad_seed<false>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, rays0, ns, s0, sk);
+        check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
+        check(hipStreamSynchronize(st), "seed");
+        open_frame({b, pb, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s});
       } else {
-        hipLaunchKernelGGL(k_trace, dim3(tblocks), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n, c0, nth, tout);
+        hipLaunchKernelGGL(k_primary_seed, dim3(grid), dim3(BLOCK), 0, st, cfg, rays0, ns, s0, b);
+        if (pb) check(hipMemsetAsync(pb, 0, sizeof(PathRec) * ns, st), "memset");   // empty paths
+        open_frame({b, pb, ns, s});
       }
-      check(hipGetLastError(), "k_trace launch");
-      if (stats && stats->timing) check(hipEventRecord(ev1, st), "event");
-      if (adj)
-        hipLaunchKernelGGL(k_shade<true>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
-                           bt.pbuf, offs, c0, nth, tout, sk);
-      else
-        hipLaunchKernelGGL(k_shade<false>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
-                           bt.pbuf, offs, c0, nth, tout, sk);
-      check(hipGetLastError(), "k_shade launch");
-      if (stats && stats->timing) check(hipEventRecord(ev2, st), "event");
-      check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
-      check(hipStreamSynchronize(st), "bounce");
-      if (stats && stats->timing) {
-        float ms = 0, ms2 = 0;
-        check(hipEventElapsedTime(&ms, ev0, ev1), "event time");
-        check(hipEventElapsedTime(&ms2, ev1, ev2), "event time");
-        stats->bounce_ms += ms;
-        stats->shade_ms += ms2;
+    }
+    while (!frames.empty()) {
+      if (frames.back().c0 >= frames.back().total) {
+        free_slots.push_back(frames.back().bt.slot);
+        frames.pop_back();
+        continue;
       }
-      if (stats) { stats->launches++; stats->threads += nth; }
-      uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)cap);
-      if (nout && cfg.nscenes > 1) {
-        // regroup by scene into the slot's second buffer, which the batch then uses
-        Spawn* sb = (Spawn*)arena.get("spawn_sorted" + std::to_string(s), sizeof(Spawn) * cap, st, false);
-        PathRec* spb = adj ? (PathRec*)arena.get("path_sorted" + std::to_string(s), sizeof(PathRec) * cap, st, false)
-                           : nullptr;
-        uint32_t* hist = (uint32_t*)arena.get("scene_hist", sizeof(uint32_t) * 768, st, true);
-        const uint32_t nb = (nout + SORT_BLOCK * SORT_ITEMS - 1) / (SORT_BLOCK * SORT_ITEMS);
-        hipLaunchKernelGGL(k_scene_hist, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, nout, hist);
-        hipLaunchKernelGGL(k_scene_start, dim3(1), dim3(256), 0, st, hist, hist + 256);
-        hipLaunchKernelGGL(k_scene_scatter, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, opb, nout, hist + 256, hist + 512,
-                           sb, spb);
-        check(hipGetLastError(), "scene sort launch");
-        ob = sb;
-        opb = spb;
+      const Batch bt = frames.back().bt;
+      const uint64_t* offs = frames.back().offs;
+      const uint64_t c0 = frames.back().c0;
+      const uint64_t total = frames.back().total;
+      {
+        uint32_t nth = (uint32_t)std::min<uint64_t>(win, total - c0);
+        frames.back().c0 += nth;
+        size_t cap = (size_t)nth * per_child;
+        int s = new_slot();
+        Spawn* ob = slot_buf(s, cap);
+        PathRec* opb = slot_path(s, cap);
+        TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * nth, st, false);
+        check(hipMemsetAsync(d_cnt, 0, 2 * sizeof(uint32_t), st), "memset");   // spawn count, work counter
+        Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err, opb};
+        if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
+        const uint32_t tblocks = (nth + TBLOCK - 1) / TBLOCK;
+        if (tn.trace_kernel == 1) {
+          const uint32_t nb = std::min(tblocks, pf_blocks);
+          if (cfg.nscenes && tn.wide)
+            hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                               bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+          else if (cfg.nscenes)
+            hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                               bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+          else if (tn.wide)
+            hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                               bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+          else
+            hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                               bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+        } else {
+          hipLaunchKernelGGL(k_trace, dim3(tblocks), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n, c0, nth, tout);
+        }
+        check(hipGetLastError(), "k_trace launch");
+        if (stats && stats->timing) check(hipEventRecord(ev1, st), "event");
+        if (adj)
+          hipLaunchKernelGGL(k_shade<true>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
+                             bt.pbuf, offs, c0, nth, tout, sk);
+        else
+          hipLaunchKernelGGL(k_shade<false>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
+                             bt.pbuf, offs, c0, nth, tout, sk);
+        check(hipGetLastError(), "k_shade launch");
+        if (stats && stats->timing) check(hipEventRecord(ev2, st), "event");
+        check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
+        check(hipStreamSynchronize(st), "bounce");
+        if (stats && stats->timing) {
+          float ms = 0, ms2 = 0;
+          check(hipEventElapsedTime(&ms, ev0, ev1), "event time");
+          check(hipEventElapsedTime(&ms2, ev1, ev2), "event time");
+          stats->bounce_ms += ms;
+          stats->shade_ms += ms2;
+        }
+        if (stats) { stats->launches++; stats->threads += nth; }
+        uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)cap);
+        if (nout && cfg.nscenes > 1) {
+          // regroup by scene into the slot's second buffer, which the batch then uses
+          Spawn* sb = (Spawn*)arena.get("spawn_sorted" + std::to_string(s), sizeof(Spawn) * cap, st, false);
+          PathRec* spb = adj ? (PathRec*)arena.get("path_sorted" + std::to_string(s), sizeof(PathRec) * cap, st, false)
+                             : nullptr;
+          uint32_t* hist = (uint32_t*)arena.get("scene_hist", sizeof(uint32_t) * 768, st, true);
+          const uint32_t nb = (nout + SORT_BLOCK * SORT_ITEMS - 1) / (SORT_BLOCK * SORT_ITEMS);
+          hipLaunchKernelGGL(k_scene_hist, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, nout, hist);
+          hipLaunchKernelGGL(k_scene_start, dim3(1), dim3(256), 0, st, hist, hist + 256);
+          hipLaunchKernelGGL(k_scene_scatter, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, opb, nout, hist + 256, hist + 512,
+                             sb, spb);
+          check(hipGetLastError(), "scene sort launch");
+          ob = sb;
+          opb = spb;
+        }
+        if (nout) open_frame(Batch{ob, opb, nout, s});
+        else free_slots.push_back(s);
       }
-      if (nout) open_frame(Batch{ob, opb, nout, s});
-      else free_slots.push_back(s);
     }
   }
   if (stats) {

@@ -10,6 +10,7 @@ reference's handle_called_process_error produces (anci.py:13-30).
 from __future__ import annotations
 
 import re
+import sys
 import time
 from pathlib import Path
 from typing import Sequence
@@ -27,6 +28,12 @@ _contexts: dict = {}
 
 def _rterr(msg: str, code: int = 1) -> RuntimeError:
     return RuntimeError(f"An error occurred with exit code {code}: {msg}")
+
+
+def _warn(prog: str, msg: str) -> None:
+    """A warning on stderr in the reference's form (error(WARNING, ...),
+    common/rterror.c), which the reference's wrappers let through."""
+    sys.stderr.write(f"{prog}: warning - {msg}\n")
 
 
 def _scene(octree: Path | str) -> "_lib.Context":
@@ -374,7 +381,14 @@ def _contrib_run(argv: list[str], modifiers: list, inp, scene):
     """rcontrib's record loop on the GPU: parse `argv`, map each modifier
     tuple (name, calfile, expression, nbins, binv, param) onto a bin function,
     read the rays and accumulate -c records.  `scene()` returns the committed
-    context.  Returns (matrix, options, modifier specs)."""
+    context.  Returns (matrix, options, modifier specs).
+
+    Records follow rcontrib() and done_contrib() (rt/rcontrib.c:358-427):
+    with -c N (N >= 1) every input ray, a zero-direction one included, is
+    one of its record's N rays and contributes nothing when its direction is
+    zero; a final record with k < N rays is written as the mean of those k,
+    with the reference's warning; with -c 0 zero-direction rays are ignored
+    (with its warning) and all the others sum into one record."""
     try:
         opt = _params.parse(argv, rcontrib=True)
         mods = [_modifier_spec(*m) for m in modifiers]
@@ -386,18 +400,46 @@ def _contrib_run(argv: list[str], modifiers: list, inp, scene):
     opt.params.seed = _seed(opt.uncorrelated)
     inp = inp if isinstance(inp, bytes) else bytes(inp)
     vals = _read_rays(inp, opt.inform)
-    acc = max(opt.accumulate, 1)
-    nrec = len(vals) // acc
-    if nrec * acc != len(vals):
-        vals = vals[:nrec * acc]
+    ncols = 3 * sum(m.nbins for m in mods)
     flags = (_lib.RT_IMM_IRRAD if opt.imm_irrad else 0) | (_lib.RT_LAMB_IRRAD if opt.lamb_irrad else 0)
+    good = (np.linalg.norm(vals[:, 3:], axis=1) > 0) if len(vals) else np.zeros(0, bool)
+    acc = opt.accumulate
+
+    def trace(rays, accum):
+        try:
+            ctx.set_params(opt.params)
+            return ctx.contrib(rays, mods, accum, flags)
+        except RuntimeError as e:
+            raise _rterr(str(e))
+
+    if acc <= 0:
+        if not good.all():
+            _warn("rcontrib", "dummy ray(s) ignored during accumulation")
+        rays = vals[good]
+        if len(rays) == 0:
+            return np.zeros((1, ncols)), opt, mods
+        # one record: the sum of every ray (no division, rt/rc2.c:301-302)
+        return trace(rays, len(rays)) * len(rays), opt, mods
+    n = len(vals)
+    nrec = -(-n // acc)
     if nrec == 0:
-        return np.zeros((0, 3 * sum(m.nbins for m in mods))), opt, mods
-    try:
-        ctx.set_params(opt.params)
-        out = ctx.contrib(vals, mods, acc, flags)
-    except RuntimeError as e:
-        raise _rterr(str(e))
+        return np.zeros((0, ncols)), opt, mods
+    partial = n % acc
+    if partial:
+        _warn("rcontrib", "partial accumulation in final record")
+    if good.all() and not partial:
+        return trace(vals, acc), opt, mods
+    # per-ray values, then the records on the host: zero-direction rays add
+    # nothing, the last record is the mean of the rays it has
+    per = np.zeros((n, ncols))
+    if good.any():
+        per[good] = trace(vals[good], 1)
+    out = np.zeros((nrec, ncols))
+    full = (n // acc) * acc
+    if full:
+        out[:n // acc] = per[:full].reshape(n // acc, acc, ncols).mean(1)
+    if partial:
+        out[-1] = per[full:].mean(0)
     return out, opt, mods
 
 

@@ -148,6 +148,35 @@ def c2_stat():
     _save_stat("c2_stat.npz", pts, 16, v, bounds, nrays, C2_ARGS, sensor_stride=10)
 
 
+RC_SCENE = ("void plastic gray 0 0 5 .5 .5 .5 0 0\n"
+            "gray polygon floor 0 0 12 -4 -4 0 4 -4 0 4 4 0 -4 4 0\n"
+            "void glow skyglow 0 0 4 1 1 1 0\nskyglow source sky 0 0 4 0 0 1 180\n"
+            "void glow groundglow 0 0 4 .5 .5 .5 0\ngroundglow source ground 0 0 4 0 0 -1 180\n")
+
+
+def rc_records():
+    """rcontrib's record loop (rt/rcontrib.c:379-427) at -ab 0 (exact values):
+    11 rays from above the floor, three of them with zero direction, under
+    -c 1, -c 3 (a partial final record of two rays) and -c 0, Reinhart MF:1
+    sky + ground bins; the outputs and the warnings the reference prints."""
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(11, 3))
+    d[:, 2] = np.abs(d[:, 2])
+    d[[2, 5, 9]] = 0.0
+    d[7, 2] = -abs(d[7, 2])                 # one toward the floor
+    rays = np.hstack([np.tile([0.3, -0.2, 1.0], (11, 1)), d])
+    out = {"rays": rays, "scene": np.array(RC_SCENE)}
+    with tempfile.TemporaryDirectory() as td:
+        octf = oconv(RC_SCENE, td)
+        for c in ("1", "3", "0"):
+            r = subprocess.run([str(REF / "rcontrib"), "-h", "-ab", "0", "-c", c, "-f", "reinhartb.cal", "-p", "MF=1",
+                                "-bn", "Nrbins", "-b", "rbin", "-m", "skyglow", "-b", "0", "-bn", "1", "-m",
+                                "groundglow", octf], input=rays_text(rays), capture_output=True, env=ENV, check=True)
+            out[f"c{c}"] = np.array(r.stdout.split(), dtype=np.float64).reshape(-1, 438)
+            out[f"c{c}_stderr"] = np.array(r.stderr.decode())
+    np.savez_compressed(HERE / "rc_records.npz", **out)
+
+
 def bins():
     rng = np.random.default_rng(2024)
     d = rng.normal(size=(3000, 3))
@@ -846,7 +875,7 @@ def image():
 if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
-    jobs = {"config1": config1, "c2_stat": c2_stat, "bins": bins, "stat_room": stat_room, "primitives": primitives,
+    jobs = {"config1": config1, "c2_stat": c2_stat, "rc_records": rc_records, "bins": bins, "stat_room": stat_room, "primitives": primitives,
             "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
             "mesh": mesh, "instance": instance, "sky": sky, "image": image}
     for name in sys.argv[1:] or list(jobs):

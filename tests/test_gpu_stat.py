@@ -106,3 +106,80 @@ def test_shadow_cache_ray_counts(L):
     assert np.allclose(e3, e, rtol=1e-12, atol=1e-15)
     assert c.last_nrays() > ni
     c.close()
+
+
+@pytest.mark.parametrize("c", ["1", "3", "0"])
+def test_record_loop_matches_reference(tmp_path, capfd, c):
+    """rcontrib's record loop (rt/rcontrib.c:379-427) through Rcontrib:
+    zero-direction rays give zero contributions but keep their place in a
+    record, a partial final record is the mean of its rays (with the
+    reference's warning), -c 0 sums every non-zero ray into one record
+    (rc_records.npz, -ab 0, exact)."""
+    from pyradiance_amd.rt import Rcontrib
+    _say(f"record loop -c {c}")
+    g = np.load(GOLDEN / "rc_records.npz")
+    (tmp_path / "s.rad").write_text(str(g["scene"]))
+    inp = ("\n".join(" ".join(f"{v:.17g}" for v in r) for r in g["rays"]) + "\n").encode()
+    rc = Rcontrib(inp, tmp_path / "s.rad", params=["-ab", "0", "-c", c, "-h"])
+    rc.add_modifier(modifier="skyglow", calfile="reinhartb.cal", param="MF=1", nbins="Nrbins", binv="rbin")
+    rc.add_modifier(modifier="groundglow", nbins="1", binv="0")
+    v = np.array(rc().split(), float).reshape(-1, 438)
+    ref = g[f"c{c}"]
+    assert v.shape == ref.shape
+    assert np.allclose(v, ref, rtol=1e-6, atol=1e-9)
+    err = capfd.readouterr().err
+    for w in ("partial accumulation in final record", "dummy ray(s) ignored during accumulation"):
+        assert (w in err) == (w in str(g[f"c{c}_stderr"])), w
+
+
+def test_device_rays_written_by_torch_are_ordered(L):
+    """rg_contrib_device reads rays that a torch kernel has only just
+    written on the caller's stream: the context's stream waits for that
+    stream (rg_set_stream; default the null stream) before its first read."""
+    import torch
+    _say("device rays ordered after the caller's stream")
+    rad, sens = scenes.config2(1)
+    c = L.Context(0)
+    c.load_text(rad).commit()
+    p = L.default_params(True)
+    p.ambounce, p.ambdiv, p.minweight, p.seed = 1, 64, 1e-3, 4
+    c.set_params(p)
+    mods = _sky_mods(L)
+    rays = sens[:400]
+    host = c.contrib(rays, mods, 1, L.RT_IMM_IRRAD)
+    base = torch.from_numpy(rays).cuda()
+    for _ in range(3):
+        torch.cuda.synchronize()
+        # a long kernel queue on the caller's stream, then the rays written last
+        m = torch.randn(4096, 4096, device="cuda")
+        for _k in range(8):
+            m = m @ m
+            m = m / m.norm()
+        d_rays = base * (1.0 + (0.0 * m[0, 0].double()).nan_to_num(0.0))
+        d_out = torch.full(host.shape, -1.0, dtype=torch.float64, device="cuda")
+        c.contrib_device(d_rays.data_ptr(), len(rays), mods, d_out.data_ptr(), 1, L.RT_IMM_IRRAD)
+        assert np.allclose(d_out.cpu().numpy(), host, rtol=1e-12, atol=1e-15)
+    c.close()
+
+
+def test_child_offsets_past_2_to_32(L):
+    """2^20 -I sensors under a unit glow sky at -ab 1 -ad 4096 spawn exactly
+    2^32 ambient children: their offsets need 64 bits.  Every sensor must
+    read E = pi exactly (stratified cosine sampling of a constant sky) and
+    every child must be traced."""
+    _say("2^32 children: 2^20 sensors at -ad 4096")
+    c = L.Context(0)
+    c.load_text("void glow skyglow 0 0 4 1 1 1 0\nskyglow source sky 0 0 4 0 0 1 180\n").commit()
+    p = L.default_params(True)
+    p.ambounce, p.ambdiv, p.minweight, p.seed = 1, 4096, 1e-6, 2
+    c.set_params(p)
+    m = L.RgModifier()
+    m.name, m.bin_kind, m.nbins, m.cbin = b"skyglow", 0, 1, 0
+    n = 1 << 20
+    rays = np.tile([0.0, 0.0, 0.0, 0.0, 0.0, 1.0], (n, 1))
+    out = c.contrib(rays, [m], 1, L.RT_IMM_IRRAD)
+    nr = c.last_nrays()
+    c.close()
+    _say(f"  rays {nr}")
+    assert nr == 1 << 32
+    assert np.allclose(out, np.pi, rtol=1e-7, atol=0)   # FP32 coefficients
