@@ -405,14 +405,51 @@ def boxes_rad(rng: np.random.Generator, nboxes: int, room: Room, mod_prefix: str
     ], float)
     verts = lo[:, None, None, :] + faces[None] * size[:, None, None, :]     # (n, 6, 4, 3)
     coords = verts.reshape(nboxes * 6, 12)
-    lines = []
-    for i in range(nboxes * 6):
-        b = i // 6
-        lines.append(f"{mod_prefix}_m{mats[b]} polygon {mod_prefix}{i}\n0\n0\n12 ")
-    nums = np.char.mod("%.6f", coords)
-    body = [" ".join(r) for r in nums]
-    out.append("".join(h + b + "\n" for h, b in zip(lines, body)))
+    out.append(_polygon_records(coords, mats.repeat(6), f"{mod_prefix}_m", mod_prefix).decode())
     return "".join(out)
+
+
+def _fixed6(v: np.ndarray) -> np.ndarray:
+    """Non-negative values below 1000 as fixed-width 'ddd.dddddd' ASCII
+    (uint8 array (..., 10)); the decimal value is v rounded to 1e-6, as '%.6f'
+    gives it."""
+    if v.size and (v.min() < 0 or v.max() >= 999.9999995):
+        raise ValueError("coordinate outside [0, 1000)")
+    q = np.rint(v * 1e6).astype(np.int64)
+    digits = np.empty(v.shape + (10,), np.uint8)
+    for k, pw in enumerate((10 ** 8, 10 ** 7, 10 ** 6)):
+        digits[..., k] = 48 + (q // pw) % 10
+    digits[..., 3] = ord(".")
+    for k, pw in enumerate((10 ** 5, 10 ** 4, 10 ** 3, 10 ** 2, 10, 1)):
+        digits[..., 4 + k] = 48 + (q // pw) % 10
+    return digits
+
+
+def _polygon_records(coords: np.ndarray, mats: np.ndarray, mat_prefix: str, name_prefix: str) -> bytes:
+    """Quadrilateral polygons as scene text, built as fixed-width byte records
+    with numpy (the float formatting of millions of vertices in Python is the
+    slow part of the large workloads): 'MAT polygon NAME\n0\n0\n12 x y z ...'."""
+    n = len(coords)
+    if n == 0:
+        return b""
+    mw = len(str(int(mats.max())))
+    nw = len(str(n - 1))
+    head = np.frombuffer(f"{mat_prefix}{'0' * mw} polygon {name_prefix}{'0' * nw}\n0\n0\n12".encode(), np.uint8)
+    rec = len(head) + 12 * 11 + 1
+    buf = np.empty((n, rec), np.uint8)
+    buf[:, :len(head)] = head
+    def put_int(col, width, vals):
+        for k in range(width):
+            buf[:, col + k] = 48 + (vals // 10 ** (width - 1 - k)) % 10
+    put_int(len(mat_prefix), mw, mats.astype(np.int64))
+    put_int(len(f"{mat_prefix}{'0' * mw} polygon {name_prefix}"), nw, np.arange(n, dtype=np.int64))
+    digits = _fixed6(coords)                          # (n, 12, 10)
+    body = np.empty((n, 12, 11), np.uint8)
+    body[:, :, 0] = ord(" ")
+    body[:, :, 1:] = digits
+    buf[:, len(head):len(head) + 132] = body.reshape(n, 132)
+    buf[:, -1] = ord("\n")
+    return buf.tobytes()
 
 
 def config3(seed: int = 0, nscenes: int = 64, prims=(1_000, 500_000), sensors=(100, 20_000)):

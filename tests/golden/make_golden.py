@@ -177,6 +177,93 @@ def rc_records():
     np.savez_compressed(HERE / "rc_records.npz", **out)
 
 
+ROUGH_SCENE = ("void plastic rplast 0 0 5 .6 .5 .4 .05 .15\n"
+               "void metal rmetal 0 0 5 .8 .7 .6 .9 .08\n"
+               "void trans rtrans 0 0 7 .7 .8 .9 .04 .1 .6 .5\n"
+               "rplast polygon floor 0 0 12 -10 -10 0 10 -10 0 10 10 0 -10 10 0\n"
+               "rmetal polygon wall 0 0 12 5 -3 0 5 3 0 5 3 3 5 -3 3\n"
+               "rtrans polygon pane 0 0 12 -5 -3 0.5 -5 -3 3 -5 3 3 -5 3 0.5\n"
+               "void glow skyglow 0 0 4 1 1 1 0\nskyglow source sky 0 0 4 0 0 1 180\n"
+               "void glow groundglow 0 0 4 1 1 1 0\ngroundglow source ground 0 0 4 0 0 -1 180\n")
+ROUGH_ARGS = ["-h", "-ab", "0", "-st", "0", "-c", "100", "-f", "reinhartb.cal", "-p", "MF=2", "-bn", "Nrbins",
+              "-b", "rbin", "-m", "skyglow", "-b", "0", "-bn", "1", "-m", "groundglow"]
+
+
+def rough_stat():
+    """Rough specular sampling (gaussamp, rt/normal.c:364-495): primary rays
+    onto a rough plastic floor (specularity .05, roughness .15), a rough
+    metal wall (.9, .08) and a rough trans pane (.04, .1; transmitted
+    specular .5) at several incidence angles, rcontrib -ab 0 -st 0, so every
+    value is a Gaussian lobe sample binned by direction (Reinhart MF:2 sky +
+    ground).  48 records of -c 100 per ray."""
+    rays = []
+    for th in (0, 30, 60, 75):                      # onto the floor at (0, 0, 0)
+        t = np.radians(th)
+        d = np.array([np.sin(t), 0.3 * np.sin(t), -np.cos(t)])
+        d /= np.linalg.norm(d)
+        rays.append(np.r_[-3 * d, d])
+    for th in (0, 45, 70):                          # onto the metal wall at (5, 0, 1.5)
+        t = np.radians(th)
+        d = np.array([np.cos(t), np.sin(t), -0.1])
+        d /= np.linalg.norm(d)
+        rays.append(np.r_[np.array([5, 0, 1.5]) - 2 * d, d])
+    for th in (0, 40):                              # through the pane at (-5, 0, 2)
+        t = np.radians(th)
+        d = np.array([-np.cos(t), np.sin(t), 0.05])
+        d /= np.linalg.norm(d)
+        rays.append(np.r_[np.array([-5, 0, 2]) - 2 * d, d])
+    pts = np.array(rays)
+    with tempfile.TemporaryDirectory() as td:
+        octf = oconv(ROUGH_SCENE, td)
+        v, bounds, nrays = _stat_runs_acc(octf, pts, 48, 100, ROUGH_ARGS)
+    _save_stat("rough_stat.npz", pts, 48, v, bounds, nrays, ROUGH_ARGS, scene=np.array(ROUGH_SCENE), accum=100)
+
+
+def _stat_runs_acc(octf, pts, R, acc, args):
+    """R records of `acc` in-stream rays per point (rcontrib -c acc)."""
+    import time
+    chunks = [c for c in np.array_split(np.arange(len(pts)), min(len(pts), os.cpu_count() or 1)) if len(c)]
+    procs = []
+    with tempfile.TemporaryDirectory() as td:
+        for k, ch in enumerate(chunks):
+            pf = Path(td) / f"in{k}.txt"
+            pf.write_bytes(rays_text(np.repeat(pts[ch], R * acc, axis=0)))
+            fin = open(pf, "rb")
+            fout = open(Path(td) / f"out{k}.txt", "wb")
+            procs.append((subprocess.Popen([str(REF / "rcontrib_count"), "-n", "1", *args, octf], stdin=fin,
+                                           stdout=fout, stderr=subprocess.PIPE, env=ENV), fin, fout))
+            time.sleep(1.05)
+        vals, nrays = [], []
+        for k, ((pr, fin, fout), ch) in enumerate(zip(procs, chunks)):
+            _, err = pr.communicate()
+            fin.close()
+            fout.close()
+            assert pr.returncode == 0, err.decode()
+            out = (Path(td) / f"out{k}.txt").read_bytes()
+            vals.append(np.array(out.split(), dtype=np.float64).reshape(len(ch), R, -1))
+            nrays.append([int(ln.split()[1]) for ln in err.decode().splitlines() if ln.startswith("NRAYS")][0])
+    bounds = np.array([0] + [int(c[-1]) + 1 for c in chunks])
+    return np.concatenate(vals), bounds, np.array(nrays, dtype=np.int64)
+
+
+C4_ARGS = ["-h", "-I+", "-ab", "3", "-ad", "2048", "-aa", "0", "-c", "1",
+           "-f", "reinhartb.cal", "-p", "MF=1", "-bn", "Nrbins", "-b", "rbin", "-m", "skyglow",
+           "-b", "0", "-bn", "1", "-m", "groundglow"]
+
+
+def c4_stat():
+    """Config 4's forward pass (24 rough plastic groups, specularity up to
+    .05 and roughness up to .2, glass window, binned sky; rcontrib -I+ -ab 3
+    -ad 2048 with rcontrib's other defaults): every 10th of its 1000
+    sensors, 16 in-stream estimates each."""
+    rad, sens, _ = scenes.config4(seed=0)
+    pts = sens[::10]
+    with tempfile.TemporaryDirectory() as td:
+        octf = oconv(rad, td)
+        v, bounds, nrays = _stat_runs(octf, pts, 16, C4_ARGS)
+    _save_stat("c4_stat.npz", pts, 16, v, bounds, nrays, C4_ARGS, sensor_stride=10)
+
+
 def bins():
     rng = np.random.default_rng(2024)
     d = rng.normal(size=(3000, 3))
@@ -875,7 +962,8 @@ def image():
 if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
-    jobs = {"config1": config1, "c2_stat": c2_stat, "rc_records": rc_records, "bins": bins, "stat_room": stat_room, "primitives": primitives,
+    jobs = {"config1": config1, "c2_stat": c2_stat, "rc_records": rc_records, "rough_stat": rough_stat,
+            "c4_stat": c4_stat, "bins": bins, "stat_room": stat_room, "primitives": primitives,
             "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
             "mesh": mesh, "instance": instance, "sky": sky, "image": image}
     for name in sys.argv[1:] or list(jobs):

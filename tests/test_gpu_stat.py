@@ -183,3 +183,143 @@ def test_child_offsets_past_2_to_32(L):
     _say(f"  rays {nr}")
     assert nr == 1 << 32
     assert np.allclose(out, np.pi, rtol=1e-7, atol=0)   # FP32 coefficients
+
+
+@pytest.mark.parametrize("alpha", [0.08, 0.2])
+def test_gauss_lobe_closed_form(L, alpha):
+    """gauss_sample's reflected and transmitted lobes at normal incidence
+    against their closed forms (tests/lobes.py), 2^20 samples each: every
+    altitude row within 5 binomial sigma."""
+    import lobes
+    _say(f"gaussian lobe closed form, alpha {alpha}")
+    n = 1 << 20
+    c = L.Context(0)
+    c.load_text(lobes.lobe_scene(alpha)).commit()
+    p = L.default_params(True)
+    p.ambounce, p.specthresh, p.specjitter, p.minweight, p.seed = 0, 0.0, 1.0, 1e-6, 9
+    c.set_params(p)
+    m = _sky_mods(L)[0]
+    g = L.RgModifier()
+    g.name, g.bin_kind, g.nbins, g.mf, g.rhs = b"groundglow", 1, 145, 1.0, 1.0
+    g.nrm[:] = (0, 0, 1)
+    g.up[:] = (0, 1, 0)
+    for ray, trans, cols in ((lobes.REFLECT_RAY, False, slice(0, 435)), (lobes.TRANSMIT_RAY, True, slice(435, 870))):
+        out = c.contrib(np.tile(ray, (n, 1)), [m, g], n, 0)
+        rows = lobes.row_fractions(out[0, cols][0::3])
+        _say(f"  {'T' if trans else 'R'} rows {np.round(rows, 5)}")
+        assert abs(rows.sum() - 1.0) < 1e-6
+        lobes.check(rows, n, alpha, trans)
+    c.close()
+
+
+def test_rough_specular_matches_reference(L):
+    """Rough plastic, metal and trans at several incidence angles, rcontrib
+    -ab 0 -st 0 (every value a Gaussian lobe sample, binned on a Reinhart
+    MF:2 sky), 48 records of 100 samples per ray against the reference's
+    (rough_stat.npz): within 3 sigma."""
+    _say("rough specular vs reference")
+    g = np.load(GOLDEN / "rough_stat.npz")
+    R, acc = int(g["repeats"]), int(g["accum"])
+    c = L.Context(0)
+    c.load_text(str(g["scene"])).commit()
+    p = L.default_params(True)
+    p.ambounce, p.specthresh, p.seed = 0, 0.0, 13
+    c.set_params(p)
+    rays = np.repeat(g["points"], R * acc, axis=0)
+    out = c.contrib(rays, _sky_mods(L, 2), acc, 0).reshape(len(g["points"]), R, -1)
+    nr = c.last_nrays()
+    c.close()
+    s = stat3s.summary(g["mean"], g["var"], R, out.mean(1), out.var(1, ddof=1), R)
+    _say(f"  {s}; rays {nr} vs reference {int(g['nrays'].sum())}")
+    stat3s.check(s, frac=0.015)
+    assert abs(nr - int(g["nrays"].sum())) <= 0.01 * int(g["nrays"].sum())
+
+
+def test_config4_forward_matches_reference_at_3_sigma(L):
+    """Config 4's forward pass (24 rough plastic groups, -I+ -ab 3 -ad 2048,
+    rcontrib's defaults otherwise): 100 sensors x 16 GPU estimates against
+    16 reference estimates (c4_stat.npz)."""
+    _say("config4 forward vs reference, 100 sensors x 16 estimates")
+    g = np.load(GOLDEN / "c4_stat.npz")
+    R = int(g["repeats"])
+    rad, sens, _ = scenes.config4(seed=0)
+    assert np.array_equal(sens[::int(g["sensor_stride"])], g["points"])
+    c = L.Context(0)
+    c.load_text(rad).commit()
+    p = L.default_params(True)
+    p.ambounce, p.ambdiv, p.seed = 3, 2048, 17
+    c.set_params(p)
+    out = c.contrib(np.repeat(g["points"], R, axis=0), _sky_mods(L), 1, L.RT_IMM_IRRAD)
+    out = out.reshape(len(g["points"]), R, -1)
+    nr = c.last_nrays()
+    c.close()
+    s = stat3s.summary(g["mean"], g["var"], R, out.mean(1), out.var(1, ddof=1), R)
+    _say(f"  {s}; rays {nr} vs reference {int(g['nrays'].sum())}")
+    stat3s.check(s)
+    assert abs(nr - int(g["nrays"].sum())) <= 0.01 * int(g["nrays"].sum())
+
+
+def test_config4_forward_matches_oracle_same_samples(L, oracle):
+    """Config 4's forward against the oracle drawing the same samples (8
+    sensors): record totals within 0.2 %, entries equal except where a
+    path diverged on an FP32-vs-FP64 edge decision."""
+    _say("config4 forward vs oracle, same samples")
+    rad, sens, _ = scenes.config4(seed=0)
+    pts = sens[::125]
+    c = L.Context(0)
+    c.load_text(rad).commit()
+    p = L.default_params(True)
+    p.ambounce, p.ambdiv, p.seed = 3, 2048, 17
+    c.set_params(p)
+    out = c.contrib(pts, _sky_mods(L), 1, L.RT_IMM_IRRAD)
+    nr = c.last_nrays()
+    c.close()
+    omods = [oracle.mod("skyglow", 1, 145), oracle.mod("groundglow", 0, 1)]
+    orc, onr = oracle.Scene(rad).contrib(pts, oracle.params(ambounce=3, ambdiv=2048, minweight=2e-3, dstrsrc=0.9,
+                                                            specthresh=0.02, seed=17), omods, 1, 8)
+    _say(f"  rays {nr} vs oracle {onr}")
+    assert np.allclose(out.sum(1), orc.sum(1), rtol=2e-3)
+    assert np.isclose(out, orc, rtol=1e-4, atol=1e-9).mean() > 0.98
+    assert abs(nr - onr) <= 0.002 * onr
+
+
+def test_config3_full_workload(L, oracle):
+    """Config 3 at its specified size: 64 rooms of 1k-500k primitives (6.7 M
+    in all) and 110-19,729 sensors each (278k), with a sun per room, in one
+    batched rcontrib -I+ -ab 3 -ad 1024 launch: every scene's records are
+    valid irradiance splits (non-negative, total <= pi under a unit sky),
+    and the two smallest scenes agree with the oracle drawing the same
+    samples (scene k of a batch draws the samples of seed + k)."""
+    _say("config3 full size: generating 64 scenes")
+    sc = scenes.config3(0)
+    nprim = np.array([r.count(" polygon ") for r, _ in sc])
+    _say(f"  {nprim.sum()} primitives, {sum(len(s) for _, s in sc)} sensors; committing")
+    ctxs = []
+    for rad, _ in sc:
+        c = L.Context(0)
+        c.load_text(rad).commit()
+        ctxs.append(c)
+    p = L.default_params(True)
+    p.ambounce, p.ambdiv, p.seed = 3, 1024, 31
+    ctxs[0].set_params(p)
+    mods = _sky_mods(L)
+    _say("  tracing")
+    res = L.contrib_batch(ctxs, [s for _, s in sc], mods, 1, L.RT_IMM_IRRAD)
+    nr = ctxs[0].last_nrays()
+    _say(f"  rays {nr}")
+    for k, ((_, sens), out) in enumerate(zip(sc, res)):
+        assert out.shape == (len(sens), 438), k
+        assert (out >= 0).all(), k
+        tot = out.sum(1) / 3
+        assert (tot <= np.pi * 1.02).all() and tot.max() > 1e-3, k
+    for c in ctxs:
+        c.close()
+    omods = [oracle.mod("skyglow", 1, 145), oracle.mod("groundglow", 0, 1)]
+    for k in np.argsort(nprim)[:2]:
+        rad, sens = sc[k]
+        _say(f"  scene {k} ({nprim[k]} primitives) vs oracle")
+        orc, _ = oracle.Scene(rad).contrib(sens[:4], oracle.params(ambounce=3, ambdiv=1024, minweight=2e-3,
+                                                                   dstrsrc=0.9, specthresh=0.02, seed=31 + int(k)),
+                                           omods, 1, 8)
+        assert np.allclose(res[k][:4].sum(1), orc.sum(1), rtol=2e-3), k
+        assert np.isclose(res[k][:4], orc, rtol=1e-4, atol=1e-9).mean() > 0.98, k

@@ -85,6 +85,26 @@ def test_shadow_cache_ray_counts_match_reference(oracle):
     assert abs(ni - int(g["nrays_irradiance"])) <= 2
 
 
+@pytest.mark.parametrize("alpha", [0.08, 0.2])
+def test_gauss_lobe_closed_form(oracle, alpha):
+    """gaussamp's reflected and transmitted lobes at normal incidence against
+    their closed forms (tests/lobes.py): fraction of samples per 12-degree
+    altitude row of a Reinhart sky within 5 binomial sigma; a planted
+    roughness slip (alpha^2 used where alpha is) fails the same check."""
+    import lobes
+    n = 20000
+    sc = oracle.Scene(lobes.lobe_scene(alpha))
+    mods = [oracle.mod("skyglow", 1, 145), oracle.mod("groundglow", 1, 145, nrm=(0.0, 0.0, 1.0))]
+    p = oracle.params(ambounce=0, specthresh=0.0, specjitter=1.0, minweight=1e-6, seed=5)
+    for ray, trans, cols in ((lobes.REFLECT_RAY, False, slice(0, 435)), (lobes.TRANSMIT_RAY, True, slice(435, 870))):
+        out, _ = sc.contrib(np.tile(ray, (n, 1)), p, mods, n, 0)   # one record: the mean of n samples
+        rows = lobes.row_fractions(out[0, cols][0::3])
+        assert abs(rows.sum() - 1.0) < 1e-9           # every sample lands in the lobe's hemisphere
+        lobes.check(rows, n, alpha, trans)
+        with pytest.raises(AssertionError):
+            lobes.check(rows, n, alpha * alpha, trans)
+
+
 def test_config1_planted_mistake_is_caught(oracle):
     """A wrong glass index (a plausible slip) must break the closed-form check."""
     rad, sens, sun = scenes.config1(0)
