@@ -355,6 +355,21 @@ CompiledScene compile_scene(const SceneDB& db) {
     const RObj& m = objs[mo];
     int kind = material_kind(m.type);
     if (kind < 0) objerr(m, "material type \"" + m.type + "\" is not supported on the GPU path");
+    // a material's own modifiers (patterns, textures: raytexture() and the
+    // pattern functions of rt/p_func.c, rt/t_func.c) change its colour or
+    // normal; the GPU material set has none, so such a material is refused
+    // instead of being shaded as if unmodified
+    for (int q = m.omod, guard = 0; q >= 0;) {
+      if (++guard > 10000) objerr(m, "modifier loop");
+      const RObj& pm = objs[q];
+      if (pm.type == "alias" && !pm.sargs.empty()) {
+        q = db.modifier(pm.sargs[0]);
+        if (q < 0) objerr(pm, "bad reference");
+        continue;
+      }
+      if (pm.type == "alias") { q = pm.omod; continue; }
+      objerr(m, "modifier \"" + pm.name + "\" of type \"" + pm.type + "\" (a pattern or texture) is not supported on the GPU path");
+    }
     Material M;
     memset(&M, 0, sizeof M);
     M.kind = kind;

@@ -264,6 +264,25 @@ def c4_stat():
     _save_stat("c4_stat.npz", pts, 16, v, bounds, nrays, C4_ARGS, sensor_stride=10)
 
 
+# the reference's own rcontrib test (tests/test_rcontrib.py:9-76): its
+# octree (tests/Resources/contrib.oct, kept under refres/), rays, options and
+# modifiers, the in-process manager's settings written as rcontrib options
+CONTRIB_RAYS = np.array([[10.0, 10.0, 3.0, 0.0, 0.0, 1.0], [4.0, 5.0, 3.0, 0.0, 0.0, 1.0]])
+CONTRIB_ARGS = ["-h", "-I+", "-u+", "-dj", "0.9", "-dr", "3", "-dp", "512", "-ds", "0.2", "-st", "0", "-ss", "1",
+                "-lr", "-10", "-lw", "2e-3", "-ar", "256", "-ad", "350", "-ab", "6", "-aa", "0", "-as", "0",
+                "-c", "1", "-f", "reinhartb.cal", "-p", "MF=1,rNx=0,rNy=0,rNz=-1,Ux=0,Uy=1,Uz=0,RHS=+1",
+                "-bn", "1", "-b", "if(-Dx*0-Dy*0-Dz*-1,0,-1)", "-m", "groundglow",
+                "-bn", "Nrbins", "-b", "rbin", "-m", "skyglow"]
+
+
+def contrib_stat():
+    """The reference's rcontrib test case: 48 in-stream estimates of its two
+    rays (its assertion: the result sums to more than 5)."""
+    octf = str(HERE / "refres" / "contrib.oct")
+    v, bounds, nrays = _stat_runs(octf, CONTRIB_RAYS, 48, CONTRIB_ARGS, nproc=2)
+    _save_stat("contrib_stat.npz", CONTRIB_RAYS, 48, v, bounds, nrays, CONTRIB_ARGS)
+
+
 def bins():
     rng = np.random.default_rng(2024)
     d = rng.normal(size=(3000, 3))
@@ -963,7 +982,7 @@ if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
     jobs = {"config1": config1, "c2_stat": c2_stat, "rc_records": rc_records, "rough_stat": rough_stat,
-            "c4_stat": c4_stat, "bins": bins, "stat_room": stat_room, "primitives": primitives,
+            "c4_stat": c4_stat, "contrib_stat": contrib_stat, "bins": bins, "stat_room": stat_room, "primitives": primitives,
             "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
             "mesh": mesh, "instance": instance, "sky": sky, "image": image}
     for name in sys.argv[1:] or list(jobs):

@@ -323,3 +323,62 @@ def test_config3_full_workload(L, oracle):
                                            omods, 1, 8)
         assert np.allclose(res[k][:4].sum(1), orc.sum(1), rtol=2e-3), k
         assert np.isclose(res[k][:4], orc, rtol=1e-4, atol=1e-9).mean() > 0.98, k
+
+
+def test_reference_rcontrib_test_case(L):
+    """The reference's own rcontrib test (tests/test_rcontrib.py:9-76): its
+    octree (contrib.oct: floor and ceiling under a glow sky and ground), its
+    two -I rays, -ab 6 -ad 350 -lw 2e-3 -st 0 -dj 0.9, the uniform-surface
+    ground bin if(-Dx*0-Dy*0-Dz*-1,0,-1) and Reinhart MF:1 sky bins, through
+    Rcontrib on the octree file.  Its assertion (the result sums to more
+    than 5) holds, and 48 estimates agree with 48 reference estimates
+    within 3 sigma."""
+    from pyradiance_amd.rt import Rcontrib
+    _say("reference rcontrib test case (contrib.oct)")
+    g = np.load(GOLDEN / "contrib_stat.npz")
+    args = [str(a) for a in g["args"]]                 # the reference run's options
+    opts = args[1:args.index("-f")]
+    params = args[args.index("-p") + 1]
+    rays0 = g["points"]
+
+    def run(rays):
+        inp = ("\n".join(" ".join(f"{v:.17g}" for v in r) for r in rays) + "\n").encode()
+        rc = Rcontrib(inp, GOLDEN / "refres" / "contrib.oct", params=["-h", *opts])
+        rc.add_modifier(modifier="groundglow", nbins="1", binv="if(-Dx*0-Dy*0-Dz*-1,0,-1)")
+        rc.add_modifier(modifier="skyglow", calfile="reinhartb.cal", param=params, nbins="Nrbins", binv="rbin")
+        return np.array(rc().split(), float).reshape(len(rays), -1)
+
+    one = run(rays0)
+    assert one.shape == (2, 438)
+    assert one.sum() > 5.0                           # the reference test's assertion
+    R = int(g["repeats"])
+    out = run(np.repeat(rays0, R, axis=0)).reshape(2, R, -1)
+    s = stat3s.summary(g["mean"], g["var"], R, out.mean(1), out.var(1, ddof=1), R)
+    _say(f"  sum {one.sum():.4f}; {s}")
+    stat3s.check(s, frac=0.02)
+
+
+def test_reference_rtrace_octree_is_refused_clearly(L):
+    """The reference's rtrace test octree (tests/Resources/trace.oct) holds a
+    gendaylit sky whose glow is modified by a brightfunc pattern
+    (perezlum.cal); patterns are outside the GPU material set, so rtrace()
+    raises the reference's error shape naming the modifier instead of
+    shading the sky as if unmodified."""
+    from pyradiance_amd.rt import rtrace
+    _say("reference rtrace test octree (trace.oct)")
+    rays = b"1 2 3 0 0 1\n4 5 6 0 1 0\n"
+    with pytest.raises(RuntimeError, match=r'^An error occurred with exit code 1: .*"skyfunc".*brightfunc.*not supported'):
+        rtrace(rays, GOLDEN / "refres" / "trace.oct", params=["-ab", "0"])
+
+
+def test_octree_file_through_rtrace(L, tmp_path):
+    """A top-level octree file (the reference oconv's, frozen) through
+    rtrace(): the ceiling seen from below is black at -ab 0 (no sources),
+    a ray escaping sideways sees the glow sky (radiance 1)."""
+    from pyradiance_amd.rt import rtrace
+    _say("rtrace on contrib.oct")
+    out = rtrace(b"10 10 3 0 0 1\n10 10 3 1 0 0.01\n", GOLDEN / "refres" / "contrib.oct",
+                 params=["-ab", "0", "-h"])
+    v = np.array(out.split(), float).reshape(2, 3)
+    assert np.allclose(v[0], 0.0)
+    assert np.allclose(v[1], 1.0)

@@ -143,3 +143,14 @@ def test_instance_materials(sc):
     assert names.count("glassy") == 7
     assert names.count("wood") == 2 * 2 and names.count("steel") == 2 * 4 and names.count("frosted") == 2
     assert names.count("paint") == 2 and names.count("base") == 1
+
+
+def test_reference_octrees_host(sc):
+    """The reference's test octrees (tests/Resources/, kept under
+    golden/refres/): contrib.oct (floor, ceiling, glow sky and ground)
+    compiles to its two polygons; trace.oct's gendaylit sky glow is modified
+    by a brightfunc pattern, which the GPU material set does not have, and is
+    refused by name rather than shaded as if unmodified."""
+    assert compile_files(sc, [GOLDEN / "refres" / "contrib.oct"]) == 2
+    with pytest.raises(RuntimeError, match=r'"skyfunc" of type "brightfunc"'):
+        compile_files(sc, [GOLDEN / "refres" / "trace.oct"])
