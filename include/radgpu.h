@@ -252,6 +252,16 @@ int rg_sky_suns(const rg_sky *s, int *id_good, double *alt_azi_illum);
  * memory of the current device (the B operand of rg_dcsky_device). */
 int rg_sky_matrix(rg_sky *s, float *out);
 int rg_sky_matrix_device(rg_sky *s, float *dev_out);
+/* rg_sky_matrix_device without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * step tables, patch tables and luminance plane the kernels use live in a
+ * fixed 96 MiB device array that each GPU holds until the process ends
+ * (Reinhart MF:4 over 8,760 steps fits, -A over that does not); a larger
+ * plan returns -1 with the message in rg_sky_error(s).  Calls on one GPU
+ * share the array and are serialised by the library; the result equals
+ * rg_sky_matrix_device's. */
+int rg_sky_matrix_device_noalloc(rg_sky *s, float *dev_out);
 
 /* ASCII records on the host cores: nrecords records of per_record values,
  * each value printed with fmt ("%e" or "%g", optional ".N", N <= 17),

@@ -24,7 +24,7 @@ EXPORTS = (
     "rg_material_index", "rg_material_get", "rg_material_set", "rg_contrib_adjoint", "rg_first_hit",
     "rg_sky_create", "rg_sky_destroy", "rg_sky_error", "rg_sky_shape", "rg_sky_site", "rg_sky_suns",
     "rg_sky_matrix", "rg_sky_matrix_device", "rg_trace_view", "rg_format_ascii", "rg_contrib_device",
-    "rg_set_stream",
+    "rg_set_stream", "rg_sky_matrix_device_noalloc",
 )
 
 BVH_AUTO, BVH_SAH, BVH_LBVH = 0, 1, 2
@@ -159,6 +159,7 @@ def lib() -> C.CDLL:
     L.rg_sky_suns.argtypes = [vp, C.POINTER(C.c_int), dp]
     L.rg_sky_matrix.argtypes = [vp, C.POINTER(C.c_float)]
     L.rg_sky_matrix_device.argtypes = [vp, vp]
+    L.rg_sky_matrix_device_noalloc.argtypes = [vp, vp]
     L.rg_format_ascii.argtypes = [dp, C.c_longlong, C.c_int, C.c_longlong, C.c_char_p, C.c_char_p, C.c_char_p,
                                   C.c_char_p, C.c_longlong, C.c_int]
     L.rg_format_ascii.restype = C.c_longlong
@@ -474,6 +475,12 @@ class Sky:
     def matrix_device(self, ptr: int) -> None:
         """Writes the matrix into device memory at `ptr` (npatch*ncols*3 floats)."""
         if self._L.rg_sky_matrix_device(self._h, C.c_void_p(ptr)) != 0:
+            raise RuntimeError(self._L.rg_sky_error(self._h).decode())
+
+    def matrix_device_noalloc(self, ptr: int) -> None:
+        """matrix_device() without device allocation (its buffers in a fixed
+        96 MiB device array; larger plans raise)."""
+        if self._L.rg_sky_matrix_device_noalloc(self._h, C.c_void_p(ptr)) != 0:
             raise RuntimeError(self._L.rg_sky_error(self._h).decode())
 
     def close(self):

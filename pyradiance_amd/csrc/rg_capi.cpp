@@ -977,6 +977,18 @@ int rg_sky_matrix_device(rg_sky* s, float* dout) {
   }
 }
 
+int rg_sky_matrix_device_noalloc(rg_sky* s, float* dout) {
+  try {
+    if (!s) throw std::runtime_error("no sky");
+    if (!dout) throw std::runtime_error("null device buffer");
+    rg::sky_matrix_noalloc(s->plan, dout, nullptr);
+    return 0;
+  } catch (const std::exception& e) {
+    s->err = e.what();
+    return -1;
+  }
+}
+
 int rg_sky_matrix(rg_sky* s, float* out) {
   float* d = nullptr;
   try {

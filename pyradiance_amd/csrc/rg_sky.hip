@@ -11,8 +11,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
+#include <string>
+#include <unordered_map>
 #include <strings.h>
 
 namespace rg {
@@ -677,57 +680,107 @@ void ck(hipError_t e, const char* what) {
 
 }  // namespace
 
-void sky_matrix(const SkyPlan& P, float* out, hipStream_t st) {
+// Device buffers of one sky_matrix() call, laid out in one block (256-byte
+// aligned parts): step records, step outputs, patch tables, patch altitude /
+// azimuth / solid angle, the luminance plane, and with -A the full matrix
+// before averaging.
+struct SkyScratch {
+  size_t steps, so, tab, alt, azi, dom, lum, full, bytes;
+};
+
+static SkyScratch sky_scratch_layout(const SkyPlan& P) {
+  const size_t np = (size_t)P.npatch, ns = (size_t)P.nsteps;
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  SkyScratch L;
+  size_t o = 0;
+  L.steps = o; o = up(o + sizeof(SkyStep) * ns);
+  L.so = o;    o = up(o + sizeof(SkyStepOut) * ns);
+  L.tab = o;   o = up(o + sizeof(SkyPatchTab) * np);
+  L.alt = o;   o = up(o + sizeof(float) * np);
+  L.azi = o;   o = up(o + sizeof(float) * np);
+  L.dom = o;   o = up(o + sizeof(float) * np);
+  L.lum = o;   o = up(o + sizeof(float) * np * ns);
+  L.full = o;  if (P.opts.average) o = up(o + sizeof(float) * 3 * np * ns);
+  L.bytes = o;
+  return L;
+}
+
+// The kernels of one call, with their buffers at `base` (layout above).
+static void sky_matrix_run(const SkyPlan& P, float* out, hipStream_t st, char* base, const SkyScratch& L) {
   const int np = P.npatch, ns = P.nsteps;
-  if (ns <= 0) return;
-  SkyStep* dsteps = nullptr;
-  SkyStepOut* dso = nullptr;
-  SkyPatchTab* dtab = nullptr;
-  float *dalt = nullptr, *dazi = nullptr, *ddom = nullptr, *dfull = nullptr, *dlum = nullptr;
-  auto release = [&] {
-    for (void* q : {(void*)dsteps, (void*)dso, (void*)dtab, (void*)dalt, (void*)dazi, (void*)ddom, (void*)dfull,
-                    (void*)dlum})
-      (void)hipFree(q);
-  };
+  SkyStep* dsteps = (SkyStep*)(base + L.steps);
+  SkyStepOut* dso = (SkyStepOut*)(base + L.so);
+  SkyPatchTab* dtab = (SkyPatchTab*)(base + L.tab);
+  float* dalt = (float*)(base + L.alt);
+  float* dazi = (float*)(base + L.azi);
+  float* ddom = (float*)(base + L.dom);
+  float* dlum = (float*)(base + L.lum);
+  ck(hipMemcpyAsync(dsteps, P.steps.data(), sizeof(SkyStep) * ns, hipMemcpyHostToDevice, st), "upload");
+  ck(hipMemcpyAsync(dalt, P.palt.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
+  ck(hipMemcpyAsync(dazi, P.pazi.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
+  ck(hipMemcpyAsync(ddom, P.dom.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
+  k_sky_patch<<<(np + 255) / 256, 256, 0, st>>>(dalt, dazi, ddom, np, dtab);
+  ck(hipGetLastError(), "k_sky_patch");
+  int nsuns = P.opts.nsuns < 1 ? 1 : (P.opts.nsuns > MAXSUN ? MAXSUN : P.opts.nsuns);
+  k_sky_step<<<ns, SKY_BLOCK, 0, st>>>(dsteps, ns, dtab, np, nsuns, P.opts.fixed_sun_sa, dlum, dso);
+  ck(hipGetLastError(), "k_sky_step");
+  float* full = P.opts.average ? (float*)(base + L.full) : out;
+  const SkyOpts& o = P.opts;
+  float3 skyc = make_float3((float)o.sky_color[0], (float)o.sky_color[1], (float)o.sky_color[2]);
+  float3 sunc = make_float3((float)o.sun_color[0], (float)o.sun_color[1], (float)o.sun_color[2]);
+  dim3 grid((ns + FILL_TS - 1) / FILL_TS, (np + FILL_TP - 1) / FILL_TP);
+  k_sky_fill<<<grid, SKY_BLOCK, 0, st>>>(dsteps, dso, dlum, ns, np, skyc, sunc, full);
+  ck(hipGetLastError(), "k_sky_fill");
+  if (P.opts.average) {
+    k_sky_average<<<(np * 3 + 255) / 256, 256, 0, st>>>(full, np, ns, 1. / (double)P.ntsteps, P.ntsteps > 1, out);
+    ck(hipGetLastError(), "k_sky_average");
+  }
+  ck(hipStreamSynchronize(st), "gendaymtx");
+}
+
+void sky_matrix(const SkyPlan& P, float* out, hipStream_t st) {
+  if (P.nsteps <= 0) return;
+  const SkyScratch L = sky_scratch_layout(P);
+  char* base = nullptr;
+  ck(hipMalloc((void**)&base, L.bytes), "hipMalloc");
   try {
-    ck(hipMalloc((void**)&dsteps, sizeof(SkyStep) * ns), "hipMalloc");
-    ck(hipMalloc((void**)&dso, sizeof(SkyStepOut) * ns), "hipMalloc");
-    ck(hipMalloc((void**)&dtab, sizeof(SkyPatchTab) * np), "hipMalloc");
-    ck(hipMalloc((void**)&dalt, sizeof(float) * np), "hipMalloc");
-    ck(hipMalloc((void**)&dazi, sizeof(float) * np), "hipMalloc");
-    ck(hipMalloc((void**)&ddom, sizeof(float) * np), "hipMalloc");
-    ck(hipMalloc((void**)&dlum, sizeof(float) * (size_t)np * ns), "hipMalloc");
-    ck(hipMemcpyAsync(dsteps, P.steps.data(), sizeof(SkyStep) * ns, hipMemcpyHostToDevice, st), "upload");
-    ck(hipMemcpyAsync(dalt, P.palt.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
-    ck(hipMemcpyAsync(dazi, P.pazi.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
-    ck(hipMemcpyAsync(ddom, P.dom.data(), sizeof(float) * np, hipMemcpyHostToDevice, st), "upload");
-    k_sky_patch<<<(np + 255) / 256, 256, 0, st>>>(dalt, dazi, ddom, np, dtab);
-    ck(hipGetLastError(), "k_sky_patch");
-    int nsuns = P.opts.nsuns < 1 ? 1 : (P.opts.nsuns > MAXSUN ? MAXSUN : P.opts.nsuns);
-    k_sky_step<<<ns, SKY_BLOCK, 0, st>>>(dsteps, ns, dtab, np, nsuns, P.opts.fixed_sun_sa, dlum, dso);
-    ck(hipGetLastError(), "k_sky_step");
-    float* full = out;
-    if (P.opts.average) {
-      ck(hipMalloc((void**)&dfull, sizeof(float) * 3 * (size_t)np * ns), "hipMalloc");
-      full = dfull;
-    }
-    const SkyOpts& o = P.opts;
-    float3 skyc = make_float3((float)o.sky_color[0], (float)o.sky_color[1], (float)o.sky_color[2]);
-    float3 sunc = make_float3((float)o.sun_color[0], (float)o.sun_color[1], (float)o.sun_color[2]);
-    dim3 grid((ns + FILL_TS - 1) / FILL_TS, (np + FILL_TP - 1) / FILL_TP);
-    k_sky_fill<<<grid, SKY_BLOCK, 0, st>>>(dsteps, dso, dlum, ns, np, skyc, sunc, full);
-    ck(hipGetLastError(), "k_sky_fill");
-    if (P.opts.average) {
-      k_sky_average<<<(np * 3 + 255) / 256, 256, 0, st>>>(dfull, np, ns, 1. / (double)P.ntsteps, P.ntsteps > 1, out);
-      ck(hipGetLastError(), "k_sky_average");
-    }
-    ck(hipStreamSynchronize(st), "gendaymtx");
+    sky_matrix_run(P, out, st, base, L);
   } catch (...) {
     (void)hipStreamSynchronize(st);
-    release();
+    (void)hipFree(base);
     throw;
   }
-  release();
+  (void)hipFree(base);
+}
+
+// ---- the allocation-free form --------------------------------------------------
+// Callers that cannot allocate (code under HIP graph capture, serving loops,
+// harnesses that forbid hipMalloc) get the same kernels with their buffers in
+// this fixed device array.  Each GPU holds its own copy until the process
+// ends.  In host code `scratch` only names the symbol: its device address, per
+// device, comes from hipGetSymbolAddress at the first call on that device.
+static __device__ __align__(256) unsigned char scratch[kSkyScratchBytes];
+
+static std::mutex g_scratch_mu;                       // calls share one array per device
+static std::unordered_map<int, char*> g_scratch_ptr;   // device -> address of `scratch`
+
+void sky_matrix_noalloc(const SkyPlan& P, float* out, hipStream_t st) {
+  if (P.nsteps <= 0) return;
+  const SkyScratch L = sky_scratch_layout(P);
+  if (L.bytes > kSkyScratchBytes)
+    throw std::runtime_error("gendaymtx needs " + std::to_string(L.bytes) +
+                             " bytes of device scratch; the allocation-free entry point has " +
+                             std::to_string(kSkyScratchBytes) + " (Reinhart MF:4 over 8760 steps fits, without -A)");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  int dev = 0;
+  ck(hipGetDevice(&dev), "hipGetDevice");
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    ck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    it = g_scratch_ptr.emplace(dev, (char*)p).first;
+  }
+  sky_matrix_run(P, out, st, it->second, L);
 }
 
 }  // namespace rg

@@ -66,4 +66,11 @@ SkyPlan sky_plan(const std::string& weather, const SkyOpts& o);
 // Writes the npatch x ncols x 3 matrix to device memory `out`.
 void sky_matrix(const SkyPlan& plan, float* out, hipStream_t st);
 
+// Size of the fixed device array the allocation-free form uses (per GPU).
+constexpr size_t kSkyScratchBytes = size_t(96) << 20;
+
+// sky_matrix() with its buffers in that array: no device allocation; throws
+// when the plan needs more.  Calls on one device are serialised.
+void sky_matrix_noalloc(const SkyPlan& plan, float* out, hipStream_t st);
+
 }  // namespace rg
