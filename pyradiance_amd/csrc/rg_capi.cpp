@@ -396,6 +396,7 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   cfg.nrays = d_nr;
   cfg.err = d_err;
   cfg.accum = accum;
+  cfg.child_order = c->tuning.child_order;
   cfg.nscenes = 0;
   cfg.scenes = nullptr;
   cfg.scene_start = nullptr;
@@ -820,6 +821,9 @@ int rg_set_tuning(rg_ctx* c, const char* key, long long value) {
     } else if (k == "wide") {
       if (value != 0 && value != 1) throw std::runtime_error("wide is 0 or 1");
       t.wide = (int)value;
+    } else if (k == "child_order") {
+      if (value != 0 && value != 1) throw std::runtime_error("child_order is 0 or 1");
+      t.child_order = (int)value;
     } else if (k == "shadow_cache") {
       if (value != 0 && value != 1) throw std::runtime_error("shadow_cache is 0 or 1");
       t.shadow_cache = (int)value;

@@ -1136,9 +1136,25 @@ __device__ __forceinline__ bool build_ray(const RunCfg& c, const Spawn& sp, uint
   r.scene = sp.scene;
   if (sp.ns) {
     // stratified cosine-weighted hemisphere sample (rt/ambcomp.c:177-216)
-    uint32_t ns = sp.ns;
-    uint32_t i = ci / ns, j = ci % ns;
-    r.key = child_key(sp.key, ((sp.flags & SPF_BACK) ? SLOT_HEMI_BACK : SLOT_HEMI_FRONT) + ci);
+    const uint32_t ns = sp.ns;
+    uint32_t i, j;
+    if (c.child_order) {
+      // children in strips of 8 stratum rows, column by column within a
+      // strip: 64 consecutive children (a wavefront) cover an 8 x 8 block of
+      // strata, a compact patch of the hemisphere, instead of a row of
+      // strata sweeping across it
+      const uint32_t strip = min(ci / (8u * ns), (ns - 1u) / 8u);
+      const uint32_t r0 = ci - 8u * ns * strip;
+      const uint32_t h = min(8u, ns - 8u * strip);
+      j = r0 / h;
+      i = 8u * strip + r0 % h;
+    } else {
+      i = ci / ns;
+      j = ci % ns;
+    }
+    // the sample of stratum (i, j) is drawn from its own key, whatever order
+    // the children are handed out in
+    r.key = child_key(sp.key, ((sp.flags & SPF_BACK) ? SLOT_HEMI_BACK : SLOT_HEMI_FRONT) + i * ns + j);
     double ss0 = rnd(r.key, RN_AMB0), ss1 = rnd(r.key, RN_AMB1);
     double spt[2];
     square2disk(spt, (j + ss1) / ns, (i + ss0) / ns);

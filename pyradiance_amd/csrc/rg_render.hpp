@@ -89,6 +89,7 @@ struct Tuning {
   uint32_t max_big = 8;          // room-sized primitives kept out of the BVH when there are at most this many (applies at commit)
   int wide = 1;                  // persistent traversal walks the four-wide tree when the scene has one
   int shadow_cache = 1;          // pre-empt shadow rays by the reference's shadow cache (rt/srcobstr.c)
+  int child_order = 1;           // hemisphere children in 8 x 8 blocks of strata (1) or by stratum row (0)
 };
 
 // max_srcs: the largest source count over the scenes of a batch (sizes the

@@ -93,6 +93,7 @@ struct RunCfg {
   int32_t ncols;       // output doubles per sensor
   int32_t nbinspec;
   int32_t accum;       // rays per output record (rcontrib -c)
+  int32_t child_order; // hemisphere children handed out 0: by stratum row, 1: in 8 x 8 blocks of strata
   int32_t nscenes;     // >0: batched launch over `scenes`
   const DevScene* scenes;      // per-scene tables of a batched launch
   const uint32_t* scene_start; // first input ray of each scene (nscenes+1)

@@ -467,3 +467,20 @@ def test_empty_and_smallest_inputs(L, tmp_path):
     B = np.ones((300, 1, 3), dtype=np.float32)
     for mode in (1, 3):
         assert np.allclose(L.dcsky(A, B, mode), 300.0, rtol=1e-5)
+
+
+def test_child_order_changes_only_the_order(L):
+    """Handing a spawn's hemisphere children out in 8 x 8 blocks of strata
+    instead of by stratum row changes which rays share a wavefront, not
+    which samples are drawn: same records (to summation order), same rays."""
+    _say("child order: blocks of strata vs rows")
+    rad, sens = scenes.config2(0)
+    c = _ctx(L, rad)
+    c.set_params(_rc_params(L, ambounce=3, ambdiv=1024, minweight=1e-3, seed=8))
+    res = []
+    for order in (0, 1):
+        c.set_tuning(child_order=order)
+        res.append((c.contrib(sens[::25], _sky_mods(L), 1, L.RT_IMM_IRRAD), c.last_nrays()))
+    c.close()
+    assert res[0][1] == res[1][1]
+    assert np.allclose(res[0][0], res[1][0], rtol=1e-9, atol=1e-15)
