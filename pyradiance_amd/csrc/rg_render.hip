@@ -46,6 +46,7 @@ struct Sink {
   uint32_t cap;
   uint32_t* err;
   PathRec* pbuf;       // adjoint replay: path record of each spawn (else null)
+  uint32_t* counts;    // child count of each spawn, compact (the scan's input)
 };
 
 // Append a spawn; in adjoint replay also its path: the parent's plus `entry`
@@ -54,6 +55,7 @@ __device__ __forceinline__ void emit(const Sink& sk, const Spawn& s, const RayS&
   uint32_t i = wave_alloc(sk.count);
   if (i < sk.cap) {
     sk.buf[i] = s;
+    sk.counts[i] = s.count;
     if (sk.pbuf) sk.pbuf[i] = path_push(r.path, entry);
   } else {
     atomicOr(sk.err, ERR_CAPACITY);
@@ -1086,7 +1088,7 @@ __global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene Sarg, RunCfg c, c
 
 // Generation 0 for radiance rays: primary rays as explicit spawns.
 __global__ void __launch_bounds__(BLOCK) k_primary_seed(RunCfg c, const double* rays, uint32_t nsens, uint32_t s0,
-                                                        Spawn* out) {
+                                                        Spawn* out, uint32_t* counts) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nsens) return;
   const double* ry = rays + 6 * (size_t)i;
@@ -1111,6 +1113,7 @@ __global__ void __launch_bounds__(BLOCK) k_primary_seed(RunCfg c, const double* 
   s.scene = (uint8_t)scene;
   s.key = sensor_key(c.p.seed + scene, s0 + i - (c.nscenes ? c.scene_start[scene] : 0));
   out[i] = s;
+  counts[i] = 1u;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_counts(const Spawn* sp, uint32_t n, uint32_t* counts) {
@@ -1755,6 +1758,72 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& are
   }
 }
 
+// The same scan with its length n = min(*d_n, cap) read on the device, so
+// that it follows the kernel that produced the counts without a host round
+// trip: persistent blocks walk the tiles, one block scans the tile sums (and
+// writes the total to *d_total and out[n]), persistent blocks add the tile
+// offsets.
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_tiles_dev(const uint32_t* in, uint64_t* out, const uint32_t* d_n,
+                                                               uint32_t cap, uint64_t* tile_sums) {
+  __shared__ uint64_t sh[SCAN_BLOCK];
+  const uint32_t n = min(*d_n, cap);
+  const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const size_t base = (size_t)tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint64_t v[SCAN_ITEMS], s = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = base + k < n ? (uint64_t)in[base + k] : 0; s += v[k]; }
+    uint64_t total;
+    uint64_t ex = block_excl_scan64(s, sh, total);
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      if (base + k < n) out[base + k] = ex;
+      ex += v[k];
+    }
+    if (threadIdx.x == 0) tile_sums[tile] = total;
+  }
+}
+
+constexpr int SUMS_BLOCK = 1024;
+__global__ void __launch_bounds__(SUMS_BLOCK) k_scan_sums_dev(const uint32_t* d_n, uint32_t cap, uint64_t* tile_sums,
+                                                              uint64_t* out, uint64_t* d_total) {
+  __shared__ uint64_t sh[SUMS_BLOCK];
+  const uint32_t n = min(*d_n, cap);
+  const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  const uint32_t per = (ntiles + SUMS_BLOCK - 1) / SUMS_BLOCK;
+  const uint32_t t = threadIdx.x, b0 = t * per, b1 = min(b0 + per, ntiles);
+  uint64_t s = 0;
+  for (uint32_t k = b0; k < b1; ++k) s += tile_sums[k];
+  sh[t] = s;
+  __syncthreads();
+  for (int off = 1; off < SUMS_BLOCK; off <<= 1) {
+    uint64_t a = (int)t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  uint64_t ex = sh[t] - s;
+  for (uint32_t k = b0; k < b1; ++k) {
+    const uint64_t v = tile_sums[k];
+    tile_sums[k] = ex;
+    ex += v;
+  }
+  if (t == SUMS_BLOCK - 1) {
+    out[n] = sh[t];
+    *d_total = sh[t];
+  }
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add_dev(uint64_t* out, const uint32_t* d_n, uint32_t cap,
+                                                             const uint64_t* tile_offs) {
+  const uint32_t n = min(*d_n, cap);
+  const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const size_t base = (size_t)tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const uint64_t a = tile_offs[tile];
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+      if (base + k < n) out[base + k] += a;
+  }
+}
+
 // out[i] = sum(in[0..i)) in 64 bits, out[n] = total.  `out` holds n+1 entries.
 template <class TI>
 static void exclusive_scan64(const TI* in, uint64_t* out, uint32_t n, DevArena& arena, hipStream_t st, int level) {
@@ -1823,18 +1892,28 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   // of a generation.
   struct Frame { Batch bt; uint64_t* offs; uint64_t total; uint64_t c0; };
   std::vector<Frame> frames;
-  auto open_frame = [&](const Batch& b) {
+  // compact child counts and their exclusive scan, per slot
+  auto slot_counts = [&](int s, size_t cap) -> uint32_t* {
+    return (uint32_t*)arena.get("counts" + std::to_string(s), sizeof(uint32_t) * (cap + 1), st, false);
+  };
+  auto slot_offs = [&](int s, size_t cap) -> uint64_t* {
+    return (uint64_t*)arena.get("offs" + std::to_string(s), sizeof(uint64_t) * (cap + 1), st, false);
+  };
+  // A batch whose size the host knows: scan its child counts (recounted
+  // from the records after a scene regroup), read the total.
+  auto open_frame = [&](const Batch& b, bool recount) {
     if (b.n == 0) { free_slots.push_back(b.slot); return; }
     if (++gens > max_batches) throw std::runtime_error("ray tree did not terminate");
-    uint32_t* counts = (uint32_t*)arena.get("counts" + std::to_string(b.slot), sizeof(uint32_t) * (b.n + 1), st, false);
-    uint64_t* offs = (uint64_t*)arena.get("offs" + std::to_string(b.slot), sizeof(uint64_t) * (b.n + 1), st, false);
-    hipLaunchKernelGGL(k_counts, dim3((b.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, b.buf, b.n, counts);
+    uint32_t* counts = slot_counts(b.slot, b.n);
+    uint64_t* offs = slot_offs(b.slot, b.n);
+    if (recount) hipLaunchKernelGGL(k_counts, dim3((b.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, b.buf, b.n, counts);
     exclusive_scan64<uint32_t>(counts, offs, b.n, arena, st, 0);
     check(hipMemcpyAsync(h_cnt, offs + b.n, sizeof(uint64_t), hipMemcpyDeviceToHost, st), "total");
     check(hipStreamSynchronize(st), "scan");
     const uint64_t total = (uint64_t)h_cnt[0] | ((uint64_t)h_cnt[1] << 32);   // little-endian 64-bit total
     frames.push_back({b, offs, total, 0});
   };
+  const uint32_t scan_grid = 2u * (uint32_t)std::max(1, ncu);
   // Generation 0 per chunk of sensors (buffers are sized to the work at hand
   // and kept by the arena for later calls, so a context that only ever
   // traces a few sensors stays small); each chunk's tree is walked to the
@@ -1848,19 +1927,20 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       int s = new_slot();
       Spawn* b = slot_buf(s, batch_cap);
       PathRec* pb = slot_path(s, batch_cap);
+      uint32_t* bc = slot_counts(s, batch_cap);
       check(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t), st), "memset");
       uint32_t grid = (ns + BLOCK - 1) / BLOCK;
       if (imm_irrad) {
-        Sink sk{b, d_cnt, (uint32_t)batch_cap, cfg.err, pb};
+        Sink sk{b, d_cnt, (uint32_t)batch_cap, cfg.err, pb, bc};
         if (adj) hipLaunchKernelGGL(k_irrad_seed<true>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, rays0, ns, s0, sk);
         else hipLaunchKernelGGL(k_irrad_seed<false>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, rays0, ns, s0, sk);
         check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
         check(hipStreamSynchronize(st), "seed");
-        open_frame({b, pb, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s});
+        open_frame({b, pb, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s}, false);
       } else {
-        hipLaunchKernelGGL(k_primary_seed, dim3(grid), dim3(BLOCK), 0, st, cfg, rays0, ns, s0, b);
+        hipLaunchKernelGGL(k_primary_seed, dim3(grid), dim3(BLOCK), 0, st, cfg, rays0, ns, s0, b, bc);
         if (pb) check(hipMemsetAsync(pb, 0, sizeof(PathRec) * ns, st), "memset");   // empty paths
-        open_frame({b, pb, ns, s});
+        open_frame({b, pb, ns, s}, false);
       }
     }
     while (!frames.empty()) {
@@ -1880,9 +1960,10 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
         int s = new_slot();
         Spawn* ob = slot_buf(s, cap);
         PathRec* opb = slot_path(s, cap);
+        uint32_t* ocnt = slot_counts(s, cap);
         TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * nth, st, false);
         check(hipMemsetAsync(d_cnt, 0, 2 * sizeof(uint32_t), st), "memset");   // spawn count, work counter
-        Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err, opb};
+        Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err, opb, ocnt};
         if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
         const uint32_t tblocks = (nth + TBLOCK - 1) / TBLOCK;
         if (tn.trace_kernel == 1) {
@@ -1912,7 +1993,20 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
                              bt.pbuf, offs, c0, nth, tout, sk);
         check(hipGetLastError(), "k_shade launch");
         if (stats && stats->timing) check(hipEventRecord(ev2, st), "event");
-        check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
+        if (cfg.nscenes <= 1) {
+          // the new batch's child counts scanned on the device right after
+          // shading: one read-back (spawn count and child total) per window
+          uint64_t* ooffs = slot_offs(s, cap);
+          uint64_t* sums = (uint64_t*)arena.get("scan_dev_sums", sizeof(uint64_t) * (cap / SCAN_TILE + 2), st, false);
+          hipLaunchKernelGGL(k_scan_tiles_dev, dim3(scan_grid), dim3(SCAN_BLOCK), 0, st, ocnt, ooffs, d_cnt,
+                             (uint32_t)cap, sums);
+          hipLaunchKernelGGL(k_scan_sums_dev, dim3(1), dim3(SUMS_BLOCK), 0, st, d_cnt, (uint32_t)cap, sums, ooffs,
+                             (uint64_t*)(d_cnt + 2));
+          hipLaunchKernelGGL(k_scan_add_dev, dim3(scan_grid), dim3(SCAN_BLOCK), 0, st, ooffs, d_cnt, (uint32_t)cap,
+                             sums);
+          check(hipGetLastError(), "scan launch");
+        }
+        check(hipMemcpyAsync(h_cnt, d_cnt, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
         check(hipStreamSynchronize(st), "bounce");
         if (stats && stats->timing) {
           float ms = 0, ms2 = 0;
@@ -1923,7 +2017,17 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
         }
         if (stats) { stats->launches++; stats->threads += nth; }
         uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)cap);
-        if (nout && cfg.nscenes > 1) {
+        if (cfg.nscenes <= 1) {
+          if (nout) {
+            if (++gens > max_batches) throw std::runtime_error("ray tree did not terminate");
+            const uint64_t total = (uint64_t)h_cnt[2] | ((uint64_t)h_cnt[3] << 32);
+            frames.push_back({Batch{ob, opb, nout, s}, slot_offs(s, cap), total, 0});
+          } else {
+            free_slots.push_back(s);
+          }
+          continue;
+        }
+        if (nout) {
           // regroup by scene into the slot's second buffer, which the batch then uses
           Spawn* sb = (Spawn*)arena.get("spawn_sorted" + std::to_string(s), sizeof(Spawn) * cap, st, false);
           PathRec* spb = adj ? (PathRec*)arena.get("path_sorted" + std::to_string(s), sizeof(PathRec) * cap, st, false)
@@ -1935,11 +2039,10 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
           hipLaunchKernelGGL(k_scene_scatter, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, opb, nout, hist + 256, hist + 512,
                              sb, spb);
           check(hipGetLastError(), "scene sort launch");
-          ob = sb;
-          opb = spb;
+          open_frame(Batch{sb, spb, nout, s}, true);
+        } else {
+          free_slots.push_back(s);
         }
-        if (nout) open_frame(Batch{ob, opb, nout, s});
-        else free_slots.push_back(s);
       }
     }
   }
