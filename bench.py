@@ -55,6 +55,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """CPUs this process may run on: the affinity mask, capped by a cgroup
+    CPU quota when one is set.  Returns (usable, in the mask, quota or None)."""
+    n = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = max(1, -(-int(q) // int(per)))
+    except (OSError, ValueError):
+        try:
+            q = int(Path("/sys/fs/cgroup/cpu/cpu.cfs_quota_us").read_text())
+            per = int(Path("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read_text())
+            if q > 0:
+                quota = max(1, -(-q // per))
+        except (OSError, ValueError):
+            pass
+    return (min(n, quota) if quota else n), n, quota
+
+
 def workload(seed: int = 0):
     rad, sensors = scenes.config2(seed)
     mods = []
@@ -75,8 +95,10 @@ def gpu_params():
     return p
 
 
-def cpu_baseline(rad: str, sensors: np.ndarray, nsample: int, cores: int):
-    """Reference rcontrib (oracle/_ref) on `nsample` sensors of the same workload."""
+def cpu_baseline(rad: str, sensors: np.ndarray, nsample: int, cores: int, gpu_count=None):
+    """Reference rcontrib (oracle/_ref) on `nsample` sensors of the same
+    workload, over `cores` processes; gpu_count(sample) -> the GPU's ray
+    count on the same sample, reported beside the reference's."""
     ref = ROOT / "oracle" / "_ref"
     if not (ref / "rcontrib").exists() or not (ref / "oconv").exists():
         return None
@@ -133,14 +155,21 @@ def cpu_baseline(rad: str, sensors: np.ndarray, nsample: int, cores: int):
             walls.append(w)
             loads.append(wall(inp.split(b"\n")[0] + b"\n")[0])
         w, load = float(np.median(walls)), float(np.median(loads))
+        usable, visible, quota = host_cores()
+        gr = int(gpu_count(sample)) if gpu_count else None
         return {
             "value": nrays / max(w - load, 1e-9) / 1e6,
             "unit": "Mrays/s",
             "cores": cores,
             "kind": "reference",
-            "sample": f"{nsample} of the 2500 sensors ({nrays} rays, counted exactly by {len(procs)} parallel "
-                      f"-n 1 counting runs in {t_count:.1f}s); rcontrib -n {cores} median wall {w:.2f}s of 3 "
-                      f"minus median octree load {load:.2f}s of 3 (one-sensor run)",
+            "sample": f"{nsample} of the 2500 sensors, {nsample / cores:.0f} records per process ({nrays} rays, "
+                      f"counted exactly by {len(procs)} parallel -n 1 counting runs in {t_count:.1f}s); rcontrib "
+                      f"-n {cores} median wall {w:.2f}s of 3 minus median octree load {load:.2f}s of 3 "
+                      f"(one-sensor run)",
+            "host_cpus": {"affinity": visible, "cgroup_quota": quota, "used": cores},
+            "rays_reference": nrays,
+            "rays_gpu_same_sample": gr,
+            "rays_gpu_over_reference": gr / nrays if gr else None,
         }
 
 
@@ -197,6 +226,19 @@ def committed_traffic(bvh: str, walk: str):
     if kv.get("bvh") != bvh or kv.get("walk", "binary") != walk or "traffic_bytes_per_ray" not in kv:
         return None
     return float(kv["traffic_bytes_per_ray"])
+
+
+def committed_lane_util():
+    """VALU lane utilisation of the persistent traversal kernel from the
+    committed SQ counter summary (its machine-readable line
+    `lane_utilisation: <fraction>`), or None."""
+    f = ROOT / "profiles" / "k_trace_config2_sq_occupancy.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("lane_utilisation:"):
+            return float(line.partition(":")[2])
+    return None
 
 
 def dcsky_bench(dc: np.ndarray, seed: int = 0):
@@ -315,6 +357,108 @@ def batched_bench(nsun: int = 255, nscene2: int = 4):
                             "batched_ms": t_batch * 1e3, "plain_Mrays_s": nr / t_plain / 1e6,
                             "batched_Mrays_s": nb / t_batch / 1e6, "speedup": t_plain / t_batch}
     res["config3"] = config3_bench()
+    return res
+
+
+def config1_bench(nsun: int = 1000, cores: int = 1, cpu: bool = True):
+    """Config 1 over `nsun` sun positions (scenes.config1(seed), seeds 0 ..
+    nsun-1): rtrace -I -ab 0 -dj 0 -ds 0, 77 sensors per sun.
+
+    GPU, per call: load + commit + trace of one sun's scene, one call after
+    another.  GPU, steady state: the committed scenes traced in batches of
+    up to 250 (rg_trace_batch).  Reference: oconv + rtrace for every sun,
+    the suns spread over `cores` concurrent pipelines; per call is the
+    median of one sun's oconv + rtrace wall, whole job the wall of all of
+    them, trace-only the whole-job wall minus that of the same pipelines
+    tracing one sensor each (octree build and load).  Rays: the
+    reference's own counter (rtrace_count's NRAYS, rt/raytrace.c:606)
+    beside the GPU's, and every irradiance against the reference's."""
+    import concurrent.futures as cf
+    p = _lib.default_params(False)
+    p.ambounce, p.dstrsrc, p.srcsizerat = 0, 0.0, 0.0
+    work = [scenes.config1(s) for s in range(nsun)]
+    ctxs = [_lib.Context(0) for _ in range(nsun)]
+    # per call, sequential
+    vals, t_call, g_rays = [], [], []
+    for c, (rad, sens, _) in zip(ctxs, work):
+        t0 = time.perf_counter()
+        c.load_text(rad).commit()
+        c.set_params(p)
+        v = c.trace(sens, _lib.RT_IMM_IRRAD)
+        t_call.append(time.perf_counter() - t0)
+        vals.append(v)
+        g_rays.append(c.last_nrays())
+    # steady state: batched traces of the committed scenes
+    _lib.trace_batch(ctxs[:250], [w[1] for w in work[:250]], _lib.RT_IMM_IRRAD)       # warm-up
+    t0 = time.perf_counter()
+    nb = 0
+    for k in range(0, nsun, 250):
+        out = _lib.trace_batch(ctxs[k:k + 250], [w[1] for w in work[k:k + 250]], _lib.RT_IMM_IRRAD)
+        nb += ctxs[k].last_nrays()
+        for i, o in enumerate(out):
+            assert np.array_equal(o, vals[k + i])
+    t_batch = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    # rays_gpu: every call fills its scene's shadow cache (as every
+    # reference run does); the batched traces reuse the filled caches
+    res = {"suns": nsun, "sensors_per_sun": 77, "rays_gpu": int(sum(g_rays)), "rays_gpu_batched": int(nb),
+           "gpu_per_call_ms": 1e3 * float(np.mean(t_call)), "gpu_per_call_median_ms": 1e3 * float(np.median(t_call)),
+           "gpu_whole_job_s": float(np.sum(t_call)), "gpu_batched_trace_s": t_batch}
+    ref = ROOT / "oracle" / "_ref"
+    if not cpu or not (ref / "rtrace").exists():
+        return res
+    env = dict(os.environ, RAYPATH=f".:{ref / 'lib'}")
+    args = ["-h", "-I", "-ab", "0", "-dj", "0", "-ds", "0"]
+    with tempfile.TemporaryDirectory() as td:
+        def job(k, prog="rtrace", nsens=77):
+            rad, sens, _ = work[k]
+            radf, octf = Path(td) / f"s{k}.rad", Path(td) / f"s{k}_{prog}_{nsens}.oct"
+            radf.write_text(rad)
+            inp = "\n".join(" ".join(f"{v:.17g}" for v in r) for r in sens[:nsens]).encode() + b"\n"
+            t0 = time.perf_counter()
+            with open(octf, "wb") as f:
+                subprocess.run([str(ref / "oconv"), "-f", str(radf)], stdout=f, check=True)
+            r = subprocess.run([str(ref / prog), *args, str(octf)], input=inp, capture_output=True, env=env,
+                               check=True)
+            t = time.perf_counter() - t0
+            octf.unlink()
+            return t, r.stdout, r.stderr
+
+        def batch(prog, nsens):
+            t0 = time.perf_counter()
+            with cf.ThreadPoolExecutor(cores) as ex:
+                outs = list(ex.map(lambda k: job(k, prog, nsens), range(nsun)))
+            return time.perf_counter() - t0, outs
+        t_all, outs = batch("rtrace", 77)
+        t_load, _ = batch("rtrace", 1)
+        _, couts = batch("rtrace_count", 77)
+    ref_vals = np.concatenate([np.array(o[1].split(), dtype=np.float64).reshape(-1, 3) for o in outs])
+    gpu_vals = np.concatenate(vals)
+    rel = np.abs(gpu_vals - ref_vals) / np.maximum(np.abs(ref_vals), 1e-30)
+    ref_rays = [int(ln.split()[1]) for o in couts for ln in o[2].decode().splitlines() if ln.startswith("NRAYS")]
+    usable, visible, quota = host_cores()
+    cpu_call = float(np.median([o[0] for o in outs]))
+    # per sun: one call (both sides build the scene and its shadow cache
+    # and trace); steady state: the traces alone (GPU: batched over
+    # committed scenes; reference: whole-job wall beyond the same pipelines
+    # run with one sensor each)
+    res.update({
+        "cpu_cores": cores, "host_cpus": {"affinity": visible, "cgroup_quota": quota, "used": cores},
+        "cpu_per_call_median_ms": 1e3 * cpu_call,
+        "cpu_whole_job_s": t_all, "cpu_octree_build_and_load_s": t_load,
+        "cpu_whole_job_Mrays_s": sum(ref_rays) / t_all / 1e6,
+        "gpu_whole_job_Mrays_s": sum(g_rays) / float(np.sum(t_call)) / 1e6,
+        "gpu_steady_ms_per_sun": 1e3 * t_batch / nsun,
+        "cpu_steady_ms_per_sun": 1e3 * max(t_all - t_load, 0.0) / nsun,
+        "rays_reference": int(sum(ref_rays)),
+        "rays_equal": bool(len(ref_rays) == nsun and sum(ref_rays) == sum(g_rays)),
+        "rays_equal_per_sun": int(sum(int(a == b) for a, b in zip(ref_rays, g_rays))),
+        "max_rel_diff": float(rel.max()), "values": int(rel.size),
+        "speedup_per_call": cpu_call / float(np.median(t_call)),
+        "speedup_whole_job": t_all / float(np.sum(t_call)),
+        "speedup_steady_state": max(t_all - t_load, 0.0) / t_batch,
+    })
     return res
 
 
@@ -676,9 +820,11 @@ def main():
     ap.add_argument("--no-sky", action="store_true", help="skip the gendaymtx measurement")
     ap.add_argument("--no-rpict", action="store_true", help="skip the rpict measurement")
     ap.add_argument("--no-rflux", action="store_true", help="skip the rfluxmtx measurement")
-    ap.add_argument("--only", choices=["sky", "rpict", "rflux", "config5"], default=None,
+    ap.add_argument("--only", choices=["sky", "rpict", "rflux", "config1", "config5"], default=None,
                     help="run one of the gendaymtx / rpict / rfluxmtx measurements alone and print its JSON")
-    ap.add_argument("--cpu-sample", type=int, default=480)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="config-2 sensors timed on the CPU (default: 20 records per host core, at least 480)")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 1000-sun config-1 comparison")
     ap.add_argument("--sensors", type=int, default=2500)
     ap.add_argument("--bvh", choices=["auto", "sah", "lbvh"], default="auto")
     ap.add_argument("--max-big", type=int, default=None, help="room-sized primitives kept out of the BVH")
@@ -695,7 +841,8 @@ def main():
             copy_calibration(args.copy_calibration)
         cpu_leg = not args.no_cpu_baseline
         fn = {"sky": lambda: sky_bench(cpu=cpu_leg), "rpict": lambda: rpict_bench(cpu=cpu_leg),
-              "rflux": lambda: rflux_bench(cpu=cpu_leg, cores=min(os.cpu_count() or 1, 16)),
+              "rflux": lambda: rflux_bench(cpu=cpu_leg, cores=host_cores()[0]),
+              "config1": lambda: config1_bench(cores=host_cores()[0], cpu=cpu_leg),
               "config5": config5_bench}[args.only]
         print(json.dumps(fn()))
         return
@@ -815,14 +962,23 @@ def main():
         log(f"rpict: {pict}")
     rfx = None
     if not args.no_rflux:
-        rfx = rflux_bench(cpu=not args.no_cpu_baseline, cores=min(os.cpu_count() or 1, 16))
+        rfx = rflux_bench(cpu=not args.no_cpu_baseline, cores=host_cores()[0])
         log(f"rfluxmtx: {rfx}")
+    c1 = None
+    if not args.no_config1:
+        c1 = config1_bench(cores=host_cores()[0], cpu=not args.no_cpu_baseline)
+        log(f"config 1 over 1000 suns: {c1}")
 
     cpu = None
     if not args.no_cpu_baseline:
-        cores = min(os.cpu_count() or 1, 16)
+        cores = host_cores()[0]
+        nsample = min(len(sensors), args.cpu_sample or max(480, 20 * cores))
+
+        def gpu_count(sample):
+            ctx.contrib(sample, mods, 1, flags)
+            return ctx.last_nrays()
         try:
-            cpu = cpu_baseline(rad, sensors, args.cpu_sample, cores)
+            cpu = cpu_baseline(rad, sensors, nsample, cores, gpu_count)
         except Exception as e:  # the GPU figure stands on its own
             log(f"cpu baseline failed: {e}")
 
@@ -852,8 +1008,15 @@ def main():
                    "bvh": bvh_name,
                    "launch_shape": tuned["best"] if tuned else tune.DEFAULT,
                    "per_thread_kernel_ms": tuned["tried"][0][1] if tuned else None},
+        # bound: the roofline the algorithmic bytes are priced against;
+        # limiter: what the counters show limits the kernel (instruction
+        # issue with a third of the lanes active: DRAM traffic is a few
+        # percent of HBM peak, dram_frac)
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "limiter": "issue",
+                     "dram_frac": (traffic / (avg_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                     "lane_utilisation": committed_lane_util(),
                      "kernel": "k_trace", "avg_launch_ms": avg_launch_ms,
                      "shade_kernel_ms_per_step": st_t["shade_ms"] / nsteps,
                      "trace_kernel_ms_per_step": st_t["trace_ms"] / nsteps,
@@ -868,6 +1031,7 @@ def main():
         "gendaymtx": sky,
         "rpict": pict,
         "rfluxmtx": rfx,
+        "config1": c1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -129,13 +129,20 @@ int rg_material_set(rg_ctx *ctx, int index, const double *f, int n);
 /* Path-replay adjoint of rg_contrib (no reference counterpart).  For the
  * loss whose derivative with respect to each record's binned irradiance
  * E[rec][c] = sum_col out[rec][col][c] * lcol[col][c] is dlde[rec][c]
- * (nrec x 3), adds into grad[i][0..4] (nmats x 5; zeroed here) the
- * derivative with respect to plastic material mats[i]'s real arguments
- * (red, green, blue reflectance, specularity, roughness), tracing the same
- * rays with the same random numbers as rg_contrib.  lcol: 3 * sum of nbins
- * doubles (radiance of every output column).  Sampling decisions are held
- * fixed, so the result is the exact derivative of the estimator for these
- * random numbers; roughness only moves sample directions and has none. */
+ * (nrec x 3), adds into grad[i][0..6] (nmats x 7; zeroed here) the
+ * derivative with respect to material mats[i]'s real arguments: red, green,
+ * blue reflectance, specularity, roughness, and for trans the
+ * transmissivity and transmitted specularity (columns 5 and 6, zero for
+ * plastic and metal).  Plastic, metal and trans materials (rt/normal.c
+ * m_normal) can be differentiated.  It traces the same rays with the same
+ * random numbers as rg_contrib.  lcol: 3 * sum of nbins doubles (radiance
+ * of every output column).  Every deposit is differentiated, shadow-ray
+ * (direct light) deposits included.  Sampling decisions are held fixed, and
+ * a rough material's Gaussian lobe samples add their likelihood-ratio score
+ * for the roughness (rt/normal.c:364-495 gaussamp): the roughness column is
+ * exact in expectation, not sample by sample.  That score needs untruncated
+ * lobe samples: with a rough differentiated material, -ss in [0, 1) is an
+ * error. */
 int rg_contrib_adjoint(rg_ctx *ctx, const double *rays, long long nrec, int accum, int flags,
                        const rg_modifier *mods, int nmods, const double *dlde, const double *lcol,
                        const int *mats, int nmats, double *grad);

@@ -288,7 +288,9 @@ class Context:
 
     def contrib_adjoint(self, rays: np.ndarray, mods: list, dlde: np.ndarray, lcol: np.ndarray, mats: list,
                         accum: int = 1, flags: int = 0) -> np.ndarray:
-        """d(loss)/d(plastic arguments) of materials `mats` (indices), (len(mats), 5).
+        """d(loss)/d(material arguments) of materials `mats` (indices),
+        (len(mats), 7): red, green, blue, specularity, roughness, and for
+        trans transmissivity and transmitted specularity (zero otherwise).
 
         dlde: (nrec, 3) derivative of the loss with respect to each record's
         E = sum over columns of contribution x lcol; lcol: (3 * sum nbins,)
@@ -300,7 +302,7 @@ class Context:
         dlde = np.ascontiguousarray(dlde, dtype=np.float64).reshape(nrec, 3)
         lcol = np.ascontiguousarray(lcol, dtype=np.float64).reshape(ncol)
         mi = (C.c_int * max(len(mats), 1))(*[int(m) for m in mats])
-        grad = np.zeros((len(mats), 5))
+        grad = np.zeros((len(mats), 7))
         self._check(self._L.rg_contrib_adjoint(self._h, _dptr(rays), nrec, accum, flags, arr, len(mods), _dptr(dlde),
                                                _dptr(lcol), mi, len(mats), _dptr(grad)))
         return grad

@@ -10,11 +10,15 @@ product with the bin radiances L[bin][channel]:
 Gradients:
   * with respect to L: DC itself (E is linear in L), so dLoss/dL[b][c] =
     sum_s dLoss/dE[s][c] * DC[s][b][c];
-  * with respect to the plastic arguments of chosen material groups (red,
-    green, blue reflectance, specularity, roughness): rg_contrib_adjoint, a
-    second pass over the same rays with the same random numbers that adds,
-    at every binned deposit, the derivative of the deposit's path
-    coefficient weighted by dLoss/dE and L.  Per-bounce state is
+  * with respect to the real arguments of chosen plastic, metal or trans
+    material groups (red, green, blue reflectance, specularity, roughness,
+    and for trans transmissivity and transmitted specularity):
+    rg_contrib_adjoint, a second pass over the same rays with the same
+    random numbers that adds, at every binned deposit (shadow-ray deposits
+    included), the derivative of the deposit's path coefficient weighted by
+    dLoss/dE and L.  The roughness derivative adds the likelihood-ratio
+    score of the Gaussian lobe samples (rt/normal.c:364-495), so it is exact
+    in expectation rather than sample by sample.  Per-bounce state is
     recomputed by the replay, not stored.
 
 PyTorch is used here only for autograd; the work runs in libradgpu.so.
@@ -42,17 +46,22 @@ class DiffScene:
         self.flags = flags
         self.groups = list(groups)
         self.mats = [self.ctx.material_index(g) for g in self.groups]
+        self.nargs = []
         for m in self.mats:
             kind, _ = self.ctx.material_get(m)
-            if kind != 0:
-                raise ValueError("only plastic material groups can be differentiated")
+            if kind not in (0, 1, 2):
+                raise ValueError("only plastic, metal and trans material groups can be differentiated")
+            self.nargs.append(7 if kind == 2 else 5)
+        self.width = max(self.nargs, default=5)
         self.nbins = sum(m.nbins for m in mods)
 
     def set_theta(self, theta: np.ndarray) -> None:
-        """Plastic arguments per group, (ngroups, 5); negative values clip to 0."""
-        theta = np.asarray(theta, dtype=np.float64).reshape(len(self.mats), 5)
-        for m, th in zip(self.mats, np.maximum(theta, 0.0)):
-            self.ctx.material_set(m, th)
+        """Material arguments per group, (ngroups, 5) or (ngroups, 7) when a
+        group is trans (plastic and metal groups use the first 5 columns);
+        negative values clip to 0."""
+        theta = np.asarray(theta, dtype=np.float64).reshape(len(self.mats), self.width)
+        for m, k, th in zip(self.mats, self.nargs, np.maximum(theta, 0.0)):
+            self.ctx.material_set(m, th[:k])
 
     def coefficients(self) -> np.ndarray:
         """DC matrix (nsensors, nbins, 3) for the current arguments."""
@@ -60,16 +69,17 @@ class DiffScene:
         return dc.reshape(self.rays.shape[0], self.nbins, 3)
 
     def adjoint(self, dlde: np.ndarray, sky: np.ndarray) -> np.ndarray:
-        """dLoss/dtheta (ngroups, 5) for dLoss/dE = dlde (nsensors, 3)."""
-        return self.ctx.contrib_adjoint(self.rays, self.mods, dlde, np.asarray(sky, np.float64).ravel(),
-                                        self.mats, 1, self.flags)
+        """dLoss/dtheta (ngroups, 5 or 7, as set_theta) for dLoss/dE = dlde (nsensors, 3)."""
+        g = self.ctx.contrib_adjoint(self.rays, self.mods, dlde, np.asarray(sky, np.float64).ravel(),
+                                     self.mats, 1, self.flags)
+        return g[:, :self.width]
 
     def close(self):
         self.ctx.close()
 
 
 class Irradiance(torch.autograd.Function):
-    """E = irradiance(theta, sky): theta (ngroups, 5), sky (nbins, 3) -> (nsensors, 3)."""
+    """E = irradiance(theta, sky): theta (ngroups, 5 or 7), sky (nbins, 3) -> (nsensors, 3)."""
 
     @staticmethod
     def forward(ctx, theta: torch.Tensor, sky: torch.Tensor, scene: DiffScene):

@@ -297,7 +297,7 @@ int rg_set_params(rg_ctx* c, const rg_params* p) {
 }
 
 // Adjoint replay request: loss weights per record, radiance per output
-// column, material -> gradient row map, host gradient [rows][5].
+// column, material -> gradient row map, host gradient [rows][ADJ_COLS].
 struct AdjSpec {
   const double* dlde;
   const double* lcol;
@@ -432,7 +432,7 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
     double* d_w = (double*)c->arena.get("adj_w", sizeof(double) * std::max<size_t>(w.size(), 1), c->st, false);
     double* d_L = (double*)c->arena.get("adj_L", sizeof(double) * std::max(ncols, 1), c->st, false);
     int32_t* d_row = (int32_t*)c->arena.get("adj_row", sizeof(int32_t) * adj->row.size(), c->st, false);
-    d_grad = (double*)c->arena.get("adj_grad", sizeof(double) * 5 * std::max(adj->nrows, 1), c->st, true);
+    d_grad = (double*)c->arena.get("adj_grad", sizeof(double) * ADJ_COLS * std::max(adj->nrows, 1), c->st, true);
     if (!w.empty()) hipck(hipMemcpyAsync(d_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, c->st), "adj");
     hipck(hipMemcpyAsync(d_L, adj->lcol, sizeof(double) * ncols, hipMemcpyHostToDevice, c->st), "adj");
     hipck(hipMemcpyAsync(d_row, adj->row.data(), sizeof(int32_t) * adj->row.size(), hipMemcpyHostToDevice, c->st),
@@ -479,7 +479,8 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
     hipck(hipMemcpyAsync(out, d_out, sizeof(double) * outn, hipMemcpyDeviceToHost, c->st), "out");
   }
   if (adj)
-    hipck(hipMemcpyAsync(adj->grad, d_grad, sizeof(double) * 5 * adj->nrows, hipMemcpyDeviceToHost, c->st), "grad");
+    hipck(hipMemcpyAsync(adj->grad, d_grad, sizeof(double) * ADJ_COLS * adj->nrows, hipMemcpyDeviceToHost, c->st),
+          "grad");
   hipck(hipStreamSynchronize(c->st), "render");
   if (batch) {
     for (size_t i = 0; i < batch->size(); ++i) c->last_nrays += shadow_end((*batch_ctx)[i], (*batch)[i]);
@@ -489,9 +490,9 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   if (herr & ERR_STACK) throw std::runtime_error("BVH traversal stack overflow");
   if (herr & ERR_CAPACITY) throw std::runtime_error("spawn buffer overflow");
   if (herr & ERR_NOMAT) throw std::runtime_error("surface without material");
-  if (herr & ERR_PATH) throw std::runtime_error("adjoint: more than 7 differentiable vertices on one path");
-  if (herr & ERR_ADJ_DIRECT)
-    throw std::runtime_error("adjoint: a tracked modifier is reached by shadow rays (direct light is not differentiated)");
+  if (herr & ERR_PATH)
+    throw std::runtime_error("adjoint: the differentiated vertices of one path need more than " +
+                             std::to_string(PATH_WORDS) + " record words");
   if (!out || device_io) return;
   if (accum > 1) {
     double s = 1.0 / accum;
@@ -665,7 +666,12 @@ int rg_contrib_adjoint(rg_ctx* c, const double* rays, long long nrec, int accum,
     for (int i = 0; i < nmats; ++i) {
       int mi = mats[i];
       if (mi < 0 || mi >= (int)c->cs.mats.size() || mi == c->cs.lamb) throw std::runtime_error("bad material index");
-      if (c->cs.mats[mi].kind != MK_PLASTIC) throw std::runtime_error("adjoint: only plastic materials are differentiated");
+      const Material& M = c->cs.mats[mi];
+      if (M.kind != MK_PLASTIC && M.kind != MK_METAL && M.kind != MK_TRANS)
+        throw std::runtime_error("adjoint: only plastic, metal and trans materials are differentiated");
+      // the lobe score assumes untruncated samples (-ss >= 1 or < 0)
+      if (M.f[4] * M.f[4] > FTINY && c->params.specjitter >= 0.0 && c->params.specjitter < 1.0)
+        throw std::runtime_error("adjoint: a rough material's roughness derivative needs -ss 1 (untruncated lobe samples)");
       if (mi > 0xfff) throw std::runtime_error("adjoint: material index beyond 4095");
       if (adj.row[mi] >= 0) throw std::runtime_error("duplicate material in the gradient list");
       adj.row[mi] = i;

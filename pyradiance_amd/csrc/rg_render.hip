@@ -50,8 +50,8 @@ struct Sink {
 };
 
 // Append a spawn; in adjoint replay also its path: the parent's plus `entry`
-// (0: the parent's vertex is not a differentiable one).
-__device__ __forceinline__ void emit(const Sink& sk, const Spawn& s, const RayS& r, uint32_t entry = 0) {
+// (none: the parent's vertex is not a differentiated one).
+__device__ __forceinline__ void emit(const Sink& sk, const Spawn& s, const RayS& r, const PathAdd& entry) {
   uint32_t i = wave_alloc(sk.count);
   if (i < sk.cap) {
     sk.buf[i] = s;
@@ -61,6 +61,8 @@ __device__ __forceinline__ void emit(const Sink& sk, const Spawn& s, const RayS&
     atomicOr(sk.err, ERR_CAPACITY);
   }
 }
+
+__device__ __forceinline__ void emit(const Sink& sk, const Spawn& s, const RayS& r) { emit(sk, s, r, path_none()); }
 
 __device__ __forceinline__ void deposit(const RunCfg& c, uint32_t sensor, int col, C3 v) {
   double* o = c.out + (size_t)sensor * c.ncols + col;
@@ -368,66 +370,134 @@ __device__ __forceinline__ int source_hit(const DevScene& S, const RayS& r) {
 #define FRESNE(ci) (exp(-5.85 * (ci)) - 0.00202943064)
 constexpr double FRESTHRESH = 0.017999;
 
+// Own coefficient of one differentiated vertex, own[c], and its derivative
+// dOwn[c][q] with respect to the material's real arguments q (red, green,
+// blue, specularity, roughness and, for trans, transmissivity and
+// transmitted specularity), as m_normal() forms them (rt/normal.c:177-360):
+//   rspec = spec + fest (1 - spec)           (fest: Fresnel estimate, pure specular only)
+//   trans = f5 (1 - rspec), tspec = trans f6, tdiff = trans - tspec   (trans only)
+//   rdiff = 1 - trans - rspec
+//   scolor = rspec (plastic, trans); metal: rho rspec, or fest + rho spec (1 - fest) when fest > 0.
+// Shadow-ray entries carry dirnorm()'s factors with these colour terms taken
+// out (rt/normal.c:72-173): the reflection side is rho rdiff a + scolor b,
+// the transmission side rho tdiff a + rho tspec b, and b's roughness
+// derivative is b g.
+__device__ __forceinline__ void vertex_own(const Material& M, uint32_t kind, double fest, const uint32_t* pay,
+                                           double own[3], double dOwn[3][ADJ_COLS]) {
+  const double sp = M.f[3];
+  const double rspec = sp + fest * (1.0 - sp);
+  double d_rspec[ADJ_COLS] = {0, 0, 0, 1.0 - fest, 0, 0, 0};
+  double trans = 0, tspec = 0, tdiff = 0;
+  double d_trans[ADJ_COLS] = {}, d_tspec[ADJ_COLS] = {}, d_tdiff[ADJ_COLS] = {}, d_rdiff[ADJ_COLS];
+  if (M.kind == MK_TRANS) {
+    trans = M.f[5] * (1.0 - rspec);
+    d_trans[3] = -M.f[5] * (1.0 - fest);
+    d_trans[5] = 1.0 - rspec;
+    tspec = trans * M.f[6];
+    for (int q = 0; q < ADJ_COLS; ++q) d_tspec[q] = M.f[6] * d_trans[q];
+    d_tspec[6] += trans;
+    tdiff = trans - tspec;
+    for (int q = 0; q < ADJ_COLS; ++q) d_tdiff[q] = d_trans[q] - d_tspec[q];
+  }
+  const double rdiff = 1.0 - trans - rspec;
+  for (int q = 0; q < ADJ_COLS; ++q) d_rdiff[q] = -d_trans[q] - d_rspec[q];
+  const double a = kind >= PV_DIR_R ? (double)__uint_as_float(pay[0]) : 0.0;
+  const double b = kind >= PV_DIR_R ? (double)__uint_as_float(pay[1]) : 0.0;
+  const double g = kind >= PV_DIR_R ? (double)__uint_as_float(pay[2]) : 0.0;
+  for (int ch = 0; ch < 3; ++ch) {
+    const double rho = M.f[ch];
+    // scolor and its derivative
+    double sc, dsc[ADJ_COLS] = {};
+    if (M.kind != MK_METAL) {
+      sc = rspec;
+      dsc[3] = d_rspec[3];
+    } else if (fest > FTINY) {
+      sc = fest + rho * sp * (1.0 - fest);
+      dsc[ch] = sp * (1.0 - fest);
+      dsc[3] = rho * (1.0 - fest);
+    } else {
+      sc = rho * rspec;
+      dsc[ch] = rspec;
+      dsc[3] = rho * d_rspec[3];
+    }
+    // rho * X for the X the kind uses (and a second term on some kinds)
+    double x = 0, dx[ADJ_COLS] = {}, x2 = 0, dx2[ADJ_COLS] = {};
+    double wx = 1.0, wsc = 0.0, wx2 = 0.0;       // own = wx rho x + wsc scolor + wx2 rho x2
+    switch (kind) {
+      case PV_AMB: x = rdiff; for (int q = 0; q < ADJ_COLS; ++q) dx[q] = d_rdiff[q]; break;
+      case PV_AMB_RBLT: x = rdiff; for (int q = 0; q < ADJ_COLS; ++q) dx[q] = d_rdiff[q]; wsc = 1.0; break;
+      case PV_SPEC:
+      case PV_GSPEC: wx = 0.0; wsc = 1.0; break;
+      case PV_TSPEC:
+      case PV_GTRANS: x = tspec; for (int q = 0; q < ADJ_COLS; ++q) dx[q] = d_tspec[q]; break;
+      case PV_AMB_BACK: x = tdiff; for (int q = 0; q < ADJ_COLS; ++q) dx[q] = d_tdiff[q]; break;
+      case PV_AMB_BACK_TBLT: x = trans; for (int q = 0; q < ADJ_COLS; ++q) dx[q] = d_trans[q]; break;
+      case PV_DIR_R: x = rdiff; for (int q = 0; q < ADJ_COLS; ++q) dx[q] = d_rdiff[q]; wx = a; wsc = b; break;
+      default:       // PV_DIR_T
+        x = tdiff; for (int q = 0; q < ADJ_COLS; ++q) dx[q] = d_tdiff[q]; wx = a;
+        x2 = tspec; for (int q = 0; q < ADJ_COLS; ++q) dx2[q] = d_tspec[q]; wx2 = b;
+        break;
+    }
+    own[ch] = wx * rho * x + wsc * sc + wx2 * rho * x2;
+    for (int q = 0; q < ADJ_COLS; ++q) dOwn[ch][q] = wx * rho * dx[q] + wsc * dsc[q] + wx2 * rho * dx2[q];
+    dOwn[ch][ch] += wx * x + wx2 * x2;
+    if (kind == PV_DIR_R) dOwn[ch][4] += b * sc * g;
+    else if (kind == PV_DIR_T) dOwn[ch][4] += b * rho * tspec * g;
+  }
+}
+
 // Path-replay adjoint of one binned deposit.  The deposit is
 //   contr_c = L-independent path coefficient; loss weight w_c = dLoss/dE_c * L_c * contr_c,
 // and contr_c is a product of the own coefficients of the path's vertices,
 // times minweight / weight at every Russian roulette survival.  For each
-// plastic vertex the derivative of log(own_c) with respect to (rho_r, rho_g,
-// rho_b, specularity) follows from how m_normal() forms it
-// (rt/normal.c:177-360): ambient a_c = rho_c * rdiff (+ rspec when the
-// specular part is folded in below -st), specular own = rspec, with
-// rdiff = 1 - rspec and rspec = spec + fest * (1 - spec) for a pure
-// specular surface.  A roulette survival divides by the largest channel of
-// each own coefficient since the previous reset, which subtracts that
-// channel's derivative.  Sampling decisions (strata counts, roulette, lobe
-// directions) are held fixed: the result is the exact derivative of the
-// estimator for the fixed random numbers, so roughness has none here.
+// differentiated vertex the derivative of log(own_c) follows from
+// vertex_own(); a roulette survival divides by the largest channel of each
+// own coefficient since the previous reset, which subtracts that channel's
+// derivative.  Sampling decisions (strata counts, roulette) are held fixed.
+// Gaussian lobe samples (gaussamp(), rt/normal.c:364-495) draw the lobe's
+// tan^2 as alpha^2 u with u exponential, so a sample's density moves with
+// the roughness alpha: their entries carry the score d log p / d alpha =
+// (2 / alpha)(u - 1) of every trial up to the accepted one, which makes the
+// roughness derivative of the estimator's expectation exact in the mean
+// (likelihood-ratio form); the lobe directions themselves are held fixed.
 __device__ __forceinline__ void adjoint_deposit(const DevScene& S, const RunCfg& c, const RayS& r, int col, C3 contr) {
-  if (r.crtype & SHADOW) { atomicOr(c.err, ERR_ADJ_DIRECT); return; }
-  if (r.path.hdr & 256u) { atomicOr(c.err, ERR_PATH); return; }
+  if (r.path.hdr & 1024u) { atomicOr(c.err, ERR_PATH); return; }
   const double* aw = c.adj_w + 3 * (size_t)r.sensor;
   const double* L = c.adj_L + col;
   const double w[3] = {aw[0] * L[0] * contr.r, aw[1] * L[1] * contr.g, aw[2] * L[2] * contr.b};
   if (w[0] == 0.0 && w[1] == 0.0 && w[2] == 0.0) return;
-  const uint32_t n = r.path.hdr & 15u;
-  for (uint32_t i = 0; i < n; ++i) {
+  const uint32_t n = r.path.hdr & 31u;
+  uint32_t i = 0;
+  while (i < n) {
     const uint32_t e = r.path.e[i];
     const int mi = (int)(e & 0xfffu);
-    const uint32_t kind = (e >> 12) & 7u;
+    const uint32_t kind = (e >> 12) & 15u;
+    const int np = path_payload(kind);
+    const uint32_t* pay = &r.path.e[i + 1 < (uint32_t)PATH_WORDS ? i + 1 : i];
+    i += 1u + (uint32_t)np;
     const int row = c.grad_row[mi];
     if (row < 0) continue;
     const Material& M = S.mats[mi];
-    const double rho[3] = {M.f[0], M.f[1], M.f[2]};
-    const double sp = M.f[3];
-    const double fest = (double)(e >> 16) * (1.0 / 65535.0);
-    const double rspec = sp + fest * (1.0 - sp);
-    const double drs = 1.0 - fest;            // d rspec / d spec
-    const double rdiff = 1.0 - rspec;
-    double D[3][4] = {};                      // d log own_c / d(rho_r, rho_g, rho_b, spec)
-    double own[3];
-    for (int ch = 0; ch < 3; ++ch) {
-      if (kind == PV_SPEC) {
-        own[ch] = rspec;
-        D[ch][3] = rspec > 0 ? drs / rspec : 0.0;
-      } else {
-        double a = rho[ch] * rdiff + (kind == PV_AMB_RBLT ? rspec : 0.0);
-        own[ch] = a;
-        if (a > 0) {
-          D[ch][ch] = rdiff / a;
-          D[ch][3] = (kind == PV_AMB_RBLT ? (1.0 - rho[ch]) : -rho[ch]) * drs / a;
-        }
-      }
-    }
-    if (e & 0x8000u) {                        // roulette divided by the largest channel
+    const double fest = (double)(e >> 17) * (1.0 / 32767.0);
+    double own[3], dOwn[3][ADJ_COLS], D[3][ADJ_COLS];
+    vertex_own(M, kind, fest, pay, own, dOwn);
+    for (int ch = 0; ch < 3; ++ch)
+      for (int q = 0; q < ADJ_COLS; ++q) D[ch][q] = own[ch] > 0 ? dOwn[ch][q] / own[ch] : 0.0;
+    if (e & PV_RR) {                          // roulette divided by the largest channel
       int cm = 0;
       if ((float)own[1] > (float)own[cm]) cm = 1;
       if ((float)own[2] > (float)own[cm]) cm = 2;
-      double Dm[4] = {D[cm][0], D[cm][1], D[cm][2], D[cm][3]};
+      double Dm[ADJ_COLS];
+      for (int q = 0; q < ADJ_COLS; ++q) Dm[q] = D[cm][q];
       for (int ch = 0; ch < 3; ++ch)
-        for (int q = 0; q < 4; ++q) D[ch][q] -= Dm[q];
+        for (int q = 0; q < ADJ_COLS; ++q) D[ch][q] -= Dm[q];
     }
-    double* g = c.grad + 5 * (size_t)row;
-    for (int q = 0; q < 4; ++q) {
+    if (kind == PV_GSPEC || kind == PV_GTRANS) {
+      const double sc = (double)__uint_as_float(pay[0]);
+      for (int ch = 0; ch < 3; ++ch) D[ch][4] += sc;
+    }
+    double* g = c.grad + ADJ_COLS * (size_t)row;
+    for (int q = 0; q < ADJ_COLS; ++q) {
       double v = w[0] * D[0][q] + w[1] * D[1][q] + w[2] * D[2][q];
       if (v != 0.0) unsafeAtomicAdd(g + q, v);
     }
@@ -496,21 +566,39 @@ struct NormDat {
   C3 mcolor, scolor;
   V3 prdir, pnorm;
   double alpha2, rdiff, rspec, trans, tdiff, tspec, pdot;
+  double rough;     // the material's roughness argument (alpha2 = rough^2)
+};
+
+// dirnorm()'s factors with the material's colour terms taken out, for the
+// adjoint: side 0 (reflection) or 1 (transmission), diffuse factor a, lobe
+// factor b and d log(lobe) / d roughness g (see vertex_own()).
+struct DirParts {
+  int side;
+  double a, b, g;
 };
 
 // dirnorm() of rt/normal.c:72-173
-__device__ __forceinline__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdir, V3 ldir, double omega) {
+__device__ __forceinline__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdir, V3 ldir, double omega,
+                                      DirParts* pt = nullptr) {
   C3 scval = c3(0, 0, 0);
   double ldot = dot(np.pnorm, ldir);
+  if (pt) { pt->side = ldot < 0.0; pt->a = pt->b = pt->g = 0.0; }
   if (ldot < 0.0 ? np.trans <= FTINY : np.trans >= 1.0 - FTINY) return scval;
-  double lrdiff = np.rdiff, ltdiff = np.tdiff;
+  double lrdiff = np.rdiff, ltdiff = np.tdiff, kf = 1.0;
   if ((np.specfl & SP_PURE) && np.rspec >= FRESTHRESH && ((lrdiff > FTINY) | (ltdiff > FTINY))) {
     double dtmp = 1. - FRESNE(fabs(ldot));
     lrdiff *= dtmp;
     ltdiff *= dtmp;
+    kf = dtmp;
   }
-  if ((ldot > FTINY) & (lrdiff > FTINY)) scval = scval + np.mcolor * (ldot * omega * lrdiff * (1.0 / PI_D));
-  if ((ldot < -FTINY) & (ltdiff > FTINY)) scval = scval + np.mcolor * (-ldot * omega * ltdiff * (1.0 / PI_D));
+  if ((ldot > FTINY) & (lrdiff > FTINY)) {
+    scval = scval + np.mcolor * (ldot * omega * lrdiff * (1.0 / PI_D));
+    if (pt) pt->a = ldot * omega * kf * (1.0 / PI_D);
+  }
+  if ((ldot < -FTINY) & (ltdiff > FTINY)) {
+    scval = scval + np.mcolor * (-ldot * omega * ltdiff * (1.0 / PI_D));
+    if (pt) pt->a = -ldot * omega * kf * (1.0 / PI_D);
+  }
   if ((ldot > FTINY) & ((np.specfl & (SP_REFL | SP_PURE)) == SP_REFL)) {
     double dtmp = np.alpha2;
     if (np.specfl & SP_FLAT) dtmp += (1. - c.p.dstrsrc) * omega * (0.25 / PI_D);
@@ -519,13 +607,22 @@ __device__ __forceinline__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdi
     d2 *= d2;
     double d3 = dot(vtmp, vtmp);
     double d4 = (d3 - d2) / d2;
+    const double aw = dtmp;
     dtmp = exp(-d4 / dtmp) * d3 / (PI_D * d2 * d2 * dtmp);
-    if (dtmp > FTINY) scval = scval + np.scolor * (dtmp * ldot * omega);
+    if (dtmp > FTINY) {
+      scval = scval + np.scolor * (dtmp * ldot * omega);
+      // d log(lobe) / d alpha2 = d4 / aw^2 - 1 / aw; d alpha2 / d roughness = 2 roughness
+      if (pt) { pt->b = dtmp * ldot * omega; pt->g = 2.0 * np.rough * (d4 / (aw * aw) - 1.0 / aw); }
+    }
   }
   if ((ldot < -FTINY) & ((np.specfl & (SP_TRAN | SP_PURE)) == SP_TRAN)) {
     double dtmp = np.alpha2 + omega * (1.0 / PI_D);
-    dtmp = exp((2. * dot(np.prdir, ldir) - 2.) / dtmp) / (PI_D * dtmp);
-    if (dtmp > FTINY) scval = scval + np.mcolor * (dtmp * np.tspec * omega * sqrt(-ldot / np.pdot));
+    const double bw = dtmp, cw = 2. * dot(np.prdir, ldir) - 2.;
+    dtmp = exp(cw / dtmp) / (PI_D * dtmp);
+    if (dtmp > FTINY) {
+      scval = scval + np.mcolor * (dtmp * np.tspec * omega * sqrt(-ldot / np.pdot));
+      if (pt) { pt->b = dtmp * omega * sqrt(-ldot / np.pdot); pt->g = 2.0 * np.rough * (-cw / (bw * bw) - 1.0 / bw); }
+    }
   }
   return scval;
 }
@@ -605,8 +702,9 @@ __device__ __forceinline__ bool shadow_blocked(const DevScene& S, int sn, V3 o, 
 // direct() of rt/source.c:398-557 with every source tested (no shadow
 // threshold / untested-source statistics): one shadow spawn per source with
 // non-zero potential.
+// mi: material of the vertex when it is differentiated (adjoint replay), else -1.
 __device__ __forceinline__ void direct_light(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, const NormDat& nd,
-                             const Sink& sk) {
+                             const Sink& sk, int mi = -1, double fest = 0.0) {
   for (int sn = 0; sn < S.nsrcs; ++sn) {
     const Source& src = S.srcs[sn];
     if (src.flags & SSKIP) continue;
@@ -669,7 +767,8 @@ __device__ __forceinline__ void direct_light(const DevScene& S, const RunCfg& c,
     }
     // the shadow cache decides before anything else (rt/source.c:437-446)
     if (shadow_blocked(S, sn, hs.rop, dir)) continue;
-    C3 coef = dirnorm(nd, c, r.d, dir, dom);
+    DirParts parts;
+    C3 coef = dirnorm(nd, c, r.d, dir, dom, mi >= 0 ? &parts : nullptr);
     if (!(cmax(coef) > 0.0)) continue;
     // potential: the ray must reach the source (srcvalue, rt/source.c:261-299)
     RayS sr = r;
@@ -697,14 +796,22 @@ __device__ __forceinline__ void direct_light(const DevScene& S, const RunCfg& c,
     s.crc = (float)cmax(coef);
     s.rsrc = (int16_t)sn;
     set_dir(s, dir);
-    emit(sk, s, r);
+    if (mi >= 0) {
+      PathAdd e = path_entry(mi, parts.side ? PV_DIR_T : PV_DIR_R, fest);
+      e.w[1] = __float_as_uint((float)parts.a);
+      e.w[2] = __float_as_uint((float)parts.b);
+      e.w[3] = __float_as_uint((float)parts.g);
+      emit(sk, s, r, e);
+    } else {
+      emit(sk, s, r);
+    }
   }
 }
 
 // multambient() + samp_hemi() set-up (rt/ambient.c:229-330, rt/ambcomp.c:351-424)
 // for -aa 0: one hemisphere spawn of n*n stratified children.
 __device__ __forceinline__ void ambient(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, C3 aval, V3 nrm,
-                        const Sink& sk, uint32_t entry = 0) {
+                        const Sink& sk, PathAdd entry = path_none()) {
   const Params& p = c.p;
   bool dumb = (p.ambdiv <= 0) || (r.adepth >= p.ambounce);
   double rdot = dot(nrm, hs.ron);
@@ -742,23 +849,30 @@ __device__ __forceinline__ void ambient(const DevScene& S, const RunCfg& c, cons
   }
 }
 
-// gaussamp() of rt/normal.c:364-495 with one sample per component (-ss <= 1.5)
+// gaussamp() of rt/normal.c:364-495 with one sample per component (-ss <= 1.5).
+// mi >= 0 (adjoint replay of a differentiated material): each child's path
+// entry carries the roughness score of its trials (see adjoint_deposit()):
+// trial k draws tan^2 = alpha^2 u_k with u_k = -ln rv1 exponential, whose
+// log density -ln alpha^2 - u_k has derivative (2 / alpha)(u_k - 1).
 __device__ __forceinline__ void gauss_sample(const DevScene& S, const RunCfg& c, const RayS& r, const HitS& hs, const NormDat& np,
-                             const Sink& sk, uint32_t e_spec) {
+                             const Sink& sk, int mi) {
   if ((np.specfl & (SP_REFL | SP_RBLT)) != SP_REFL && (np.specfl & (SP_TRAN | SP_TBLT)) != SP_TRAN) return;
   V3 u, v;
   tangent_frame(np.pnorm, rnd(r.key, RN_GFRAME), u, v);
+  const double sfac = mi >= 0 ? 2.0 / np.rough : 0.0;
   if ((np.specfl & (SP_REFL | SP_RBLT)) == SP_REFL) {
     C3 sc = np.scolor;
     Spawn s = make_child(r, hs.rop, RSPECULAR, &sc, SLOT_RSPEC, 0);
     bool ok = false;
     V3 dir;
+    double score = 0.0;
     for (int ntrials = 0; ntrials < 10; ++ntrials) {
       double rv0 = rnd(s.key, RN_GAUSS + 2 * ntrials), rv1 = rnd(s.key, RN_GAUSS + 2 * ntrials + 1);
       double d = 2.0 * PI_D * rv0;
       double cosp = cos(d), sinp = sin(d);
       if ((0. <= c.p.specjitter) & (c.p.specjitter < 1.)) rv1 = 1.0 - c.p.specjitter * rv1;
       d = (rv1 <= FTINY) ? 1.0 : sqrt(np.alpha2 * -log(rv1));
+      if (mi >= 0) score += sfac * (d * d / np.alpha2 - 1.0);
       V3 h = np.pnorm + (u * cosp + v * sinp) * d;
       d = -2.0 * dot(h, r.d) / (1.0 + d * d);
       dir = r.d + h * d;
@@ -767,26 +881,46 @@ __device__ __forceinline__ void gauss_sample(const DevScene& S, const RunCfg& c,
       ok = true;
       break;
     }
-    if (ok) { set_dir(s, dir); emit(sk, s, r, e_spec); }
+    if (ok) {
+      set_dir(s, dir);
+      if (mi >= 0) {
+        PathAdd e = path_entry(mi, PV_GSPEC, 0.0);
+        e.w[1] = __float_as_uint((float)score);
+        emit(sk, s, r, e);
+      } else {
+        emit(sk, s, r);
+      }
+    }
   }
   if ((np.specfl & (SP_TRAN | SP_TBLT)) == SP_TRAN) {
     C3 tc = np.mcolor * np.tspec;
     Spawn s = make_child(r, hs.rop, TSPECULAR, &tc, SLOT_TSPEC, 0);
     bool ok = false;
     V3 dir;
+    double score = 0.0;
     for (int ntrials = 0; ntrials < 10; ++ntrials) {
       double rv0 = rnd(s.key, RN_GAUSS + 2 * ntrials), rv1 = rnd(s.key, RN_GAUSS + 2 * ntrials + 1);
       double d = 2.0 * PI_D * rv0;
       double cosp = cos(d), sinp = sin(d);
       if ((0. <= c.p.specjitter) & (c.p.specjitter < 1.)) rv1 = 1.0 - c.p.specjitter * rv1;
       d = (rv1 <= FTINY) ? 1.0 : sqrt(np.alpha2 * -log(rv1));
+      if (mi >= 0) score += sfac * (d * d / np.alpha2 - 1.0);
       dir = np.prdir + (u * cosp + v * sinp) * d;
       if (dot(dir, hs.ron) >= -FTINY) continue;
       normalize(dir);
       ok = true;
       break;
     }
-    if (ok) { set_dir(s, dir); emit(sk, s, r); }
+    if (ok) {
+      set_dir(s, dir);
+      if (mi >= 0) {
+        PathAdd e = path_entry(mi, PV_GTRANS, 0.0);
+        e.w[1] = __float_as_uint((float)score);
+        emit(sk, s, r, e);
+      } else {
+        emit(sk, s, r);
+      }
+    }
   }
 }
 
@@ -807,6 +941,7 @@ __device__ __forceinline__ void shade_normal(const DevScene& S, const RunCfg& c,
   NormDat nd;
   nd.mcolor = c3((float)m.f[0], (float)m.f[1], (float)m.f[2]);
   nd.specfl = 0;
+  nd.rough = m.f[4];
   nd.alpha2 = m.f[4] * m.f[4];
   if (nd.alpha2 <= FTINY) nd.specfl |= SP_PURE;
   const bool hastexture = dot(hs.pert, hs.pert) > FTINY * FTINY;
@@ -844,11 +979,13 @@ __device__ __forceinline__ void shade_normal(const DevScene& S, const RunCfg& c,
     nd.tdiff = nd.tspec = nd.trans = 0.0;
   }
   nd.rdiff = 1.0 - nd.trans - nd.rspec;
+  // adjoint path entries of this vertex (differentiated materials only)
+  const bool dv = c.adj && mi >= 0 && c.grad_row[mi] >= 0;
   if ((nd.specfl & (SP_TRAN | SP_PURE | SP_TBLT)) == (SP_TRAN | SP_PURE)) {
     C3 tc = nd.mcolor * nd.tspec;
     Spawn s = make_child(r, hs.rop, TRANS, &tc, SLOT_TRANS, 0);
     set_dir(s, nd.prdir);
-    emit(sk, s, r);
+    emit(sk, s, r, dv ? path_entry(mi, PV_TSPEC, fest) : path_none());
   }
   if (r.crtype & SHADOW) return;
   nd.scolor = c3(0, 0, 0);
@@ -864,9 +1001,6 @@ __device__ __forceinline__ void shade_normal(const DevScene& S, const RunCfg& c,
     }
     if (!(nd.specfl & SP_PURE) && c.p.specthresh >= nd.rspec - FTINY) nd.specfl |= SP_RBLT;
   }
-  // adjoint path entries of this vertex (plastic only)
-  const bool dv = c.adj && mi >= 0 && m.kind == MK_PLASTIC;
-  const uint32_t e_spec = dv ? path_entry(mi, PV_SPEC, fest) : 0u;
   if ((nd.specfl & (SP_REFL | SP_PURE | SP_RBLT)) == (SP_REFL | SP_PURE)) {
     C3 sc = nd.scolor;
     Spawn s = make_child(r, hs.rop, REFLECTED, &sc, SLOT_REFL, 0);
@@ -874,21 +1008,22 @@ __device__ __forceinline__ void shade_normal(const DevScene& S, const RunCfg& c,
     if (hastexture && dot(dir, hs.ron) <= FTINY) dir = r.d + hs.ron * (2. * hs.rod);   // penetration
     normalize(dir);
     set_dir(s, dir);
-    emit(sk, s, r, e_spec);
+    emit(sk, s, r, dv ? path_entry(mi, PV_SPEC, fest) : path_none());
   }
   if ((nd.specfl & SP_PURE) && nd.rdiff <= FTINY && nd.tdiff <= FTINY) return;
-  if (!(nd.specfl & SP_PURE)) gauss_sample(S, c, r, hs, nd, sk, e_spec);
+  if (!(nd.specfl & SP_PURE)) gauss_sample(S, c, r, hs, nd, sk, dv ? mi : -1);
   if (nd.rdiff > FTINY) {
     C3 a = nd.mcolor * nd.rdiff;
     if (nd.specfl & SP_RBLT) a = a + nd.scolor;
-    const uint32_t e_amb = dv ? path_entry(mi, (nd.specfl & SP_RBLT) ? PV_AMB_RBLT : PV_AMB, fest) : 0u;
-    ambient(S, c, r, hs, a, nd.pnorm, sk, e_amb);
+    ambient(S, c, r, hs, a, nd.pnorm, sk,
+            dv ? path_entry(mi, (nd.specfl & SP_RBLT) ? PV_AMB_RBLT : PV_AMB, fest) : path_none());
   }
   if (nd.tdiff > FTINY) {
     C3 a = nd.mcolor * ((nd.specfl & SP_TBLT) ? nd.trans : nd.tdiff);
-    ambient(S, c, r, hs, a, -nd.pnorm, sk);
+    ambient(S, c, r, hs, a, -nd.pnorm, sk,
+            dv ? path_entry(mi, (nd.specfl & SP_TBLT) ? PV_AMB_BACK_TBLT : PV_AMB_BACK, fest) : path_none());
   }
-  direct_light(S, c, r, hs, nd, sk);
+  direct_light(S, c, r, hs, nd, sk, dv ? mi : -1, fest);
 }
 
 // m_glass() of rt/glass.c:46-165

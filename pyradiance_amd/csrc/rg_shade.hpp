@@ -35,39 +35,78 @@ static_assert(sizeof(Spawn) == 96, "Spawn layout");
 enum : uint8_t { SPF_RR = 1, SPF_FORCE = 2, SPF_BACK = 4 };
 
 // Adjoint bookkeeping (path replay): the differentiable vertices a ray's
-// coefficient was formed at.  Each entry is one plastic vertex whose own
-// coefficient fed the ray:
-//   bits 0-11  material index          bits 12-14 kind (PV_*)
-//   bit 15     Russian roulette after this vertex rescaled by 1/weight
-//   bits 16-31 Fresnel estimate fest of a pure-specular vertex, x 65535
-// hdr: bits 0-3 entry count, 4-7 first entry since the last roulette reset,
-// bit 8 overflow (more than PATH_VERTS differentiable vertices).
-constexpr int PATH_VERTS = 7;
-enum : uint32_t { PV_AMB = 1, PV_AMB_RBLT = 2, PV_SPEC = 3 };
+// coefficient was formed at, as a list of words.  An entry is a header word
+// followed by 0, 1 or 3 payload words (float bits), by kind:
+//   header bits 0-11  material index      bits 12-15 kind (PV_*)
+//          bit 16     Russian roulette after this vertex rescaled by 1/weight
+//          bits 17-31 Fresnel estimate fest of a pure-specular vertex, x 32767
+//   PV_GSPEC, PV_GTRANS: 1 payload, the score d log p / d roughness summed
+//     over the lobe trials up to the accepted one;
+//   PV_DIR_R, PV_DIR_T: 3 payloads, the direct-light factors of dirnorm()
+//     with the material's colour terms taken out: diffuse a, lobe b, and
+//     d log(lobe) / d roughness.
+// hdr: bits 0-4 word count, 5-9 first word since the last roulette reset,
+// bit 10 overflow (more than PATH_WORDS words).
+constexpr int PATH_WORDS = 15;
+constexpr int ADJ_COLS = 7;      // gradient columns per material
+enum : uint32_t {
+  PV_AMB = 1,            // front ambient: rho * rdiff
+  PV_AMB_RBLT = 2,       // front ambient with the specular part folded in: rho * rdiff + scolor
+  PV_SPEC = 3,           // mirror reflection: scolor
+  PV_GSPEC = 4,          // Gaussian lobe reflection sample: scolor
+  PV_TSPEC = 5,          // straight transmission: rho * tspec
+  PV_GTRANS = 6,         // Gaussian lobe transmission sample: rho * tspec
+  PV_AMB_BACK = 7,       // back ambient: rho * tdiff
+  PV_AMB_BACK_TBLT = 8,  // back ambient with the transmitted lobe folded in: rho * trans
+  PV_DIR_R = 9,          // shadow ray on the reflection side: rho * rdiff * a + scolor * b
+  PV_DIR_T = 10          // shadow ray on the transmission side: rho * tdiff * a + rho * tspec * b
+};
+constexpr uint32_t PV_RR = 1u << 16;
+RG_HD int path_payload(uint32_t kind) {
+  return (kind == PV_GSPEC || kind == PV_GTRANS) ? 1 : ((kind == PV_DIR_R || kind == PV_DIR_T) ? 3 : 0);
+}
 struct alignas(16) PathRec {
-  uint32_t e[PATH_VERTS];
+  uint32_t e[PATH_WORDS];
   uint32_t hdr;
 };
-static_assert(sizeof(PathRec) == 32, "PathRec layout");
+static_assert(sizeof(PathRec) == 64, "PathRec layout");
 
-RG_HD uint32_t path_entry(int mat, uint32_t kind, double fest) {
-  uint32_t q = (uint32_t)(fest <= 0 ? 0.0 : (fest >= 1 ? 65535.0 : fest * 65535.0 + 0.5));
-  return (uint32_t)(mat & 0xfff) | (kind << 12) | (q << 16);
+// An entry to append: header word and its payload.
+struct PathAdd {
+  uint32_t w[4];
+  int n;               // 0: nothing to append
+};
+RG_HD PathAdd path_entry(int mat, uint32_t kind, double fest) {
+  PathAdd a;
+  uint32_t q = (uint32_t)(fest <= 0 ? 0.0 : (fest >= 1 ? 32767.0 : fest * 32767.0 + 0.5));
+  a.w[0] = (uint32_t)(mat & 0xfff) | (kind << 12) | (q << 17);
+  a.w[1] = a.w[2] = a.w[3] = 0u;
+  a.n = 1 + path_payload(kind);
+  return a;
 }
-RG_HD PathRec path_push(PathRec p, uint32_t entry) {
-  if (!entry) return p;
-  uint32_t n = p.hdr & 15u;
-  if (n >= (uint32_t)PATH_VERTS) { p.hdr |= 256u; return p; }
-  p.e[n] = entry;
-  p.hdr = (p.hdr & ~15u) | (n + 1);
+RG_HD PathAdd path_none() {
+  PathAdd a;
+  a.w[0] = a.w[1] = a.w[2] = a.w[3] = 0u;
+  a.n = 0;
+  return a;
+}
+RG_HD PathRec path_push(PathRec p, const PathAdd& a) {
+  if (!a.n) return p;
+  uint32_t n = p.hdr & 31u;
+  if (n + (uint32_t)a.n > (uint32_t)PATH_WORDS) { p.hdr |= 1024u; return p; }
+  for (int i = 0; i < a.n; ++i) p.e[n + i] = a.w[i];
+  p.hdr = (p.hdr & ~31u) | (n + (uint32_t)a.n);
   return p;
 }
 // A roulette survivor's coefficient was divided by its weight, the product
 // of the own-coefficient maxima since the last reset: flag those vertices.
 RG_HD void path_roulette(PathRec& p) {
-  uint32_t n = p.hdr & 15u, seg = (p.hdr >> 4) & 15u;
-  for (uint32_t i = seg; i < n; ++i) p.e[i] |= 0x8000u;
-  p.hdr = (p.hdr & ~0xf0u) | (n << 4);
+  uint32_t n = p.hdr & 31u, i = (p.hdr >> 5) & 31u;
+  while (i < n) {
+    p.e[i] |= PV_RR;
+    i += 1u + (uint32_t)path_payload((p.e[i] >> 12) & 15u);
+  }
+  p.hdr = (p.hdr & ~(31u << 5)) | (n << 5);
 }
 
 // State of one traced ray, reconstructed from its spawn record.
@@ -108,10 +147,11 @@ struct RunCfg {
   const double* adj_w;         // [nsensor][3] d(loss)/d(E), already / accum
   const double* adj_L;         // [ncols] radiance of each output column
   const int32_t* grad_row;     // material index -> grad row (-1: not differentiated)
-  double* grad;                // [rows][5]: rho_r, rho_g, rho_b, specularity, roughness
+  double* grad;                // [rows][ADJ_COLS]: rho_r, rho_g, rho_b, specularity, roughness, transmissivity,
+                               // transmitted specularity (the last two for trans only)
 };
 
-enum : uint32_t { ERR_STACK = 1, ERR_CAPACITY = 2, ERR_NOMAT = 4, ERR_GEN = 8, ERR_PATH = 16, ERR_ADJ_DIRECT = 32 };
+enum : uint32_t { ERR_STACK = 1, ERR_CAPACITY = 2, ERR_NOMAT = 4, ERR_GEN = 8, ERR_PATH = 16 };
 
 // ---- bins (util/reinhartb.cal:19-47, util/klems_full.cal:10-43,
 //      util/disk2square.cal:34-90, util/cieskyscan.cal:21-53) ------------

@@ -1,6 +1,8 @@
 """Path-replay adjoint of the differentiable irradiance workload (the
 config-1 room with 24 plastic groups and a binned sky): gradients against
-central finite differences with common random numbers, the sky gradient
+central finite differences with common random numbers, at the workload's
+size too; the roughness derivative against the closed form of the Gaussian
+lobe; metal, trans and shadow-ray (direct light) deposits; the sky gradient
 against the coefficient matrix, and the torch.autograd wrapper."""
 
 import numpy as np
@@ -32,7 +34,7 @@ def _sky_mods(L):
     return [m, g]
 
 
-def _params(L, ab, ad, seed, roulette):
+def _params(L, ab, ad, seed, roulette, minweight=1e-7):
     p = L.default_params(True)
     p.ambounce, p.ambdiv, p.seed = ab, ad, seed
     # Without roulette (-lr > 0) a ray is dropped below -lw; a specular
@@ -42,16 +44,16 @@ def _params(L, ab, ad, seed, roulette):
     # against 0.99 at h = 1e-4 and 1e-5: the cut-off, not the derivative).
     # The cut-off is therefore set below every path weight here.  With
     # roulette on, a weight limit that the second-bounce rays fall below.
-    p.minweight = 5e-3 if roulette else 1e-7
+    p.minweight = 5e-3 if roulette else minweight
     p.maxdepth = -10 if roulette else 8
     return p
 
 
-def _scene(L, nsens, ab, ad, seed=3, roulette=False):
+def _scene(L, nsens, ab, ad, seed=3, roulette=False, minweight=1e-7):
     from pyradiance_amd.adjoint import DiffScene
     theta = scenes.config4_theta(0)
     rad, sens, groups = scenes.config4(theta)
-    sc = DiffScene(rad, sens[:nsens], groups, _sky_mods(L), _params(L, ab, ad, seed, roulette))
+    sc = DiffScene(rad, sens[:nsens], groups, _sky_mods(L), _params(L, ab, ad, seed, roulette, minweight))
     return sc, theta
 
 
@@ -80,7 +82,6 @@ def test_material_gradient_matches_central_differences(L, roulette, h, floor):
     sc.set_theta(theta)
     g = sc.adjoint(w, sky)
     assert g.shape == (24, 5)
-    assert np.all(g[:, 4] == 0.0)                   # roughness only moves directions
     gmax = np.abs(g).max()
     assert gmax > 0
     checked = 0
@@ -142,4 +143,192 @@ def test_adjoint_replays_the_forward_paths(L):
     assert np.all(g[:, 0] >= 0)
     total = float((g[:, 0] * theta[:, 0]).sum())
     assert 0 < total < e_red
+    sc.close()
+
+
+def _lobe_mods(L):
+    sky = _sky_mods(L)[0]
+    g = L.RgModifier()
+    g.name, g.bin_kind, g.nbins, g.mf, g.rhs = b"groundglow", 1, 145, 1.0, 1.0
+    g.nrm[:] = (0, 0, 1)
+    g.up[:] = (0, 1, 0)
+    return [sky, g]
+
+
+@pytest.mark.parametrize("alpha", [0.08, 0.2])
+def test_roughness_gradient_closed_form(L, alpha):
+    """The roughness derivative of a Gaussian lobe's binned output against
+    the derivative of its closed form (tests/lobes.py): a rough white metal
+    mirror (reflection) and a rough clear trans pane (transmission) seen at
+    normal incidence, -ab 0 -st 0, loss = sum over altitude rows of
+    v_row x (row's share).  The score-function estimate averaged over 8
+    seeds of 2^17 samples must sit within 4 standard errors of
+    d/dalpha sum_row v_row P_row(alpha).  The other columns are exact:
+    own = rho x rspec (metal) and rho x trans x tspec (trans) at all-ones
+    arguments give d/drho_red = d/dspec = E (metal) and d/drho_red = E,
+    d/dspec = -E, d/dtrans = d/dtspec = E (trans)."""
+    import lobes
+    _say(f"roughness gradient vs closed-form lobe, alpha {alpha}")
+    n, seeds = 1 << 17, 8
+    c = L.Context(0)
+    c.load_text(lobes.lobe_scene(alpha)).commit()
+    mods = _lobe_mods(L)
+    v = np.array([0.3, 1.7, -0.8, 2.2, 0.5, -1.1, 1.4, 0.9])       # weight per altitude row
+    vb = np.zeros(145)
+    for r, (a, b) in enumerate(zip(lobes.ROW_START[:-1], lobes.ROW_START[1:])):
+        vb[a:b] = v[r]
+    dlde = np.array([[1.0, 0.0, 0.0]])
+    for ray, trans, name, half in ((lobes.REFLECT_RAY, False, "lobem", 0), (lobes.TRANSMIT_RAY, True, "lobet", 1)):
+        mat = c.material_index(name)
+        lcol = np.zeros((290, 3))
+        lcol[145 * half:145 * half + 145, 0] = vb
+        ga = []
+        for s in range(seeds):
+            p = L.default_params(True)
+            p.ambounce, p.specthresh, p.specjitter, p.minweight, p.seed = 0, 0.0, 1.0, 1e-6, 101 + s
+            c.set_params(p)
+            g = c.contrib_adjoint(np.tile(ray, (n, 1)), mods, dlde, lcol.ravel(), [mat], n, 0)[0]
+            e = c.contrib(np.tile(ray, (n, 1)), mods, n, 0).reshape(290, 3)[:, 0] @ lcol[:, 0]
+            if trans:
+                assert np.allclose(g[[0, 3, 5, 6]], [e, -e, e, e], rtol=1e-6, atol=1e-12), (g, e)
+            else:
+                assert np.allclose(g[[0, 3]], [e, e], rtol=1e-6, atol=1e-12), (g, e)
+                assert g[5] == 0.0 and g[6] == 0.0
+            ga.append(g[4])
+        h = 1e-5
+        want = (v @ lobes.expected_rows(alpha + h, trans) - v @ lobes.expected_rows(alpha - h, trans)) / (2 * h)
+        ga = np.array(ga)
+        mean, se = ga.mean(), ga.std(ddof=1) / np.sqrt(seeds)
+        _say(f"  {'T' if trans else 'R'}: adjoint {mean:.4f} +- {se:.4f}, closed form {want:.4f}")
+        assert abs(mean - want) <= 4 * se + 1e-3 * abs(want), (mean, se, want)
+        assert se < 0.1 * abs(want)                   # the check has power
+    c.close()
+
+
+MIXED = {2: "metal", 9: "metal", 5: "trans", 21: "trans"}     # group -> material type
+
+
+def _mixed_scene(L, nsens, ab, ad, specthresh, seed=3):
+    """Config 4's room with two groups turned to metal and two to trans
+    (transmissivity 0.3, transmitted specularity 0.5) and a sun tracked
+    beside the sky and ground, so shadow-ray deposits enter the loss."""
+    import re
+    from pyradiance_amd.adjoint import DiffScene
+    theta = np.zeros((24, 7))
+    theta[:, :5] = scenes.config4_theta(0)
+    rad, sens, groups = scenes.config4(theta[:, :5])
+    for gi, kind in MIXED.items():
+        name = groups[gi]
+        args = theta[gi, :5]
+        if kind == "trans":
+            theta[gi, 5:] = (0.3, 0.5)
+            args = theta[gi]
+        txt = f"void {kind} {name}\n0\n0\n{len(args)} " + " ".join(f"{x:.9g}" for x in args) + "\n"
+        rad, k = re.subn(rf"void plastic {name}\n0\n0\n5 [^\n]*\n", txt, rad)
+        assert k == 1
+    rad += scenes.sun_rad(scenes.sun_direction(40.0, 180.0), radiance=(1e4, 1e4, 1e4))
+    sun = L.RgModifier()
+    sun.name, sun.bin_kind, sun.nbins, sun.cbin = b"solar", 0, 1, 0
+    p = _params(L, ab, ad, seed, False)
+    p.specthresh = specthresh
+    sc = DiffScene(rad, sens[:nsens], groups, _sky_mods(L) + [sun], p)
+    assert sc.width == 7
+    return sc, theta
+
+
+# -st 0.02 (lobe samples drawn): every column but the roughness, whose
+# derivative is a likelihood-ratio score that finite differences with common
+# random numbers do not see.  -st 1 (every specular part folded into the
+# ambient value, no lobe samples): the roughness then only enters dirnorm()'s
+# lobe on shadow rays, a smooth function, so it is checked too.
+@pytest.mark.parametrize("st,cols", [(0.02, (0, 1, 2, 3, 5, 6)), (1.0, (0, 1, 2, 3, 4, 5, 6))])
+def test_metal_trans_and_direct_gradients_match_central_differences(L, st, cols):
+    _say(f"metal/trans/direct adjoint vs central differences, -st {st}")
+    h = 1e-3
+    sc, theta = _mixed_scene(L, 120, 2, 256, st)
+    rng = np.random.default_rng(7)
+    sky = np.concatenate([rng.lognormal(0, 0.5, (146, 3)), [[2.0, 1.5, 1.0]]])
+    w = rng.uniform(0.5, 1.5, (120, 3))
+    sc.set_theta(theta)
+    dc = sc.coefficients()
+    assert dc[:, 146].sum() > 0                      # the sun is reached by shadow rays
+    g = sc.adjoint(w, sky)
+    assert g.shape == (24, 7)
+    gmax = np.abs(g).max()
+    checked = 0
+    for grp in (1, 2, 5, 9, 13, 21):
+        for q in cols:
+            if q >= 5 and MIXED.get(grp) != "trans":
+                assert g[grp, q] == 0.0
+                continue
+            if q == 4 and theta[grp, 4] ** 2 <= 1e-6:
+                continue
+            tp, tm = theta.copy(), theta.copy()
+            tp[grp, q] += h
+            tm[grp, q] -= h
+            fd = (_loss_parts(sc, tp, w, sky)[0] - _loss_parts(sc, tm, w, sky)[0]) / (2 * h)
+            ad = g[grp, q]
+            tol = 0.03 * max(abs(fd), abs(ad)) + 0.005 * gmax
+            assert abs(fd - ad) <= tol, (grp, q, fd, ad)
+            checked += 1
+    assert checked >= 24
+    sc.close()
+
+
+def test_config4_workload_gradient(L):
+    """Config 4 at its workload: 1000 sensors, -ab 3 -ad 2048.  Reflectance
+    and specularity gradients of every group against central differences
+    with common random numbers (h = 1e-3, no roulette); the roughness
+    gradient, a likelihood-ratio estimate, against central differences
+    (h = 1e-2) over 8 seeds, per group within 4 combined standard errors."""
+    _say("config 4 workload: adjoint vs central differences")
+    # -ad 2048 starts the first bounce at weight 1/2025, and chains of
+    # specular children (weight x specularity each) reach a 1e-7 cut-off,
+    # where a finite step moves paths across it: the cut-off goes to 1e-12.
+    sc, theta = _scene(L, 1000, 3, 2048, minweight=1e-12)
+    rng = np.random.default_rng(11)
+    sky = rng.lognormal(0, 0.5, (146, 3))
+    w = rng.uniform(0.5, 1.5, (1000, 3))
+    sc.set_theta(theta)
+    g = sc.adjoint(w, sky)
+    gmax = np.abs(g[:, :4]).max()
+    h = 1e-3
+    worst = 0.0
+    for grp in range(24):
+        if abs(theta[grp, 3] - 0.02) < 2 * h:        # specularity at the -st cut-off
+            continue
+        for q in range(4):
+            if theta[grp, q] < 2 * h:                    # the step would clip at zero
+                continue
+            tp, tm = theta.copy(), theta.copy()
+            tp[grp, q] += h
+            tm[grp, q] -= h
+            fd = (_loss_parts(sc, tp, w, sky)[0] - _loss_parts(sc, tm, w, sky)[0]) / (2 * h)
+            err = abs(fd - g[grp, q])
+            worst = max(worst, err / (0.03 * max(abs(fd), abs(g[grp, q])) + 0.005 * gmax))
+            assert err <= 0.03 * max(abs(fd), abs(g[grp, q])) + 0.005 * gmax, (grp, q, fd, g[grp, q])
+    _say(f"  rho/spec: worst error / tolerance {worst:.3f}")
+    # roughness: 8 seeds of adjoint and of central differences
+    hr, seeds = 1e-2, 8
+    rough = [k for k in range(24) if theta[k, 4] > 2 * hr]
+    ga, fa = [], []
+    for s in range(seeds):
+        p = _params(L, 3, 2048, 50 + s, False, 1e-12)
+        sc.ctx.set_params(p)
+        sc.set_theta(theta)
+        ga.append(sc.adjoint(w, sky)[rough, 4])
+        f = []
+        for grp in rough:
+            tp, tm = theta.copy(), theta.copy()
+            tp[grp, 4] += hr
+            tm[grp, 4] -= hr
+            f.append((_loss_parts(sc, tp, w, sky)[0] - _loss_parts(sc, tm, w, sky)[0]) / (2 * hr))
+        fa.append(f)
+    ga, fa = np.array(ga), np.array(fa)
+    se = np.sqrt(ga.var(0, ddof=1) / seeds + fa.var(0, ddof=1) / seeds)
+    z = (ga.mean(0) - fa.mean(0)) / np.maximum(se, 1e-30)
+    _say(f"  roughness, {len(rough)} groups: |z| max {np.abs(z).max():.2f}, mean z^2 {np.mean(z * z):.2f}; "
+         f"groups with |adjoint| > 2 se: {int((np.abs(ga.mean(0)) > 2 * ga.std(0, ddof=1) / np.sqrt(seeds)).sum())}")
+    assert np.all(np.abs(z) <= 4.0), z
+    assert np.mean(z * z) < 3.0
     sc.close()
