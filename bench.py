@@ -108,7 +108,7 @@ def cpu_baseline(rad: str, sensors: np.ndarray, nsample: int, cores: int, gpu_co
         octf = Path(td) / "room.oct"
         radf.write_text(rad)
         with open(octf, "wb") as f:
-            subprocess.run([str(ref / "oconv"), "-f", str(radf)], stdout=f, check=True)
+            subprocess.run([str(ref / "oconv"), str(radf)], stdout=f, check=True)
         sample = sensors[:nsample]
         inp = "\n".join(" ".join(f"{v:.17g}" for v in r) for r in sample).encode()
         args = ["-h", "-I+", "-ab", "4", "-ad", "4096", "-lw", "1e-4", "-aa", "0", "-c", "1",
@@ -418,7 +418,7 @@ def config1_bench(nsun: int = 1000, cores: int = 1, cpu: bool = True):
             inp = "\n".join(" ".join(f"{v:.17g}" for v in r) for r in sens[:nsens]).encode() + b"\n"
             t0 = time.perf_counter()
             with open(octf, "wb") as f:
-                subprocess.run([str(ref / "oconv"), "-f", str(radf)], stdout=f, check=True)
+                subprocess.run([str(ref / "oconv"), str(radf)], stdout=f, check=True)
             r = subprocess.run([str(ref / prog), *args, str(octf)], input=inp, capture_output=True, env=env,
                                check=True)
             t = time.perf_counter() - t0
@@ -728,7 +728,7 @@ def rpict_bench(xres: int = 1024, yres: int = 1024, reps: int = 3, cpu: bool = T
         if cpu and (ref / "rpict").exists() and (ref / "oconv").exists():
             octf = Path(td) / "room.oct"
             with open(octf, "wb") as f:
-                subprocess.run([str(ref / "oconv"), "-f", str(radf)], stdout=f, check=True)
+                subprocess.run([str(ref / "oconv"), str(radf)], stdout=f, check=True)
 
             def run(x, y):
                 t0 = time.perf_counter()

@@ -147,11 +147,19 @@ static void vcross(double *r, const double *a, const double *b) {
   double t[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
   r[0] = t[0]; r[1] = t[1]; r[2] = t[2];
 }
+/* normalize() of common/fvect.c:132-155, with its first-order reciprocal
+ * within FTINY of unit length, in the operation order of the reference's
+ * -O3 -ffast-math build: |v|^2 = (x^2 + z^2) + y^2, 2 - (|v|^2 + 1) / 2 */
 static double vnorm(double *v) {
-  double l2 = DOT(v, v), l, s;
-  if (l2 <= 0) return 0;
-  l = sqrt(l2);
-  s = 1.0 / l;
+  double l2 = (v[0] * v[0] + v[2] * v[2]) + v[1] * v[1], l, s;
+  if (l2 == 0.0) return 0;
+  if ((l2 <= 1.0 + FTINY) & (l2 >= 1.0 - FTINY)) {
+    l = (l2 + 1.0) * 0.5;
+    s = 2.0 - l;
+  } else {
+    l = sqrt(l2);
+    s = 1.0 / l;
+  }
   v[0] *= s; v[1] *= s; v[2] *= s;
   return l;
 }
@@ -371,6 +379,16 @@ static void scene_cube(ro_scene *s) {
   for (k = 0; k < 3; k++)
     if (hi[k] - lo[k] > s->cusize) s->cusize = hi[k] - lo[k];
   for (k = 0; k < 3; k++) s->cuorg[k] = (hi[k] + lo[k] - s->cusize) * .5;
+  /* as read back from the octree: 12 significant digits (ot/writeoct.c:44-47) */
+  {
+    char buf[40];
+    for (k = 0; k < 3; k++) {
+      snprintf(buf, sizeof buf, "%.12g", s->cuorg[k]);
+      s->cuorg[k] = strtod(buf, NULL);
+    }
+    snprintf(buf, sizeof buf, "%.12g", s->cusize);
+    s->cusize = strtod(buf, NULL);
+  }
 }
 
 ro_scene *ro_scene_parse(const char *text, char *err, int errlen) {
@@ -827,8 +845,12 @@ static int cache_cell(const Ctx *c, int sn, const ORay *r) {
     int ax = C->ax, ax1 = (ax + 1) % 3, ax2 = (ax + 2) % 3;
     double t = (C->o[ax] - r->rorg[ax]) / S->sloc[ax];
     if (t <= FTINY) return -1;
-    ondx = 2 * SHADCACHE * (int)(2 * SHADCACHE * C->e1 * (r->rorg[ax1] + t * S->sloc[ax1] - C->o[ax1]));
-    ondx += (int)(2 * SHADCACHE * C->e2 * (r->rorg[ax2] + t * S->sloc[ax2] - C->o[ax2]));
+    /* in the operation order of the reference's -O3 -ffast-math build of
+     * rt/srcobstr.c:216-221 (reassociated to ((t sl - o) + ro) (e 40)):
+     * shadow-ray origins on the cube's centre lines fall exactly on a cell
+     * boundary, where only that order gives the reference's cell */
+    ondx = 2 * SHADCACHE * (int)(((t * S->sloc[ax1] - C->o[ax1]) + r->rorg[ax1]) * (C->e1 * (2 * SHADCACHE)));
+    ondx += (int)(((t * S->sloc[ax2] - C->o[ax2]) + r->rorg[ax2]) * (C->e2 * (2 * SHADCACHE)));
     if ((ondx < 0) | (ondx >= 4 * SHADCACHE * SHADCACHE)) return -1;
   } else if (S->flags & SFLAT) {
     double sd[3], sd0m, sd1m;
@@ -944,9 +966,11 @@ static void cache_init(Ctx *c, int sn) {
     for (k = 0; k < 3; k++) rdir[k] = -S->sloc[k];
     for (i = 2 * SHADCACHE; i--;)
       for (j = 2 * SHADCACHE; j--;) {
+        /* (i + .5) / (2 SHADCACHE e1) as the reference build evaluates it:
+         * (i + .5) * 0.025 / e1 (rt/srcobstr.c:127-131) */
         memcpy(rorg, C->o, sizeof rorg);
-        rorg[ax1] += (i + .5) / (2 * SHADCACHE * C->e1);
-        rorg[ax2] += (j + .5) / (2 * SHADCACHE * C->e2);
+        rorg[ax1] += (i + .5) * (1.0 / (2 * SHADCACHE)) / C->e1;
+        rorg[ax2] += (j + .5) * (1.0 / (2 * SHADCACHE)) / C->e2;
         cast_shadow(c, sn, rorg, rdir);
       }
   } else if (S->flags & SFLAT) {
@@ -1635,9 +1659,11 @@ static void eval_ray(Ctx *c, const double *ry, long ridx, long sensor, int imm_i
   r.ro = -1;
   if (imm_irrad) {
     for (k = 0; k < 3; k++) {
+      /* rtcompute() and rayirrad() (rt/rtrace.c:416-449): origin 1.1e-4
+       * out along the normal, the pretended hit 1e-5 back from there */
       r.rorg[k] = org[k] + dir[k] * 1.1e-4;
       r.rdir[k] = -dir[k];
-      r.rop[k] = org[k] + dir[k] * 1e-4;
+      r.rop[k] = r.rorg[k] + r.rdir[k] * 1e-5;
       r.ron[k] = dir[k];
     }
     r.rot = 1e-5;

@@ -342,21 +342,26 @@ static void shadow_begin(rg_ctx* c, DevScene& S, bool on) {
   hipck(hipMemsetAsync(S.src_tested, 0, sizeof(uint32_t) * c->cs.srcs.size(), c->st), "tested flags");
 }
 
-// After a call: the cast rays of every source tested for the first time
-// since the commit, as the reference's counter counts them.
+// The cast rays of every source of c flagged in `tested` for the first
+// time since the commit, as the reference's counter counts them.
+static unsigned long long shadow_count(rg_ctx* c, const uint32_t* tested) {
+  unsigned long long add = 0;
+  for (size_t k = 0; k < c->cs.srcs.size(); ++k)
+    if (tested[k] && !c->cast_counted[k]) {
+      c->cast_counted[k] = 1;
+      add += c->cast_count[k];
+    }
+  return add;
+}
+
+// After a call: shadow_count of the flags the call left.
 static unsigned long long shadow_end(rg_ctx* c, const DevScene& S) {
   if (!S.shadow_on) return 0;
   const size_t ns = c->cs.srcs.size();
   std::vector<uint32_t> tested(ns);
   hipck(hipMemcpyAsync(tested.data(), S.src_tested, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost, c->st), "tested");
   hipck(hipStreamSynchronize(c->st), "tested");
-  unsigned long long add = 0;
-  for (size_t k = 0; k < ns; ++k)
-    if (tested[k] && !c->cast_counted[k]) {
-      c->cast_counted[k] = 1;
-      add += c->cast_count[k];
-    }
-  return add;
+  return shadow_count(c, tested.data());
 }
 
 // device_io: `rays` and `out` are device memory of the context's device
@@ -402,11 +407,28 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   cfg.scene_start = nullptr;
   // shadow caches of the scenes traced (filled at the first call)
   const bool shadow = c->tuning.shadow_cache != 0;
+  // a batch's tested-source flags: one buffer on the launching stream,
+  // cleared and read back once for all its scenes
+  uint32_t* d_tested = nullptr;
+  std::vector<size_t> tested_at;
   if (batch) {
+    size_t nt = 0;
     for (size_t i = 0; i < batch->size(); ++i) {
-      shadow_begin((*batch_ctx)[i], (*batch)[i], shadow);
-      hipck(hipStreamSynchronize((*batch_ctx)[i]->st), "shadow cache");
+      rg_ctx* bc = (*batch_ctx)[i];
+      DevScene& S = (*batch)[i];
+      S.shadow_on = 0;
+      tested_at.push_back(nt);
+      if (!shadow || bc->cs.scache.empty()) continue;
+      shadow_fill(bc);     // first call after a commit only (synchronises its stream)
+      S.shadow_on = 1;
+      nt += bc->cs.srcs.size();
     }
+    if (nt) {
+      d_tested = (uint32_t*)c->arena.get("batch_tested", sizeof(uint32_t) * nt, c->st, true);
+      for (size_t i = 0; i < batch->size(); ++i)
+        if ((*batch)[i].shadow_on) (*batch)[i].src_tested = (RG_GLOBAL uint32_t*)(d_tested + tested_at[i]);
+    }
+    tested_at.push_back(nt);
   } else {
     shadow_begin(c, c->dscene, shadow);
   }
@@ -483,7 +505,12 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
           "grad");
   hipck(hipStreamSynchronize(c->st), "render");
   if (batch) {
-    for (size_t i = 0; i < batch->size(); ++i) c->last_nrays += shadow_end((*batch_ctx)[i], (*batch)[i]);
+    if (d_tested) {
+      std::vector<uint32_t> tested(tested_at.back());
+      hipck(hipMemcpy(tested.data(), d_tested, sizeof(uint32_t) * tested.size(), hipMemcpyDeviceToHost), "tested");
+      for (size_t i = 0; i < batch->size(); ++i)
+        if ((*batch)[i].shadow_on) c->last_nrays += shadow_count((*batch_ctx)[i], tested.data() + tested_at[i]);
+    }
   } else {
     c->last_nrays += shadow_end(c, c->dscene);
   }

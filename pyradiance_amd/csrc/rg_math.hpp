@@ -25,13 +25,25 @@ RG_HD V3 cross(V3 a, V3 b) {
   return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 RG_HD double comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-// normalize() of common/fvect.c: returns the length, leaves zero vectors alone
+// normalize() of common/fvect.c:132-155: returns the length, leaves zero
+// vectors alone; within FTINY of unit length it takes the first-order
+// reciprocal 2 - (1 + |v|^2) / 2.  The arithmetic is in the order the
+// reference's -O3 -ffast-math build evaluates it, |v|^2 = (x^2 + z^2) + y^2,
+// without fused multiply-adds: source directions feed the shadow cache's
+// cell indices, where the last bit can decide a cell.
 RG_HD double normalize(V3& v) {
-  double l2 = dot(v, v);
-  if (l2 <= 0.0) return 0.0;
-  double l = sqrt(l2);
-  double inv = 1.0 / l;
-  v = v * inv;
+#pragma clang fp contract(off)
+  double l2 = (v.x * v.x + v.z * v.z) + v.y * v.y;
+  if (l2 == 0.0) return 0.0;
+  double l, inv;
+  if ((l2 <= 1.0 + 1e-6) & (l2 >= 1.0 - 1e-6)) {
+    l = (l2 + 1.0) * 0.5;
+    inv = 2.0 - l;
+  } else {
+    l = sqrt(l2);
+    inv = 1.0 / l;
+  }
+  v = V3{v.x * inv, v.y * inv, v.z * inv};
   return l;
 }
 RG_HD V3 load3(const double* p) { return V3{p[0], p[1], p[2]}; }

@@ -631,6 +631,9 @@ __device__ __forceinline__ C3 dirnorm(const NormDat& np, const RunCfg& c, V3 rdi
 // srcobstructp() (rt/srcobstr.c:196-278): the cell of a ray from o along d
 // aimed at source src; -1 when the ray falls outside the grid.
 __device__ __forceinline__ int shadow_cell(const Source& src, const SrcCache& C, V3 o, V3 d) {
+  // no fused multiply-adds: shadow-ray origins on the cube's centre lines
+  // fall exactly on a cell boundary, where the rounding decides the cell
+#pragma clang fp contract(off)
   constexpr int S = kShadCache;
   int ondx;
   if (C.kind == SC_DISTANT) {
@@ -639,8 +642,10 @@ __device__ __forceinline__ int shadow_cell(const Source& src, const SrcCache& C,
     const double ro[3] = {o.x, o.y, o.z};
     const double t = (C.o[ax] - ro[ax]) / sl[ax];
     if (t <= FTINY) return -1;
-    ondx = 2 * S * (int)(2 * S * C.e1 * (ro[ax1] + t * sl[ax1] - C.o[ax1]));
-    ondx += (int)(2 * S * C.e2 * (ro[ax2] + t * sl[ax2] - C.o[ax2]));
+    // in the operation order of the reference's -O3 -ffast-math build
+    // (reassociated to ((t sl - o) + ro) (e 2S))
+    ondx = 2 * S * (int)(((t * sl[ax1] - C.o[ax1]) + ro[ax1]) * (C.e1 * (2 * S)));
+    ondx += (int)(((t * sl[ax2] - C.o[ax2]) + ro[ax2]) * (C.e2 * (2 * S)));
     if ((ondx < 0) | (ondx >= 4 * S * S)) return -1;
   } else if (C.kind == SC_FLAT) {
     const double sd0 = -dot(d, load3(C.u)), sd1 = -dot(d, load3(C.v)), sd2 = -dot(d, load3(src.snorm));
@@ -1196,7 +1201,10 @@ __global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene Sarg, RunCfg c, c
   V3 org = load3(ry), dir = load3(ry + 3);
   normalize(dir);
   RayS r;
-  r.o = org + dir * 1.1e-4;
+  {
+#pragma clang fp contract(off)
+    r.o = V3{org.x + dir.x * 1.1e-4, org.y + dir.y * 1.1e-4, org.z + dir.z * 1.1e-4};
+  }
   r.d = -dir;
   r.coef = c3(1, 1, 1);
   r.w = 1.0f;
@@ -1210,8 +1218,13 @@ __global__ void __launch_bounds__(BLOCK) k_irrad_seed(DevScene Sarg, RunCfg c, c
   r.key = sensor_key(c.p.seed + scene, s0 + i - (c.nscenes ? c.scene_start[scene] : 0));
   if (ADJ) r.path = PathRec{};
   HitS hs;
+  // rayirrad() (rt/rtrace.c:416-431): the pretended hit 1e-5 back along
+  // the ray, computed as the reference does (shadow-cache cells read it)
   hs.rot = 1e-5;
-  hs.rop = org + dir * 1e-4;
+  {
+#pragma clang fp contract(off)
+    hs.rop = V3{r.o.x + r.d.x * 1e-5, r.o.y + r.d.y * 1e-5, r.o.z + r.d.z * 1e-5};
+  }
   hs.ron = dir;
   hs.rod = 1.0;
   hs.prim = -1;

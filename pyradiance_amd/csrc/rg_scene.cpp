@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <unordered_map>
@@ -30,10 +31,21 @@ inline void cross3(double* r, const double* a, const double* b) {
   r[1] = a[2] * b[0] - a[0] * b[2];
   r[2] = a[0] * b[1] - a[1] * b[0];
 }
+// normalize() of common/fvect.c:132-155, first-order reciprocal near unit
+// length included, in the operation order of the reference's -O3
+// -ffast-math build ((x^2 + z^2) + y^2; source directions feed the shadow
+// cache's cell indices)
 inline double norm3(double* v) {
-  double l2 = dot3(v, v);
-  if (l2 <= 0) return 0;
-  double l = sqrt(l2), inv = 1.0 / l;
+  double l2 = (v[0] * v[0] + v[2] * v[2]) + v[1] * v[1];
+  if (l2 == 0.0) return 0;
+  double l, inv;
+  if ((l2 <= 1.0 + FTINY) & (l2 >= 1.0 - FTINY)) {
+    l = (l2 + 1.0) * 0.5;
+    inv = 2.0 - l;
+  } else {
+    l = sqrt(l2);
+    inv = 1.0 / l;
+  }
   v[0] *= inv; v[1] *= inv; v[2] *= inv;
   return l;
 }
@@ -268,9 +280,11 @@ static void shadow_cache_setup(CompiledScene& cs) {
       for (int a = 0; a < 3; ++a) dir[a] = -src.sloc[a];
       for (int i = 2 * S; i--;)
         for (int j = 2 * S; j--;) {
+          // (i + .5) / (2 S e1) as the reference's -O3 -ffast-math build
+          // evaluates it (rt/srcobstr.c:127-131): (i + .5) * 0.025 / e1
           for (int a = 0; a < 3; ++a) org[a] = C.o[a];
-          org[ax1] += (i + .5) / (2 * S * C.e1);
-          org[ax2] += (j + .5) / (2 * S * C.e2);
+          org[ax1] += (i + .5) * (1.0 / (2 * S)) / C.e1;
+          org[ax2] += (j + .5) * (1.0 / (2 * S)) / C.e2;
           cast((int)sn, org, dir);
         }
     } else if (C.kind == SC_FLAT) {
@@ -716,6 +730,15 @@ CompiledScene compile_scene(const SceneDB& db) {
       cs.cube_size = std::max(cs.cube_size, cb_hi[a] - cb_lo[a]);
     }
     for (int a = 0; a < 3; ++a) cs.cuorg[a] = (cb_hi[a] + cb_lo[a] - cs.cube_size) * .5;
+    // the reference reads the cube back from the octree oconv wrote, where
+    // it is text of 12 significant digits (ot/writeoct.c:44-47)
+    char buf[40];
+    for (int a = 0; a < 3; ++a) {
+      snprintf(buf, sizeof buf, "%.12g", cs.cuorg[a]);
+      cs.cuorg[a] = strtod(buf, nullptr);
+    }
+    snprintf(buf, sizeof buf, "%.12g", cs.cube_size);
+    cs.cube_size = strtod(buf, nullptr);
   }
   // FP32 bounds, widened so that the FP32 slab test stays conservative
   double eps = 1e-5 + 4e-6 * ext;

@@ -5,6 +5,7 @@ built into oracle/_ref (rtrace, rcontrib, oconv) from the reference sources.
 
 Fixtures (inputs and the reference's outputs only):
   c1_seed{0,1,2}.npz     config-1 room: sensors, rtrace -I -ab 0 -dj 0 -ds 0 values
+  c1_counts.npz          config 1's reference ray count for sun seeds 0 .. 999
   bins_reinhart.npz      sky directions and the rcontrib bin (MF 1, 2, 4)
   bins_klems.npz         directions and the klems_full.cal kbinS / kbinD bins
   stat_room.npz          48 repeats of 4 sensors, rcontrib -ab 2 -ad 256 bins
@@ -61,11 +62,16 @@ def rays_text(rays: np.ndarray) -> bytes:
     return ("\n".join(" ".join(f"{v:.17g}" for v in r) for r in rays) + "\n").encode()
 
 
-def oconv(rad: str, td: str) -> str:
+def oconv(rad: str, td: str, frozen: bool = False) -> str:
+    """Octree of `rad` by the reference oconv.  A plain octree makes rtrace
+    and rcontrib read the scene text itself, at full precision, as the GPU
+    path does; a frozen one (-f) stores real arguments in Radiance's packed
+    float form (about 31 bits of mantissa), which moves source directions
+    and shadow-cache cells by ~1e-10."""
     radf, octf = Path(td) / "s.rad", Path(td) / "s.oct"
     radf.write_text(rad)
     with open(octf, "wb") as f:
-        subprocess.run([str(REF / "oconv"), "-f", str(radf)], stdout=f, check=True)
+        subprocess.run([str(REF / "oconv"), *(["-f"] if frozen else []), str(radf)], stdout=f, check=True)
     return str(octf)
 
 
@@ -88,6 +94,26 @@ def config1():
             assert np.array_equal(np.array(r.stdout.split(), dtype=np.float64).reshape(-1, 3), v)
             nrays = [int(ln.split()[1]) for ln in r.stderr.decode().splitlines() if ln.startswith("NRAYS")][0]
         np.savez_compressed(HERE / f"c1_seed{seed}.npz", sensors=sens, sun=sun, values=v, nrays=nrays)
+
+
+def config1_counts(n: int = 1000):
+    """The reference's ray count of config 1 (rtrace_count NRAYS: the
+    shadow-cache fill toward the sun plus the traced shadow rays and their
+    continuations) for sun seeds 0 .. n-1.  Sensors on the room's centre
+    lines project exactly onto shadow-cache cell boundaries, so these counts
+    pin the cell arithmetic bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(seed):
+        rad, sens, _ = scenes.config1(seed)
+        with tempfile.TemporaryDirectory() as td:
+            octf = oconv(rad, td)
+            r = subprocess.run([str(REF / "rtrace_count"), "-h", "-I", "-ab", "0", "-dj", "0", "-ds", "0", octf],
+                               input=rays_text(sens), capture_output=True, env=ENV, check=True)
+        return [int(ln.split()[1]) for ln in r.stderr.decode().splitlines() if ln.startswith("NRAYS")][0]
+    with ThreadPoolExecutor(os.cpu_count() or 1) as ex:
+        counts = list(ex.map(one, range(n)))
+    np.savez_compressed(HERE / "c1_counts.npz", seeds=np.arange(n), nrays=np.array(counts, dtype=np.int64))
 
 
 def _stat_runs(octf: str, pts: np.ndarray, R: int, args: list, nproc: int | None = None):
@@ -981,7 +1007,7 @@ def image():
 if __name__ == "__main__":
     if not (REF / "rcontrib").exists():
         sys.exit("build the reference tools first: make -C oracle ref")
-    jobs = {"config1": config1, "c2_stat": c2_stat, "rc_records": rc_records, "rough_stat": rough_stat,
+    jobs = {"config1": config1, "config1_counts": config1_counts, "c2_stat": c2_stat, "rc_records": rc_records, "rough_stat": rough_stat,
             "c4_stat": c4_stat, "contrib_stat": contrib_stat, "bins": bins, "stat_room": stat_room, "primitives": primitives,
             "shading": shading, "bins_flux": bins_flux, "rflux": rflux, "rflux_send": rflux_send,
             "mesh": mesh, "instance": instance, "sky": sky, "image": image}

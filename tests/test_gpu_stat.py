@@ -67,6 +67,29 @@ def test_config2_matches_reference_at_3_sigma(L):
     assert abs(nr - ref_n) <= 0.01 * ref_n
 
 
+def test_config1_ray_counts_1000_suns(L):
+    """Config 1 (rtrace -I -ab 0 -dj 0 -ds 0) for sun seeds 0 .. 999: the
+    GPU's ray count equals the reference's counter (c1_counts.npz, plain
+    octrees) for every sun, the cases where a sensor projects exactly onto
+    a shadow-cache cell boundary included."""
+    _say("config 1 ray counts, 1000 suns")
+    g = np.load(GOLDEN / "c1_counts.npz")
+    p = L.default_params(False)
+    p.ambounce, p.dstrsrc, p.srcsizerat = 0, 0.0, 0.0
+    bad = []
+    for seed in range(len(g["seeds"])):
+        rad, sens, _ = scenes.config1(seed)
+        c = L.Context(0)
+        c.load_text(rad).commit()
+        c.set_params(p)
+        c.trace(sens, L.RT_IMM_IRRAD)
+        if c.last_nrays() != int(g["nrays"][seed]):
+            bad.append((seed, c.last_nrays(), int(g["nrays"][seed])))
+        c.close()
+    _say(f"  differing suns: {bad[:10]}")
+    assert not bad
+
+
 def test_shadow_cache_ray_counts(L):
     """The shadow cache (rt/srcobstr.c) on the local-source fixture: the GPU
     counts the cache's cast rays and skips the shadow rays it settles, so

@@ -68,6 +68,24 @@ def test_config1_oracle_matches_reference(oracle, seed):
     assert 1600 < nrays < 1600 + 2 * len(sens) + 100
 
 
+# Suns whose config-1 count hinges on a sensor on the room's centre line
+# projecting exactly onto a shadow-cache cell boundary (cell coordinate 20
+# to within an ulp): the count is the reference's only with the reference's
+# arithmetic (its normalize(): first-order reciprocal near unit length,
+# (x^2 + z^2) + y^2; the octree cube to 12 digits; the cell index and cast
+# origins in the order its build evaluates them) and the scene read at full
+# precision (a frozen octree's packed arguments move these cells).
+TIE_SUNS = [172, 297, 518, 536, 614, 636, 641, 662, 710, 902, 973]
+
+
+def test_config1_ray_counts_on_cell_boundaries(oracle):
+    g = np.load(GOLDEN / "c1_counts.npz")
+    for seed in TIE_SUNS + [0, 1, 2, 3, 4]:
+        rad, sens, _ = scenes.config1(seed)
+        _, nrays = oracle.Scene(rad).trace(sens, oracle.params(ambounce=0, dstrsrc=0.0, srcsizerat=0.0), 8)
+        assert nrays == int(g["nrays"][seed]), seed
+
+
 def test_shadow_cache_ray_counts_match_reference(oracle):
     """The reference counts the rays its shadow cache casts (40 x 40 toward
     a distant source, 1200 from a flat one, 2400 from a sphere) and skips
