@@ -365,8 +365,8 @@ def config1_bench(nsun: int = 1000, cores: int = 1, cpu: bool = True):
     nsun-1): rtrace -I -ab 0 -dj 0 -ds 0, 77 sensors per sun.
 
     GPU, per call: load + commit + trace of one sun's scene, one call after
-    another.  GPU, steady state: the committed scenes traced in batches of
-    up to 250 (rg_trace_batch).  Reference: oconv + rtrace for every sun,
+    another.  GPU, steady state: the committed scenes traced in one batched
+    call (rg_trace_batch).  Reference: oconv + rtrace for every sun,
     the suns spread over `cores` concurrent pipelines; per call is the
     median of one sun's oconv + rtrace wall, whole job the wall of all of
     them, trace-only the whole-job wall minus that of the same pipelines
@@ -388,16 +388,15 @@ def config1_bench(nsun: int = 1000, cores: int = 1, cpu: bool = True):
         t_call.append(time.perf_counter() - t0)
         vals.append(v)
         g_rays.append(c.last_nrays())
-    # steady state: batched traces of the committed scenes
-    _lib.trace_batch(ctxs[:250], [w[1] for w in work[:250]], _lib.RT_IMM_IRRAD)       # warm-up
+    # steady state: one batched trace of all the committed scenes
+    sens_all = [w[1] for w in work]
+    _lib.trace_batch(ctxs, sens_all, _lib.RT_IMM_IRRAD)       # warm-up
     t0 = time.perf_counter()
-    nb = 0
-    for k in range(0, nsun, 250):
-        out = _lib.trace_batch(ctxs[k:k + 250], [w[1] for w in work[k:k + 250]], _lib.RT_IMM_IRRAD)
-        nb += ctxs[k].last_nrays()
-        for i, o in enumerate(out):
-            assert np.array_equal(o, vals[k + i])
+    out = _lib.trace_batch(ctxs, sens_all, _lib.RT_IMM_IRRAD)
     t_batch = time.perf_counter() - t0
+    nb = ctxs[0].last_nrays()
+    for i, o in enumerate(out):
+        assert np.array_equal(o, vals[i])
     for c in ctxs:
         c.close()
     # rays_gpu: every call fills its scene's shadow cache (as every

@@ -110,11 +110,13 @@ int rg_set_stream(rg_ctx *ctx, void *stream);
 /* Batched rcontrib: the records of nctx scenes (one committed context each,
  * same device) traced in one wavefront.  rays holds scene 0's nrec[0]*accum
  * rays, then scene 1's, ...; out holds the records in the same order.
- * Parameters and the ray counter are those of ctxs[0]. */
+ * Parameters and the ray counter are those of ctxs[0].  At most 4096
+ * scenes per call. */
 int rg_contrib_batch(rg_ctx *const *ctxs, int nctx, const double *rays, const long long *nrec, int accum,
                      int flags, const rg_modifier *mods, int nmods, double *out);
 
-/* Batched rtrace: n[i] rays of scene i, concatenated; out in the same order. */
+/* Batched rtrace: n[i] rays of scene i, concatenated; out in the same order
+ * (at most 4096 scenes per call). */
 int rg_trace_batch(rg_ctx *const *ctxs, int nctx, const double *rays, const long long *n, int flags, double *out);
 
 /* Materials of the committed scene (no reference counterpart: the reference

@@ -47,6 +47,12 @@ struct rg_ctx {
   std::vector<char> cast_counted;
   hipStream_t caller = nullptr;   // device-memory inputs are read after the work queued here
   hipEvent_t caller_ev = nullptr;
+  // pinned host staging: a call's inputs go up and its small results come
+  // back through it in one transfer each way
+  char* stage = nullptr;
+  size_t stage_bytes = 0;
+  const double* d_cast_rays = nullptr;   // shadow-cache cast rays (in the scene blob)
+  const int32_t* d_cast_src = nullptr;
 };
 
 static thread_local std::string g_create_err;
@@ -84,6 +90,54 @@ static T* upload(DevArena& a, const std::string& name, const std::vector<T>& v, 
   if (!v.empty()) hipck(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st), "upload");
   return p;
 }
+
+// The context's pinned staging buffer, at least `bytes` long.  Every call
+// ends with its stream synchronised, so no transfer uses it when it grows.
+static char* stage_buf(rg_ctx* c, size_t bytes) {
+  if (c->stage_bytes < bytes) {
+    if (c->stage) (void)hipHostFree(c->stage);
+    c->stage = nullptr;
+    c->stage_bytes = 0;
+    hipck(hipHostMalloc((void**)&c->stage, bytes, 0), "pinned staging");
+    c->stage_bytes = bytes;
+  }
+  return c->stage;
+}
+
+// Host arrays laid out in one device block (256-byte aligned parts), filled
+// through the pinned staging buffer and sent in one copy.  A part with no
+// source is zero-filled; one with fewer source bytes than its size has a
+// zero tail.
+struct Pack {
+  struct Part {
+    size_t off, bytes, src_bytes;
+    const void* src;
+  };
+  std::vector<Part> parts;
+  size_t total = 0;
+  size_t add(const void* src, size_t src_bytes, size_t bytes) {
+    const size_t off = (total + 255) & ~(size_t)255;
+    parts.push_back({off, bytes, src_bytes, src});
+    total = off + std::max<size_t>(bytes, 16);
+    return off;
+  }
+  template <class T>
+  size_t add(const std::vector<T>& v) { return add(v.data(), sizeof(T) * v.size(), sizeof(T) * v.size()); }
+  size_t zero(size_t bytes) { return add(nullptr, 0, bytes); }
+  // stage the parts and queue the copy to dev (total bytes)
+  void send(rg_ctx* c, void* dev) const {
+    char* h = stage_buf(c, total);
+    size_t at = 0;
+    for (const Part& p : parts) {
+      if (p.off > at) memset(h + at, 0, p.off - at);
+      if (p.src_bytes) memcpy(h + p.off, p.src, p.src_bytes);
+      const size_t end = p.off + std::max<size_t>(p.bytes, 16);
+      if (end > p.off + p.src_bytes) memset(h + p.off + p.src_bytes, 0, end - p.off - p.src_bytes);
+      at = end;
+    }
+    if (total) hipck(hipMemcpyAsync(dev, h, total, hipMemcpyHostToDevice, c->st), "upload");
+  }
+};
 
 extern "C" {
 
@@ -138,6 +192,7 @@ void rg_ctx_destroy(rg_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   if (c->caller_ev) (void)hipEventDestroy(c->caller_ev);
+  if (c->stage) (void)hipHostFree(c->stage);
   c->arena.release();
   c->scene_mem.release();
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -170,23 +225,12 @@ int rg_scene_commit(rg_ctx* c) {
     c->cs = compile_scene(c->db);
     CompiledScene& cs = c->cs;
     DevScene& S = c->dscene;
-    S.prims = to_global(upload(c->scene_mem, "prims", cs.prims, c->st));
-    S.geom = to_global(upload(c->scene_mem, "geom", cs.geom, c->st));
-    S.mats = to_global(upload(c->scene_mem, "mats", cs.mats, c->st));
-    S.srcs = to_global(upload(c->scene_mem, "srcs", cs.srcs, c->st));
     S.nprims = (int)cs.prims.size();
     S.nmats = (int)cs.mats.size();
     S.nsrcs = (int)cs.srcs.size();
     S.lamb = cs.lamb;
     S.cusize = cs.cusize;
-    S.scache = to_global(upload(c->scene_mem, "scache", cs.scache, c->st));
-    S.shadow_obs = to_global((unsigned long long*)c->scene_mem.get(
-        "shadow_obs", sizeof(unsigned long long) * std::max<int32_t>(cs.ncells, 1), c->st, true));
-    S.src_tested = (RG_GLOBAL uint32_t*)c->scene_mem.get("src_tested", sizeof(uint32_t) * std::max<size_t>(cs.srcs.size(), 1),
-                                               c->st, true);
     S.shadow_on = 0;
-    upload(c->scene_mem, "cast_rays", cs.cast_rays, c->st);
-    upload(c->scene_mem, "cast_src", cs.cast_src, c->st);
     c->shadow_filled = false;
     c->cast_count.assign(cs.srcs.size(), 0);
     c->cast_counted.assign(cs.srcs.size(), 0);
@@ -226,39 +270,53 @@ int rg_scene_commit(rg_ctx* c) {
         slo[a] = std::min(slo[a], cs.lo[3 * k + a]);
         shi[a] = std::max(shi[a], cs.hi[3 * k + a]);
       }
-    uint32_t n = (uint32_t)small_ids.size();
-    float* dlo = upload(c->scene_mem, "blo", slo_v, c->st);
-    float* dhi = upload(c->scene_mem, "bhi", shi_v, c->st);
-    int32_t* dids = upload(c->scene_mem, "bids", small_ids, c->st);
-    S.big = to_global(upload(c->scene_mem, "big", big, c->st));
+    const uint32_t n = (uint32_t)small_ids.size();
+    const bool sah = n && (c->bvh_builder == RG_BVH_SAH || (c->bvh_builder == RG_BVH_AUTO && n <= kSahMaxPrims));
+    // host SAH tree (leaf refs hold primitive indices) and its four-wide
+    // form for the persistent walk, kept when every path's pushes (at most
+    // three per four-wide node) fit the traversal stack
+    std::vector<BVHNode> hn;
+    std::vector<BVH4Node> h4;
+    int root = -1;
+    bool wide = false;
+    if (sah) {
+      root = build_sah_bvh(slo_v.data(), shi_v.data(), small_ids.data(), n, hn, kMaxTreeDepth);
+      const int d4 = collapse_bvh4(hn, root, h4);
+      wide = root == 0 && !h4.empty() && 3 * d4 <= kTraceStackEntries;   // both walks start at node 0
+    }
+    // every table in one device block, sent in one copy
+    Pack pk;
+    const size_t o_prims = pk.add(cs.prims), o_geom = pk.add(cs.geom), o_mats = pk.add(cs.mats),
+                 o_srcs = pk.add(cs.srcs), o_scache = pk.add(cs.scache),
+                 o_obs = pk.zero(sizeof(unsigned long long) * std::max<int32_t>(cs.ncells, 1)),
+                 o_tested = pk.zero(sizeof(uint32_t) * std::max<size_t>(cs.srcs.size(), 1)),
+                 o_cast = pk.add(cs.cast_rays), o_cast_src = pk.add(cs.cast_src), o_big = pk.add(big),
+                 o_nodes = pk.add(hn.data(), sizeof(BVHNode) * hn.size(), sizeof(BVHNode) * std::max<uint32_t>(n, 1)),
+                 o_leaf = pk.add(sah ? small_ids.data() : nullptr, sah ? sizeof(int32_t) * n : 0,
+                                 sizeof(int32_t) * std::max<uint32_t>(n, 1)),
+                 o_nodes4 = wide ? pk.add(h4) : 0, o_blo = sah ? 0 : pk.add(slo_v), o_bhi = sah ? 0 : pk.add(shi_v),
+                 o_bids = sah ? 0 : pk.add(small_ids);
+    char* blob = (char*)c->scene_mem.get("blob", pk.total, c->st, false);
+    pk.send(c, blob);
+    S.prims = to_global((const Prim*)(blob + o_prims));
+    S.geom = to_global((const double*)(blob + o_geom));
+    S.mats = to_global((const Material*)(blob + o_mats));
+    S.srcs = to_global((const Source*)(blob + o_srcs));
+    S.scache = to_global((const SrcCache*)(blob + o_scache));
+    S.shadow_obs = to_global((unsigned long long*)(blob + o_obs));
+    S.src_tested = (RG_GLOBAL uint32_t*)(blob + o_tested);
+    c->d_cast_rays = (const double*)(blob + o_cast);
+    c->d_cast_src = (const int32_t*)(blob + o_cast_src);
+    S.big = to_global((const int32_t*)(blob + o_big));
     S.nbig = (int)big.size();
     S.nbvh = (int)n;
-    S.nodes4 = nullptr;
-    BVHNode* nodes = (BVHNode*)c->scene_mem.get("nodes", sizeof(BVHNode) * std::max<uint32_t>(n, 1), c->st, true);
-    int32_t* leaf = (int32_t*)c->scene_mem.get("leaf", sizeof(int32_t) * std::max<uint32_t>(n, 1), c->st, false);
-    bool sah = c->bvh_builder == RG_BVH_SAH || (c->bvh_builder == RG_BVH_AUTO && n <= kSahMaxPrims);
-    if (n == 0) {
-      S.root = -1;
-    } else if (sah) {
-      // host SAH tree: leaf refs already hold primitive indices
-      std::vector<BVHNode> hn;
-      S.root = build_sah_bvh(slo_v.data(), shi_v.data(), small_ids.data(), n, hn, kMaxTreeDepth);
-      if (!hn.empty())
-        hipck(hipMemcpyAsync(nodes, hn.data(), sizeof(BVHNode) * hn.size(), hipMemcpyHostToDevice, c->st), "nodes");
-      hipck(hipMemcpyAsync(leaf, small_ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st), "leaves");
-      // four-wide form for the persistent walk, when every path's pushes
-      // (at most three per four-wide node) fit the traversal stack
-      std::vector<BVH4Node> h4;
-      const int d4 = collapse_bvh4(hn, S.root, h4);
-      if (S.root == 0 && !h4.empty() && 3 * d4 <= kTraceStackEntries) {   // both walks start at node 0
-        BVH4Node* n4 = (BVH4Node*)c->scene_mem.get("nodes4", sizeof(BVH4Node) * h4.size(), c->st, false);
-        hipck(hipMemcpyAsync(n4, h4.data(), sizeof(BVH4Node) * h4.size(), hipMemcpyHostToDevice, c->st), "nodes4");
-        S.nodes4 = to_global(n4);
-      }
-      hipck(hipStreamSynchronize(c->st), "nodes");
-    } else {
-      S.root = build_lbvh(dlo, dhi, dids, n, slo, shi, nodes, leaf, c->arena, c->st);
-    }
+    S.nodes4 = wide ? to_global((const BVH4Node*)(blob + o_nodes4)) : nullptr;
+    BVHNode* nodes = (BVHNode*)(blob + o_nodes);
+    int32_t* leaf = (int32_t*)(blob + o_leaf);
+    if (n == 0) S.root = -1;
+    else if (sah) S.root = root;
+    else S.root = build_lbvh((float*)(blob + o_blo), (float*)(blob + o_bhi), (int32_t*)(blob + o_bids), n, slo, shi,
+                             nodes, leaf, c->arena, c->st);
     S.nodes = to_global(nodes);
     S.leaf_prim = to_global(leaf);
     hipck(hipStreamSynchronize(c->st), "scene commit");
@@ -311,39 +369,9 @@ __global__ void k_scale_out(double* v, size_t n, double s) {
   if (i < n) v[i] *= s;
 }
 
-// Fills the context's shadow cache once per commit (rt/srcobstr.c:63-183).
-static void shadow_fill(rg_ctx* c) {
-  if (c->shadow_filled) return;
-  const CompiledScene& cs = c->cs;
-  const uint32_t n = (uint32_t)cs.cast_src.size();
-  unsigned long long* d_cnt =
-      (unsigned long long*)c->arena.get("cast_count", sizeof(unsigned long long) * std::max<size_t>(cs.srcs.size(), 1),
-                                        c->st, true);
-  uint32_t* d_err = (uint32_t*)c->arena.get("cast_err", 8, c->st, true);
-  shadow_cast(c->dscene, (const double*)c->scene_mem.get("cast_rays", sizeof(double) * std::max<size_t>(cs.cast_rays.size(), 1), c->st, false),
-              (const int32_t*)c->scene_mem.get("cast_src", sizeof(int32_t) * std::max<size_t>(n, 1), c->st, false), n,
-              (unsigned long long*)c->dscene.shadow_obs, d_cnt, d_err, c->st);
-  uint32_t herr = 0;
-  if (!cs.srcs.empty())
-    hipck(hipMemcpyAsync(c->cast_count.data(), d_cnt, sizeof(unsigned long long) * cs.srcs.size(), hipMemcpyDeviceToHost,
-                         c->st), "cast counts");
-  hipck(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c->st), "cast err");
-  hipck(hipStreamSynchronize(c->st), "shadow cache");
-  if (herr & ERR_STACK) throw std::runtime_error("BVH traversal stack overflow");
-  c->shadow_filled = true;
-}
-
-// Before a call: the cache in use (or not) and no source tested yet.
-static void shadow_begin(rg_ctx* c, DevScene& S, bool on) {
-  S.shadow_on = 0;
-  if (!on || c->cs.scache.empty()) return;
-  shadow_fill(c);
-  S.shadow_on = 1;
-  hipck(hipMemsetAsync(S.src_tested, 0, sizeof(uint32_t) * c->cs.srcs.size(), c->st), "tested flags");
-}
-
 // The cast rays of every source of c flagged in `tested` for the first
-// time since the commit, as the reference's counter counts them.
+// time since the commit, as the reference's counter counts them
+// (the cache is filled by rt/srcobstr.c:63-183 at a source's first test).
 static unsigned long long shadow_count(rg_ctx* c, const uint32_t* tested) {
   unsigned long long add = 0;
   for (size_t k = 0; k < c->cs.srcs.size(); ++k)
@@ -352,16 +380,6 @@ static unsigned long long shadow_count(rg_ctx* c, const uint32_t* tested) {
       add += c->cast_count[k];
     }
   return add;
-}
-
-// After a call: shadow_count of the flags the call left.
-static unsigned long long shadow_end(rg_ctx* c, const DevScene& S) {
-  if (!S.shadow_on) return 0;
-  const size_t ns = c->cs.srcs.size();
-  std::vector<uint32_t> tested(ns);
-  hipck(hipMemcpyAsync(tested.data(), S.src_tested, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost, c->st), "tested");
-  hipck(hipStreamSynchronize(c->st), "tested");
-  return shadow_count(c, tested.data());
 }
 
 // device_io: `rays` and `out` are device memory of the context's device
@@ -379,68 +397,88 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   long long nrec = nrays / accum;
   size_t outn = (size_t)nrec * ncols;
   double* d_out = (double*)c->arena.get("out", sizeof(double) * std::max<size_t>(outn, 1), c->st, true);
-  double* d_rays = (double*)c->arena.get("rays", sizeof(double) * 6 * std::max<long long>(nrays, 1), c->st, false);
-  unsigned long long* d_nr = (unsigned long long*)c->arena.get("nrays", 8, c->st, true);
-  uint32_t* d_err = (uint32_t*)c->arena.get("err", 8, c->st, true);
-  BinSpec* d_bins = nullptr;
-  if (!bins.empty()) {
-    d_bins = (BinSpec*)c->arena.get("bins", sizeof(BinSpec) * bins.size(), c->st, false);
-    hipck(hipMemcpyAsync(d_bins, bins.data(), sizeof(BinSpec) * bins.size(), hipMemcpyHostToDevice, c->st), "bins");
+  // the scenes of this call (a batch's, or the context's own as a copy
+  // that carries the call's shadow-cache fields)
+  DevScene Sd = c->dscene;
+  std::vector<rg_ctx*> sc_ctx;
+  std::vector<DevScene*> sc_dev;
+  if (batch) {
+    for (size_t i = 0; i < batch->size(); ++i) {
+      sc_ctx.push_back((*batch_ctx)[i]);
+      sc_dev.push_back(&(*batch)[i]);
+    }
+  } else {
+    sc_ctx.push_back(c);
+    sc_dev.push_back(&Sd);
   }
-  if (nrays && rays)      // rays == nullptr: already generated in the "rays" buffer on the device
-    hipck(hipMemcpyAsync(d_rays, rays, sizeof(double) * 6 * nrays,
-                         device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st), "rays");
+  // The call block, cleared once and read back once: rays traced (u64),
+  // error flags (u32), then per scene with a shadow cache the closest-hit
+  // queries of its cast rays per source (u64, when this call fills the
+  // cache: the first call after a commit, as the reference fills a source's
+  // cache at its first shadow test) and its tested-source flags (u32).
+  const bool shadow = c->tuning.shadow_cache != 0;
+  const size_t nsc = sc_ctx.size();
+  std::vector<size_t> cast_at(nsc, 0), tested_at(nsc, 0);
+  std::vector<char> fill(nsc, 0);
+  size_t blk = 16;
+  for (size_t i = 0; i < nsc; ++i) {
+    rg_ctx* bc = sc_ctx[i];
+    DevScene& S = *sc_dev[i];
+    S.shadow_on = 0;
+    if (!shadow || bc->cs.scache.empty()) continue;
+    S.shadow_on = 1;
+    const size_t ns = bc->cs.srcs.size();
+    fill[i] = !bc->shadow_filled;
+    cast_at[i] = blk;
+    blk += 8 * ns;
+    tested_at[i] = blk;
+    blk += (4 * ns + 7) & ~(size_t)7;
+  }
+  char* d_blk = (char*)c->arena.get("call", blk, c->st, true);
+  for (size_t i = 0; i < nsc; ++i)
+    if (sc_dev[i]->shadow_on) sc_dev[i]->src_tested = (RG_GLOBAL uint32_t*)(d_blk + tested_at[i]);
+  // inputs in one copy: bin specs, a batch's scene tables, the rays
+  Pack in;
+  const size_t o_bins = bins.empty() ? 0 : in.add(bins);
+  const size_t o_sc = batch ? in.add(*batch) : 0, o_ss = batch ? in.add(*batch_start) : 0;
+  const bool host_rays = nrays && rays && !device_io;
+  const size_t o_rays = host_rays ? in.add(rays, sizeof(double) * 6 * nrays, sizeof(double) * 6 * nrays) : 0;
+  // results come back through the staging buffer when small enough to
+  // stage (large records go straight to the caller's memory)
+  const bool stage_out = outn && out && !device_io && outn * sizeof(double) <= ((size_t)64 << 20);
+  const size_t o_hout = (blk + 255) & ~(size_t)255;
+  stage_buf(c, std::max(in.total, o_hout + (stage_out ? sizeof(double) * outn : 0)));
+  char* d_in = nullptr;
+  if (in.total) {
+    d_in = (char*)c->arena.get("call_in", in.total, c->st, false);
+    in.send(c, d_in);
+  }
+  double* d_rays = host_rays ? (double*)(d_in + o_rays)
+                             : (double*)c->arena.get("rays", sizeof(double) * 6 * std::max<long long>(nrays, 1), c->st,
+                                                      false);
+  if (nrays && rays && device_io)
+    hipck(hipMemcpyAsync(d_rays, rays, sizeof(double) * 6 * nrays, hipMemcpyDeviceToDevice, c->st), "rays");
   RunCfg cfg;
   cfg.p = c->params;
   cfg.mode = mode;
   cfg.irrad = (flags & 16) ? 1 : 0;
   cfg.ncols = ncols;
   cfg.nbinspec = (int)bins.size();
-  cfg.bins = d_bins;
+  cfg.bins = bins.empty() ? nullptr : (const BinSpec*)(d_in + o_bins);
   cfg.out = d_out;
-  cfg.nrays = d_nr;
-  cfg.err = d_err;
+  cfg.nrays = (unsigned long long*)d_blk;
+  cfg.err = (uint32_t*)(d_blk + 8);
   cfg.accum = accum;
   cfg.child_order = c->tuning.child_order;
-  cfg.nscenes = 0;
-  cfg.scenes = nullptr;
-  cfg.scene_start = nullptr;
-  // shadow caches of the scenes traced (filled at the first call)
-  const bool shadow = c->tuning.shadow_cache != 0;
-  // a batch's tested-source flags: one buffer on the launching stream,
-  // cleared and read back once for all its scenes
-  uint32_t* d_tested = nullptr;
-  std::vector<size_t> tested_at;
-  if (batch) {
-    size_t nt = 0;
-    for (size_t i = 0; i < batch->size(); ++i) {
-      rg_ctx* bc = (*batch_ctx)[i];
-      DevScene& S = (*batch)[i];
-      S.shadow_on = 0;
-      tested_at.push_back(nt);
-      if (!shadow || bc->cs.scache.empty()) continue;
-      shadow_fill(bc);     // first call after a commit only (synchronises its stream)
-      S.shadow_on = 1;
-      nt += bc->cs.srcs.size();
-    }
-    if (nt) {
-      d_tested = (uint32_t*)c->arena.get("batch_tested", sizeof(uint32_t) * nt, c->st, true);
-      for (size_t i = 0; i < batch->size(); ++i)
-        if ((*batch)[i].shadow_on) (*batch)[i].src_tested = (RG_GLOBAL uint32_t*)(d_tested + tested_at[i]);
-    }
-    tested_at.push_back(nt);
-  } else {
-    shadow_begin(c, c->dscene, shadow);
-  }
-  if (batch) {
-    DevScene* d_sc = (DevScene*)c->arena.get("batch_scenes", sizeof(DevScene) * batch->size(), c->st, false);
-    uint32_t* d_ss = (uint32_t*)c->arena.get("batch_start", sizeof(uint32_t) * batch_start->size(), c->st, false);
-    hipck(hipMemcpyAsync(d_sc, batch->data(), sizeof(DevScene) * batch->size(), hipMemcpyHostToDevice, c->st), "scenes");
-    hipck(hipMemcpyAsync(d_ss, batch_start->data(), sizeof(uint32_t) * batch_start->size(), hipMemcpyHostToDevice,
-                         c->st), "scene starts");
-    cfg.nscenes = (int)batch->size();
-    cfg.scenes = d_sc;
-    cfg.scene_start = d_ss;
+  cfg.nscenes = batch ? (int)batch->size() : 0;
+  cfg.scenes = batch ? (const DevScene*)(d_in + o_sc) : nullptr;
+  cfg.scene_start = batch ? (const uint32_t*)(d_in + o_ss) : nullptr;
+  // shadow caches filled by this call: their cast rays go first on the stream
+  for (size_t i = 0; i < nsc; ++i) {
+    if (!fill[i]) continue;
+    rg_ctx* bc = sc_ctx[i];
+    shadow_cast(*sc_dev[i], bc->d_cast_rays, bc->d_cast_src, (uint32_t)bc->cs.cast_src.size(),
+                (unsigned long long*)sc_dev[i]->shadow_obs, (unsigned long long*)(d_blk + cast_at[i]), cfg.err, c->st);
   }
   cfg.adj = 0;
   cfg.adj_w = nullptr;
@@ -485,18 +523,20 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
       if (!b.nodes4) tn.wide = 0;
   if (nrays) {
     std::lock_guard<std::mutex> lock(g_render_mu);
-    render(c->dscene, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, tn, render_arena(c->device), c->st,
+    render(Sd, cfg, d_rays, (uint32_t)nrays, (flags & 8) != 0, max_srcs, tn, render_arena(c->device), c->st,
            &c->last_stats);
   }
   if (d_vis) hipck(hipMemcpyAsync(c->last_visits, d_vis, 16, hipMemcpyDeviceToHost, c->st), "visits");
   else c->last_visits[0] = c->last_visits[1] = 0;
-  uint32_t herr = 0;
-  hipck(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c->st), "err");
-  hipck(hipMemcpyAsync(&c->last_nrays, d_nr, 8, hipMemcpyDeviceToHost, c->st), "nrays");
+  // one read-back of the call block (and of staged records)
+  char* h = c->stage;
+  hipck(hipMemcpyAsync(h, d_blk, blk, hipMemcpyDeviceToHost, c->st), "call block");
   if (outn && out && device_io) {
     if (accum > 1)
       k_scale_out<<<(unsigned)((outn + 255) / 256), 256, 0, c->st>>>(d_out, outn, 1.0 / accum);
     hipck(hipMemcpyAsync(out, d_out, sizeof(double) * outn, hipMemcpyDeviceToDevice, c->st), "out");
+  } else if (stage_out) {
+    hipck(hipMemcpyAsync(h + o_hout, d_out, sizeof(double) * outn, hipMemcpyDeviceToHost, c->st), "out");
   } else if (outn && out) {
     hipck(hipMemcpyAsync(out, d_out, sizeof(double) * outn, hipMemcpyDeviceToHost, c->st), "out");
   }
@@ -504,16 +544,18 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
     hipck(hipMemcpyAsync(adj->grad, d_grad, sizeof(double) * ADJ_COLS * adj->nrows, hipMemcpyDeviceToHost, c->st),
           "grad");
   hipck(hipStreamSynchronize(c->st), "render");
-  if (batch) {
-    if (d_tested) {
-      std::vector<uint32_t> tested(tested_at.back());
-      hipck(hipMemcpy(tested.data(), d_tested, sizeof(uint32_t) * tested.size(), hipMemcpyDeviceToHost), "tested");
-      for (size_t i = 0; i < batch->size(); ++i)
-        if ((*batch)[i].shadow_on) c->last_nrays += shadow_count((*batch_ctx)[i], tested.data() + tested_at[i]);
+  uint32_t herr = 0;
+  memcpy(&c->last_nrays, h, 8);
+  memcpy(&herr, h + 8, 4);
+  for (size_t i = 0; i < nsc; ++i) {
+    rg_ctx* bc = sc_ctx[i];
+    if (fill[i]) {
+      memcpy(bc->cast_count.data(), h + cast_at[i], sizeof(unsigned long long) * bc->cs.srcs.size());
+      bc->shadow_filled = !(herr & ERR_STACK);
     }
-  } else {
-    c->last_nrays += shadow_end(c, c->dscene);
+    if (sc_dev[i]->shadow_on) c->last_nrays += shadow_count(bc, (const uint32_t*)(h + tested_at[i]));
   }
+  if (stage_out) memcpy(out, h + o_hout, sizeof(double) * outn);
   if (herr & ERR_STACK) throw std::runtime_error("BVH traversal stack overflow");
   if (herr & ERR_CAPACITY) throw std::runtime_error("spawn buffer overflow");
   if (herr & ERR_NOMAT) throw std::runtime_error("surface without material");
@@ -760,7 +802,8 @@ int rg_contrib_batch(rg_ctx* const* ctxs, int nctx, const double* rays, const lo
     return -1;
   }
   RG_TRY(c0, {
-    if (nctx > 255) throw std::runtime_error("at most 255 scenes per batch");
+    if (nctx > kMaxBatchScenes)
+      throw std::runtime_error("at most " + std::to_string(kMaxBatchScenes) + " scenes per batch");
     std::unordered_map<std::string, int> slot;
     int ncols = 0;
     std::vector<BinSpec> bins = bin_specs(mods, nmods, slot, ncols);
@@ -788,7 +831,8 @@ int rg_trace_batch(rg_ctx* const* ctxs, int nctx, const double* rays, const long
     return -1;
   }
   RG_TRY(c0, {
-    if (nctx > 255) throw std::runtime_error("at most 255 scenes per batch");
+    if (nctx > kMaxBatchScenes)
+      throw std::runtime_error("at most " + std::to_string(kMaxBatchScenes) + " scenes per batch");
     std::vector<DevScene> scenes;
     std::vector<uint32_t> start(1, 0);
     long long total = 0;

@@ -280,7 +280,7 @@ __device__ __forceinline__ Spawn make_child(const RayS& r, V3 org, uint16_t rt, 
   else { s.rlvl = (uint8_t)r.rlvl; s.rsrc = (int16_t)r.rsrc; }
   s.adepth = (uint8_t)r.adepth;
   s.flags = flags;
-  s.scene = (uint8_t)r.scene;
+  s.scene = (uint16_t)r.scene;
   s.key = child_key(r.key, slot);
   return s;
 }
@@ -1258,7 +1258,7 @@ __global__ void __launch_bounds__(BLOCK) k_primary_seed(RunCfg c, const double* 
   s.adepth = 0;
   s.flags = SPF_FORCE;
   const int scene = scene_of_ray(c, s0 + i);
-  s.scene = (uint8_t)scene;
+  s.scene = (uint16_t)scene;
   s.key = sensor_key(c.p.seed + scene, s0 + i - (c.nscenes ? c.scene_start[scene] : 0));
   out[i] = s;
   counts[i] = 1u;
@@ -1739,9 +1739,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RG_S
 constexpr int SORT_BLOCK = 256;
 constexpr int SORT_ITEMS = 4;
 
+constexpr int NSC = kMaxBatchScenes;        // histogram bins (scenes)
+constexpr int SC_PER_T = NSC / SORT_BLOCK;   // bins per thread
+
 __global__ void __launch_bounds__(SORT_BLOCK) k_scene_hist(const Spawn* in, uint32_t n, uint32_t* hist) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  __shared__ uint32_t h[NSC];
+  for (int q = 0; q < SC_PER_T; ++q) h[threadIdx.x + q * SORT_BLOCK] = 0;
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * SORT_BLOCK * SORT_ITEMS + threadIdx.x;
   for (int k = 0; k < SORT_ITEMS; ++k) {
@@ -1749,23 +1752,32 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_scene_hist(const Spawn* in, uint
     if (i < n) atomicAdd(&h[in[i].scene], 1u);
   }
   __syncthreads();
-  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+  for (int q = 0; q < SC_PER_T; ++q) {
+    const int b = threadIdx.x + q * SORT_BLOCK;
+    if (h[b]) atomicAdd(&hist[b], h[b]);
+  }
 }
 
-// exclusive scan of the 256 scene counts (one block)
-__global__ void __launch_bounds__(256) k_scene_start(const uint32_t* hist, uint32_t* start) {
-  __shared__ uint32_t sh[256];
+// exclusive scan of the NSC scene counts (one block of 1024, 4 each)
+__global__ void __launch_bounds__(1024) k_scene_start(const uint32_t* hist, uint32_t* start) {
+  static_assert(NSC == 4 * 1024, "scan shape");
+  __shared__ uint32_t sh[1024];
   const int t = threadIdx.x;
-  uint32_t v = hist[t];
-  sh[t] = v;
+  const uint32_t v0 = hist[4 * t], v1 = hist[4 * t + 1], v2 = hist[4 * t + 2], v3 = hist[4 * t + 3];
+  const uint32_t tot = v0 + v1 + v2 + v3;
+  sh[t] = tot;
   __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
+  for (int off = 1; off < 1024; off <<= 1) {
     uint32_t a = t >= off ? sh[t - off] : 0;
     __syncthreads();
     sh[t] += a;
     __syncthreads();
   }
-  start[t] = sh[t] - v;
+  const uint32_t ex = sh[t] - tot;
+  start[4 * t] = ex;
+  start[4 * t + 1] = ex + v0;
+  start[4 * t + 2] = ex + v0 + v1;
+  start[4 * t + 3] = ex + v0 + v1 + v2;
 }
 
 // Each block reserves one range per scene it holds, then places its records
@@ -1773,9 +1785,9 @@ __global__ void __launch_bounds__(256) k_scene_start(const uint32_t* hist, uint3
 __global__ void __launch_bounds__(SORT_BLOCK) k_scene_scatter(const Spawn* in, const PathRec* pin, uint32_t n,
                                                               const uint32_t* start, uint32_t* fill, Spawn* out,
                                                               PathRec* pout) {
-  __shared__ uint32_t h[256];
-  __shared__ uint32_t b[256];
-  h[threadIdx.x] = 0;
+  __shared__ uint32_t h[NSC];
+  __shared__ uint32_t b[NSC];
+  for (int q = 0; q < SC_PER_T; ++q) h[threadIdx.x + q * SORT_BLOCK] = 0;
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * SORT_BLOCK * SORT_ITEMS + threadIdx.x;
   uint32_t sc[SORT_ITEMS], rank[SORT_ITEMS];
@@ -1785,7 +1797,10 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_scene_scatter(const Spawn* in, c
     rank[k] = i < n ? atomicAdd(&h[sc[k]], 1u) : 0u;
   }
   __syncthreads();
-  if (h[threadIdx.x]) b[threadIdx.x] = start[threadIdx.x] + atomicAdd(&fill[threadIdx.x], h[threadIdx.x]);
+  for (int q = 0; q < SC_PER_T; ++q) {
+    const int c = threadIdx.x + q * SORT_BLOCK;
+    if (h[c]) b[c] = start[c] + atomicAdd(&fill[c], h[c]);
+  }
   __syncthreads();
   for (int k = 0; k < SORT_ITEMS; ++k) {
     size_t i = base + (size_t)k * SORT_BLOCK;
@@ -2180,12 +2195,12 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
           Spawn* sb = (Spawn*)arena.get("spawn_sorted" + std::to_string(s), sizeof(Spawn) * cap, st, false);
           PathRec* spb = adj ? (PathRec*)arena.get("path_sorted" + std::to_string(s), sizeof(PathRec) * cap, st, false)
                              : nullptr;
-          uint32_t* hist = (uint32_t*)arena.get("scene_hist", sizeof(uint32_t) * 768, st, true);
+          uint32_t* hist = (uint32_t*)arena.get("scene_hist", sizeof(uint32_t) * 3 * NSC, st, true);
           const uint32_t nb = (nout + SORT_BLOCK * SORT_ITEMS - 1) / (SORT_BLOCK * SORT_ITEMS);
           hipLaunchKernelGGL(k_scene_hist, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, nout, hist);
-          hipLaunchKernelGGL(k_scene_start, dim3(1), dim3(256), 0, st, hist, hist + 256);
-          hipLaunchKernelGGL(k_scene_scatter, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, opb, nout, hist + 256, hist + 512,
-                             sb, spb);
+          hipLaunchKernelGGL(k_scene_start, dim3(1), dim3(1024), 0, st, hist, hist + NSC);
+          hipLaunchKernelGGL(k_scene_scatter, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, opb, nout, hist + NSC,
+                             hist + 2 * NSC, sb, spb);
           check(hipGetLastError(), "scene sort launch");
           open_frame(Batch{sb, spb, nout, s}, true);
         } else {

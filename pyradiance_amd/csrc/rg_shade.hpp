@@ -26,11 +26,12 @@ struct alignas(16) Spawn {
   int16_t rsrc;      // child's aimed source
   uint8_t rlvl;      // child's reflection level
   uint8_t adepth;    // child's ambient recursion depth
-  uint8_t flags;     // SPF_*
-  uint8_t scene;     // scene of a batched launch (0 otherwise)
+  uint16_t flags : 3;    // SPF_*
+  uint16_t scene : 13;   // scene of a batched launch (0 otherwise; at most kMaxBatchScenes)
   uint64_t key;      // explicit: child key; hemisphere: parent key
 };
 static_assert(sizeof(Spawn) == 96, "Spawn layout");
+constexpr int kMaxBatchScenes = 4096;   // scenes of one batched launch
 
 enum : uint8_t { SPF_RR = 1, SPF_FORCE = 2, SPF_BACK = 4 };
 
