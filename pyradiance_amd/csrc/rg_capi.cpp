@@ -369,6 +369,27 @@ __global__ void k_scale_out(double* v, size_t n, double s) {
   if (i < n) v[i] *= s;
 }
 
+// Fixed-point bin sums (int64 multiples of 1 / fix) to doubles, in place.
+__global__ void k_fixed_out(double* v, size_t n, double inv_fix) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const long long q = ((const long long*)v)[i];
+    v[i] = (double)q * inv_fix;
+  }
+}
+
+// Scale of the fixed-point bin sums of a coefficient-mode call: a ray's
+// coefficients over all its bins add up to at most 16 (a Lambertian
+// irradiance record reaches pi, and every material below it multiplies by
+// at most 1), so a record of `accum` rays stays below 16 accum, and the
+// scale 2^(58 - ceil(log2 accum)) keeps that below 2^62 while resolving
+// 2^-58 of a single-ray record.
+static double fixed_scale(int accum) {
+  int e = 0;
+  while ((1LL << e) < accum) ++e;
+  return ldexp(1.0, 58 - e);
+}
+
 // The cast rays of every source of c flagged in `tested` for the first
 // time since the commit, as the reference's counter counts them
 // (the cache is filled by rt/srcobstr.c:63-183 at a source's first test).
@@ -466,6 +487,9 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   cfg.nbinspec = (int)bins.size();
   cfg.bins = bins.empty() ? nullptr : (const BinSpec*)(d_in + o_bins);
   cfg.out = d_out;
+  // rcontrib coefficients (not -V values, nor rtrace radiances, whose range
+  // the host cannot bound) accumulate in fixed point
+  cfg.fix = (mode == 1 && !c->params.contrib && !adj) ? fixed_scale(accum) : 0.0;
   cfg.nrays = (unsigned long long*)d_blk;
   cfg.err = (uint32_t*)(d_blk + 8);
   cfg.accum = accum;
@@ -528,6 +552,8 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   }
   if (d_vis) hipck(hipMemcpyAsync(c->last_visits, d_vis, 16, hipMemcpyDeviceToHost, c->st), "visits");
   else c->last_visits[0] = c->last_visits[1] = 0;
+  if (cfg.fix > 0.0 && outn)
+    k_fixed_out<<<(unsigned)((outn + 255) / 256), 256, 0, c->st>>>(d_out, outn, 1.0 / cfg.fix);
   // one read-back of the call block (and of staged records)
   char* h = c->stage;
   hipck(hipMemcpyAsync(h, d_blk, blk, hipMemcpyDeviceToHost, c->st), "call block");
@@ -559,6 +585,8 @@ static void run(rg_ctx* c, const double* rays, long long nrays, int accum, int f
   if (herr & ERR_STACK) throw std::runtime_error("BVH traversal stack overflow");
   if (herr & ERR_CAPACITY) throw std::runtime_error("spawn buffer overflow");
   if (herr & ERR_NOMAT) throw std::runtime_error("surface without material");
+  if (herr & ERR_FIXED)
+    throw std::runtime_error("a contribution coefficient beyond the fixed-point range (coefficients above 16 per ray)");
   if (herr & ERR_PATH)
     throw std::runtime_error("adjoint: the differentiated vertices of one path need more than " +
                              std::to_string(PATH_WORDS) + " record words");

@@ -64,8 +64,23 @@ __device__ __forceinline__ void emit(const Sink& sk, const Spawn& s, const RayS&
 
 __device__ __forceinline__ void emit(const Sink& sk, const Spawn& s, const RayS& r) { emit(sk, s, r, path_none()); }
 
+// Bin accumulation.  With c.fix > 0 each channel is rounded to a multiple
+// of 1 / c.fix and added as a 64-bit integer: integer sums do not depend on
+// the order the deposits arrive in, so a record's value is the same bit for
+// bit whatever the launch shape or scheduling.  A deposit beyond 2^60 / fix
+// flags ERR_FIXED (the range the host chose from the record's bound).
 __device__ __forceinline__ void deposit(const RunCfg& c, uint32_t sensor, int col, C3 v) {
   double* o = c.out + (size_t)sensor * c.ncols + col;
+  if (c.fix > 0.0) {
+    unsigned long long* q = (unsigned long long*)o;
+    const double x[3] = {(double)v.r * c.fix, (double)v.g * c.fix, (double)v.b * c.fix};
+    for (int k = 0; k < 3; ++k) {
+      if (!(fabs(x[k]) < 0x1p60)) { atomicOr(c.err, ERR_FIXED); continue; }
+      const long long n = llrint(x[k]);
+      if (n) atomicAdd(q + k, (unsigned long long)n);
+    }
+    return;
+  }
   unsafeAtomicAdd(o + 0, (double)v.r);
   unsafeAtomicAdd(o + 1, (double)v.g);
   unsafeAtomicAdd(o + 2, (double)v.b);
@@ -2097,9 +2112,34 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
         Sink sk{b, d_cnt, (uint32_t)batch_cap, cfg.err, pb, bc};
         if (adj) hipLaunchKernelGGL(k_irrad_seed<true>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, rays0, ns, s0, sk);
         else hipLaunchKernelGGL(k_irrad_seed<false>, dim3(grid), dim3(BLOCK), 0, st, S, cfg, rays0, ns, s0, sk);
-        check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
-        check(hipStreamSynchronize(st), "seed");
-        open_frame({b, pb, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s}, false);
+        if (cfg.nscenes <= 1) {
+          // child counts scanned on the device: one read-back of the spawn
+          // count and the child total
+          uint64_t* offs = slot_offs(s, batch_cap);
+          uint64_t* sums =
+              (uint64_t*)arena.get("scan_dev_sums", sizeof(uint64_t) * (batch_cap / SCAN_TILE + 2), st, false);
+          hipLaunchKernelGGL(k_scan_tiles_dev, dim3(scan_grid), dim3(SCAN_BLOCK), 0, st, bc, offs, d_cnt,
+                             (uint32_t)batch_cap, sums);
+          hipLaunchKernelGGL(k_scan_sums_dev, dim3(1), dim3(SUMS_BLOCK), 0, st, d_cnt, (uint32_t)batch_cap, sums, offs,
+                             (uint64_t*)(d_cnt + 2));
+          hipLaunchKernelGGL(k_scan_add_dev, dim3(scan_grid), dim3(SCAN_BLOCK), 0, st, offs, d_cnt,
+                             (uint32_t)batch_cap, sums);
+          check(hipGetLastError(), "scan launch");
+          check(hipMemcpyAsync(h_cnt, d_cnt, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
+          check(hipStreamSynchronize(st), "seed");
+          const uint32_t nseed = std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap);
+          if (nseed) {
+            if (++gens > max_batches) throw std::runtime_error("ray tree did not terminate");
+            const uint64_t total = (uint64_t)h_cnt[2] | ((uint64_t)h_cnt[3] << 32);
+            frames.push_back({Batch{b, pb, nseed, s}, offs, total, 0});
+          } else {
+            free_slots.push_back(s);
+          }
+        } else {
+          check(hipMemcpyAsync(h_cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
+          check(hipStreamSynchronize(st), "seed");
+          open_frame({b, pb, std::min<uint32_t>(h_cnt[0], (uint32_t)batch_cap), s}, false);
+        }
       } else {
         hipLaunchKernelGGL(k_primary_seed, dim3(grid), dim3(BLOCK), 0, st, cfg, rays0, ns, s0, b, bc);
         if (pb) check(hipMemsetAsync(pb, 0, sizeof(PathRec) * ns, st), "memset");   // empty paths

@@ -139,6 +139,8 @@ struct RunCfg {
   const uint32_t* scene_start; // first input ray of each scene (nscenes+1)
   const BinSpec* bins; // per tracked-modifier slot
   double* out;         // [nsensor][ncols]
+  double fix;          // > 0: deposits add up as int64 multiples of 1 / fix in `out` (order-independent,
+                       // so results repeat bit for bit); 0: FP64 atomics
   unsigned long long* nrays;
   unsigned long long* visits;  // optional: [0] BVH nodes visited, [1] primitives tested
   uint32_t* err;       // error flags
@@ -152,7 +154,7 @@ struct RunCfg {
                                // transmitted specularity (the last two for trans only)
 };
 
-enum : uint32_t { ERR_STACK = 1, ERR_CAPACITY = 2, ERR_NOMAT = 4, ERR_GEN = 8, ERR_PATH = 16 };
+enum : uint32_t { ERR_STACK = 1, ERR_CAPACITY = 2, ERR_NOMAT = 4, ERR_GEN = 8, ERR_PATH = 16, ERR_FIXED = 32 };
 
 // ---- bins (util/reinhartb.cal:19-47, util/klems_full.cal:10-43,
 //      util/disk2square.cal:34-90, util/cieskyscan.cal:21-53) ------------

@@ -407,8 +407,30 @@ def test_batched_scenes_match_individual_runs(L):
         single = c.contrib(sens, mods, 1, L.RT_IMM_IRRAD)
         total += c.last_nrays()
         assert single.shape == res[k].shape
-        assert np.allclose(single, res[k], rtol=1e-9, atol=1e-13), k
+        # coefficient bins accumulate in fixed point: bit for bit
+        assert np.array_equal(single, res[k]), k
     assert nb == total
+
+
+def test_bins_repeat_bit_for_bit_across_launch_shapes(L):
+    """rcontrib coefficients add up in 64-bit fixed point, so the order in
+    which deposits arrive (launch shape, persistent refill, window size,
+    child order) leaves every record unchanged to the last bit."""
+    _say("bin accumulation: identical across launch shapes")
+    rad, sens = scenes.config2(0, nboxes=40)
+    sens = sens[::10]
+    c = _ctx(L, rad)
+    c.set_params(_rc_params(L, ambounce=2, ambdiv=512, minweight=1e-3, seed=5))
+    mods = _sky_mods(L)
+    outs = []
+    for shape in (dict(trace_kernel=1, refill=16, window=1 << 21), dict(trace_kernel=1, refill=48, window=1 << 12),
+                  dict(trace_kernel=0, window=1 << 14, child_order=0)):
+        c.set_tuning(**shape)
+        outs.append(c.contrib(sens, mods, 1, L.RT_IMM_IRRAD))
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+    assert outs[0].sum() > 0
+    c.close()
 
 
 def test_batched_rtrace_config1_sun_positions(L):
