@@ -352,7 +352,8 @@ class Context:
         self._check(self._L.rg_set_profiling(self._h, level))
 
     def set_tuning(self, **kw):
-        """Launch shape: trace_kernel (0/1), refill, window, blocks_per_cu."""
+        """Launch shape: trace_kernel (0/1), refill, window, blocks_per_cu,
+        wide, max_big, shadow_cache, child_order, inflight (see rg_set_tuning)."""
         for k, v in kw.items():
             self._check(self._L.rg_set_tuning(self._h, k.encode(), int(v)))
         return self

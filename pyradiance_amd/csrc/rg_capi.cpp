@@ -929,6 +929,9 @@ int rg_set_tuning(rg_ctx* c, const char* key, long long value) {
     } else if (k == "child_order") {
       if (value != 0 && value != 1) throw std::runtime_error("child_order is 0 or 1");
       t.child_order = (int)value;
+    } else if (k == "inflight") {
+      if (value != 1 && value != 2) throw std::runtime_error("inflight is 1 or 2");
+      t.inflight = (int)value;
     } else if (k == "shadow_cache") {
       if (value != 0 && value != 1) throw std::runtime_error("shadow_cache is 0 or 1");
       t.shadow_cache = (int)value;

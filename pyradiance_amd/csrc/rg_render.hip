@@ -2034,7 +2034,7 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   const uint32_t pf_blocks = (uint32_t)std::max(1, ncu) * std::max<uint32_t>(tn.blocks_per_cu, 1u);
   const long max_batches = 1L << 22;
   uint32_t* d_cnt = (uint32_t*)arena.get("ctr", 64, st, true);
-  uint32_t* h_cnt = arena.host_u32(8);
+  uint32_t* h_cnt = arena.host_u32(16);   // [0, 8): frame totals and seeds; [8, 16): windows in flight
 
   const bool adj = cfg.adj != 0;
   struct Batch { Spawn* buf; PathRec* pbuf; uint32_t n; int slot; };
@@ -2058,12 +2058,27 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   const uint32_t slot_children = (uint32_t)std::max<size_t>(64, kSlotBytes / (sizeof(Spawn) * per_child));
   const uint32_t win = std::min(window, slot_children);
   long gens = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-  if (stats && stats->timing) {
-    check(hipEventCreate(&ev0), "event");
-    check(hipEventCreate(&ev1), "event");
-    check(hipEventCreate(&ev2), "event");
+  // Events: two per window in flight (its read-back, and the trace / shade
+  // timing when asked for), reused in turn.
+  struct Events {
+    std::vector<hipEvent_t> ev;
+    ~Events() { for (hipEvent_t e : ev) (void)hipEventDestroy(e); }
+  } evs;
+  const bool timing = stats && stats->timing;
+  constexpr int kInFlight = 2;
+  for (int k = 0; k < kInFlight; ++k) {
+    hipEvent_t e;
+    check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    evs.ev.push_back(e);
+    if (timing)
+      for (int q = 0; q < 3; ++q) {
+        check(hipEventCreate(&e), "event");
+        evs.ev.push_back(e);
+      }
   }
+  const int ev_per = timing ? 4 : 1;
+  static_assert(8 + 4 * kInFlight <= 16, "host counters");
+  uint32_t* h_win = h_cnt + 8;   // per in-flight window: spawn count, work counter, child total (64 bits)
   // Depth first, one window at a time: after each window its children are
   // opened and walked before the next window of the same batch, so live
   // spawn buffers are bounded by the depth of the ray tree, not by the size
@@ -2146,116 +2161,136 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
         open_frame({b, pb, ns, s}, false);
       }
     }
-    while (!frames.empty()) {
-      if (frames.back().c0 >= frames.back().total) {
-        free_slots.push_back(frames.back().bt.slot);
-        frames.pop_back();
-        continue;
-      }
+    // With tn.inflight = 2 windows are pipelined: while the GPU runs one, the
+    // host waits for the counts of the one before and pushes its children's
+    // frame.  All work is on one stream, so windows run in the order they
+    // were queued and a slot freed on the host is only overwritten after the
+    // kernels reading it.  Batched launches regroup each window's spawns by
+    // scene before their frame opens, and run one window at a time.
+    struct Pending { Batch out; size_t cap; uint32_t nth; int pair; };
+    std::vector<Pending> inflight;
+    int next_pair = 0;
+    const size_t depth = cfg.nscenes <= 1 ? (size_t)std::max(1, std::min(tn.inflight, kInFlight)) : 1;
+    auto launch_window = [&]() {
       const Batch bt = frames.back().bt;
       const uint64_t* offs = frames.back().offs;
       const uint64_t c0 = frames.back().c0;
       const uint64_t total = frames.back().total;
-      {
-        uint32_t nth = (uint32_t)std::min<uint64_t>(win, total - c0);
-        frames.back().c0 += nth;
-        size_t cap = (size_t)nth * per_child;
-        int s = new_slot();
-        Spawn* ob = slot_buf(s, cap);
-        PathRec* opb = slot_path(s, cap);
-        uint32_t* ocnt = slot_counts(s, cap);
-        TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * nth, st, false);
-        check(hipMemsetAsync(d_cnt, 0, 2 * sizeof(uint32_t), st), "memset");   // spawn count, work counter
-        Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err, opb, ocnt};
-        if (stats && stats->timing) check(hipEventRecord(ev0, st), "event");
-        const uint32_t tblocks = (nth + TBLOCK - 1) / TBLOCK;
-        if (tn.trace_kernel == 1) {
-          const uint32_t nb = std::min(tblocks, pf_blocks);
-          if (cfg.nscenes && tn.wide)
-            hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                               bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
-          else if (cfg.nscenes)
-            hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                               bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
-          else if (tn.wide)
-            hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                               bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
-          else
-            hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                               bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
-        } else {
-          hipLaunchKernelGGL(k_trace, dim3(tblocks), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n, c0, nth, tout);
-        }
-        check(hipGetLastError(), "k_trace launch");
-        if (stats && stats->timing) check(hipEventRecord(ev1, st), "event");
-        if (adj)
-          hipLaunchKernelGGL(k_shade<true>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
-                             bt.pbuf, offs, c0, nth, tout, sk);
+      uint32_t nth = (uint32_t)std::min<uint64_t>(win, total - c0);
+      frames.back().c0 += nth;
+      size_t cap = (size_t)nth * per_child;
+      int s = new_slot();
+      Spawn* ob = slot_buf(s, cap);
+      PathRec* opb = slot_path(s, cap);
+      uint32_t* ocnt = slot_counts(s, cap);
+      uint64_t* ooffs = slot_offs(s, cap);
+      TraceOut* tout = (TraceOut*)arena.get("trace_out", sizeof(TraceOut) * nth, st, false);
+      const int pair = next_pair;
+      next_pair = (next_pair + 1) % kInFlight;
+      hipEvent_t* e = &evs.ev[pair * ev_per];
+      check(hipMemsetAsync(d_cnt, 0, 2 * sizeof(uint32_t), st), "memset");   // spawn count, work counter
+      Sink sk{ob, d_cnt, (uint32_t)cap, cfg.err, opb, ocnt};
+      if (timing) check(hipEventRecord(e[1], st), "event");
+      const uint32_t tblocks = (nth + TBLOCK - 1) / TBLOCK;
+      if (tn.trace_kernel == 1) {
+        const uint32_t nb = std::min(tblocks, pf_blocks);
+        if (cfg.nscenes && tn.wide)
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+        else if (cfg.nscenes)
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+        else if (tn.wide)
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
         else
-          hipLaunchKernelGGL(k_shade<false>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
-                             bt.pbuf, offs, c0, nth, tout, sk);
-        check(hipGetLastError(), "k_shade launch");
-        if (stats && stats->timing) check(hipEventRecord(ev2, st), "event");
-        if (cfg.nscenes <= 1) {
-          // the new batch's child counts scanned on the device right after
-          // shading: one read-back (spawn count and child total) per window
-          uint64_t* ooffs = slot_offs(s, cap);
-          uint64_t* sums = (uint64_t*)arena.get("scan_dev_sums", sizeof(uint64_t) * (cap / SCAN_TILE + 2), st, false);
-          hipLaunchKernelGGL(k_scan_tiles_dev, dim3(scan_grid), dim3(SCAN_BLOCK), 0, st, ocnt, ooffs, d_cnt,
-                             (uint32_t)cap, sums);
-          hipLaunchKernelGGL(k_scan_sums_dev, dim3(1), dim3(SUMS_BLOCK), 0, st, d_cnt, (uint32_t)cap, sums, ooffs,
-                             (uint64_t*)(d_cnt + 2));
-          hipLaunchKernelGGL(k_scan_add_dev, dim3(scan_grid), dim3(SCAN_BLOCK), 0, st, ooffs, d_cnt, (uint32_t)cap,
-                             sums);
-          check(hipGetLastError(), "scan launch");
-        }
-        check(hipMemcpyAsync(h_cnt, d_cnt, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
-        check(hipStreamSynchronize(st), "bounce");
-        if (stats && stats->timing) {
-          float ms = 0, ms2 = 0;
-          check(hipEventElapsedTime(&ms, ev0, ev1), "event time");
-          check(hipEventElapsedTime(&ms2, ev1, ev2), "event time");
-          stats->bounce_ms += ms;
-          stats->shade_ms += ms2;
-        }
-        if (stats) { stats->launches++; stats->threads += nth; }
-        uint32_t nout = std::min<uint32_t>(h_cnt[0], (uint32_t)cap);
-        if (cfg.nscenes <= 1) {
-          if (nout) {
-            if (++gens > max_batches) throw std::runtime_error("ray tree did not terminate");
-            const uint64_t total = (uint64_t)h_cnt[2] | ((uint64_t)h_cnt[3] << 32);
-            frames.push_back({Batch{ob, opb, nout, s}, slot_offs(s, cap), total, 0});
-          } else {
-            free_slots.push_back(s);
-          }
-          continue;
-        }
-        if (nout) {
-          // regroup by scene into the slot's second buffer, which the batch then uses
-          Spawn* sb = (Spawn*)arena.get("spawn_sorted" + std::to_string(s), sizeof(Spawn) * cap, st, false);
-          PathRec* spb = adj ? (PathRec*)arena.get("path_sorted" + std::to_string(s), sizeof(PathRec) * cap, st, false)
-                             : nullptr;
-          uint32_t* hist = (uint32_t*)arena.get("scene_hist", sizeof(uint32_t) * 3 * NSC, st, true);
-          const uint32_t nb = (nout + SORT_BLOCK * SORT_ITEMS - 1) / (SORT_BLOCK * SORT_ITEMS);
-          hipLaunchKernelGGL(k_scene_hist, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, nout, hist);
-          hipLaunchKernelGGL(k_scene_start, dim3(1), dim3(1024), 0, st, hist, hist + NSC);
-          hipLaunchKernelGGL(k_scene_scatter, dim3(nb), dim3(SORT_BLOCK), 0, st, ob, opb, nout, hist + NSC,
-                             hist + 2 * NSC, sb, spb);
-          check(hipGetLastError(), "scene sort launch");
-          open_frame(Batch{sb, spb, nout, s}, true);
-        } else {
-          free_slots.push_back(s);
-        }
+          hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
+                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+      } else {
+        hipLaunchKernelGGL(k_trace, dim3(tblocks), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n, c0, nth, tout);
       }
+      check(hipGetLastError(), "k_trace launch");
+      if (timing) check(hipEventRecord(e[2], st), "event");
+      if (adj)
+        hipLaunchKernelGGL(k_shade<true>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
+                           bt.pbuf, offs, c0, nth, tout, sk);
+      else
+        hipLaunchKernelGGL(k_shade<false>, dim3((nth + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, S, cfg, bt.buf,
+                           bt.pbuf, offs, c0, nth, tout, sk);
+      check(hipGetLastError(), "k_shade launch");
+      if (timing) check(hipEventRecord(e[3], st), "event");
+      if (cfg.nscenes <= 1) {
+        // the new batch's child counts scanned on the device right after
+        // shading: one read-back (spawn count and child total) per window
+        uint64_t* sums = (uint64_t*)arena.get("scan_dev_sums", sizeof(uint64_t) * (cap / SCAN_TILE + 2), st, false);
+        hipLaunchKernelGGL(k_scan_tiles_dev, dim3(scan_grid), dim3(SCAN_BLOCK), 0, st, ocnt, ooffs, d_cnt,
+                           (uint32_t)cap, sums);
+        hipLaunchKernelGGL(k_scan_sums_dev, dim3(1), dim3(SUMS_BLOCK), 0, st, d_cnt, (uint32_t)cap, sums, ooffs,
+                           (uint64_t*)(d_cnt + 2));
+        hipLaunchKernelGGL(k_scan_add_dev, dim3(scan_grid), dim3(SCAN_BLOCK), 0, st, ooffs, d_cnt, (uint32_t)cap,
+                           sums);
+        check(hipGetLastError(), "scan launch");
+      }
+      check(hipMemcpyAsync(h_win + 4 * pair, d_cnt, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "count");
+      check(hipEventRecord(e[0], st), "event");
+      inflight.push_back({Batch{ob, opb, 0, s}, cap, nth, pair});
+    };
+    auto finish_window = [&]() {
+      const Pending pw = inflight.front();
+      inflight.erase(inflight.begin());
+      hipEvent_t* e = &evs.ev[pw.pair * ev_per];
+      check(hipEventSynchronize(e[0]), "bounce");
+      const uint32_t* h = h_win + 4 * pw.pair;
+      if (timing) {
+        float ms = 0, ms2 = 0;
+        check(hipEventElapsedTime(&ms, e[1], e[2]), "event time");
+        check(hipEventElapsedTime(&ms2, e[2], e[3]), "event time");
+        stats->bounce_ms += ms;
+        stats->shade_ms += ms2;
+      }
+      if (stats) { stats->launches++; stats->threads += pw.nth; }
+      const int s = pw.out.slot;
+      const uint32_t nout = std::min<uint32_t>(h[0], (uint32_t)pw.cap);
+      if (!nout) {
+        free_slots.push_back(s);
+        return;
+      }
+      if (cfg.nscenes <= 1) {
+        if (++gens > max_batches) throw std::runtime_error("ray tree did not terminate");
+        const uint64_t total = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
+        frames.push_back({Batch{pw.out.buf, pw.out.pbuf, nout, s}, slot_offs(s, pw.cap), total, 0});
+        return;
+      }
+      // regroup by scene into the slot's second buffer, which the batch then uses
+      const size_t cap = pw.cap;
+      Spawn* sb = (Spawn*)arena.get("spawn_sorted" + std::to_string(s), sizeof(Spawn) * cap, st, false);
+      PathRec* spb = adj ? (PathRec*)arena.get("path_sorted" + std::to_string(s), sizeof(PathRec) * cap, st, false)
+                         : nullptr;
+      uint32_t* hist = (uint32_t*)arena.get("scene_hist", sizeof(uint32_t) * 3 * NSC, st, true);
+      const uint32_t nb = (nout + SORT_BLOCK * SORT_ITEMS - 1) / (SORT_BLOCK * SORT_ITEMS);
+      hipLaunchKernelGGL(k_scene_hist, dim3(nb), dim3(SORT_BLOCK), 0, st, pw.out.buf, nout, hist);
+      hipLaunchKernelGGL(k_scene_start, dim3(1), dim3(1024), 0, st, hist, hist + NSC);
+      hipLaunchKernelGGL(k_scene_scatter, dim3(nb), dim3(SORT_BLOCK), 0, st, pw.out.buf, pw.out.pbuf, nout,
+                         hist + NSC, hist + 2 * NSC, sb, spb);
+      check(hipGetLastError(), "scene sort launch");
+      open_frame(Batch{sb, spb, nout, s}, true);
+    };
+    while (!frames.empty() || !inflight.empty()) {
+      while (!frames.empty() && frames.back().c0 >= frames.back().total) {
+        free_slots.push_back(frames.back().bt.slot);
+        frames.pop_back();
+      }
+      if (!frames.empty() && inflight.size() < depth) {
+        launch_window();
+        continue;
+      }
+      if (!inflight.empty()) finish_window();
     }
   }
   if (stats) {
     stats->generations = gens;
     stats->node_bytes = (tn.trace_kernel == 1 && tn.wide) ? (int)sizeof(BVH4Node) : (int)sizeof(BVHNode);
   }
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
-  if (ev2) (void)hipEventDestroy(ev2);
 }
 
 }  // namespace rg

@@ -90,6 +90,7 @@ struct Tuning {
   int wide = 1;                  // persistent traversal walks the four-wide tree when the scene has one
   int shadow_cache = 1;          // pre-empt shadow rays by the reference's shadow cache (rt/srcobstr.c)
   int child_order = 1;           // hemisphere children in 8 x 8 blocks of strata (1) or by stratum row (0)
+  int inflight = 1;              // windows queued ahead of the host's read-back (1 or 2)
 };
 
 // max_srcs: the largest source count over the scenes of a batch (sizes the

@@ -415,7 +415,8 @@ def test_batched_scenes_match_individual_runs(L):
 def test_bins_repeat_bit_for_bit_across_launch_shapes(L):
     """rcontrib coefficients add up in 64-bit fixed point, so the order in
     which deposits arrive (launch shape, persistent refill, window size,
-    child order) leaves every record unchanged to the last bit."""
+    child order, windows in flight) leaves every record unchanged to the
+    last bit."""
     _say("bin accumulation: identical across launch shapes")
     rad, sens = scenes.config2(0, nboxes=40)
     sens = sens[::10]
@@ -424,7 +425,8 @@ def test_bins_repeat_bit_for_bit_across_launch_shapes(L):
     mods = _sky_mods(L)
     outs = []
     for shape in (dict(trace_kernel=1, refill=16, window=1 << 21), dict(trace_kernel=1, refill=48, window=1 << 12),
-                  dict(trace_kernel=0, window=1 << 14, child_order=0)):
+                  dict(trace_kernel=0, window=1 << 14, child_order=0),
+                  dict(trace_kernel=1, refill=32, window=1 << 13, child_order=1, inflight=2)):
         c.set_tuning(**shape)
         outs.append(c.contrib(sens, mods, 1, L.RT_IMM_IRRAD))
     for o in outs[1:]:
