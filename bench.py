@@ -199,11 +199,12 @@ def dcsky_full_error(dA, dB, dC, M: int, K: int, N: int, rows: int = 2048):
     return mx, tot / max(cnt, 1), cnt
 
 
-def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, mode: int = 1):
+def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, mode: int | None = None):
     """DC x sky product on device buffers: kernel time (HIP events on the
     null stream, where the product runs), algorithmic and issued MFMA rates,
     and the relative error of every entry against an FP64 product.  A is a
-    synthetic coefficient matrix, B the synthetic sky."""
+    synthetic coefficient matrix, B the synthetic sky.  mode None: the tile
+    form the autotuner picks for these sizes (tune.autotune_dcsky)."""
     import torch
     rng = np.random.default_rng(seed)
     A = (rng.random((M, K, 3), dtype=np.float32) * np.float32(2e-3)).astype(np.float32)
@@ -212,6 +213,10 @@ def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, mod
     dB = torch.from_numpy(B).cuda()
     del A
     dC = torch.empty((M, N, 3), dtype=torch.float32, device="cuda")
+    dtuned = None
+    if mode is None:
+        dtuned = tune.autotune_dcsky(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N)
+        mode = dtuned["best"]
     _lib.dcsky_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -229,7 +234,8 @@ def dcsky_device_bench(M: int, K: int, N: int, seed: int = 0, reps: int = 5, mod
     return {"M": M, "K": K, "N": N, "mode": mode, "kernel_ms": ms, "algorithmic_TFLOPs": flops / ms / 1e9,
             "issued_TFLOPs": 3 * flops / ms / 1e9, "mfma_util_issued": 3 * flops / ms / 1e9 / BF16_PEAK_TFLOPS,
             "GBs": byts / ms / 1e6, "hbm_frac": byts / ms / 1e6 / HBM_PEAK_GBS,
-            "max_rel_err_vs_fp64": emax, "mean_rel_err_vs_fp64": emean, "entries_checked": nent}
+            "max_rel_err_vs_fp64": emax, "mean_rel_err_vs_fp64": emean, "entries_checked": nent,
+            "tile_tuning": dtuned}
 
 
 def committed_traffic(bvh: str, walk: str):
@@ -882,6 +888,14 @@ def main():
     log(f"scene: {info}, commit+BVH {t_commit * 1e3:.1f} ms")
     ctx.set_params(gpu_params())
     flags = _lib.RT_IMM_IRRAD
+    scene_tuned = None
+    if not args.no_autotune and args.bvh == "auto" and args.max_big is None:
+        # tree construction (builder, room-sized primitives kept out of the
+        # tree) timed on the workload itself, host buffers, before the
+        # launch shape
+        scene_tuned = tune.autotune_scene(ctx, ctx.commit, lambda: ctx.contrib(sensors, mods, 1, flags))
+        log(f"scene autotune: best {scene_tuned['best']} {scene_tuned['ms']:.1f} ms; "
+            f"tried {[(c, round(m, 1), round(cm, 1)) for c, m, cm in scene_tuned['tried']]}")
     # sensors and records resident in HBM for the timed steps (rg_contrib_device)
     import torch
     d_sens = torch.from_numpy(np.ascontiguousarray(sensors, dtype=np.float64)).cuda()
@@ -953,12 +967,10 @@ def main():
     dcs = dcsky_bench(out)
     dcs["device"] = dcsky_device_bench(2500, 146, 8760)
     log(f"dcsky: {dcs}")
-    log(f"dcsky with LDS-staged tiles: {dcsky_device_bench(2500, 146, 8760, mode=3)}")
     dc5 = c5 = None
     if not args.no_config5:
         dc5 = dcsky_device_bench(20000, 2306, 8760)
         log(f"dcsky config 5 size: {dc5}")
-        log(f"dcsky config 5 size without LDS staging: {dcsky_device_bench(20000, 2306, 8760, mode=2)}")
         c5 = config5_bench()
         log(f"config 5 end to end: {c5}")
 
@@ -1005,6 +1017,8 @@ def main():
             log(f"cpu baseline failed: {e}")
 
     bvh_name = args.bvh if args.bvh != "auto" else ("sah" if info["primitives"] <= (4 << 20) else "lbvh")
+    if scene_tuned:
+        bvh_name = "sah" if scene_tuned["best"]["builder"] == _lib.BVH_SAH else "lbvh"
     walk = "wide" if st_c.get("node_bytes") == 128 else "binary"
     traffic_ray = committed_traffic(bvh_name, walk)
     # HBM bytes per launch = measured bytes per ray x rays per launch of this run
@@ -1029,6 +1043,7 @@ def main():
                    "host_buffer_Mrays_s": host_rate,
                    "bvh": bvh_name,
                    "launch_shape": tuned["best"] if tuned else tune.DEFAULT,
+                   "tree": scene_tuned["best"] if scene_tuned else None,
                    "per_thread_kernel_ms": tuned["tried"][0][1] if tuned else None},
         # bound: the roofline the algorithmic bytes are priced against;
         # limiter: what the counters show limits the kernel (instruction

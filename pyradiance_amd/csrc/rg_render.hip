@@ -2051,11 +2051,16 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
   auto slot_path = [&](int s, size_t cap) -> PathRec* {
     return adj ? (PathRec*)arena.get("path" + std::to_string(s), sizeof(PathRec) * cap, st, false) : nullptr;
   };
-  // Spawn buffers: at most kSlotBytes each (a window's children each append
+  // Spawn buffers: at most slot_bytes each (a window's children each append
   // at most per_child records), so capacities fit 32 bits and scenes with
   // many light sources shrink their windows instead of failing to allocate.
-  constexpr size_t kSlotBytes = size_t(4) << 30;
-  const uint32_t slot_children = (uint32_t)std::max<size_t>(64, kSlotBytes / (sizeof(Spawn) * per_child));
+  // A slot is held to 1/24 of the device's memory (about six slots are live
+  // at the deepest point of a ray tree, each with its scan and trace
+  // records), and to 16 GiB.
+  size_t dev_mem = 0;
+  check(hipDeviceTotalMem(&dev_mem, dev), "device memory");
+  const size_t slot_bytes = std::max<size_t>(size_t(64) << 20, std::min<size_t>(size_t(16) << 30, dev_mem / 24));
+  const uint32_t slot_children = (uint32_t)std::max<size_t>(64, slot_bytes / (sizeof(Spawn) * per_child));
   const uint32_t win = std::min(window, slot_children);
   long gens = 0;
   // Events: two per window in flight (its read-back, and the trace / shade

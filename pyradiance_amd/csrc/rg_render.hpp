@@ -84,7 +84,7 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, DevArena& are
 struct Tuning {
   int trace_kernel = 1;          // 0: one child per thread, 1: persistent with dynamic ray fetch
   uint32_t refill = 16;          // idle lanes that make a persistent wavefront fetch new rays
-  uint32_t window = 1u << 21;    // children per generation launch
+  uint32_t window = 1u << 24;    // children per generation launch (at most a spawn slot's share of device memory)
   uint32_t blocks_per_cu = 4;    // persistent grid: blocks of 256 per compute unit
   uint32_t max_big = 8;          // room-sized primitives kept out of the BVH when there are at most this many (applies at commit)
   int wide = 1;                  // persistent traversal walks the four-wide tree when the scene has one
