@@ -1346,6 +1346,23 @@ __device__ __forceinline__ uint32_t find_spawn(const uint64_t* offs, uint32_t ns
   return lo;
 }
 
+// Spawn of the first child of every 64-child chunk of a window [c0, c0 +
+// nth), and at index nchunk the spawn of the window's last child: one search
+// per chunk, all in parallel, ahead of the walk.  The walk's refill then
+// finds a child's spawn between its chunk's two entries (usually the same
+// record) instead of searching the whole scan from a single lane, a chain of
+// about log2(nspawn) dependent loads that every lane of the wavefront waited
+// for.
+constexpr uint32_t CHUNK_LOG = 6;
+__global__ void __launch_bounds__(BLOCK) k_chunk_spawn(const uint64_t* offs, uint32_t nspawn, uint64_t c0, uint32_t nth,
+                                                       uint32_t* first) {
+  const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t nch = (nth + (1u << CHUNK_LOG) - 1) >> CHUNK_LOG;
+  if (k > nch) return;
+  const uint64_t gid = c0 + (k < nch ? (uint64_t)k << CHUNK_LOG : (uint64_t)nth - 1);
+  first[k] = find_spawn(offs, nspawn, gid);
+}
+
 // Traversal result of one child ray (prim -2: not traced).
 struct alignas(16) TraceOut {
   double t;
@@ -1410,15 +1427,16 @@ k_trace(DevScene Sarg, RunCfg c, const Spawn* in, const uint64_t* offs, uint32_t
 // indices from `work` with one atomic and the idle lanes start new rays,
 // while the others carry on.  Lanes therefore do not idle until the longest
 // ray of their wavefront ends, which is what bounds the one-ray-per-thread
-// kernel (its VALU lane utilisation is about a third).  Same admission,
-// closest-hit walk and output record as k_trace.  BATCH: rays of several
+// kernel (its VALU lane utilisation is about a third).  A child's spawn lies
+// between the entries of its 64-child chunk in `first` (k_chunk_spawn).  Same
+// admission, closest-hit walk and output record as k_trace.  BATCH: rays of several
 // scenes (each lane keeps its ray's scene tables); otherwise the kernel
 // argument's scene, held in scalar registers.  WIDE: the walk fetches the
 // four-wide tree (four child boxes per node; the children hit are taken
 // nearest first, the others pushed farthest first).
 template <int BS, bool BATCH, bool WIDE>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(RG_TRACE_WAVES)))
-k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint64_t* offs, uint32_t nspawn, uint64_t c0,
+k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint64_t* offs, const uint32_t* first, uint64_t c0,
            uint32_t nthreads, TraceOut* outp, uint32_t* work, uint32_t refill) {
   __shared__ int32_t stk[LSTACK * BS];
   __shared__ float stn[LSTACK * BS];
@@ -1451,13 +1469,12 @@ k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint64_t* offs, uint3
       if ((uint64_t)base + nidle >= nthreads) exhausted = true;
       if (tcur < 0) {
         const uint32_t my = base + lane_rank(idle);
-        const uint64_t gid0 = c0 + base;
-        uint32_t s0 = 0;
-        if (wl == leader && base < nthreads) s0 = find_spawn(offs, nspawn, gid0);
-        s0 = __shfl(s0, leader);
         if (my < nthreads) {
           const uint64_t gid = c0 + my;
-          uint32_t lo = s0, hi = (uint32_t)min((uint64_t)nspawn, (uint64_t)s0 + (gid - gid0) + 1);
+          // last spawn with offs <= gid, between the spawn of this chunk's
+          // first child and that of the next chunk's (or the window's last child)
+          const uint32_t ck = my >> CHUNK_LOG;
+          uint32_t lo = first[ck], hi = first[ck + 1] + 1;
           while (hi - lo > 1) {
             uint32_t mid = (lo + hi) >> 1;
             if (offs[mid] <= gid) lo = mid; else hi = mid;
@@ -2198,19 +2215,22 @@ void render(const DevScene& S, RunCfg cfg, const double* d_rays, uint32_t nsens,
       if (timing) check(hipEventRecord(e[1], st), "event");
       const uint32_t tblocks = (nth + TBLOCK - 1) / TBLOCK;
       if (tn.trace_kernel == 1) {
+        const uint32_t nch = (nth + (1u << CHUNK_LOG) - 1) >> CHUNK_LOG;
+        uint32_t* first = (uint32_t*)arena.get("chunk_spawn", sizeof(uint32_t) * (nch + 1), st, false);
+        hipLaunchKernelGGL(k_chunk_spawn, dim3((nch + BLOCK) / BLOCK), dim3(BLOCK), 0, st, offs, bt.n, c0, nth, first);
         const uint32_t nb = std::min(tblocks, pf_blocks);
         if (cfg.nscenes && tn.wide)
           hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+                             first, c0, nth, tout, d_cnt + 1, tn.refill);
         else if (cfg.nscenes)
           hipLaunchKernelGGL((k_trace_pf<TBLOCK, true, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+                             first, c0, nth, tout, d_cnt + 1, tn.refill);
         else if (tn.wide)
           hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, true>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+                             first, c0, nth, tout, d_cnt + 1, tn.refill);
         else
           hipLaunchKernelGGL((k_trace_pf<TBLOCK, false, false>), dim3(nb), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs,
-                             bt.n, c0, nth, tout, d_cnt + 1, tn.refill);
+                             first, c0, nth, tout, d_cnt + 1, tn.refill);
       } else {
         hipLaunchKernelGGL(k_trace, dim3(tblocks), dim3(TBLOCK), 0, st, S, cfg, bt.buf, offs, bt.n, c0, nth, tout);
       }
