@@ -209,9 +209,13 @@ int rg_last_stats(rg_ctx *ctx, double *out, int n);
 /* DC x sky product, replaces cm_multiply() (util/cmatrix.c:421-474):
  * C[M][N][3] = A[M][K][3] . B[K][N][3], each channel independently.
  * Host pointers.  mode 0 = FP32 accumulate, 1 = BF16 hi/lo split MFMA
- * (LDS-staged 128 x 128 tiles when K > 256, per-wavefront operand loads
- * otherwise), 2 / 3 = the split product forced to per-wavefront loads /
- * LDS-staged tiles. */
+ * (K <= 160: rows of A held in registers, sky columns streamed through LDS;
+ * K > 256: LDS-staged 128 x 128 tiles; otherwise per-wavefront operand
+ * loads), 2 / 3 / 4 = the split product forced to per-wavefront loads /
+ * LDS-staged tiles / the short-K form (K <= 160).  The split planes live in
+ * a buffer kept per device and stream, grown as needed and held until the
+ * process ends (2 * 3 * (M + N) * ceil(K / 32) * 32 * 2 bytes at most, with
+ * K rounded up to 64 for the staged form). */
 int rg_dcsky(const float *A, const float *B, float *C, int M, int K, int N, int mode);
 /* Same with device pointers on the current HIP device. */
 int rg_dcsky_device(const float *A, const float *B, float *C, int M, int K, int N, int mode);

@@ -419,7 +419,10 @@ def trace_batch(ctxs: list, rays: list, flags: int = 0) -> list:
 
 
 def dcsky(A: np.ndarray, B: np.ndarray, mode: int = 1) -> np.ndarray:
-    """C[m,n,:] = sum_k A[m,k,:] * B[k,n,:] per channel on the GPU."""
+    """C[m,n,:] = sum_k A[m,k,:] * B[k,n,:] per channel on the GPU.
+    mode 0: FP32 FMA; 1: BF16 hi/lo split MFMA in the form chosen by K;
+    2 / 3 / 4: the split product with operands loaded per wavefront / staged
+    through LDS in 128 x 128 tiles / rows of A in registers (K <= 160)."""
     A = np.ascontiguousarray(A, dtype=np.float32)
     B = np.ascontiguousarray(B, dtype=np.float32)
     M, K, _ = A.shape

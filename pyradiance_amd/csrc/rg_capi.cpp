@@ -971,7 +971,7 @@ int rg_last_stats(rg_ctx* c, double* out, int n) {
 int rg_dcsky_device(const float* A, const float* B, float* C, int M, int K, int N, int mode) {
   try {
     if (M <= 0 || K <= 0 || N <= 0) throw std::runtime_error("bad matrix dimensions");
-    if (mode < 0 || mode > 3) throw std::runtime_error("bad mode");
+    if (mode < 0 || mode > 4) throw std::runtime_error("bad mode");
     dcsky(A, B, C, M, K, N, mode, nullptr);
     hipck(hipGetLastError(), "dcsky launch");
     hipck(hipDeviceSynchronize(), "dcsky");

@@ -11,6 +11,7 @@
 // K step is fetched while the current one is on the matrix cores.
 // mode 0: plain FP32 FMA, tiled through LDS (accuracy baseline).
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include "rg_render.hpp"
 
 namespace rg {
@@ -289,6 +290,139 @@ __global__ void __launch_bounds__(64 * WM * WN) k_dcsky_lds(const unsigned short
   }
 }
 
+// Short-K form (K <= 160, e.g. 146 Klems patches): the product is bound by
+// the output write, and the operands are small enough that a wavefront keeps
+// its 16 rows of A (hi and lo, all K, three channels) in registers for the
+// whole launch.  A block of 8 wavefronts owns a slab of 128 rows of A and
+// walks a range of sky columns in chunks of 32: each chunk's B planes (hi/lo,
+// three channels, all K) are staged into LDS once, double-buffered (the next
+// chunk's global loads are in flight during this chunk's MFMAs), and every
+// wavefront multiplies its rows by them and stores its 16 x 32 x 3 outputs
+// before the next chunk.  No K loop over global memory, one barrier per chunk.
+// Every lane issues the same loads and stores in every chunk (pieces past
+// the chunk load a valid piece again; outputs outside C go to `sink`), so the
+// wait for the next chunk's loads never waits for this chunk's stores.
+constexpr int SW = 8;                  // wavefronts per block
+constexpr int SROWS = 16 * SW;         // rows of A per block (a slab)
+constexpr int SCOLS = 32;              // sky columns per staged chunk
+constexpr int SHORT_KS = 5;            // K steps of 32 the short form covers (Kp <= 160)
+constexpr int SINK_FLOATS = 64 * 12;   // one wavefront's worth of redirected outputs
+
+template <int KS, bool VEC>
+__global__ void __launch_bounds__(64 * SW) k_dcsky_short(const unsigned short* Ah, const unsigned short* Al,
+                                                         const unsigned short* Bh, const unsigned short* Bl, float* C,
+                                                         float* sink, int M, int N, int nslab, int nrange) {
+  constexpr int Kp = 32 * KS;
+  constexpr int ROW = Kp + 8;                  // padded LDS row (bf16)
+  constexpr int PL = SCOLS * ROW;              // one (channel, hi|lo) tile
+  constexpr int NT = 64 * SW;
+  constexpr int CPR = Kp / 8;                  // 16-byte pieces per row
+  constexpr int NPC = 6 * SCOLS * CPR;         // pieces per chunk
+  constexpr int NLD = (NPC + NT - 1) / NT;     // pieces per thread
+  __shared__ __attribute__((aligned(16))) unsigned short lds[2][6][PL];   // [buffer][2 c + lo]
+  // block -> (slab, column range); blocks run on XCD b % 8, and the blocks
+  // of one XCD take consecutive q: all slabs of one column range, which
+  // share its B chunks in that XCD's L2
+  const int per = (int)((gridDim.x + 7) / 8);
+  const int q = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (q >= nslab * nrange) return;
+  const int slab = q % nslab, range = q / nslab;
+  const int nchunk = (N + SCOLS - 1) / SCOLS;
+  const int cb = (int)((long long)range * nchunk / nrange), ce = (int)((long long)(range + 1) * nchunk / nrange);
+  if (cb >= ce) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  const int m = slab * SROWS + wave * 16 + r16;
+  const int mc = min(m, M - 1);                // clamped for loads; rows >= M are not stored
+  bf16x8 ah[3][KS], al[3][KS];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const size_t o = ((size_t)c * M + mc) * Kp + s * 32 + kq;
+      ah[c][s] = ld8(Ah + o);
+      al[c][s] = ld8(Al + o);
+    }
+  // staging piece j of this thread: plane pl = 2 c + lo, row, 16-byte part
+  int soff[NLD], spl[NLD], srow[NLD], spart[NLD];
+#pragma unroll
+  for (int j = 0; j < NLD; ++j) {
+    const int p = min(t + j * NT, NPC - 1);
+    const int rem = p % (SCOLS * CPR);
+    spl[j] = p / (SCOLS * CPR);
+    srow[j] = rem / CPR;
+    spart[j] = rem % CPR;
+    soff[j] = srow[j] * ROW + spart[j] * 8;
+  }
+  auto gload = [&](int chunk, u16x8* reg) {
+    const int n0 = chunk * SCOLS;
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+      const unsigned short* src = (spl[j] & 1) ? Bl : Bh;
+      reg[j] = *(const u16x8*)(src + ((size_t)(spl[j] >> 1) * N + min(n0 + srow[j], N - 1)) * Kp + spart[j] * 8);
+    }
+  };
+  auto lstore = [&](int buf, const u16x8* reg) {
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) *(u16x8*)&lds[buf][spl[j]][soff[j]] = reg[j];
+  };
+  u16x8 reg[NLD];
+  gload(cb, reg);
+  lstore(0, reg);
+  // every load so far (A and the first chunk) complete here, so that no wait
+  // inside the loop, where this chunk's stores are also outstanding, is
+  // counted against them
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  float* const mysink = sink + lane * 12;
+  for (int ch = cb; ch < ce; ++ch) {
+    const int buf = (ch - cb) & 1;
+    gload(min(ch + 1, ce - 1), reg);
+    __builtin_amdgcn_sched_barrier(0);     // the next chunk's loads go out before the MFMAs
+    f32x4 acc[3][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c][0] = acc[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int row = 16 * b + r16;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][2 * c][row * ROW + s * 32 + kq]);
+          const bf16x8 bl = __builtin_bit_cast(bf16x8, *(const u16x8*)&lds[buf][2 * c + 1][row * ROW + s * 32 + kq]);
+          acc[c][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah[c][s], acc[c][b], 0, 0, 0);
+          acc[c][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah[c][s], acc[c][b], 0, 0, 0);
+          acc[c][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al[c][s], acc[c][b], 0, 0, 0);
+        }
+      }
+    // D layout: lane holds D[n = 4 * (lane >> 4) + i][m = lane & 15], 12
+    // consecutive floats of row m
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int nq = ch * SCOLS + 16 * b + (lane >> 4) * 4;
+      const f32x4 r = acc[0][b], g = acc[1][b], u = acc[2][b];
+      if (VEC) {   // N a multiple of 4: nq < N implies nq + 3 < N, 16-byte aligned
+        float* dst = (m < M && nq < N) ? C + ((size_t)m * N + nq) * 3 : mysink;
+        float4* d4 = (float4*)dst;
+        d4[0] = make_float4(r[0], g[0], u[0], r[1]);
+        d4[1] = make_float4(g[1], u[1], r[2], g[2]);
+        d4[2] = make_float4(u[2], r[3], g[3], u[3]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float* dst = (m < M && nq + i < N) ? C + ((size_t)m * N + nq + i) * 3 : mysink + 3 * i;
+          dst[0] = r[i];
+          dst[1] = g[i];
+          dst[2] = u[i];
+        }
+      }
+    }
+    lstore(buf ^ 1, reg);    // the last chunk stages a copy of itself into the idle buffer
+    __syncthreads();
+  }
+}
+
 // FP32 reference path: 16x16 threads, 64x64 tile, LDS staged.
 __global__ void __launch_bounds__(256) k_dcsky_f32(const float* A, const float* B, float* C, int M, int K, int N) {
   __shared__ float As[3][64][17];
@@ -328,25 +462,102 @@ static void dck(hipError_t e, const char* w) {
   if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
 }
 
+// Split planes of a call, kept per (device, stream) and grown as needed (a
+// stream-ordered allocation and free per call cost about as much as the whole
+// product at K = 146).  The lock is held while a call queues its kernels, so a
+// buffer is only replaced after the work queued on it has finished.
+struct DcScratch {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+static std::mutex g_dc_mu;
+static std::map<std::pair<int, hipStream_t>, DcScratch> g_dc_scratch;
+
+static unsigned short* dc_scratch(size_t bytes, hipStream_t st) {
+  int dev = 0;
+  dck(hipGetDevice(&dev), "dcsky device");
+  DcScratch& s = g_dc_scratch[{dev, st}];
+  if (s.bytes < bytes) {
+    if (s.p) {
+      dck(hipStreamSynchronize(st), "dcsky scratch sync");
+      dck(hipFree(s.p), "dcsky scratch free");
+      s.p = nullptr;
+      s.bytes = 0;
+    }
+    dck(hipMalloc(&s.p, bytes), "dcsky scratch");
+    s.bytes = bytes;
+  }
+  return (unsigned short*)s.p;
+}
+
+// Column ranges per slab for the short-K form: one block per CU at a time
+// (its LDS), so the fewest ranges that keep at least 90 % of the CUs busy
+// over whole rounds of blocks.
+static int short_ranges(int nslab, int nchunk) {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  ncu = std::max(ncu, 1);
+  int best = 1;
+  double beff = 0.0;
+  for (int r = 1; r <= std::min(nchunk, 64); ++r) {
+    const long long G = (long long)nslab * r;
+    const long long rounds = (G + ncu - 1) / ncu;
+    const double eff = (double)G / (double)(rounds * ncu);
+    if (eff > beff + 1e-9) { beff = eff; best = r; }
+    if (eff >= 0.9) break;
+  }
+  return best;
+}
+
+template <int KS>
+static void launch_short(const unsigned short* Ah, const unsigned short* Al, const unsigned short* Bh,
+                         const unsigned short* Bl, float* C, float* sink, int M, int N, hipStream_t st) {
+  const int nslab = (M + SROWS - 1) / SROWS, nchunk = (N + SCOLS - 1) / SCOLS;
+  const int nrange = short_ranges(nslab, nchunk);
+  const long long G = (long long)nslab * nrange;
+  if (G > 0x7fffffffLL - 8) throw std::runtime_error("dcsky: matrix too large");
+  const unsigned nblk = (unsigned)((G + 7) / 8 * 8);
+  if (N % 4 == 0)
+    hipLaunchKernelGGL((k_dcsky_short<KS, true>), dim3(nblk), dim3(64 * SW), 0, st, Ah, Al, Bh, Bl, C, sink, M, N,
+                       nslab, nrange);
+  else
+    hipLaunchKernelGGL((k_dcsky_short<KS, false>), dim3(nblk), dim3(64 * SW), 0, st, Ah, Al, Bh, Bl, C, sink, M, N,
+                       nslab, nrange);
+}
+
 void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mode, hipStream_t st) {
   if (mode == 0) {
     hipLaunchKernelGGL(k_dcsky_f32, dim3((N + 63) / 64, (M + 63) / 64), dim3(256), 0, st, A, B, C, M, K, N);
     return;
   }
-  // K padded to whole stages of the kernel that will run (zeros beyond K)
-  const bool staged = mode == 3 || (mode == 1 && K > 256);
+  // Short K (a few K steps) is bound by the output write: K <= 160 keeps A in
+  // registers (mode 4 forces that form), up to 256 the per-wavefront kernel;
+  // long K is MFMA work and needs the LDS-staged tiles.  mode 2 forces the
+  // per-wavefront kernel, mode 3 the staged one.
+  const int k32 = (K + 31) / 32 * 32;
+  if (mode == 4 && k32 > 32 * SHORT_KS) throw std::runtime_error("dcsky: mode 4 needs K <= 160");
+  const bool shortk = mode == 4 || (mode == 1 && k32 <= 32 * SHORT_KS);
+  const bool staged = !shortk && (mode == 3 || (mode == 1 && K > 256));
   const int kstep = staged ? LKS : 32;
   int Kp = (K + kstep - 1) / kstep * kstep;
   size_t na = (size_t)3 * M * Kp, nb = (size_t)3 * N * Kp;
-  unsigned short* buf = nullptr;
-  dck(hipMallocAsync((void**)&buf, 2 * (na + nb) * sizeof(unsigned short), st), "dcsky scratch");
+  std::lock_guard<std::mutex> lk(g_dc_mu);
+  // split planes, then (16-byte aligned) the short form's output sink
+  const size_t plane_bytes = (2 * (na + nb) * sizeof(unsigned short) + 255) / 256 * 256;
+  unsigned short* buf = dc_scratch(plane_bytes + SINK_FLOATS * sizeof(float), st);
   unsigned short *Ah = buf, *Al = buf + na, *Bh = buf + 2 * na, *Bl = buf + 2 * na + nb;
+  float* sink = (float*)((char*)buf + plane_bytes);
   hipLaunchKernelGGL(k_split_a, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, st, A, M, K, Kp, Ah, Al);
   hipLaunchKernelGGL(k_split_b, dim3((N + 31) / 32, Kp / 32, 3), dim3(32, 8), 0, st, B, K, N, Kp, Bh, Bl);
-  // Short K (a few K steps) is bound by latency and the output write: the
-  // per-wavefront kernel is as fast there; long K is MFMA work and needs the
-  // LDS-staged tiles.  mode 2 forces the former, mode 3 the latter.
-  if (!staged) {
+  if (shortk) {
+    switch (Kp / 32) {
+      case 1: launch_short<1>(Ah, Al, Bh, Bl, C, sink, M, N, st); break;
+      case 2: launch_short<2>(Ah, Al, Bh, Bl, C, sink, M, N, st); break;
+      case 3: launch_short<3>(Ah, Al, Bh, Bl, C, sink, M, N, st); break;
+      case 4: launch_short<4>(Ah, Al, Bh, Bl, C, sink, M, N, st); break;
+      default: launch_short<5>(Ah, Al, Bh, Bl, C, sink, M, N, st); break;
+    }
+  } else if (!staged) {
     const int Mt = (M + BM - 1) / BM, Nt = (N + BN - 1) / BN;
     const long long tiles = (long long)Mt * Nt;
     if (tiles > 0x7fffffffLL - 8) throw std::runtime_error("dcsky: matrix too large");
@@ -360,7 +571,6 @@ void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mo
     hipLaunchKernelGGL((k_dcsky_lds<4, 4, 2, 2>), dim3(nblk), dim3(1024), 0, st, Ah, Al, Bh, Bl, C, M, N, Kp, Mt,
                        Nt);
   }
-  dck(hipFreeAsync(buf, st), "dcsky scratch free");
 }
 
 }  // namespace rg

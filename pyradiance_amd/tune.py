@@ -85,11 +85,13 @@ def autotune_scene(ctx: "_lib.Context", commit, run, cands: list | None = None) 
 def autotune_dcsky(A_ptr: int, B_ptr: int, C_ptr: int, M: int, K: int, N: int, reps: int = 10) -> dict:
     """Time the DC x sky product's tile forms on device buffers of the
     caller's own sizes (mode 2: operands loaded per wavefront, 32 x 64 tiles;
-    mode 3: 128 x 128 tiles staged through LDS) and return the fastest mode
-    (the same product: both accumulate the same hi/lo split in FP32)."""
+    mode 3: 128 x 128 tiles staged through LDS; mode 4, for K <= 160: rows of
+    A held in registers, sky columns streamed through LDS) and return the
+    fastest mode (the same product: all accumulate the same hi/lo split in
+    FP32)."""
     import torch
     tried = []
-    for mode in (2, 3):
+    for mode in ((2, 3, 4) if K <= 160 else (2, 3)):
         _lib.dcsky_device(A_ptr, B_ptr, C_ptr, M, K, N, mode)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

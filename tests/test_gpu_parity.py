@@ -225,14 +225,16 @@ def test_klems_bins_match_reference(L):
 def test_dcsky_against_fp64(L):
     _say("dcsky")
     rng = np.random.default_rng(0)
-    for (M, K, N) in [(37, 19, 51), (250, 146, 877), (64, 2305, 96), (300, 77, 130), (129, 33, 257)]:
+    for (M, K, N) in [(37, 19, 51), (250, 146, 877), (64, 2305, 96), (300, 77, 130), (129, 33, 257),
+                      (513, 160, 1203), (130, 64, 2048)]:
         A = rng.random((M, K, 3), dtype=np.float32)
         B = rng.lognormal(0, 1, (K, N, 3)).astype(np.float32)
         ref = np.einsum("mkc,knc->mnc", A.astype(np.float64), B.astype(np.float64), optimize=True)
         c32 = L.dcsky(A, B, 0)
         e32 = np.abs(c32 - ref).max() / np.abs(ref).max()
         assert e32 < 1e-5, (M, K, N, e32)
-        for mode in (1, 2, 3):       # automatic, direct and LDS-staged split-BF16 products
+        # automatic, direct, LDS-staged and (K <= 160) short-K split-BF16 products
+        for mode in ((1, 2, 3, 4) if K <= 160 else (1, 2, 3)):
             cbf = L.dcsky(A, B, mode)
             ebf = (np.abs(cbf - ref) / np.abs(ref)).max()
             assert ebf < 1e-4, (M, K, N, mode, ebf)
@@ -485,7 +487,7 @@ def test_empty_and_smallest_inputs(L, tmp_path):
     # 1 x 1 x 1 and 1 x K x 1 products
     A = np.full((1, 1, 3), 2.0, dtype=np.float32)
     B = np.full((1, 1, 3), 3.0, dtype=np.float32)
-    for mode in (0, 1, 2, 3):
+    for mode in (0, 1, 2, 3, 4):
         assert np.allclose(L.dcsky(A, B, mode), 6.0, rtol=1e-5)
     A = np.ones((1, 300, 3), dtype=np.float32)
     B = np.ones((300, 1, 3), dtype=np.float32)
