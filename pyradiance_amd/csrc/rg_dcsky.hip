@@ -27,40 +27,48 @@ __device__ __forceinline__ unsigned short f2bf(float f) {   // round to nearest 
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 
-// A[M][K][3] -> hi/lo planes [3][M][Kp]
-__global__ void k_split_a(const float* A, int M, int K, int Kp, unsigned short* hi, unsigned short* lo) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t tot = (size_t)3 * M * Kp;
-  if (i >= tot) return;
-  int k = (int)(i % Kp);
-  size_t r = i / Kp;
-  int m = (int)(r % M);
-  int c = (int)(r / M);
-  float x = k < K ? A[((size_t)m * K + k) * 3 + c] : 0.0f;
-  unsigned short h = f2bf(x);
-  hi[i] = h;
-  lo[i] = f2bf(x - bf2f(h));
-}
-
-// B[K][N][3] -> hi/lo planes [3][N][Kp] (transposed through LDS)
-__global__ void k_split_b(const float* B, int K, int N, int Kp, unsigned short* hi, unsigned short* lo) {
-  __shared__ float tile[32][33];
-  int c = blockIdx.z;
-  int n0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
-  int tx = threadIdx.x, ty = threadIdx.y;   // 32 x 8
+// Both operands to hi/lo planes in one launch: blocks [0, nba) split
+// A[M][K][3] into [3][M][Kp] (one element per thread), the others take a
+// 32 (k) x 32 (n) tile of B[K][N][3] with its three channels (96 contiguous
+// floats per k row) through LDS into [3][N][Kp] (K-contiguous per channel).
+// K is zero-padded to Kp.
+__global__ void __launch_bounds__(256) k_split(const float* A, const float* B, int M, int K, int N, int Kp,
+                                               unsigned nba, unsigned short* Ah, unsigned short* Al,
+                                               unsigned short* Bh, unsigned short* Bl) {
+  __shared__ float tile[3][32][33];
+  if (blockIdx.x < nba) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)3 * M * Kp) return;
+    const int k = (int)(i % Kp);
+    const size_t r = i / Kp;
+    const int m = (int)(r % M), c = (int)(r / M);
+    const float x = k < K ? A[((size_t)m * K + k) * 3 + c] : 0.0f;
+    const unsigned short h = f2bf(x);
+    Ah[i] = h;
+    Al[i] = f2bf(x - bf2f(h));
+    return;
+  }
+  const unsigned bt = blockIdx.x - nba;
+  const int ntn = (N + 31) / 32;
+  const int n0 = (int)(bt % ntn) * 32, k0 = (int)(bt / ntn) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
   for (int j = ty; j < 32; j += 8) {
-    int k = k0 + j, n = n0 + tx;
-    tile[j][tx] = (k < K && n < N) ? B[((size_t)k * N + n) * 3 + c] : 0.0f;
+    const int k = k0 + j;
+#pragma unroll
+    for (int e = tx; e < 96; e += 32) {
+      const int n = n0 + e / 3;
+      tile[e % 3][j][e / 3] = (k < K && n < N) ? B[((size_t)k * N + n) * 3 + e % 3] : 0.0f;
+    }
   }
   __syncthreads();
-  for (int j = ty; j < 32; j += 8) {
-    int n = n0 + j, k = k0 + tx;
+  for (int j = ty; j < 96; j += 8) {
+    const int c = j / 32, n = n0 + j % 32, k = k0 + tx;
     if (n < N && k < Kp) {
-      float x = tile[tx][j];
-      unsigned short h = f2bf(x);
-      size_t o = ((size_t)c * N + n) * Kp + k;
-      hi[o] = h;
-      lo[o] = f2bf(x - bf2f(h));
+      const float x = tile[c][tx][j % 32];
+      const unsigned short h = f2bf(x);
+      const size_t o = ((size_t)c * N + n) * Kp + k;
+      Bh[o] = h;
+      Bl[o] = f2bf(x - bf2f(h));
     }
   }
 }
@@ -547,8 +555,10 @@ void dcsky(const float* A, const float* B, float* C, int M, int K, int N, int mo
   unsigned short* buf = dc_scratch(plane_bytes + SINK_FLOATS * sizeof(float), st);
   unsigned short *Ah = buf, *Al = buf + na, *Bh = buf + 2 * na, *Bl = buf + 2 * na + nb;
   float* sink = (float*)((char*)buf + plane_bytes);
-  hipLaunchKernelGGL(k_split_a, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, st, A, M, K, Kp, Ah, Al);
-  hipLaunchKernelGGL(k_split_b, dim3((N + 31) / 32, Kp / 32, 3), dim3(32, 8), 0, st, B, K, N, Kp, Bh, Bl);
+  const size_t nba = (na + 255) / 256, nbb = (size_t)((N + 31) / 32) * (Kp / 32);
+  if (nba + nbb > 0x7fffffffu) throw std::runtime_error("dcsky: matrix too large");
+  hipLaunchKernelGGL(k_split, dim3((unsigned)(nba + nbb)), dim3(256), 0, st, A, B, M, K, N, Kp, (unsigned)nba, Ah, Al,
+                     Bh, Bl);
   if (shortk) {
     switch (Kp / 32) {
       case 1: launch_short<1>(Ah, Al, Bh, Bl, C, sink, M, N, st); break;
