@@ -1888,21 +1888,29 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(uint32_t* out, uint32_t
 }
 
 // The same in 64 bits, for child counts whose totals can pass 2^32 (a
-// million -I sensors at -ad 4096 already spawn 2^32 children).
+// million -I sensors at -ad 4096 already spawn 2^32 children): an inclusive
+// scan within each wavefront by lane shuffles, the wavefront totals through
+// LDS, two barriers per tile.  sh holds at least SCAN_BLOCK / 64 entries.
 __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* sh, uint64_t& total) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
-    uint64_t a = t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += a;
-    __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t a = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += a;
   }
-  total = sh[SCAN_BLOCK - 1];
-  uint64_t incl = sh[t];
+  if (lane == 63) sh[w] = incl;
   __syncthreads();
-  return incl - v;
+  uint64_t pre = 0;
+  total = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_BLOCK / 64; ++k) {
+    const uint64_t x = sh[k];
+    pre += k < w ? x : 0;
+    total += x;
+  }
+  __syncthreads();      // sh is rewritten by the next call
+  return pre + incl - v;
 }
 
 template <class TI>
@@ -1980,30 +1988,40 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_tiles_dev(const uint32_t* i
 constexpr int SUMS_BLOCK = 1024;
 __global__ void __launch_bounds__(SUMS_BLOCK) k_scan_sums_dev(const uint32_t* d_n, uint32_t cap, uint64_t* tile_sums,
                                                               uint64_t* out, uint64_t* d_total) {
-  __shared__ uint64_t sh[SUMS_BLOCK];
+  __shared__ uint64_t sh[SUMS_BLOCK / 64];
   const uint32_t n = min(*d_n, cap);
   const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   const uint32_t per = (ntiles + SUMS_BLOCK - 1) / SUMS_BLOCK;
-  const uint32_t t = threadIdx.x, b0 = t * per, b1 = min(b0 + per, ntiles);
+  const uint32_t t = threadIdx.x, b0 = min(t * per, ntiles), b1 = min(b0 + per, ntiles);
   uint64_t s = 0;
   for (uint32_t k = b0; k < b1; ++k) s += tile_sums[k];
-  sh[t] = s;
-  __syncthreads();
-  for (int off = 1; off < SUMS_BLOCK; off <<= 1) {
-    uint64_t a = (int)t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += a;
-    __syncthreads();
+  // exclusive scan of the per-thread sums: lane shuffles, then the 16
+  // wavefront totals through LDS
+  const int lane = t & 63, w = t >> 6;
+  uint64_t incl = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t a = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += a;
   }
-  uint64_t ex = sh[t] - s;
+  if (lane == 63) sh[w] = incl;
+  __syncthreads();
+  uint64_t pre = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < SUMS_BLOCK / 64; ++k) {
+    const uint64_t x = sh[k];
+    pre += k < w ? x : 0;
+    total += x;
+  }
+  uint64_t ex = pre + incl - s;
   for (uint32_t k = b0; k < b1; ++k) {
     const uint64_t v = tile_sums[k];
     tile_sums[k] = ex;
     ex += v;
   }
   if (t == SUMS_BLOCK - 1) {
-    out[n] = sh[t];
-    *d_total = sh[t];
+    out[n] = total;
+    *d_total = total;
   }
 }
 
