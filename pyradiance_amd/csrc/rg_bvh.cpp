@@ -7,8 +7,12 @@
 #include "rg_bvh.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
+#include <string>
+#include <thread>
 
 namespace rg {
 
@@ -44,6 +48,7 @@ struct Task {
   int32_t parent;      // parent node (-1: root)
   int side;            // 0: left slot, 1: right slot
   int depth;
+  int32_t node;        // this range's node (ranges of two or more refs)
 };
 
 constexpr int NBINS = 32;
@@ -64,6 +69,94 @@ void set_child(BVHNode& N, int side, const Box& bx, int32_t ref) {
 
 }  // namespace
 
+// One range: its node (boxes of both children are written by the children
+// themselves into their slots of this node), and the two child ranges pushed
+// onto `out`.  Nodes are numbered in depth-first preorder with the left
+// subtree first: a range of k refs holds k - 1 internal nodes, so the left
+// child's node follows its parent and the right child's follows the left
+// subtree.  Ranges are disjoint, so ranges of different subtrees can be
+// processed by different threads.
+static void process_range(std::vector<Ref>& refs, std::vector<BVHNode>& nodes, const Task& t, int max_depth,
+                          int median_depth, std::vector<Task>& out) {
+  Box bx, cb;
+  bx.reset();
+  cb.reset();
+  for (uint32_t i = t.b; i < t.e; ++i) {
+    bx.grow(refs[i].lo, refs[i].hi);
+    cb.grow(refs[i].c, refs[i].c);
+  }
+  if (t.e - t.b == 1) {
+    set_child(nodes[t.parent], t.side, bx, ~refs[t.b].id);
+    return;
+  }
+  const int32_t me = t.node;
+  if (t.depth > max_depth) throw std::runtime_error("BVH deeper than the traversal stack");
+  std::memset(&nodes[me], 0, sizeof(BVHNode));
+  if (t.parent >= 0) set_child(nodes[t.parent], t.side, bx, me);
+
+  int axis = 0;
+  float ext[3];
+  for (int a = 0; a < 3; ++a) ext[a] = cb.hi[a] - cb.lo[a];
+  if (ext[1] > ext[axis]) axis = 1;
+  if (ext[2] > ext[axis]) axis = 2;
+  uint32_t mid = t.b + (t.e - t.b) / 2;
+  bool done = false;
+  if (ext[axis] > 0.0f && t.depth < median_depth) {
+    // binned SAH over all three axes
+    double best = 1e300;
+    int best_axis = -1, best_split = 0;
+    for (int a = 0; a < 3; ++a) {
+      if (!(ext[a] > 0.0f)) continue;
+      Box bins[NBINS];
+      uint32_t cnt[NBINS] = {};
+      for (int k = 0; k < NBINS; ++k) bins[k].reset();
+      const float scale = NBINS / ext[a];
+      for (uint32_t i = t.b; i < t.e; ++i) {
+        int k = bin_of(refs[i].c[a], cb.lo[a], scale);
+        cnt[k]++;
+        bins[k].grow(refs[i].lo, refs[i].hi);
+      }
+      double rarea[NBINS];
+      uint32_t rcnt[NBINS];
+      Box acc;
+      acc.reset();
+      uint32_t c = 0;
+      for (int k = NBINS - 1; k > 0; --k) {
+        acc.grow(bins[k]);
+        c += cnt[k];
+        rarea[k] = acc.area();
+        rcnt[k] = c;
+      }
+      acc.reset();
+      c = 0;
+      for (int k = 0; k < NBINS - 1; ++k) {
+        acc.grow(bins[k]);
+        c += cnt[k];
+        if (c == 0 || rcnt[k + 1] == 0) continue;
+        double cost = acc.area() * c + rarea[k + 1] * rcnt[k + 1];
+        if (cost < best) { best = cost; best_axis = a; best_split = k + 1; }
+      }
+    }
+    if (best_axis >= 0) {
+      const int a = best_axis;
+      const float scale = NBINS / ext[a];
+      const float l0 = cb.lo[a];
+      auto it = std::partition(refs.begin() + t.b, refs.begin() + t.e,
+                               [&](const Ref& r) { return bin_of(r.c[a], l0, scale) < best_split; });
+      mid = (uint32_t)(it - refs.begin());
+      done = mid > t.b && mid < t.e;
+    }
+  }
+  if (!done) {
+    mid = t.b + (t.e - t.b) / 2;
+    std::nth_element(refs.begin() + t.b, refs.begin() + mid, refs.begin() + t.e,
+                     [axis](const Ref& x, const Ref& y) { return x.c[axis] < y.c[axis]; });
+  }
+  // right first so that a stack pops the left subtree first
+  out.push_back({mid, t.e, me, 1, t.depth + 1, me + (int32_t)(mid - t.b)});
+  out.push_back({t.b, mid, me, 0, t.depth + 1, me + 1});
+}
+
 int32_t build_sah_bvh(const float* lo, const float* hi, const int32_t* ids, uint32_t n, std::vector<BVHNode>& nodes,
                       int max_depth) {
   nodes.clear();
@@ -79,92 +172,56 @@ int32_t build_sah_bvh(const float* lo, const float* hi, const int32_t* ids, uint
     }
     r.id = ids[i];
   }
-  nodes.reserve(n - 1);
-  std::vector<Task> stack;
-  stack.push_back({0, n, -1, 0, 0});
+  nodes.resize(n - 1);
   const int median_depth = std::max(1, max_depth - 26);   // leaves room for log2(n) median levels
-  while (!stack.empty()) {
-    Task t = stack.back();
-    stack.pop_back();
-    Box bx, cb;
-    bx.reset();
-    cb.reset();
-    for (uint32_t i = t.b; i < t.e; ++i) {
-      bx.grow(refs[i].lo, refs[i].hi);
-      cb.grow(refs[i].c, refs[i].c);
+  // The top of the tree is split on this thread, largest range first, until
+  // there are enough ranges for the host threads; each remaining range's
+  // subtree is then built by whichever thread takes it.  Every range is split
+  // as the one-thread build splits it and its nodes land at the indices the
+  // one-thread build gives them, so the tree is the same for any thread count.
+  unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const uint32_t kSerial = 1u << 16;     // ranges below this size are built by one thread
+  std::vector<Task> front{{0, n, -1, 0, 0, 0}}, kids;
+  if (nthr > 1 && n > kSerial) {
+    for (;;) {
+      auto big = std::max_element(front.begin(), front.end(),
+                                  [](const Task& x, const Task& y) { return x.e - x.b < y.e - y.b; });
+      if (big->e - big->b < kSerial || front.size() >= 8 * (size_t)nthr) break;
+      const Task t = *big;
+      front.erase(big);
+      kids.clear();
+      process_range(refs, nodes, t, max_depth, median_depth, kids);
+      front.insert(front.end(), kids.begin(), kids.end());
     }
-    if (t.e - t.b == 1) {
-      set_child(nodes[t.parent], t.side, bx, ~refs[t.b].id);
-      continue;
-    }
-    int32_t me = (int32_t)nodes.size();
-    if (t.depth > max_depth) throw std::runtime_error("BVH deeper than the traversal stack");
-    nodes.emplace_back();
-    std::memset(&nodes.back(), 0, sizeof(BVHNode));
-    if (t.parent >= 0) set_child(nodes[t.parent], t.side, bx, me);
-
-    int axis = 0;
-    float ext[3];
-    for (int a = 0; a < 3; ++a) ext[a] = cb.hi[a] - cb.lo[a];
-    if (ext[1] > ext[axis]) axis = 1;
-    if (ext[2] > ext[axis]) axis = 2;
-    uint32_t mid = t.b + (t.e - t.b) / 2;
-    bool done = false;
-    if (ext[axis] > 0.0f && t.depth < median_depth) {
-      // binned SAH over all three axes
-      double best = 1e300;
-      int best_axis = -1, best_split = 0;
-      for (int a = 0; a < 3; ++a) {
-        if (!(ext[a] > 0.0f)) continue;
-        Box bins[NBINS];
-        uint32_t cnt[NBINS] = {};
-        for (int k = 0; k < NBINS; ++k) bins[k].reset();
-        const float scale = NBINS / ext[a];
-        for (uint32_t i = t.b; i < t.e; ++i) {
-          int k = bin_of(refs[i].c[a], cb.lo[a], scale);
-          cnt[k]++;
-          bins[k].grow(refs[i].lo, refs[i].hi);
-        }
-        double rarea[NBINS];
-        uint32_t rcnt[NBINS];
-        Box acc;
-        acc.reset();
-        uint32_t c = 0;
-        for (int k = NBINS - 1; k > 0; --k) {
-          acc.grow(bins[k]);
-          c += cnt[k];
-          rarea[k] = acc.area();
-          rcnt[k] = c;
-        }
-        acc.reset();
-        c = 0;
-        for (int k = 0; k < NBINS - 1; ++k) {
-          acc.grow(bins[k]);
-          c += cnt[k];
-          if (c == 0 || rcnt[k + 1] == 0) continue;
-          double cost = acc.area() * c + rarea[k + 1] * rcnt[k + 1];
-          if (cost < best) { best = cost; best_axis = a; best_split = k + 1; }
-        }
-      }
-      if (best_axis >= 0) {
-        const int a = best_axis;
-        const float scale = NBINS / ext[a];
-        const float l0 = cb.lo[a];
-        auto it = std::partition(refs.begin() + t.b, refs.begin() + t.e,
-                                 [&](const Ref& r) { return bin_of(r.c[a], l0, scale) < best_split; });
-        mid = (uint32_t)(it - refs.begin());
-        done = mid > t.b && mid < t.e;
-      }
-    }
-    if (!done) {
-      mid = t.b + (t.e - t.b) / 2;
-      std::nth_element(refs.begin() + t.b, refs.begin() + mid, refs.begin() + t.e,
-                       [axis](const Ref& x, const Ref& y) { return x.c[axis] < y.c[axis]; });
-    }
-    // right first so the left subtree follows its parent in memory
-    stack.push_back({mid, t.e, me, 1, t.depth + 1});
-    stack.push_back({t.b, mid, me, 0, t.depth + 1});
+    std::sort(front.begin(), front.end(), [](const Task& x, const Task& y) { return x.e - x.b > y.e - y.b; });
+  } else {
+    nthr = 1;
   }
+  std::atomic<size_t> next{0};
+  std::mutex err_mu;
+  std::string err;
+  auto worker = [&]() {
+    std::vector<Task> stack;
+    try {
+      for (size_t k; (k = next.fetch_add(1)) < front.size();) {
+        stack.assign(1, front[k]);
+        while (!stack.empty()) {
+          const Task t = stack.back();
+          stack.pop_back();
+          process_range(refs, nodes, t, max_depth, median_depth, stack);
+        }
+      }
+    } catch (const std::exception& e) {
+      std::lock_guard<std::mutex> lk(err_mu);
+      if (err.empty()) err = e.what();
+      next.store(front.size());
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned i = 1; i < nthr; ++i) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  if (!err.empty()) throw std::runtime_error(err);
   return 0;
 }
 

@@ -5,6 +5,8 @@
 // path builds its own BVH from the primitives.
 #include "scene_io.hpp"
 
+#include <charconv>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -61,16 +63,27 @@ void SceneDB::add(RObj&& o) {
 // ---- .rad text ------------------------------------------------------------
 
 namespace {
+// isspace() of the "C" locale
+inline bool is_ws(char c) { return c == ' ' || (c >= '\t' && c <= '\r'); }
+
 struct Lexer {
   const std::string& s;
   size_t p = 0;
   explicit Lexer(const std::string& t) : s(t) {}
-  void skip_ws() { while (p < s.size() && isspace((unsigned char)s[p])) ++p; }
+  void skip_ws() { while (p < s.size() && is_ws(s[p])) ++p; }
   bool eof() { skip_ws(); return p >= s.size(); }
   // fgetword(): whitespace-delimited word, quotes group words (common/fgetword.c)
   bool word(std::string& w) {
     skip_ws();
     if (p >= s.size()) return false;
+    // common case: an unquoted word, copied in one piece
+    size_t e = p;
+    while (e < s.size() && !is_ws(s[e]) && s[e] != '"' && s[e] != '\'') ++e;
+    if (e == s.size() || is_ws(s[e])) {
+      w.assign(s, p, e - p);
+      p = e;
+      return true;
+    }
     w.clear();
     char q = 0;
     while (p < s.size()) {
@@ -79,7 +92,7 @@ struct Lexer {
         if (c == q) { q = 0; ++p; continue; }
         w.push_back(c); ++p; continue;
       }
-      if (isspace((unsigned char)c)) break;
+      if (is_ws(c)) break;
       if (c == '"' || c == '\'') { q = c; ++p; continue; }
       w.push_back(c); ++p;
     }
@@ -103,12 +116,23 @@ struct Lexer {
   }
 };
 
+// Plain decimal words go through std::from_chars (correctly rounded, as
+// glibc's strtod is, so the values are the same); anything else (a sign
+// '+', hexadecimal, inf / nan) through the C library as before.
 bool parse_int(const std::string& w, long& v) {
+  const char* b = w.data();
+  const char* e = b + w.size();
+  auto r = std::from_chars(b, e, v, 10);
+  if (r.ec == std::errc() && r.ptr == e && !w.empty()) return true;
   char* end = nullptr;
   v = strtol(w.c_str(), &end, 10);
   return end && *end == '\0' && !w.empty();
 }
 bool parse_flt(const std::string& w, double& v) {
+  const char* b = w.data();
+  const char* e = b + w.size();
+  auto r = std::from_chars(b, e, v);
+  if (r.ec == std::errc() && r.ptr == e && !w.empty()) return true;
   char* end = nullptr;
   v = strtod(w.c_str(), &end);
   return end && *end == '\0' && !w.empty();
@@ -173,6 +197,7 @@ void SceneDB::parse_stream(const std::string& text, const std::string& where, in
       }
       if (!lx.word(w) || !parse_int(w, n) || n != 0) throw RadError("(" + where + "): bad arguments for " + o.name);
       if (!lx.word(w) || !parse_int(w, n) || n < 0) throw RadError("(" + where + "): bad arguments for " + o.name);
+      o.fargs.reserve((size_t)std::min<long>(n, 1L << 20));
       for (long i = 0; i < n; ++i) {
         double v;
         if (!lx.word(w) || !parse_flt(w, v)) throw RadError("(" + where + "): bad arguments for " + o.name);
