@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -179,7 +180,12 @@ int32_t build_sah_bvh(const float* lo, const float* hi, const int32_t* ids, uint
   // subtree is then built by whichever thread takes it.  Every range is split
   // as the one-thread build splits it and its nodes land at the indices the
   // one-thread build gives them, so the tree is the same for any thread count.
+  // RG_BVH_THREADS caps the thread count (1: the one-thread build)
   unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (const char* env = std::getenv("RG_BVH_THREADS")) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v >= 1) nthr = std::min<unsigned>(nthr, (unsigned)v);
+  }
   const uint32_t kSerial = 1u << 16;     // ranges below this size are built by one thread
   std::vector<Task> front{{0, n, -1, 0, 0, 0}}, kids;
   if (nthr > 1 && n > kSerial) {

@@ -222,6 +222,33 @@ def test_klems_bins_match_reference(L):
         assert np.array_equal(got, g[name]), name
 
 
+def test_sah_tree_independent_of_host_threads(L, monkeypatch):
+    """The host SAH build splits the top of the tree on one thread and hands
+    the subtrees below to the host threads; every range is split and
+    numbered as the one-thread build does it, so first hits (distance,
+    normal, primitive) agree bit for bit whatever the thread count.  150k
+    random triangles, above the size the build starts threads at."""
+    _say("sah threads")
+    rng = np.random.default_rng(5)
+    n = 150_000
+    c0 = rng.random((n, 3)) * 20.0
+    parts = ["void plastic m 0 0 5 .5 .5 .5 0 0"]
+    for i in range(n):
+        v = c0[i] + rng.normal(0, 0.15, (3, 3))
+        parts.append(f"m polygon t{i} 0 0 9 " + " ".join(f"{x:.9g}" for x in v.ravel()))
+    rad = "\n".join(parts) + "\n"
+    rays = np.hstack([rng.random((20_000, 3)) * 20.0, rng.normal(0, 1, (20_000, 3))])
+    out = []
+    for threads in ("1", "16"):
+        monkeypatch.setenv("RG_BVH_THREADS", threads)
+        c = _ctx(L, rad, L.BVH_SAH)
+        out.append(c.first_hit(rays))
+        c.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    assert (out[0][2] >= 0).mean() > 0.3
+
+
 def test_dcsky_against_fp64(L):
     _say("dcsky")
     rng = np.random.default_rng(0)
