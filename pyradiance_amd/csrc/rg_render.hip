@@ -13,6 +13,8 @@
 
 namespace rg {
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 constexpr int BLOCK = 128;       // shading and seed kernels
 constexpr int TBLOCK = 256;      // traversal kernel
 #ifndef RG_LSTACK
@@ -1107,8 +1109,8 @@ __device__ __forceinline__ void shade_glass(const RunCfg& c, const RayS& r, HitS
 }
 
 // rayshade() for a surface hit, followed by rcontrib's trace callback.
-__device__ __forceinline__ void shade_hit(const DevScene& S, const RunCfg& c, const RayS& r, const HitRec& h, const Sink& sk) {
-  const Prim& P = S.prims[h.prim];
+__device__ __forceinline__ void shade_hit(const DevScene& S, const RunCfg& c, const RayS& r, const HitRec& h, const Sink& sk,
+                                          const Prim& P) {
   HitS hs;
   hs.rot = h.t;
   hs.rop = r.o + r.d * h.t;
@@ -1750,18 +1752,58 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RG_S
   if (t >= nthreads) return;
   c.adj = ADJ;
   if (!ADJ) sk.pbuf = nullptr;
-  const TraceOut o = tr[t];
+  // The loads a shaded ray starts with are issued together and waited for
+  // together: the 32-byte trace record (two 16-byte loads), then its spawn
+  // record, the spawn's first child index and, with one scene, the hit
+  // primitive's record.  The empty asm statements take the loaded vectors as
+  // inputs, which keeps the compiler from sinking a load below the early exit
+  // or to its first use, where each would cost a round trip of its own.
+  TraceOut o;
+  {
+    static_assert(sizeof(TraceOut) == 2 * sizeof(u32x4), "TraceOut layout");
+    const u32x4* tp = reinterpret_cast<const u32x4*>(tr + t);
+    const u32x4 a = tp[0], b = tp[1];
+    asm volatile("" :: "v"(a), "v"(b));
+    __builtin_memcpy(&o, &a, sizeof a);
+    __builtin_memcpy(reinterpret_cast<char*>(&o) + sizeof a, &b, sizeof b);
+  }
   if (o.prim == -2) return;
-  uint32_t si = o.spawn;
+  const uint32_t si = o.spawn;
+  Spawn sp;
+  uint64_t off;
+  Prim P{};
+  const bool early = !c.nscenes && o.prim >= 0;
+  {
+    static_assert(sizeof(Spawn) == 6 * sizeof(u32x4), "Spawn layout");
+    const u32x4* spp = reinterpret_cast<const u32x4*>(in + si);
+    const u32x4 s0 = spp[0], s1 = spp[1], s2 = spp[2], s3 = spp[3], s4 = spp[4], s5 = spp[5];
+    off = offs[si];
+    u32x4 p0 = {0u, 0u, 0u, 0u}, p1 = {0u, 0u, 0u, 0u};
+    if (early) {
+      const u32x4* pp = reinterpret_cast<const u32x4*>(Sarg.prims + o.prim);
+      p0 = pp[0];
+      p1 = pp[1];
+    }
+    asm volatile("" :: "v"(s0), "v"(s1), "v"(s2), "v"(s3), "v"(s4), "v"(s5), "v"(off), "v"(p0), "v"(p1));
+    const u32x4 sv[6] = {s0, s1, s2, s3, s4, s5};
+    __builtin_memcpy(&sp, sv, sizeof sp);
+    static_assert(sizeof(Prim) == 2 * sizeof(u32x4), "Prim layout");
+    const u32x4 pv[2] = {p0, p1};
+    __builtin_memcpy(&P, pv, sizeof P);
+  }
   RayS r;
-  build_ray(c, in[si], (uint32_t)(c0 + t - offs[si]), r, ADJ ? pin + si : nullptr);
+  build_ray(c, sp, (uint32_t)(c0 + t - off), r, ADJ ? pin + si : nullptr);
   HitRec h;
   h.t = o.t;
   h.prim = o.prim;
   h.rod = 1.0;
   const DevScene& S = scene_of(Sarg, c, r.scene);
-  if (h.prim >= 0) shade_hit(S, c, r, h, sk);
-  else shade_miss(S, c, r);
+  if (h.prim >= 0) {
+    if (!early) P = S.prims[h.prim];
+    shade_hit(S, c, r, h, sk, P);
+  } else {
+    shade_miss(S, c, r);
+  }
 }
 
 // ---- grouping a batched generation by scene ----------------------------------
