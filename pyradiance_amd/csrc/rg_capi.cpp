@@ -259,6 +259,13 @@ int rg_scene_commit(rg_ctx* c) {
       for (size_t k = 0; k < cs.prims.size(); ++k)
         if (!isbig[k]) small_ids.push_back((int32_t)k);
     }
+    std::vector<BigPlane> bigpl(kBigPlanes, BigPlane{{0.0, 0.0, 0.0}, 0.0});
+    for (size_t k = 0; k < big.size() && k < (size_t)kBigPlanes; ++k) {
+      const Prim& P = cs.prims[(size_t)big[k]];
+      if (P.kind != PK_POLYGON) continue;
+      const double* g = cs.geom.data() + P.goff;
+      bigpl[k] = BigPlane{{g[0], g[1], g[2]}, g[3]};
+    }
     std::vector<float> slo_v, shi_v;
     slo_v.reserve(3 * small_ids.size());
     shi_v.reserve(3 * small_ids.size());
@@ -290,7 +297,7 @@ int rg_scene_commit(rg_ctx* c) {
                  o_srcs = pk.add(cs.srcs), o_scache = pk.add(cs.scache),
                  o_obs = pk.zero(sizeof(unsigned long long) * std::max<int32_t>(cs.ncells, 1)),
                  o_tested = pk.zero(sizeof(uint32_t) * std::max<size_t>(cs.srcs.size(), 1)),
-                 o_cast = pk.add(cs.cast_rays), o_cast_src = pk.add(cs.cast_src), o_big = pk.add(big),
+                 o_cast = pk.add(cs.cast_rays), o_cast_src = pk.add(cs.cast_src), o_big = pk.add(big), o_bigpl = pk.add(bigpl),
                  o_nodes = pk.add(hn.data(), sizeof(BVHNode) * hn.size(), sizeof(BVHNode) * std::max<uint32_t>(n, 1)),
                  o_leaf = pk.add(sah ? small_ids.data() : nullptr, sah ? sizeof(int32_t) * n : 0,
                                  sizeof(int32_t) * std::max<uint32_t>(n, 1)),
@@ -308,6 +315,7 @@ int rg_scene_commit(rg_ctx* c) {
     c->d_cast_rays = (const double*)(blob + o_cast);
     c->d_cast_src = (const int32_t*)(blob + o_cast_src);
     S.big = to_global((const int32_t*)(blob + o_big));
+    S.bigpl = to_global((const BigPlane*)(blob + o_bigpl));
     S.nbig = (int)big.size();
     S.nbvh = (int)n;
     S.nodes4 = wide ? to_global((const BVH4Node*)(blob + o_nodes4)) : nullptr;

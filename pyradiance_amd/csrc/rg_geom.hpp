@@ -31,6 +31,18 @@ struct alignas(16) BVH4Node {
   int32_t pad[4];
 };
 
+// Planes of the first kBigPlanes room-sized primitives, in one contiguous
+// table: the walk estimates their hit distances from these before any
+// primitive test, with one load per entry instead of the chain big index ->
+// primitive record -> geometry.  Polygons hold their plane (normal, offset);
+// other shapes, and entries past the primitive count, hold zeros (the
+// estimate then defers to the full test).
+constexpr int kBigPlanes = 8;
+struct alignas(16) BigPlane {
+  double n[3];
+  double off;
+};
+
 struct DevScene {
   const RG_GLOBAL Prim* prims;
   const RG_GLOBAL double* geom;
@@ -40,6 +52,7 @@ struct DevScene {
   const RG_GLOBAL BVH4Node* nodes4;    // four-wide form of `nodes` (root 0); null: none
   const RG_GLOBAL int32_t* leaf_prim;   // sorted leaf position -> primitive index
   const RG_GLOBAL int32_t* big;         // primitives kept out of the BVH, tested first
+  const RG_GLOBAL BigPlane* bigpl;      // kBigPlanes entries, see BigPlane
   int32_t nbig;
   int32_t nbvh;                 // primitives inside the BVH
   int32_t nprims;

@@ -113,27 +113,29 @@ __device__ __forceinline__ uint64_t pack_entry(int32_t ref, float t) {
 // would reject it and every later one (t > hit + FTINY), so they are skipped.
 // Planes the ray nearly parallels, and other shapes, get the full test.
 // te: this lane's NF estimates, in LDS column-major (te[k * BS] for entry k).
-constexpr int NBIG_ORDER = 8;
+constexpr int NBIG_ORDER = kBigPlanes;
 template <int BS>
 __device__ __forceinline__ void big_hits(const DevScene& S, V3 o, V3 d, HitRec& h, uint32_t& nprim, float* te) {
   constexpr int NF = NBIG_ORDER;
   const int nb = S.nbig;
   uint32_t full = 0;        // big primitives that get the full test unordered
-#pragma unroll 1
+  // all NF plane records requested at once (the table always holds NF)
+  BigPlane bp[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) bp[k] = S.bigpl[k];
+#pragma unroll
   for (int k = 0; k < NF; ++k) {
     te[k * BS] = INFINITY;
     if (k < nb) {
-      const Prim& P = S.prims[S.big[k]];
+      // a zero normal (not a polygon) fails the test below: full test
+      const double* g = bp[k].n;
+      double rdot = -(d.x * g[0] + d.y * g[1] + d.z * g[2]);
+      double num = o.x * g[0] + o.y * g[1] + o.z * g[2] - bp[k].off;
       bool f = true;
-      if (P.kind == PK_POLYGON) {
-        const double* g = S.geom + P.goff;
-        double rdot = -(d.x * g[0] + d.y * g[1] + d.z * g[2]);
-        double num = o.x * g[0] + o.y * g[1] + o.z * g[2] - g[3];
-        if (fabs(rdot) > 4 * FTINY) {
-          float t = (float)num / (float)rdot;
-          if (t > -1e-3f) te[k * BS] = t;   // behind the origin: the FP64 test rejects t <= FTINY
-          f = false;
-        }
+      if (fabs(rdot) > 4 * FTINY) {
+        float t = (float)num / (float)rdot;
+        if (t > -1e-3f) te[k * BS] = t;   // behind the origin: the FP64 test rejects t <= FTINY
+        f = false;
       }
       if (f) full |= 1u << k;
     }
