@@ -1350,6 +1350,21 @@ __device__ __forceinline__ uint32_t find_spawn(const uint64_t* offs, uint32_t ns
   return lo;
 }
 
+// Spawn record si and its first child index, requested together and waited
+// for together (the empty asm statement takes the loaded vectors as inputs,
+// which keeps the compiler from splitting the record's fields over several
+// round trips at their first uses).
+__device__ __forceinline__ void load_spawn(const Spawn* in, const uint64_t* offs, uint32_t si, Spawn& sp,
+                                           uint64_t& off) {
+  static_assert(sizeof(Spawn) == 6 * sizeof(u32x4), "Spawn layout");
+  const u32x4* spp = reinterpret_cast<const u32x4*>(in + si);
+  const u32x4 s0 = spp[0], s1 = spp[1], s2 = spp[2], s3 = spp[3], s4 = spp[4], s5 = spp[5];
+  off = offs[si];
+  asm volatile("" :: "v"(s0), "v"(s1), "v"(s2), "v"(s3), "v"(s4), "v"(s5), "v"(off));
+  const u32x4 sv[6] = {s0, s1, s2, s3, s4, s5};
+  __builtin_memcpy(&sp, sv, sizeof sp);
+}
+
 // Spawn of the first child of every 64-child chunk of a window [c0, c0 +
 // nth), and at index nchunk the spawn of the window's last child: one search
 // per chunk, all in parallel, ahead of the walk.  The walk's refill then
@@ -1485,7 +1500,10 @@ k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint64_t* offs, const
           }
           si = lo;
           RayS r;
-          if (!build_ray(c, in[si], (uint32_t)(gid - offs[si]), r)) {
+          Spawn srec;
+          uint64_t soff;
+          load_spawn(in, offs, si, srec, soff);
+          if (!build_ray(c, srec, (uint32_t)(gid - soff), r)) {
             TraceOut to;
             to.spawn = si; to.pad = 0; to.prim = -2; to.t = 0;
             to.dx = to.dy = to.dz = 0.f;
