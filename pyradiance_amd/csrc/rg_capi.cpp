@@ -259,12 +259,17 @@ int rg_scene_commit(rg_ctx* c) {
       for (size_t k = 0; k < cs.prims.size(); ++k)
         if (!isbig[k]) small_ids.push_back((int32_t)k);
     }
-    std::vector<BigPlane> bigpl(kBigPlanes, BigPlane{{0.0, 0.0, 0.0}, 0.0});
+    std::vector<BigPlane> bigpl(kBigPlanes, BigPlane{{0.0, 0.0, 0.0}, 0.0, -1, 0, 0, 0});
     for (size_t k = 0; k < big.size() && k < (size_t)kBigPlanes; ++k) {
       const Prim& P = cs.prims[(size_t)big[k]];
+      bigpl[k].prim = big[k];
+      bigpl[k].kind = P.kind;
+      bigpl[k].goff = P.goff;
+      bigpl[k].nv = P.nv;
       if (P.kind != PK_POLYGON) continue;
       const double* g = cs.geom.data() + P.goff;
-      bigpl[k] = BigPlane{{g[0], g[1], g[2]}, g[3]};
+      for (int a = 0; a < 3; ++a) bigpl[k].n[a] = g[a];
+      bigpl[k].off = g[3];
     }
     std::vector<float> slo_v, shi_v;
     slo_v.reserve(3 * small_ids.size());

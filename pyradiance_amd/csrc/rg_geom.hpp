@@ -34,13 +34,18 @@ struct alignas(16) BVH4Node {
 // Planes of the first kBigPlanes room-sized primitives, in one contiguous
 // table: the walk estimates their hit distances from these before any
 // primitive test, with one load per entry instead of the chain big index ->
-// primitive record -> geometry.  Polygons hold their plane (normal, offset);
-// other shapes, and entries past the primitive count, hold zeros (the
-// estimate then defers to the full test).
+// primitive record -> geometry, and tests a polygon among them from the
+// entry's plane, geometry offset and vertex count.  Polygons hold their
+// plane (normal, offset); other shapes, and entries past the primitive
+// count, hold zeros (the estimate then defers to the full test).
 constexpr int kBigPlanes = 8;
 struct alignas(16) BigPlane {
   double n[3];
   double off;
+  int32_t prim;      // primitive index (-1: no entry)
+  int32_t kind;      // PrimKind
+  int32_t goff;      // geometry offset (doubles)
+  int32_t nv;        // polygon vertex count
 };
 
 struct DevScene {
@@ -189,6 +194,19 @@ RG_HD bool tri_baryc(double wt[3], V3 p, V3 v1, V3 v2, V3 v3) {
 }
 
 // Test one primitive; updates h when it wins.
+// Polygon pi (plane normal n and offset off, geometry g, nv vertices):
+// o_face.c's plane distance, hit rejection and inside test.
+RG_HD void hit_polygon(const DevScene& S, int pi, V3 n, double off, const double* g, int nv, V3 o, V3 d,
+                       HitRec& h) {
+  double rdot = -dot(d, n);
+  if ((rdot <= FTINY) & (rdot >= -FTINY)) return;
+  double t = (dot(o, n) - off) / rdot;
+  if (ray_reject(S, pi, t, rdot, h)) return;
+  V3 p = o + d * t;
+  if (!in_face(p, g, nv)) return;
+  h.t = t; h.rod = rdot; h.prim = pi;
+}
+
 RG_HD void hit_prim(const DevScene& S, int pi, V3 o, V3 d, HitRec& h) {
   const Prim& P = S.prims[pi];
   const double* g = S.geom + P.goff;
@@ -211,17 +229,9 @@ RG_HD void hit_prim(const DevScene& S, int pi, V3 o, V3 d, HitRec& h) {
       h.t = t; h.rod = rod; h.prim = pi;
       return;
     }
-    case PK_POLYGON: {
-      V3 n = load3(g);
-      double rdot = -dot(d, n);
-      if ((rdot <= FTINY) & (rdot >= -FTINY)) return;
-      double t = (dot(o, n) - g[3]) / rdot;
-      if (ray_reject(S, pi, t, rdot, h)) return;
-      V3 p = o + d * t;
-      if (!in_face(p, g, P.nv)) return;
-      h.t = t; h.rod = rdot; h.prim = pi;
+    case PK_POLYGON:
+      hit_polygon(S, pi, load3(g), g[3], g, P.nv, o, d, h);
       return;
-    }
     case PK_SPHERE:
     case PK_BUBBLE: {
       double a = dot(d, d);

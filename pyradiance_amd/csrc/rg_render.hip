@@ -162,7 +162,17 @@ __device__ __forceinline__ void big_hits(const DevScene& S, V3 o, V3 d, HitRec& 
       te[bk * BS] = INFINITY;
       pk = bk;
     }
-    hit_prim(S, S.big[pk], o, d, h);
+    if (pk < NF) {
+      // plane, geometry offset and vertex count from the table entry (one
+      // load, where the record's own chain is three)
+      const BigPlane B = S.bigpl[pk];
+      if (B.kind == PK_POLYGON)
+        hit_polygon(S, B.prim, v3(B.n[0], B.n[1], B.n[2]), B.off, S.geom + B.goff, B.nv, o, d, h);
+      else
+        hit_prim(S, B.prim, o, d, h);
+    } else {
+      hit_prim(S, S.big[pk], o, d, h);
+    }
     ++nprim;
   }
 }
