@@ -327,15 +327,14 @@ RG_HD void hit_prim(const DevScene& S, int pi, V3 o, V3 d, HitRec& h) {
 }
 
 // Surface normal at a hit (the reference's ron), FP64.
-RG_HD V3 prim_normal(const DevScene& S, int pi, V3 p) {
-  const Prim& P = S.prims[pi];
-  const double* g = S.geom + P.goff;
-  switch (P.kind) {
+RG_HD V3 prim_normal_k(const DevScene& S, int kind, int goff, V3 p) {
+  const double* g = S.geom + goff;
+  switch (kind) {
     case PK_POLYGON: return load3(g);
     case PK_TRIANGLE: return load3(g + 9);
     case PK_SPHERE:
     case PK_BUBBLE: {
-      double rad = g[3] * (1 - 2 * (P.kind == PK_BUBBLE));
+      double rad = g[3] * (1 - 2 * (kind == PK_BUBBLE));
       return (p - load3(g)) * (1.0 / rad);
     }
     case PK_RING: return load3(g + 3);
@@ -345,13 +344,13 @@ RG_HD V3 prim_normal(const DevScene& S, int pi, V3 p) {
       V3 rdx = p - p0;
       double hb = dot(rdx, ad);
       V3 nrm;
-      if ((P.kind == PK_CYLINDER) | (P.kind == PK_TUBE)) {
-        double aa = (P.kind == PK_CYLINDER) ? r0 : -r0;
+      if ((kind == PK_CYLINDER) | (kind == PK_TUBE)) {
+        double aa = (kind == PK_CYLINDER) ? r0 : -r0;
         nrm = (rdx - ad * hb) * (1.0 / aa);
       } else {
         double cc = r1 - r0;
         double aa = r0 + hb * cc / al;
-        if (P.kind == PK_CUP) { cc = -cc; aa = -aa; }
+        if (kind == PK_CUP) { cc = -cc; aa = -aa; }
         nrm = (rdx - ad * hb) * (1.0 / aa);
         nrm = (nrm * al - ad * cc) * (1.0 / g[9]);
       }
@@ -361,15 +360,18 @@ RG_HD V3 prim_normal(const DevScene& S, int pi, V3 p) {
     }
   }
 }
+RG_HD V3 prim_normal(const DevScene& S, int pi, V3 p) {
+  const Prim& P = S.prims[pi];
+  return prim_normal_k(S, P.kind, P.goff, p);
+}
 
 // Normal perturbation at a hit (the reference's pert): mesh triangles with
 // vertex normals interpolate them barycentrically (rt/o_mesh.c:192-204);
 // everything else is unperturbed.
-RG_HD V3 prim_pert(const DevScene& S, int pi, V3 p, V3 ron) {
-  const Prim& P = S.prims[pi];
+RG_HD V3 prim_pert_k(const DevScene& S, int kind, int goff, V3 p, V3 ron) {
   V3 z = {0.0, 0.0, 0.0};
-  if (P.kind != PK_TRIANGLE) return z;
-  const double* g = S.geom + P.goff;
+  if (kind != PK_TRIANGLE) return z;
+  const double* g = S.geom + goff;
   if (!((int)g[12] & TRI_NORMALS)) return z;
   double wt[3];
   if (!tri_baryc(wt, p, load3(g), load3(g + 3), load3(g + 6))) return z;
@@ -379,10 +381,17 @@ RG_HD V3 prim_pert(const DevScene& S, int pi, V3 p, V3 ron) {
   pn = pn * (1.0 / sqrt(l2));
   return pn - ron;
 }
+RG_HD V3 prim_pert(const DevScene& S, int pi, V3 p, V3 ron) {
+  const Prim& P = S.prims[pi];
+  return prim_pert_k(S, P.kind, P.goff, p, ron);
+}
 
+RG_HD bool prim_phong_k(const DevScene& S, int kind, int goff) {
+  return kind == PK_TRIANGLE && ((int)S.geom[goff + 12] & TRI_PHONG);
+}
 RG_HD bool prim_phong(const DevScene& S, int pi) {
   const Prim& P = S.prims[pi];
-  return P.kind == PK_TRIANGLE && ((int)S.geom[P.goff + 12] & TRI_PHONG);
+  return prim_phong_k(S, P.kind, P.goff);
 }
 
 }  // namespace rg

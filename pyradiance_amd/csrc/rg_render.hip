@@ -1126,12 +1126,12 @@ __device__ __forceinline__ void shade_hit(const DevScene& S, const RunCfg& c, co
   HitS hs;
   hs.rot = h.t;
   hs.rop = r.o + r.d * h.t;
-  hs.ron = prim_normal(S, h.prim, hs.rop);
+  hs.ron = prim_normal_k(S, P.kind, P.goff, hs.rop);
   hs.rod = -dot(r.d, hs.ron);
-  hs.pert = prim_pert(S, h.prim, hs.rop, hs.ron);
+  hs.pert = prim_pert_k(S, P.kind, P.goff, hs.rop, hs.ron);
   hs.prim = h.prim;
   hs.flat = P.flat & PF_FLAT;
-  hs.phong = prim_phong(S, h.prim);
+  hs.phong = prim_phong_k(S, P.kind, P.goff);
   if (P.mat < 0) { atomicOr(c.err, ERR_NOMAT); return; }
   const Material& m = S.mats[P.mat];
   bool zeroed = false;
