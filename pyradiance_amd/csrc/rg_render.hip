@@ -1504,6 +1504,15 @@ k_trace_pf(DevScene Sarg, RunCfg c, const Spawn* in, const uint64_t* offs, const
           // first child and that of the next chunk's (or the window's last child)
           const uint32_t ck = my >> CHUNK_LOG;
           uint32_t lo = first[ck], hi = first[ck + 1] + 1;
+          if (hi - lo > 2) {
+            // a run of one-child spawns (the later generations) puts child
+            // gid at spawn lo + (gid - offs[lo]): try that spawn first, two
+            // dependent loads instead of a search of six or seven
+            const uint64_t o0 = offs[lo];
+            const uint32_t g = (uint32_t)min((uint64_t)(hi - 2), (uint64_t)lo + (gid - o0));
+            const uint64_t og = offs[g], og1 = offs[g + 1];
+            if (og <= gid && gid < og1) { lo = g; hi = g + 1; }
+          }
           while (hi - lo > 1) {
             uint32_t mid = (lo + hi) >> 1;
             if (offs[mid] <= gid) lo = mid; else hi = mid;
