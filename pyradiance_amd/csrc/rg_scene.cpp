@@ -360,7 +360,16 @@ CompiledScene compile_scene(const SceneDB& db) {
     }
     return -1;
   };
-  auto findmaterial = [&](int oi) -> int { return findmaterial_from(oi, objs[oi].omod); };
+  // the chain depends only on the modifier: remembered per modifier object
+  // (a large scene has millions of surfaces and a few hundred modifiers)
+  std::vector<int> mat_of_mod(nobj, INT32_MIN);
+  auto findmaterial = [&](int oi) -> int {
+    const int m = objs[oi].omod;
+    if (m < 0) return findmaterial_from(oi, m);
+    int& r = mat_of_mod[m];
+    if (r == INT32_MIN) r = findmaterial_from(oi, m);
+    return r;
+  };
 
   std::function<int(int)> material_index = [&](int mo) -> int {
     if (mo < 0) return -1;
@@ -434,7 +443,14 @@ CompiledScene compile_scene(const SceneDB& db) {
     }
   };
 
-  // pass 1: primitives
+  // pass 1: primitives (tables reserved from the argument counts: a
+  // polygon's geometry is its 3 nv coordinates plus 5 plane values)
+  {
+    size_t nf = 0;
+    for (const RObj& o : objs) nf += o.fargs.size() + 5;
+    cs.prims.reserve((size_t)nobj);
+    cs.geom.reserve(nf);
+  }
   std::vector<int> obj_prim(nobj, -1);
   std::vector<std::array<double, 6>> dbounds;
   // One surface object -> one primitive (-1 if degenerate).  mo: material
