@@ -4,6 +4,8 @@
 // the nearest box entry for random rays agrees with brute force.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <random>
 #include <tuple>
 #include <vector>
@@ -234,4 +236,30 @@ extern "C" int check_bvh4(int n, int nrays, unsigned seed, int max_depth, int cl
     if (ti != bi) bad |= 64;
   }
   return bad;
+}
+
+// The tree built by one host thread and by up to 16 (RG_BVH_THREADS) for the
+// same boxes: returns 0 when the node arrays are identical byte for byte.
+extern "C" int check_sah_threads(int n, unsigned seed, int clustered) {
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<float> U(0.0f, 1.0f);
+  std::vector<float> lo(3 * n), hi(3 * n);
+  std::vector<int32_t> ids(n);
+  for (int i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      float c = clustered ? (float)(i % 13) : U(rng) * 10.0f;
+      float h = 0.001f + U(rng) * 0.2f;
+      lo[3 * i + a] = c - h;
+      hi[3 * i + a] = c + h;
+    }
+    ids[i] = i;
+  }
+  std::vector<BVHNode> one, many;
+  setenv("RG_BVH_THREADS", "1", 1);
+  const int32_t r1 = rg::build_sah_bvh(lo.data(), hi.data(), ids.data(), (uint32_t)n, one, 60);
+  setenv("RG_BVH_THREADS", "16", 1);
+  const int32_t r2 = rg::build_sah_bvh(lo.data(), hi.data(), ids.data(), (uint32_t)n, many, 60);
+  unsetenv("RG_BVH_THREADS");
+  if (r1 != r2 || one.size() != many.size()) return 1;
+  return std::memcmp(one.data(), many.data(), one.size() * sizeof(BVHNode)) != 0 ? 2 : 0;
 }

@@ -24,6 +24,22 @@ int fail(char* err, int errlen, const std::string& m) {
 
 extern "C" {
 
+// Reads scene text as the library does and copies the real arguments of
+// object `obj` (at most cap); returns their count, or -1 with the message.
+int scene_text_fargs(const char* text, int obj, double* out, int cap, char* err, int errlen) {
+  try {
+    SceneDB db;
+    db.load_rad_text(text, "scene text");
+    if (obj < 0 || obj >= (int)db.objs.size()) return fail(err, errlen, "no such object");
+    const auto& f = db.objs[(size_t)obj].fargs;
+    for (size_t i = 0; i < f.size() && (int)i < cap; ++i) out[i] = f[i];
+    return (int)f.size();
+  } catch (const std::exception& e) {
+    return fail(err, errlen, e.what());
+  }
+}
+
+
 // Loads the files in order and compiles them; returns the primitive count.
 int scene_compile(const char** files, int nfiles, char* err, int errlen) {
   try {

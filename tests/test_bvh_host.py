@@ -20,12 +20,13 @@ def bvhcheck(tmp_path_factory):
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
     out = tmp_path_factory.mktemp("bvh") / "bvhcheck.so"
-    subprocess.run(["g++", "-O2", "-shared", "-fPIC", "-std=c++17", f"-I{ROOT / 'pyradiance_amd/csrc'}",
+    subprocess.run(["g++", "-O2", "-shared", "-fPIC", "-std=c++17", "-pthread", f"-I{ROOT / 'pyradiance_amd/csrc'}",
                     str(ROOT / "tests/native/bvh_check.cpp"), str(ROOT / "pyradiance_amd/csrc/rg_bvh.cpp"),
                     "-o", str(out)], check=True)
     L = C.CDLL(str(out))
     L.check_sah.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_int)]
     L.check_bvh4.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.check_sah_threads.argtypes = [C.c_int, C.c_uint, C.c_int]
     return L
 
 
@@ -56,3 +57,12 @@ def test_four_wide_tree_holds_the_grandchildren_and_walks_like_brute_force(bvhch
     if n >= 2:
         # binary depth counts edges below the root, four-wide depth counts nodes
         assert d4.value <= (d2.value + 2) // 2 + 1
+
+
+@pytest.mark.parametrize("n,clustered", [(70000, 0), (200000, 0), (200000, 1)])
+def test_sah_tree_is_the_same_for_any_host_thread_count(bvhcheck, n, clustered):
+    """Above 65,536 boxes the builder splits the top of the tree on one
+    thread and builds the subtrees below on up to 16: each range is split
+    and its nodes numbered as the one-thread build does, so the node arrays
+    agree byte for byte."""
+    assert bvhcheck.check_sah_threads(n, 7, clustered) == 0

@@ -30,6 +30,7 @@ def sc(tmp_path_factory):
     L.scene_trace.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
     L.scene_prims.argtypes = [C.POINTER(C.c_int)]
     L.scene_mat_name.restype = C.c_char_p
+    L.scene_text_fargs.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_char_p, C.c_int]
     return L
 
 
@@ -154,3 +155,25 @@ def test_reference_octrees_host(sc):
     assert compile_files(sc, [GOLDEN / "refres" / "contrib.oct"]) == 2
     with pytest.raises(RuntimeError, match=r'"skyfunc" of type "brightfunc"'):
         compile_files(sc, [GOLDEN / "refres" / "trace.oct"])
+
+
+def test_scene_reader_numbers_match_the_c_library(sc):
+    """The scene reader takes plain decimal words through std::from_chars and
+    anything else (a leading '+', hexadecimal, inf / nan, out-of-range
+    magnitudes) through strtod, as before: every value must be the one
+    strtod gives (Python's float() is correctly rounded like both)."""
+    words = ["1", "-0", "+1.5", "0.1", "1e-320", "2.5E+3", ".5", "5.", "-7.25e-3",
+             "0x1p3", "inf", "-inf", "1e400", "-1e400", "123456789012345678901234567890",
+             "0.30000000000000004", "4.9406564584124654e-324", "2.2250738585072011e-308"]
+    text = "void plastic m 0 0 %d %s\n" % (len(words), " ".join(words))
+    out = (C.c_double * len(words))()
+    err = C.create_string_buffer(256)
+    n = sc.scene_text_fargs(text.encode(), 0, out, len(words), err, 256)
+    assert n == len(words), err.value
+    for w, v in zip(words, out):
+        want = float.fromhex(w) if w.lower().startswith(("0x", "-0x", "+0x")) else float(w)
+        assert v == want and np.signbit(v) == np.signbit(want), (w, v, want)
+    # a quoted word and a comment still parse as before
+    text = "void plastic m 0 0 5 1 2 3 4 5\n# comment\nvoid mixfunc x 4 'a b' c d e 0 0\n"
+    assert sc.scene_text_fargs(text.encode(), 0, out, 5, err, 256) == 5
+    assert list(out[:5]) == [1.0, 2.0, 3.0, 4.0, 5.0]
